@@ -1,0 +1,126 @@
+/*
+ * cafe_oracle.h -- FP64 CPU restatement of CAFE5's likelihood hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
+ * compared against.  Only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * leg of bench.py may load it.  The product (cafe5_amd/) never links or calls it.
+ *
+ * Every function follows the arithmetic order of the reference so that its
+ * results agree with a compiled reference to ~1 ulp (see oracle/_ref).
+ *
+ * Reference files followed (paths relative to the CAFE5 source tree):
+ *   src/matrix_cache.h:44-63          quantized matrix keys
+ *   src/matrix_cache.cpp:32-58        matrix::multiply
+ *   src/matrix_cache.cpp:78-163       get_from_parent_fam_size_to_c, precalculate_matrices
+ *   src/probability.cpp:55-167        lgamma cache, chooseln, birthdeath closed form
+ *   src/probability.cpp:175-234       compute_node_probability (leaf / root / internal)
+ *   src/core.cpp:134-145              inference_prune
+ *   src/base_model.cpp:53-100         base_model::infer_family_likelihoods
+ *   src/gamma_core.cpp:97-237         gamma_model prune / posterior / infer
+ *   src/gamma.cpp:15-241              PAML discrete gamma (AS91, AS32, AS70)
+ *   src/root_equilibrium_distribution.cpp:15-87   root priors (float valued)
+ *   src/poisson.cpp:21-24             poisspdf
+ */
+#ifndef CAFE_ORACLE_H
+#define CAFE_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* probability.cpp:69-80: fills the log-gamma table (glibc lgamma of integers). */
+void orc_init(void);
+
+/* probability.cpp:82-91 */
+double orc_chooseln(double n, double r);
+/* probability.cpp:104-148 */
+double orc_birthdeath(int s, int c, double log_alpha, double coeff);
+/* probability.cpp:150-167 */
+double orc_transition(double lambda, double t, int parent_size, int child_size);
+/* matrix_cache.cpp:107-111 */
+int orc_is_saturated(double t, double lambda);
+/* matrix_cache.h:49-62: quantize and de-quantize a key component. */
+double orc_key_lambda(double lambda);
+double orc_key_branch_length(double t);
+
+/* matrix_cache.cpp:113-163 for a single key at de-quantized (lambda_q, t_q);
+ * out is N*N row-major, out[s*N + c] = P(s -> c). */
+void orc_build_matrix(double lambda_q, double t_q, int N, double *out);
+
+/*
+ * Tree: n_nodes nodes, parent[i] == -1 for the root.  leaf_col[i] >= 0 marks a
+ * leaf and names its column in a family's count row; -1 marks an internal node.
+ * A node's children are taken in increasing node index (the host lists them in
+ * the reference's descendant order).
+ *
+ * Matrices: node_mat[i] indexes a pool of N*N row-major matrices and names the
+ * matrix of the branch above node i (ignored for the root).
+ *
+ * Error model: err is (B+1) x n_dev, row k = error_model::get_probs(k)
+ * (error_model.cpp:52-57); NULL for no error model.
+ */
+typedef struct {
+    int n_nodes;
+    const int *parent;
+    const int *leaf_col;
+} orc_tree;
+
+/* matrix_cache.cpp:50-56 (matrix::multiply): result[s-s_min] =
+ * sum_{c=c_min..c_max} M[s][c] v[c-c_min], ascending c, from 0.  M is N*N. */
+void orc_matvec(const double *M, int N, const double *v, int s_min, int s_max,
+                int c_min, int c_max, double *result);
+
+/* probability.cpp:184-204: leaf vector (B+1 entries) for an observed count. */
+int orc_leaf_vector(int count, const double *err, int n_dev, int B, double *out);
+
+/* core.cpp:134-145 with compute_node_probability (probability.cpp:175-234).
+ * Writes the root vector (R entries, root size j+1 at index j).  Returns 0, or
+ * -1 for malformed input. */
+int orc_prune(const orc_tree *tree, const int *counts, const double *mat_pool,
+              const int *node_mat, int N, const double *err, int n_dev,
+              int B, int R, double *out_root);
+
+/* base_model.cpp:77-95 for F families whose root vectors are given
+ * (roots: F x R).  prior[j] = double(root_equilibrium_distribution::compute(j)).
+ * Writes lnL_f and returns -sum_f lnL_f (serial, family order). */
+double orc_base_reduce(int F, int R, const double *roots, const double *prior,
+                       double *out_lnl);
+
+/* gamma_core.cpp:143-237 for F families and K categories.  roots is F x K x R.
+ * Writes cat[F*K] (category likelihood), fam[F], post[F*K], lnl[F].
+ * Returns -sum lnL, or +inf if any family has a zero-sum root vector
+ * (gamma_core.cpp:151-152, 216-225); *n_failed receives that family count. */
+double orc_gamma_reduce(int F, int K, int R, const double *roots,
+                        const double *prior, const double *cat_probs,
+                        double *out_cat, double *out_fam, double *out_post,
+                        double *out_lnl, int *n_failed);
+
+/* Whole evaluation as the reference runs it per likelihood call: build every
+ * (lambda x unique t) matrix, prune every family, reduce.  n_cat == 0 runs the
+ * base model; n_cat >= 1 the gamma model with multipliers[k] and cat_probs[k].
+ *   node_lambda[i]: lambda of the branch above node i (multi-lambda trees
+ *   give different values per branch); node_t[i]: its raw branch length.
+ * counts: F x n_leaves.  Threads: OpenMP over families when n_threads > 1.
+ * Outputs as orc_base_reduce / orc_gamma_reduce (out_cat/out_post may be NULL
+ * for the base model). */
+double orc_evaluate(const orc_tree *tree, const double *node_lambda,
+                    const double *node_t, int F, int n_leaves, const int *counts,
+                    int B, int R, const double *prior, const double *err,
+                    int n_dev, int n_cat, const double *multipliers,
+                    const double *cat_probs, double *out_lnl, double *out_cat,
+                    double *out_fam, double *out_post, int n_threads);
+
+/* gamma.cpp:225-241: K median-discretized gamma rates at shape alpha
+ * (beta = alpha); freq[k] = 1/K. */
+void orc_get_gamma(int K, double alpha, double *freq, double *rate);
+
+/* root_equilibrium_distribution.cpp: prior[j] = double(float(compute(j))), j < R */
+void orc_prior_uniform(int max_size, int R, double *prior);
+void orc_prior_poisson(double poisson_lambda, int num_values, int R, double *prior);
+/* map constructor: sizes[i] with counts[i] entries */
+void orc_prior_map(int n, const int *sizes, const int *counts, int R, double *prior);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

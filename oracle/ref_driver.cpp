@@ -1,0 +1,241 @@
+// ref_driver.cpp -- C entry points over the compiled CAFE5 reference sources.
+//
+// TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together with
+// the reference's own hot-path sources (read in place from the reference tree,
+// never copied) into oracle/_ref/libcaferef.so.  Tests use it to pin the oracle
+// (cafe_oracle.c) to the reference at ~1 ulp; nothing in the product loads it.
+//
+// Every function here only calls reference code: matrix_cache, inference_prune,
+// base_model / gamma_model::infer_family_likelihoods, get_gamma, and the
+// root_equilibrium_distribution constructors.
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "easylogging++.h"
+INITIALIZE_EASYLOGGINGPP
+
+#include "base_model.h"
+#include "clade.h"
+#include "core.h"
+#include "error_model.h"
+#include "gamma.h"
+#include "gamma_core.h"
+#include "gene_family.h"
+#include "lambda.h"
+#include "matrix_cache.h"
+#include "probability.h"
+#include "root_equilibrium_distribution.h"
+
+// The reference's executables define this engine (cafexp.cpp / test.cpp).
+std::mt19937 randomizer_engine(10);
+void init_lgamma_cache();
+double chooseln(double n, double r);
+
+#define REF_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+// Access to the protected per-family records of the reference models.
+struct base_probe : public base_model {
+    using base_model::base_model;
+    const std::vector<family_info_stash>& stash() const { return results; }
+};
+struct gamma_probe : public gamma_model {
+    using gamma_model::gamma_model;
+    const std::vector<family_info_stash>& stash() const { return results; }
+};
+
+std::vector<std::string> split_names(const char* names)
+{
+    std::vector<std::string> out;
+    std::istringstream ist(names);
+    std::string s;
+    while (ist >> s) out.push_back(s);
+    return out;
+}
+
+std::vector<gene_family> make_families(int F, const char* leaf_names, const int* counts)
+{
+    auto names = split_names(leaf_names);
+    std::vector<gene_family> fams(F);
+    for (int f = 0; f < F; ++f) {
+        fams[f].set_id("F" + std::to_string(f));
+        for (size_t i = 0; i < names.size(); ++i)
+            fams[f].set_species_size(names[i], counts[(size_t)f * names.size() + i]);
+    }
+    return fams;
+}
+
+// error model rows 0..n_rows-1 with deviations -h..h (n_dev = 2h+1)
+error_model* make_error_model(int n_dev, int n_rows, const double* rows, int max_size)
+{
+    if (!rows) return nullptr;
+    auto* em = new error_model();
+    std::vector<std::string> devs;
+    for (int i = 0; i < n_dev; ++i) devs.push_back(std::to_string(i - (n_dev - 1) / 2));
+    em->set_deviations(devs);
+    em->set_max_family_size(max_size);
+    for (int r = 0; r < n_rows; ++r)
+        em->set_probabilities(r, std::vector<double>(rows + (size_t)r * n_dev, rows + (size_t)(r + 1) * n_dev));
+    return em;
+}
+
+lambda* make_lambda(int n_lambdas, const double* lambdas, const char* lambda_tree)
+{
+    if (lambda_tree && lambda_tree[0]) {
+        clade* lt = parse_newick(lambda_tree, true);
+        auto m = lt->get_lambda_index_map();
+        delete lt;
+        return new multiple_lambda(m, std::vector<double>(lambdas, lambdas + n_lambdas));
+    }
+    return new single_lambda(lambdas[0]);
+}
+
+}  // namespace
+
+REF_API void ref_init() { init_lgamma_cache(); }
+
+REF_API double ref_chooseln(double n, double r) { return chooseln(n, r); }
+
+REF_API double ref_birthdeath(int s, int c, double log_alpha, double coeff)
+{
+    return birthdeath_rate_with_log_alpha(s, c, log_alpha, coeff);
+}
+
+REF_API double ref_transition(double lambda, double t, int s, int c)
+{
+    return the_probability_of_going_from_parent_fam_size_to_c(lambda, t, s, c);
+}
+
+// matrix_cache(N).precalculate_matrices({lambda}, {t}); get_matrix(t, lambda)
+REF_API int ref_build_matrix(double lambda, double t, int N, double* out)
+{
+    try {
+        matrix_cache cache(N);
+        cache.precalculate_matrices({lambda}, std::set<double>{t});
+        const matrix* m = cache.get_matrix(t, lambda);
+        for (int s = 0; s < N; ++s)
+            for (int c = 0; c < N; ++c) out[(size_t)s * N + c] = m->get(s, c);
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// inference_prune (core.cpp:134) on one family; out_root has R entries.
+REF_API int ref_prune(const char* newick, const char* leaf_names, const int* counts,
+                      double lambda, double multiplier, int B, int R,
+                      int n_dev, int n_err_rows, const double* err_rows, double* out_root)
+{
+    try {
+        clade* tree = parse_newick(newick);
+        auto fams = make_families(1, leaf_names, counts);
+        error_model* em = make_error_model(n_dev, n_err_rows, err_rows, B);
+        single_lambda lam(lambda);
+        matrix_cache cache(std::max(R, B) + 1);
+        cache.precalculate_matrices({lambda * multiplier}, tree->get_branch_lengths());
+        auto v = inference_prune(fams[0], cache, &lam, em, tree, multiplier, R, B);
+        std::copy(v.begin(), v.end(), out_root);
+        delete em;
+        delete tree;
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// root_equilibrium_distribution variants: kind 0 uniform(max_size),
+// 1 Poisson(poisson_lambda, max_size), 2 map(sizes, counts)
+static root_equilibrium_distribution* make_prior(int kind, int max_size, double poisson_lambda,
+                                                 int n_map, const int* map_sizes, const int* map_counts)
+{
+    if (kind == 1) return new root_equilibrium_distribution(poisson_lambda, (size_t)max_size);
+    if (kind == 2) {
+        std::map<int, int> m;
+        for (int i = 0; i < n_map; ++i) m[map_sizes[i]] = map_counts[i];
+        return new root_equilibrium_distribution(m);
+    }
+    return new root_equilibrium_distribution((size_t)max_size);
+}
+
+REF_API void ref_prior(int kind, int max_size, double poisson_lambda, int n_map,
+                       const int* map_sizes, const int* map_counts, int R, double* out)
+{
+    auto* p = make_prior(kind, max_size, poisson_lambda, n_map, map_sizes, map_counts);
+    for (int j = 0; j < R; ++j) out[j] = p->compute(j);
+    delete p;
+}
+
+// base_model::infer_family_likelihoods; returns -lnL (or +inf), out_lnl[F]
+REF_API double ref_base_model(const char* newick, const char* leaf_names, int F, const int* counts,
+                              int n_lambdas, const double* lambdas, const char* lambda_tree,
+                              int B, int R, int prior_kind, int prior_max, double poisson_lambda,
+                              int n_dev, int n_err_rows, const double* err_rows, double* out_lnl)
+{
+    try {
+        clade* tree = parse_newick(newick);
+        auto fams = make_families(F, leaf_names, counts);
+        error_model* em = make_error_model(n_dev, n_err_rows, err_rows, B);
+        lambda* lam = make_lambda(n_lambdas, lambdas, lambda_tree);
+        auto* prior = make_prior(prior_kind, prior_max, poisson_lambda, 0, nullptr, nullptr);
+        base_probe model(lam, tree, &fams, B, R, em);
+        double v = model.infer_family_likelihoods(*prior, lam);
+        const auto& st = model.stash();
+        for (int f = 0; f < F && (size_t)f < st.size(); ++f) out_lnl[f] = st[f].posterior_probability;
+        delete prior;
+        delete lam;
+        delete em;
+        delete tree;
+        return v;
+    } catch (...) {
+        return std::nan("");
+    }
+}
+
+// gamma_model with explicit categories (gamma_core.h constructor 2);
+// out arrays F*K (cat, post) and F (fam); returns -lnL or +inf
+REF_API double ref_gamma_model(const char* newick, const char* leaf_names, int F, const int* counts,
+                               int n_lambdas, const double* lambdas, const char* lambda_tree,
+                               int B, int R, int K, const double* cat_probs, const double* multipliers,
+                               int prior_kind, int prior_max, double poisson_lambda,
+                               int n_dev, int n_err_rows, const double* err_rows,
+                               double* out_cat, double* out_fam, double* out_post)
+{
+    try {
+        clade* tree = parse_newick(newick);
+        auto fams = make_families(F, leaf_names, counts);
+        error_model* em = make_error_model(n_dev, n_err_rows, err_rows, B);
+        lambda* lam = make_lambda(n_lambdas, lambdas, lambda_tree);
+        auto* prior = make_prior(prior_kind, prior_max, poisson_lambda, 0, nullptr, nullptr);
+        gamma_probe model(lam, tree, &fams, B, R,
+                          std::vector<double>(cat_probs, cat_probs + K),
+                          std::vector<double>(multipliers, multipliers + K), em);
+        double v = model.infer_family_likelihoods(*prior, lam);
+        const auto& st = model.stash();
+        for (size_t i = 0; i < st.size() && i < (size_t)F * K; ++i) {
+            out_cat[i] = st[i].category_likelihood;
+            out_post[i] = st[i].posterior_probability;
+            if (i % K == 0) out_fam[i / K] = st[i].family_likelihood;
+        }
+        delete prior;
+        delete lam;
+        delete em;
+        delete tree;
+        return v;
+    } catch (...) {
+        return std::nan("");
+    }
+}
+
+REF_API void ref_get_gamma(int K, double alpha, double* freq, double* rate)
+{
+    std::vector<double> f(K), r(K);
+    get_gamma(f, r, alpha);
+    std::copy(f.begin(), f.end(), freq);
+    std::copy(r.begin(), r.end(), rate);
+}

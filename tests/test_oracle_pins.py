@@ -1,0 +1,451 @@
+"""Pins the FP64 oracle (oracle/cafe_oracle.c) before anything is compared to it.
+
+Three kinds of checks, chosen so that a dropped term, a wrong index or a
+transposed operand fails at least one of them:
+  * the known-answer values of the reference's own unit tests (test.cpp),
+    checked to one unit in the last printed digit;
+  * exact rational brute force (Python fractions) and closed-form invariants
+    that do not depend on libm;
+  * the compiled reference (oracle/_ref) on seeded inputs, when it is built.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from cafe5_amd import workloads as wl
+from oracle_lib import ORC, Oracle, Ref
+
+
+def printed_tol(text: str) -> float:
+    """One unit in the last printed decimal of a reference value (the
+    reference's test literals are sometimes truncated, not rounded)."""
+    if "." not in text:
+        return 1.0
+    return 10.0 ** (-len(text.split(".")[1]))
+
+
+def assert_printed(value: float, text: str):
+    assert abs(value - float(text)) <= printed_tol(text) * (1 + 1e-9), (value, text)
+
+
+# --------------------------------------------------------------------------
+# exact rational transition matrices
+# --------------------------------------------------------------------------
+
+def exact_matrix(alpha: Fraction, N: int):
+    """P(s -> c) for s, c < N from the one-gene generating function
+    G(z) = (a + (1-2a) z) / (1 - a z); row s is the s-fold convolution."""
+    row1 = [alpha] + [(1 - alpha) ** 2 * alpha ** (c - 1) for c in range(1, N)]
+    rows = [[Fraction(1)] + [Fraction(0)] * (N - 1)]
+    for s in range(1, N):
+        prev = rows[-1]
+        rows.append([sum(prev[k] * row1[c - k] for k in range(c + 1)) for c in range(N)])
+    return rows
+
+
+def test_matrix_5x5_reference_values():
+    # test.cpp:595-617: lambda 0.05, t 5 -> alpha = 0.2
+    expected = [[1, 0, 0, 0, 0],
+                [0.2, 0.64, 0.128, 0.0256, 0.00512],
+                [0.04, 0.256, 0.4608, 0.17408, 0.0512],
+                [0.008, 0.0768, 0.26112, 0.36352, 0.187392],
+                [0.0016, 0.02048, 0.1024, 0.249856, 0.305562]]
+    lq, tq = Oracle.key(0.05, 5)
+    m = Oracle.matrix(lq, tq, 5)
+    for s in range(5):
+        for c in range(5):
+            assert_printed(m[s, c], repr(expected[s][c]) if s < 4 or c < 4 else "0.305562")
+    exact = exact_matrix(Fraction(1, 5), 5)
+    for s in range(5):
+        for c in range(5):
+            assert abs(m[s, c] - float(exact[s][c])) <= 1e-15 + 1e-13 * float(exact[s][c])
+
+
+@pytest.mark.parametrize("lam,t", [(0.05, 5.0), (0.002, 37.4), (0.0137, 11.0), (0.001, 1.0)])
+def test_matrix_exact_rationals(lam, t):
+    N = 24
+    lq, tq = Oracle.key(lam, t)
+    alpha = lq * tq / (1 + lq * tq)
+    exact = exact_matrix(Fraction(alpha), N)
+    m = Oracle.matrix(lq, tq, N)
+    for s in range(N):
+        for c in range(N):
+            e = float(exact[s][c])
+            assert abs(m[s, c] - e) <= 1e-300 + 1e-12 * e, (s, c, m[s, c], e)
+
+
+def test_matrix_invariants():
+    lq, tq = Oracle.key(0.004, 23.0)
+    alpha = lq * tq / (1 + lq * tq)
+    N = 400
+    m = Oracle.matrix(lq, tq, N)
+    # row 0 is the delta at 0 (matrix_cache.cpp:143-149), P(s->0) = alpha^s
+    assert m[0, 0] == 1.0 and not m[0, 1:].any()
+    for s in range(1, 60):
+        assert abs(m[s, 0] - alpha ** s) <= 1e-13 * alpha ** s
+    # rows sum to 1 when the truncation is far away
+    assert np.allclose(m[1:60].sum(axis=1), 1.0, rtol=0, atol=1e-12)
+    # detailed balance of the linear birth-death process: c P(s->c) = s P(c->s)
+    for s in range(1, 40):
+        for c in range(1, 40):
+            a, b = c * m[s, c], s * m[c, s]
+            assert abs(a - b) <= 1e-12 * max(a, b) + 1e-300
+
+
+def test_saturated_and_degenerate_matrices():
+    # lambda t > 1: all zero except (0,0) (matrix_cache.cpp:144-154)
+    m = Oracle.matrix(0.05, 25.0, 10)
+    assert m[0, 0] == 1.0 and np.count_nonzero(m) == 1
+    assert ORC.orc_is_saturated(25, 0.05) == 1          # test.cpp:1638-1643
+    assert ORC.orc_is_saturated(25, 0.01) == 0
+    # lambda t == 1: coeff == 0 -> rows >= 1 vanish (probability.cpp:157)
+    m = Oracle.matrix(0.05, 20.0, 10)
+    assert m[0, 0] == 1.0 and np.count_nonzero(m) == 1
+    # quantized t == 0 (t < 0.001): alpha == 0, coeff == 1 -> rows >= 1 vanish
+    lq, tq = Oracle.key(0.01, 0.0004)
+    assert tq == 0.0
+    m = Oracle.matrix(lq, tq, 10)
+    assert m[0, 0] == 1.0 and np.count_nonzero(m) == 1
+
+
+# --------------------------------------------------------------------------
+# known-answer tests of the reference (test.cpp)
+# --------------------------------------------------------------------------
+
+def test_transition_probability_values():
+    # test.cpp:549-562
+    assert_printed(ORC.orc_transition(0.05, 5, 5, 9), "0.0152237")
+    assert_printed(ORC.orc_transition(0.05, 5, 10, 9), "0.17573")
+    assert_printed(ORC.orc_transition(0.05, 5, 10, 10), "0.182728")
+    assert_printed(ORC.orc_transition(0.05, 1, 10, 10), "0.465565")
+    # test.cpp:579-582 (unquantized) and 564-577 (quantized keys, N = 141)
+    assert_printed(ORC.orc_transition(0.006335, 68.7105, 5, 5), "0.194661")
+    lq, tq = Oracle.key(0.006335, 68.7105)
+    assert tq == 68.71
+    assert abs(Oracle.matrix(lq, tq, 141)[5, 5] - 0.194661) <= 0.0001 * 0.194661
+    lq, tq = Oracle.key(0.006335, 68)
+    assert_printed(Oracle.matrix(lq, tq, 141)[5, 5], "0.195791")
+
+
+def test_birthdeath_closed_form_values():
+    # test.cpp:1329-1343
+    cases = [((46, 45, -3.672556, 0.949177), "-1.55455", True),
+             ((44, 46, -2.617970, 0.854098), "-2.20436", True),
+             ((43, 43, -1.686354, 0.629613), "-2.39974", True),
+             ((43, 44, -1.686354, 0.629613), "-2.44301", True),
+             ((13, 14, -2.617970, 0.854098), "-1.58253", True),
+             ((40, 42, -1.37, 0.5), "0.107933", False),
+             ((41, 34, -1.262, 0.4), "0.005714", False),
+             ((5, 5, -1.1931291703283662, 0.39345841643135504), "0.194661", False)]
+    for args, text, is_log in cases:
+        v = ORC.orc_birthdeath(*args)
+        assert_printed(math.log(v) if is_log else v, text)
+
+
+def test_chooseln_against_lgamma():
+    for n in range(0, 60):
+        for r in range(0, n + 1):
+            v = ORC.orc_chooseln(n, r)
+            if r == 0:
+                assert v == 0.0
+            elif n == 0:
+                assert v == -math.inf
+            else:
+                exact = math.log(math.comb(n, r))
+                assert abs(v - exact) <= 1e-12 * max(1.0, abs(exact))
+
+
+def test_matvec_exact_integers():
+    # test.cpp:1432-1463
+    m = np.arange(1, 10, dtype=np.float64).reshape(3, 3)
+    v = np.array([7.0, 9.0, 11.0])
+    out = np.zeros(3)
+    ORC.orc_matvec(m.ctypes.data_as(np.ctypeslib.ctypes.POINTER(np.ctypeslib.ctypes.c_double)), 3,
+                   v.ctypes.data_as(np.ctypeslib.ctypes.POINTER(np.ctypeslib.ctypes.c_double)),
+                   0, 2, 0, 2, out.ctypes.data_as(np.ctypeslib.ctypes.POINTER(np.ctypeslib.ctypes.c_double)))
+    assert out.tolist() == [58.0, 139.0, 220.0]
+    m8 = np.zeros((8, 8))
+    m8[3:6, 3:6] = m
+    out[:] = 0
+    P = np.ctypeslib.ctypes.POINTER(np.ctypeslib.ctypes.c_double)
+    ORC.orc_matvec(m8.ctypes.data_as(P), 8, v.ctypes.data_as(P), 3, 5, 3, 5, out.ctypes.data_as(P))
+    assert out.tolist() == [58.0, 139.0, 220.0]
+
+
+def _leaf(count, err, B):
+    P = np.ctypeslib.ctypes.POINTER(np.ctypeslib.ctypes.c_double)
+    out = np.zeros(B + 1)
+    e = None if err is None else np.ascontiguousarray(err, dtype=np.float64)
+    rc = ORC.orc_leaf_vector(count, None if e is None else e.ctypes.data_as(P),
+                             0 if e is None else e.shape[1], B, out.ctypes.data_as(P))
+    assert rc == 0
+    return out
+
+
+def test_leaf_vectors():
+    # test.cpp:1685-1727: one-hot at the observed count
+    v = _leaf(3, None, 20)
+    assert v.tolist() == [0.0] * 3 + [1.0] + [0.0] * 17
+    v = _leaf(6, None, 20)
+    assert v.tolist() == [0.0] * 6 + [1.0] + [0.0] * 14
+    # test.cpp:1765-1804: error model rows 0:{0,.8,.2}, 1..20:{.2,.6,.2}
+    err = np.array([[0.0, 0.8, 0.2]] + [[0.2, 0.6, 0.2]] * 20)
+    v = _leaf(3, err, 20)
+    assert v.tolist() == [0, 0, 0.2, 0.6, 0.2] + [0.0] * 16
+    # count 0: the -1 deviation falls off the vector (probability.cpp:193-194)
+    v = _leaf(0, err, 20)
+    assert v.tolist() == [0.8, 0.2] + [0.0] * 19
+
+
+def _tree_ab():
+    return wl.parse_newick("(A:1,B:3):7")
+
+
+def test_root_vector_values():
+    # test.cpp:1729-1763: tree (A:1,B:3):7, lambda 0.03, A=3, B=6, B=R=20
+    tree = _tree_ab()
+    expected = ("-19.7743 -11.6688 -5.85672 -5.66748 -6.61256 -8.59725 -12.2301 -16.4424 -20.9882 "
+                "-25.7574 -30.6888 -35.7439 -40.8971 -46.1299 -51.4289 -56.7837 -62.1863 -67.6304 "
+                "-73.1106 -78.6228").split()
+    root = Oracle.prune(tree, np.array([3, 6]), np.full(tree.n_nodes, 0.03), 20, 20)
+    for v, text in zip(np.log(root), expected):
+        assert_printed(v, text)
+
+
+def test_inference_prune_values():
+    # test.cpp:1662-1683: lambda 0.03 x multiplier 1.5 = 0.045
+    tree = _tree_ab()
+    expected = ("-17.2771 -10.0323 -5.0695 -4.91426 -5.86062 -7.75163 -10.7347 -14.2334 -18.0458 "
+                "-22.073 -26.2579 -30.5639 -34.9663 -39.4472 -43.9935 -48.595 -53.2439 -57.9338 "
+                "-62.6597 -67.4173").split()
+    root = Oracle.prune(tree, np.array([3, 6]), np.full(tree.n_nodes, 0.03 * 1.5), 20, 20)
+    for v, text in zip(np.log(root), expected):
+        assert_printed(v, text)
+
+
+def test_base_model_value():
+    # test.cpp:461-488: (A:1,B:1), 4 families, lambda 0.01, B=56, R=8,
+    # prior uniform(100) from user_data's default
+    tree = wl.parse_newick("(A:1,B:1);")
+    counts = np.array([[1, 2], [2, 1], [3, 6], [6, 3]], dtype=np.int32)
+    prior = Oracle.prior_uniform(100, 8)
+    res = Oracle.evaluate(tree, np.full(tree.n_nodes, 0.01), counts, 56, 8, prior)
+    assert_printed(res["total"], "46.56632")
+    assert abs(res["total"] + res["lnl"].sum()) < 1e-12
+
+
+def test_gamma_prune_values():
+    # test.cpp:1269-1294: categories {0.01, 0.05} x multipliers {0.1, 0.5},
+    # lambda 0.005, B=10, R=8, rootdist {1:2,2:2,3:2,4:2,5:1}
+    tree = _tree_ab()
+    prior = Oracle.prior_map([1, 2, 3, 4, 5], [2, 2, 2, 2, 1], 8)
+    res = Oracle.evaluate(tree, np.full(tree.n_nodes, 0.005), np.array([[3, 6]], dtype=np.int32),
+                          10, 8, prior, n_cat=2, multipliers=[0.1, 0.5], cat_probs=[0.01, 0.05])
+    assert_printed(math.log(res["cat"][0, 0]), "-23.04433")
+    assert_printed(math.log(res["cat"][0, 1]), "-16.68005")
+
+
+def test_gamma_saturation_gives_inf():
+    # test.cpp:1296-1311: lambda 0.9 x {0.1, 0.5} saturates the t=3 branch
+    tree = _tree_ab()
+    prior = Oracle.prior_uniform(100, 8)
+    res = Oracle.evaluate(tree, np.full(tree.n_nodes, 0.9), np.array([[3, 6]], dtype=np.int32),
+                          10, 8, prior, n_cat=2, multipliers=[0.1, 0.5], cat_probs=[1.0, 1.0])
+    assert res["total"] == math.inf
+
+
+def test_key_quantization():
+    # test.cpp:1313-1327: 31 distinct lambda keys from repeated +0.1, and the
+    # accumulated value at 3.0 maps to the same key as 3.0
+    t = 0.0
+    keys = set()
+    for _ in range(31):
+        t += 0.1
+        keys.add(ORC.orc_key_lambda(t))
+    assert len(keys) == 31
+    assert ORC.orc_key_lambda(3.0) in keys
+    # truncation, not rounding (matrix_cache.h:50-51)
+    assert ORC.orc_key_branch_length(1.001) == 1.0       # 1.001 * 1000 = 1000.9999...
+    assert ORC.orc_key_branch_length(2.5) == 2.5
+    assert ORC.orc_key_lambda(1.57e-05) == 1.5699e-05    # 15699.999... -> 15699
+
+
+def test_priors():
+    # test.cpp:490-497, 2669-2676, 2690-2738
+    u = Oracle.prior_uniform(10, 12)
+    assert u[5] == np.float32(0.1) and u[0] == np.float32(0.1) and u[10] == 0.0
+    u = Oracle.prior_uniform(125, 130)
+    assert abs(u[1] - 0.008) < 1e-9 and abs(u[99] - 0.008) < 1e-9 and u[126] == 0.0
+    m = Oracle.prior_map([1, 2], [3, 5], 4)
+    assert m[1] == 0.375
+    m = Oracle.prior_map([2, 3, 4, 6], [11, 5, 7, 2], 8)
+    assert np.allclose(m[[2, 3, 4, 5, 6]], [0.44, 0.2, 0.28, 0.0, 0.08], rtol=1e-6)
+    p = Oracle.prior_poisson(0.75, 100, 120)
+    assert_printed(p[0], "0.47237")
+    assert_printed(p[1], "0.35427")
+    assert p[11] == 0.0 and p[8] > 0.0 and p[9] == 0.0
+    assert p[0] == float(np.float32(math.exp(-0.75)))
+
+
+def test_discrete_gamma_against_quantiles():
+    # median discretization: r_k proportional to the gamma(alpha, alpha)
+    # quantile at (2k+1)/(2K), normalized to mean 1 (gamma.cpp:20-31); AS91
+    # stops at a relative step of 0.5e-6, so agreement is ~1e-6.
+    from scipy.stats import gamma as G
+    for K in (2, 4, 8):
+        for alpha in (0.2, 0.5, 1.0, 2.7):
+            f, r = Oracle.get_gamma(K, alpha)
+            q = G.ppf((2 * np.arange(K) + 1) / (2 * K), alpha, scale=1 / alpha)
+            q = q / q.mean()
+            assert np.allclose(f, 1.0 / K)
+            assert np.allclose(r, q, rtol=2e-5), (K, alpha, r, q)
+            assert abs(r.mean() - 1) < 1e-12
+
+
+# --------------------------------------------------------------------------
+# exact brute-force pruning on small trees
+# --------------------------------------------------------------------------
+
+def _exact_for(tree, lam, B):
+    mats = {}
+    for i in range(tree.n_nodes):
+        if i == tree.root:
+            continue
+        lq, tq = Oracle.key(lam, float(tree.length[i]))
+        a = lq * tq / (1 + lq * tq)
+        mats[i] = exact_matrix(Fraction(a), B + 2)
+    return mats
+
+
+def test_prune_cherry_exact():
+    tree = wl.parse_newick("(A:2,B:5);")
+    B = R = 12
+    mats = _exact_for(tree, 0.03, B)
+    counts = np.array([4, 7])
+    root = Oracle.prune(tree, counts, np.full(tree.n_nodes, 0.03), B, R)
+    a, b = [i for i in range(tree.n_nodes) if tree.leaf_col[i] >= 0]
+    for j in range(R):
+        s = j + 1                        # root index j is size j+1
+        e = mats[a][s][counts[tree.leaf_col[a]]] * mats[b][s][counts[tree.leaf_col[b]]]
+        assert abs(root[j] - float(e)) <= 1e-12 * float(e)
+
+
+def test_prune_three_leaves_exact():
+    tree = wl.parse_newick("((A:1.5,B:2.5):3,C:4);")
+    B, R = 9, 7
+    lam = 0.04
+    mats = _exact_for(tree, lam, B)
+    counts = np.array([2, 5, 3])
+    root = Oracle.prune(tree, counts, np.full(tree.n_nodes, lam), B, R)
+    leaf = {tree.names[i]: i for i in range(tree.n_nodes) if tree.leaf_col[i] >= 0}
+    inner = [i for i in range(tree.n_nodes) if tree.leaf_col[i] < 0 and i != tree.root][0]
+    A, Bn, C = leaf["A"], leaf["B"], leaf["C"]
+    for j in range(R):
+        s = j + 1
+        total = Fraction(0)
+        for x in range(B + 1):           # internal node sizes 0..B
+            total += mats[inner][s][x] * mats[A][x][2] * mats[Bn][x][5]
+        e = total * mats[C][s][3]
+        assert abs(root[j] - float(e)) <= 1e-12 * float(e)
+
+
+# --------------------------------------------------------------------------
+# the compiled reference, where it is built
+# --------------------------------------------------------------------------
+
+needs_ref = pytest.mark.skipif(not Ref.available(), reason="oracle/_ref not built (no CAFE5 tree)")
+
+
+@needs_ref
+def test_ref_matrices_bitwise():
+    rng = np.random.default_rng(7)
+    for _ in range(12):
+        lam = float(np.exp(rng.uniform(np.log(1e-4), np.log(2e-2))))
+        t = float(rng.uniform(0.1, 60))
+        N = int(rng.integers(5, 90))
+        ref = Ref.matrix(lam, t, N)
+        lq, tq = Oracle.key(lam, t)
+        assert np.array_equal(ref, Oracle.matrix(lq, tq, N))
+
+
+@needs_ref
+@pytest.mark.parametrize("use_err", [False, True])
+def test_ref_prune_bitwise(use_err):
+    tree = wl.yule_tree(9, 11)
+    B = R = 30
+    err = wl.error_table(0.05, B) if use_err else None
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        counts = rng.integers(0, B - 1, tree.n_leaves)
+        ref = Ref.prune(tree, counts, 0.004, 1.0, B, R, err)
+        got = Oracle.prune(tree, counts, np.full(tree.n_nodes, 0.004), B, R, err)
+        assert np.array_equal(ref, got)
+
+
+@needs_ref
+def test_ref_base_model_seeded():
+    w = wl.config_base(seed=1, F=150)
+    B = R = 40
+    counts = np.minimum(w.counts, B)
+    ref_total, ref_lnl = Ref.base_model(w.tree, counts, [0.002], B, R, prior_kind=0, prior_max=R)
+    prior = Oracle.prior_uniform(R, R)
+    res = Oracle.evaluate(w.tree, np.full(w.tree.n_nodes, 0.002), counts, B, R, prior)
+    assert np.array_equal(ref_lnl, res["lnl"])
+    assert ref_total == res["total"]
+
+
+@needs_ref
+def test_ref_base_model_multilambda_error_poisson():
+    w = wl.config_error_multi(seed=4, F=60, bound=40)
+    B = R = 40
+    err = wl.error_table(0.05, B)
+    ref_total, ref_lnl = Ref.base_model(w.tree, w.counts, w.lambdas, B, R, prior_kind=1,
+                                        prior_max=R, poisson_lambda=3.5, err=err,
+                                        node_class=w.node_class)
+    prior = Oracle.prior_poisson(3.5, R, R)
+    assert np.array_equal(prior, Ref.prior(1, R, 3.5, R))
+    res = Oracle.evaluate(w.tree, w.lambdas[w.node_class], w.counts, B, R, prior, err=err)
+    assert np.array_equal(ref_lnl, res["lnl"])
+    assert ref_total == res["total"]
+
+
+@needs_ref
+def test_ref_gamma_model_seeded():
+    w = wl.config_gamma(seed=3, F=40, K=4, bound=30)
+    B = R = 30
+    f, r = Oracle.get_gamma(4, 0.5)
+    rf, rr = Ref.get_gamma(4, 0.5)
+    assert np.array_equal(r, rr) and np.array_equal(f, rf)
+    ref_total, ref_cat, ref_fam, ref_post = Ref.gamma_model(w.tree, w.counts, [0.003], B, R, f, r,
+                                                            prior_kind=0, prior_max=R)
+    prior = Oracle.prior_uniform(R, R)
+    res = Oracle.evaluate(w.tree, np.full(w.tree.n_nodes, 0.003), w.counts, B, R, prior, n_cat=4,
+                          multipliers=r, cat_probs=f)
+    assert np.array_equal(ref_cat, res["cat"])
+    assert np.array_equal(ref_post, res["post"])
+    assert np.array_equal(ref_fam, res["fam"])
+    assert ref_total == res["total"]
+
+
+@needs_ref
+def test_ref_discrete_gamma_sweep():
+    for K in (1, 2, 3, 4, 5, 8, 16):
+        for alpha in np.exp(np.linspace(np.log(0.05), np.log(20.0), 23)):
+            rf, rr = Ref.get_gamma(K, float(alpha))
+            f, r = Oracle.get_gamma(K, float(alpha))
+            if K == 1:
+                continue      # get_gamma is only called for K > 1 (gamma_core.cpp:64)
+            assert np.array_equal(r, rr), (K, alpha)
+            assert np.array_equal(f, rf)
+
+
+@needs_ref
+def test_ref_priors():
+    for R in (8, 20, 101):
+        assert np.array_equal(Oracle.prior_uniform(R, R + 3), Ref.prior(0, R, 0.0, R + 3))
+        for lam in (0.75, 3.0, 9.5):
+            assert np.array_equal(Oracle.prior_poisson(lam, R, R + 10), Ref.prior(1, R, lam, R + 10))
+    s, c = [2, 3, 4, 6], [11, 5, 7, 2]
+    assert np.array_equal(Oracle.prior_map(s, c, 9), Ref.prior(2, 0, 0.0, 9, s, c))
