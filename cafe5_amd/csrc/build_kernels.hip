@@ -1,0 +1,229 @@
+// build_kernels.hip -- transition-matrix construction on the GPU
+// (replaces matrix_cache::precalculate_matrices, src/matrix_cache.cpp:113-163).
+//
+// Both kernels write the c-major pool layout of engine.h: an FP64 copy and,
+// when requested, an FP32 copy of every matrix.  Entries are those of
+// get_from_parent_fam_size_to_c (matrix_cache.cpp:78-85): row 0 is delta_0,
+// rows s >= 1 come from the birth-death process with a = lt/(1+lt) and
+// coeff = 1 - 2a; keys with coeff <= 0 or coeff == 1 (saturated, lt == 1, or
+// a == 0) keep only (0,0) = 1 (matrix_cache.cpp:144-154, probability.cpp:157).
+#include "engine.h"
+
+namespace cafe {
+
+__device__ __forceinline__ bool degenerate_key(double alpha, double coeff)
+{
+    return !(coeff > 0.0 && coeff != 1.0);
+}
+
+// ---------------------------------------------------------------------------
+// Recurrence build.  The one-gene generating function of the process is
+// G_1(z) = (a + (1-2a) z) / (1 - a z) and row s of the matrix holds the
+// coefficients of G_1(z)^s, so
+//     P_s(c) = a P_{s-1}(c) + (1-2a) P_{s-1}(c-1) + a P_s(c-1).
+// Every term is non-negative, so rounding errors stay relative (~1e-15 per
+// step) and there is no cancellation.
+//
+// One wavefront per matrix.  Lane l owns rows s = l + 64 b.  Columns advance
+// one at a time; along s the recurrence y_s = a y_{s-1} + K_s is a first-order
+// linear scan, done across lanes with a Kogge-Stone affine scan and carried
+// between 64-row blocks.  Stores for a fixed column are contiguous in s.
+// ---------------------------------------------------------------------------
+template <int NB>
+__global__ __launch_bounds__(64) void k_build_recurrence(const double* __restrict__ lam_q,
+                                                          const double* __restrict__ t_q,
+                                                          int first_slot, int N, int Np,
+                                                          long long slot_elems,
+                                                          double* __restrict__ mt64,
+                                                          float* __restrict__ mt32)
+{
+    const int key = blockIdx.x;
+    const int lane = threadIdx.x;
+    const double lt = lam_q[key] * t_q[key];
+    const double alpha = lt / (1 + lt);
+    const double coeff = 1 - 2 * alpha;
+    double* M64 = mt64 + (long long)(first_slot + key) * slot_elems;
+    float* M32 = mt32 ? mt32 + (long long)(first_slot + key) * slot_elems : nullptr;
+
+    if (degenerate_key(alpha, coeff)) {
+        for (int c = 0; c < N; ++c)
+            for (int b = 0; b < NB; ++b) {
+                const int s = lane + 64 * b;
+                if (s < N) {
+                    const double v = (s == 0 && c == 0) ? 1.0 : 0.0;
+                    M64[(long long)c * Np + s] = v;
+                    if (M32) M32[(long long)c * Np + s] = (float)v;
+                }
+            }
+        return;
+    }
+
+    // a^d for the scan distances d = 1, 2, 4, ..., 32 and a^(l+1) per lane
+    double apow[6];
+    apow[0] = alpha;
+    for (int i = 1; i < 6; ++i) apow[i] = apow[i - 1] * apow[i - 1];
+    double a_lane = 1.0;
+    {
+        int e = lane + 1;
+        double base = alpha;
+        while (e) {
+            if (e & 1) a_lane *= base;
+            base *= base;
+            e >>= 1;
+        }
+    }
+
+    double prev[NB];   // P_s(c-1) for the lane's rows
+#pragma unroll
+    for (int b = 0; b < NB; ++b) prev[b] = 0.0;
+
+    for (int c = 0; c < N; ++c) {
+        double K[NB];
+        // K_s = (1-2a) P_{s-1}(c-1) + a P_s(c-1); row 0 is delta_0 itself
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            double below = __shfl_up(prev[b], 1);
+            const double from_prev_block = b > 0 ? __shfl(prev[b > 0 ? b - 1 : 0], 63) : 0.0;
+            if (lane == 0) below = from_prev_block;
+            K[b] = coeff * below + alpha * prev[b];
+        }
+        if (lane == 0) K[0] = (c == 0) ? 1.0 : 0.0;   // P_0(c) = delta_{c0}
+
+        double carry = 0.0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            double y = K[b];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                const int d = 1 << i;
+                const double up = __shfl_up(y, d);
+                if (lane >= d) y += apow[i] * up;
+            }
+            if (b > 0) y += a_lane * carry;
+            carry = __shfl(y, 63);
+            prev[b] = y;
+            const int s = lane + 64 * b;
+            if (s < N) {
+                const double v = y < 1.0 ? y : 1.0;   // [0,1] clamp, probability.cpp:147
+                M64[(long long)c * Np + s] = v;
+                if (M32) M32[(long long)c * Np + s] = (float)v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Closed-form build: probability.cpp:104-148 term by term, one thread per
+// (s, c).  chooseln uses the host's glibc log-gamma table so the log-binomials
+// are the reference's values; terms are exp(t) * pow(coeff, j), added in
+// ascending j with separately rounded products and sums.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double chooseln_dev(const double* __restrict__ lg, int n, int r)
+{
+    if (r == 0) return 0.0;
+    if (n <= 0 || r <= 0) return -__builtin_huge_val();
+    return __dsub_rn(__dsub_rn(lg[n + 1], lg[r + 1]), lg[n - r + 1]);
+}
+
+__global__ __launch_bounds__(256) void k_build_closed_form(const double* __restrict__ lam_q,
+                                                           const double* __restrict__ t_q,
+                                                           const double* __restrict__ lg,
+                                                           int first_slot, int N, int Np,
+                                                           long long slot_elems,
+                                                           double* __restrict__ mt64,
+                                                           float* __restrict__ mt32)
+{
+    const int key = blockIdx.z;
+    const int c = blockIdx.y;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    const double lt = lam_q[key] * t_q[key];
+    const double alpha = lt / (1 + lt);
+    const double coeff = 1 - 2 * alpha;
+    double v;
+    if (s == 0) {
+        v = (c == 0) ? 1.0 : 0.0;
+    } else if (degenerate_key(alpha, coeff)) {
+        v = 0.0;
+    } else {
+        const double log_alpha = log(alpha);
+        const int m = c < s ? c : s;
+        double acc = 0.0;
+        for (int j = 0; j <= m; ++j) {
+            const double t = __dadd_rn(__dadd_rn(chooseln_dev(lg, s, j),
+                                                 chooseln_dev(lg, s + c - 1 - j, s - 1)),
+                                       __dmul_rn((double)(s + c - 2 * j), log_alpha));
+            const double p = __dmul_rn(exp(t), pow(coeff, (double)j));
+            acc = __dadd_rn(acc, p);
+        }
+        v = acc > 1.0 ? 1.0 : (acc < 0.0 ? 0.0 : acc);
+    }
+    double* M64 = mt64 + (long long)(first_slot + key) * slot_elems;
+    M64[(long long)c * Np + s] = v;
+    if (mt32) mt32[(long long)(first_slot + key) * slot_elems + (long long)c * Np + s] = (float)v;
+}
+
+// Zero-fill (used instead of hipMemsetAsync so the calls can be captured).
+__global__ void k_fill_zero_u32(uint32_t* __restrict__ p, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0u;
+}
+
+}  // namespace cafe
+
+// Host-side launchers (called from api.cpp).
+namespace cafe {
+
+hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const double* d_lg,
+                        int n, int first_slot, int N, int Np, long long slot_elems, double* mt64,
+                        float* mt32, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    if (algo == 0) {
+        // grid.z holds the key index: launch at most 65535 keys at a time
+        for (int k0 = 0; k0 < n; k0 += 65535) {
+            const int nk = n - k0 < 65535 ? n - k0 : 65535;
+            dim3 grid((N + 255) / 256, N, nk);
+            hipLaunchKernelGGL(k_build_closed_form, grid, dim3(256), 0, stream, d_lam + k0, d_t + k0,
+                               d_lg, first_slot + k0, N, Np, slot_elems, mt64, mt32);
+        }
+    } else {
+        const int nb = (N + 63) / 64;
+        if (nb <= 1)
+            hipLaunchKernelGGL(k_build_recurrence<1>, dim3(n), dim3(64), 0, stream, d_lam, d_t,
+                               first_slot, N, Np, slot_elems, mt64, mt32);
+        else if (nb <= 2)
+            hipLaunchKernelGGL(k_build_recurrence<2>, dim3(n), dim3(64), 0, stream, d_lam, d_t,
+                               first_slot, N, Np, slot_elems, mt64, mt32);
+        else if (nb <= 4)
+            hipLaunchKernelGGL(k_build_recurrence<4>, dim3(n), dim3(64), 0, stream, d_lam, d_t,
+                               first_slot, N, Np, slot_elems, mt64, mt32);
+        else if (nb <= 8)
+            hipLaunchKernelGGL(k_build_recurrence<8>, dim3(n), dim3(64), 0, stream, d_lam, d_t,
+                               first_slot, N, Np, slot_elems, mt64, mt32);
+        else if (nb <= 16)
+            hipLaunchKernelGGL(k_build_recurrence<16>, dim3(n), dim3(64), 0, stream, d_lam, d_t,
+                               first_slot, N, Np, slot_elems, mt64, mt32);
+        else if (nb <= 32)
+            hipLaunchKernelGGL(k_build_recurrence<32>, dim3(n), dim3(64), 0, stream, d_lam, d_t,
+                               first_slot, N, Np, slot_elems, mt64, mt32);
+        else
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(void* p, long long bytes, hipStream_t stream)
+{
+    const long long words = bytes / 4;
+    if (words <= 0) return hipSuccess;
+    long long blocks = (words + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_zero_u32, dim3((unsigned)blocks), dim3(256), 0, stream,
+                       (uint32_t*)p, words);
+    return hipGetLastError();
+}
+
+}  // namespace cafe

@@ -1,0 +1,99 @@
+// engine.h -- device data structures shared by the HIP kernels and the host
+// planner of libcafe_gpu.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cafe {
+
+// Matrices are stored column-major ("c-major"): element (s, c) = P(s -> c) at
+// pool[slot * slot_elems + c * Np + s].  Np = round_up(N, 16) + 16 leaves room
+// for 16-row tiles and the one-row shift of the root contraction; padding is 0.
+constexpr int kTile = 16;
+constexpr int kMaxSlots = 24;         // live partial vectors per family tile
+constexpr int kLgammaN = 4096;        // log-gamma table (integers < 4096)
+
+// One unit of work of the pruning kernels: families fam0 .. fam0+nf-1 of a
+// size bucket (bucket-sorted order) at one parameter point.
+struct TileDesc {
+    int bucket;
+    int point;
+    int fam0;
+    int nf;
+};
+
+struct BucketDev {
+    int B, R;          // max_family_size, max_root_family_size
+    int L, Lp;         // B+1 and its 16-multiple
+    int Rp;            // R rounded up to 16
+    int fam_begin;     // first family (bucket-sorted order)
+    int count;
+    int prior_off;     // offset of this bucket's prior in the prior tables
+};
+
+// Internal node of the evaluation program (Sethi-Ullman order, root last).
+struct OpDev {
+    int node;
+    int out_slot;
+    int n_child;
+    int child_begin;
+};
+
+struct ChildDev {
+    int node;       // child node (its branch matrix is node_slot[point][node])
+    int leaf_col;   // >= 0: leaf column in the count table; -1: interior
+    int in_slot;    // interior: slot holding its partial vector
+    int pad;
+};
+
+// Output flags per (point, family).
+enum : int {
+    kFlagZeroSum = 1,     // root vector sums to 0 (gamma_core.cpp:151-152)
+    kFlagRecompute64 = 2, // FP32 result not certified: recompute on BATCHED64
+    kFlagRecomputePlain = 4  // FP64 value near the subnormal range: recompute on PLAIN64
+};
+
+struct PruneParams {
+    // work list
+    const TileDesc* tiles;
+    const int* n_tiles_dev;   // number of tiles (device-resident count)
+    int* tile_counter;        // persistent-kernel work counter (zeroed before launch)
+    // model
+    const BucketDev* buckets;
+    const OpDev* ops;
+    int n_ops;
+    const ChildDev* kids;
+    int n_nodes;
+    const int* node_slot;     // [point][n_nodes] matrix slot per branch
+    // families
+    const int* counts_T;      // [leaf][F] counts, bucket-sorted family order
+    int F;                    // number of families
+    const int* perm;          // bucket-sorted index -> caller's family index
+    // matrices
+    const double* mt64;
+    const float* mt32;
+    long long slot_elems;     // Np * Np
+    int Np;
+    // error model (n_dev == 0: none); rows[k * n_dev + i]
+    const double* err64;
+    const float* err32;
+    int n_dev;
+    // root prior per bucket: prior[bucket.prior_off + j]
+    const double* prior64;
+    const float* prior32;
+    int model;                // CAFE_GPU_MODEL_*
+    // outputs [point][F] in the caller's family order
+    double* out_val;
+    int* out_flag;
+    // optional root vectors (FP64) [F][root_stride], one point only
+    double* root_out;
+    int root_stride;
+    // partial-vector scratch: per workgroup n_slots x 16 x ldv values
+    void* scratch;
+    long long scratch_per_wg;  // elements
+    int ldv;
+    int n_slots;
+};
+
+}  // namespace cafe

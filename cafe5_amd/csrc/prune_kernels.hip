@@ -1,0 +1,588 @@
+// prune_kernels.hip -- Felsenstein pruning of many gene families on MI355X
+// (replaces inference_prune / compute_node_probability, src/core.cpp:134-145,
+// src/probability.cpp:175-330, and the root reductions of base_model.cpp:77-95
+// and gamma_core.cpp:143-165).
+//
+// All pruning kernels are persistent: a fixed grid of single-wave workgroups
+// pulls TileDesc work items from a device counter until the device-resident
+// tile count is reached, so every wave exits once the list is drained.
+//
+// Per internal node u with children c (program order from the host, root last):
+//     v_u[s] = prod_c  sum_x M_c[s][x] v_c[x]            s = 0..B
+//     root:  v[j] = prod_c sum_x M_c[j+1][x] v_c[x]       j = 0..R-1
+// A leaf child contributes the column M_c[.][count] (or the error-model mix of
+// neighbouring columns); that is exactly what the reference's matrix-vector
+// product with a one-hot leaf vector evaluates to.
+#include "engine.h"
+
+namespace cafe {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { typedef floatx4 type; };
+template <> struct Vec4<double> { typedef doublex4 type; };
+
+__device__ __forceinline__ floatx4 mfma16(float a, float b, floatx4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ doublex4 mfma16(double a, double b, doublex4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ const float* pool_of(const PruneParams& p, float*) { return p.mt32; }
+__device__ __forceinline__ const double* pool_of(const PruneParams& p, double*) { return p.mt64; }
+__device__ __forceinline__ const float* err_of(const PruneParams& p, float*) { return p.err32; }
+__device__ __forceinline__ const double* err_of(const PruneParams& p, double*) { return p.err64; }
+
+__device__ __forceinline__ int fetch_tile(const PruneParams& p)
+{
+    int t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(p.tile_counter, 1);
+    return __shfl(t, 0);
+}
+
+constexpr double kLn2 = 0.69314718055994530942;
+// FP32 flush threshold (smallest normal float): the certificate charges this
+// for every quantity FP32 could have flushed to zero.
+constexpr double kTau32 = 1.1754943508222875e-38;
+// an FP32 likelihood is accepted when the flush bound is below this fraction
+constexpr double kCertEta = 1e-9;
+// below this, FP64 values may lose bits to subnormals; such families are
+// recomputed in the reference's summation order (PLAIN64)
+constexpr double kFp64Floor = 1e-290;
+
+// Running root reduction of one lane: best key over its rows.
+struct RootBest {
+    double key;   // base FP64: log x + log prior; otherwise x * prior
+    double x;     // x at the best key
+    int j;        // its index (smallest j on ties)
+    double xmax;  // max_j x_j
+};
+
+__device__ __forceinline__ void root_take(RootBest& b, double key, double x, int j)
+{
+    if (b.j < 0 || b.key < key || (b.key == key && j < b.j)) {
+        b.key = key;
+        b.x = x;
+        b.j = j;
+    }
+    b.xmax = b.xmax > x ? b.xmax : x;
+}
+
+__device__ __forceinline__ RootBest root_combine(RootBest b)
+{
+    for (int d = 16; d <= 32; d <<= 1) {
+        RootBest o;
+        o.key = __shfl_xor(b.key, d);
+        o.x = __shfl_xor(b.x, d);
+        o.j = __shfl_xor(b.j, d);
+        o.xmax = __shfl_xor(b.xmax, d);
+        if (o.j >= 0) {
+            const double xm = b.xmax > o.xmax ? b.xmax : o.xmax;
+            root_take(b, o.key, o.x, o.j);
+            b.xmax = xm;
+        }
+    }
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// Batched MFMA pruning.  One wave = 16 families (columns) of one bucket at one
+// parameter point.  For v_mfma_*_16x16x4: lane l supplies A[s0 + l%16][k] and
+// B[k][f = l%16] with k = l/16, and holds D[s0 + 4(l/16) + r][f = l%16], r<4.
+// Within each 16-wide chunk of the contraction index, MFMA j maps lane l's k
+// to x = 16 kt + 4 (l/16) + j, so every lane reads four consecutive entries of
+// the partial vectors it stored itself (no cross-lane memory traffic).
+//
+// T = float, SCALED: each node's partial is renormalised per family to a max
+// in [0.5, 1) and the exponent is carried as an integer; a per-family bound on
+// what FP32 flushing could have dropped decides whether the result is kept.
+// T = double: unscaled, so values (and underflow) follow the reference.
+// NT: 16-row tiles held in registers; SCALED needs every tile in one pass.
+// ---------------------------------------------------------------------------
+template <typename T, int NT, bool SCALED>
+__global__ __launch_bounds__(64) void k_prune_mfma(PruneParams p)
+{
+    typedef typename Vec4<T>::type V4;
+    const int lane = threadIdx.x;
+    const int fl = lane & 15;
+    const int g = lane >> 4;
+    const T* __restrict__ pool = pool_of(p, (T*)nullptr);
+    const T* __restrict__ err = err_of(p, (T*)nullptr);
+    T* slots = (T*)p.scratch + (long long)blockIdx.x * p.scratch_per_wg;
+    const int ldv = p.ldv;
+    const int Np = p.Np;
+    const int n_tiles = *p.n_tiles_dev;
+    __shared__ int s_exp[kMaxSlots][16];       // SCALED: exponent per slot/family
+    __shared__ double s_cert[kMaxSlots][16];   // SCALED: flush bound per slot/family
+
+    for (int tile = fetch_tile(p); tile < n_tiles; tile = fetch_tile(p)) {
+        const TileDesc td = p.tiles[tile];
+        const BucketDev bk = p.buckets[td.bucket];
+        const bool valid = fl < td.nf;
+        const int fs = bk.fam_begin + td.fam0 + (valid ? fl : 0);
+        const int* __restrict__ nslot = p.node_slot + (long long)td.point * p.n_nodes;
+        const int nts = bk.Lp / kTile;
+
+        for (int op = 0; op < p.n_ops; ++op) {
+            const OpDev o = p.ops[op];
+            const bool is_root = (op == p.n_ops - 1);
+            const int rows = is_root ? bk.Rp / kTile : nts;
+            const int limit = is_root ? bk.R : bk.L;
+            const int shift = is_root ? 1 : 0;
+
+            // exponents and flush bounds inherited from the children
+            int e_sum = 0;
+            double cert = 0.0;
+            if (SCALED) {
+                for (int k = 0; k < o.n_child; ++k) {
+                    const ChildDev ch = p.kids[o.child_begin + k];
+                    if (ch.leaf_col >= 0) {
+                        cert += 2.0 * kTau32;
+                    } else {
+                        e_sum += s_exp[ch.in_slot][fl];
+                        cert += s_cert[ch.in_slot][fl] + 4.0 * kTau32 * bk.Lp;
+                    }
+                }
+            }
+            RootBest rb{-__builtin_huge_val(), 0.0, -1, 0.0};
+
+            for (int t0 = 0; t0 < rows; t0 += NT) {
+                V4 acc[NT];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) acc[t] = V4{1, 1, 1, 1};
+                for (int k = 0; k < o.n_child; ++k) {
+                    const ChildDev ch = p.kids[o.child_begin + k];
+                    const T* __restrict__ M = pool + (long long)nslot[ch.node] * p.slot_elems;
+                    V4 x[NT];
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) x[t] = V4{0, 0, 0, 0};
+                    if (ch.leaf_col >= 0) {
+                        const int cnt = valid ? p.counts_T[(long long)ch.leaf_col * p.F + fs] : 0;
+                        if (p.n_dev == 0) {
+                            const T* col = M + (long long)cnt * Np + shift + 4 * g;
+#pragma unroll
+                            for (int t = 0; t < NT; ++t)
+                                if (t0 + t < rows) {
+                                    const T* q = col + kTile * (t0 + t);
+                                    x[t] = V4{q[0], q[1], q[2], q[3]};
+                                }
+                        } else {
+                            // error model: columns cnt-h .. cnt+h weighted by
+                            // get_probs(cnt), negative columns skipped
+                            const int h = (p.n_dev - 1) / 2;
+                            for (int i = 0; i < p.n_dev; ++i) {
+                                const int cc = cnt - h + i;
+                                if (cc < 0) continue;
+                                const T w = err[(long long)cnt * p.n_dev + i];
+                                const T* col = M + (long long)cc * Np + shift + 4 * g;
+#pragma unroll
+                                for (int t = 0; t < NT; ++t)
+                                    if (t0 + t < rows) {
+                                        const T* q = col + kTile * (t0 + t);
+                                        x[t] += w * V4{q[0], q[1], q[2], q[3]};
+                                    }
+                            }
+                        }
+                    } else {
+                        const T* V = slots + ((long long)ch.in_slot * 16 + fl) * ldv + 4 * g;
+                        for (int kt = 0; kt < nts; ++kt) {
+                            const V4 b = *(const V4*)(V + kTile * kt);
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                const T* a = M + (long long)(kTile * kt + 4 * g + j) * Np + shift + fl;
+#pragma unroll
+                                for (int t = 0; t < NT; ++t)
+                                    if (t0 + t < rows) x[t] = mfma16(a[kTile * (t0 + t)], b[j], x[t]);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) acc[t] *= x[t];
+                }
+                // rows past the bound hold real matrix rows: clear them
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (kTile * (t0 + t) + 4 * g + r >= limit || t0 + t >= rows) acc[t][r] = 0;
+
+                if (!is_root) {
+                    if (SCALED) {
+                        double m = 0.0;
+#pragma unroll
+                        for (int t = 0; t < NT; ++t)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) m = m > (double)acc[t][r] ? m : (double)acc[t][r];
+                        m = fmax(m, __shfl_xor(m, 16));
+                        m = fmax(m, __shfl_xor(m, 32));
+                        int e = 0;
+                        if (m > 0) (void)frexp(m, &e);
+#pragma unroll
+                        for (int t = 0; t < NT; ++t)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) acc[t][r] = ldexpf(acc[t][r], -e);
+                        __syncthreads();   // every lane has read the child slots' s_exp/s_cert
+                        if (g == 0) {
+                            s_exp[o.out_slot][fl] = e_sum + e;
+                            s_cert[o.out_slot][fl] = m > 0 ? cert * 1.01 / m + kTau32 : 1e300;
+                        }
+                        __syncthreads();
+                    }
+                    T* out = slots + ((long long)o.out_slot * 16 + fl) * ldv + 4 * g;
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+                        if (t0 + t < rows) *(V4*)(out + kTile * (t0 + t)) = acc[t];
+                } else {
+                    const double* pr = p.prior64 + bk.prior_off;
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int j = kTile * (t0 + t) + 4 * g + r;
+                            if (j >= limit || t0 + t >= rows) continue;
+                            const double xv = (double)acc[t][r];
+                            if (p.root_out && valid)
+                                p.root_out[(long long)p.perm[fs] * p.root_stride + j] =
+                                    SCALED ? ldexp(xv, e_sum) : xv;
+                            const double key = (!SCALED && p.model == 0) ? log(xv) + log(pr[j])
+                                                                         : xv * pr[j];
+                            root_take(rb, key, xv, j);
+                        }
+                }
+            }
+
+            if (is_root) {
+                rb = root_combine(rb);
+                double val;
+                int flag = 0;
+                if (SCALED) {
+                    // rb.key = x_j * prior_j in units of 2^e_sum
+                    if (!(rb.key > 0)) {
+                        flag = kFlagRecompute64;
+                        val = 0.0;
+                    } else {
+                        const double log_scale = (double)e_sum * kLn2;
+                        if (p.model == 0)
+                            val = log(rb.x) + log(p.prior64[bk.prior_off + rb.j]) + log_scale;
+                        else
+                            val = ldexp(rb.key, e_sum);
+                        // flush bound (absolute, scaled units) against the
+                        // likelihood, and FP64 range of the unscaled values
+                        if (!(cert <= kCertEta * rb.key)) flag |= kFlagRecompute64;
+                        if (!(log(rb.xmax) + log_scale >= log(kFp64Floor))) flag |= kFlagRecompute64;
+                    }
+                } else {
+                    val = rb.key;
+                    if (rb.xmax == 0.0) flag |= kFlagZeroSum;
+                    if (rb.xmax < kFp64Floor) flag |= kFlagRecomputePlain;
+                }
+                if (valid && g == 0) {
+                    p.out_val[(long long)td.point * p.F + p.perm[fs]] = val;
+                    p.out_flag[(long long)td.point * p.F + p.perm[fs]] = flag;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain FP64 pruning: the drop-in.  One wave per family at a time, lanes over
+// rows s; every matrix-vector product runs over x = 0..B in ascending order
+// with separately rounded multiply and add, as matrix::multiply
+// (matrix_cache.cpp:50-56) does, and the node product starts from 1
+// (probability.cpp:208-217, 222-231).  Partials live in per-wave scratch.
+// NQ: rows per lane (row s = lane + 64 q).
+// ---------------------------------------------------------------------------
+template <int NQ>
+__global__ __launch_bounds__(64) void k_prune_plain(PruneParams p)
+{
+    const int lane = threadIdx.x;
+    double* slots = (double*)p.scratch + (long long)blockIdx.x * p.scratch_per_wg;
+    const int ldv = p.ldv;
+    const int Np = p.Np;
+    const int n_tiles = *p.n_tiles_dev;
+
+    for (int tile = fetch_tile(p); tile < n_tiles; tile = fetch_tile(p)) {
+        const TileDesc td = p.tiles[tile];
+        const BucketDev bk = p.buckets[td.bucket];
+        const int* __restrict__ nslot = p.node_slot + (long long)td.point * p.n_nodes;
+        const int B = bk.B;
+        for (int fi = 0; fi < td.nf; ++fi) {
+            const int fs = bk.fam_begin + td.fam0 + fi;
+            double best = -__builtin_huge_val();
+            double xmax = 0.0;
+            for (int op = 0; op < p.n_ops; ++op) {
+                const OpDev o = p.ops[op];
+                const bool is_root = (op == p.n_ops - 1);
+                const int len = is_root ? bk.R : bk.L;
+                const int shift = is_root ? 1 : 0;
+                double acc[NQ];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) acc[q] = 1.0;
+                for (int k = 0; k < o.n_child; ++k) {
+                    const ChildDev ch = p.kids[o.child_begin + k];
+                    const double* __restrict__ M = p.mt64 + (long long)nslot[ch.node] * p.slot_elems + shift;
+                    double x[NQ];
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) x[q] = 0.0;
+                    if (ch.leaf_col >= 0) {
+                        const int cnt = p.counts_T[(long long)ch.leaf_col * p.F + fs];
+                        if (p.n_dev == 0) {
+#pragma unroll
+                            for (int q = 0; q < NQ; ++q) {
+                                const int s = lane + 64 * q;
+                                if (s < len) x[q] = M[(long long)cnt * Np + s];
+                            }
+                        } else {
+                            const int h = (p.n_dev - 1) / 2;
+                            for (int i = 0; i < p.n_dev; ++i) {
+                                const int cc = cnt - h + i;
+                                if (cc < 0) continue;
+                                const double w = p.err64[(long long)cnt * p.n_dev + i];
+#pragma unroll
+                                for (int q = 0; q < NQ; ++q) {
+                                    const int s = lane + 64 * q;
+                                    if (s < len) x[q] = __dadd_rn(x[q], __dmul_rn(M[(long long)cc * Np + s], w));
+                                }
+                            }
+                        }
+                    } else {
+                        const double* V = slots + (long long)ch.in_slot * ldv;
+                        for (int c = 0; c <= B; ++c) {
+                            const double v = V[c];
+                            const double* col = M + (long long)c * Np;
+#pragma unroll
+                            for (int q = 0; q < NQ; ++q) {
+                                const int s = lane + 64 * q;
+                                if (s < len) x[q] = __dadd_rn(x[q], __dmul_rn(col[s], v));
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) acc[q] = __dmul_rn(acc[q], x[q]);
+                }
+                if (!is_root) {
+                    __syncthreads();   // all lanes finished reading the child slots
+                    double* out = slots + (long long)o.out_slot * ldv;
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        const int s = lane + 64 * q;
+                        if (s < len) out[s] = acc[q];
+                    }
+                    __syncthreads();
+                } else {
+                    const double* pr = p.prior64 + bk.prior_off;
+                    int bj = -1;
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        const int j = lane + 64 * q;
+                        if (j >= len) continue;
+                        if (p.root_out)
+                            p.root_out[(long long)p.perm[fs] * p.root_stride + j] = acc[q];
+                        // base_model.cpp:86 / gamma_core.cpp:157
+                        const double key = p.model == 0 ? log(acc[q]) + log(pr[j]) : __dmul_rn(acc[q], pr[j]);
+                        if (bj < 0 || best < key) {
+                            best = key;
+                            bj = j;
+                        }
+                        xmax = xmax > acc[q] ? xmax : acc[q];
+                    }
+                    for (int d = 1; d < 64; d <<= 1) {
+                        const double ob = __shfl_xor(best, d);
+                        const double ox = __shfl_xor(xmax, d);
+                        best = ob > best ? ob : best;
+                        xmax = ox > xmax ? ox : xmax;
+                    }
+                }
+            }
+            if (lane == 0) {
+                p.out_val[(long long)td.point * p.F + p.perm[fs]] = best;
+                p.out_flag[(long long)td.point * p.F + p.perm[fs]] = xmax == 0.0 ? kFlagZeroSum : 0;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Appends every tile with a family flagged `mask` to out_tiles.
+__global__ void k_compact_flags(const TileDesc* __restrict__ tiles, const int* __restrict__ n_tiles_dev,
+                                const BucketDev* __restrict__ buckets, const int* __restrict__ perm,
+                                const int* __restrict__ flags, int F, int mask,
+                                TileDesc* __restrict__ out_tiles, int* __restrict__ out_count,
+                                int* __restrict__ fam_count)
+{
+    const int n = *n_tiles_dev;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const TileDesc td = tiles[i];
+        const BucketDev bk = buckets[td.bucket];
+        int hits = 0;
+        for (int k = 0; k < td.nf; ++k) {
+            const int fam = perm[bk.fam_begin + td.fam0 + k];
+            hits += (flags[(long long)td.point * F + fam] & mask) != 0;
+        }
+        if (hits) {
+            out_tiles[atomicAdd(out_count, 1)] = td;
+            atomicAdd(fam_count, hits);
+        }
+    }
+}
+
+// Per (trial, family) model reduction: base copies lnL; gamma combines the K
+// categories as gamma_core.cpp:190-208 (cat_k = (max_j L_j prior_j) p_k,
+// family = sum_k cat_k, posterior_k = cat_k p_k / sum_k cat_k p_k).
+__global__ void k_finalize(int model, int T, int K, int F, const double* __restrict__ val,
+                           const int* __restrict__ flag, const double* __restrict__ cat_probs,
+                           double* __restrict__ lnl, double* __restrict__ cat, double* __restrict__ post,
+                           int* __restrict__ fail)
+{
+    const long long n = (long long)T * F;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int t = (int)(i / F);
+        const int f = (int)(i % F);
+        if (model == 0) {
+            lnl[i] = val[i];
+            fail[i] = 0;
+            continue;
+        }
+        bool zero = false;
+        for (int k = 0; k < K; ++k) zero |= (flag[((long long)t * K + k) * F + f] & kFlagZeroSum) != 0;
+        fail[i] = zero ? 1 : 0;
+        double fam = 0.0, denom = 0.0;
+        for (int k = 0; k < K; ++k) {
+            const double c = zero ? 0.0 : __dmul_rn(val[((long long)t * K + k) * F + f], cat_probs[k]);
+            cat[i * K + k] = c;
+            fam = __dadd_rn(fam, c);
+            denom = __dadd_rn(denom, __dmul_rn(c, cat_probs[k]));
+        }
+        for (int k = 0; k < K; ++k)
+            post[i * K + k] = __ddiv_rn(__dmul_rn(cat[i * K + k], cat_probs[k]), denom);
+        lnl[i] = zero ? __builtin_nan("") : log(fam);
+    }
+}
+
+// total[t] = -sum_f lnl[t][f] with compensated (Neumaier) summation over
+// contiguous chunks and a two-sum tree; +inf when any gamma family failed.
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e)
+{
+    s = __dadd_rn(a, b);
+    const double bb = __dsub_rn(s, a);
+    e = __dadd_rn(__dsub_rn(a, __dsub_rn(s, bb)), __dsub_rn(b, bb));
+}
+
+__global__ __launch_bounds__(256) void k_totals(int F, const double* __restrict__ lnl,
+                                                const int* __restrict__ fail, double* __restrict__ total)
+{
+    const int t = blockIdx.x;
+    const double* x = lnl + (long long)t * F;
+    const int* fl = fail + (long long)t * F;
+    const int tid = threadIdx.x;
+    const int chunk = (F + 255) / 256;
+    const int lo = tid * chunk, hi = lo + chunk < F ? lo + chunk : F;
+    double s = 0.0, c = 0.0;
+    int bad = 0;
+    for (int i = lo; i < hi; ++i) {
+        bad |= fl[i];
+        double ns, e;
+        two_sum(s, x[i], ns, e);
+        s = ns;
+        c = __dadd_rn(c, e);
+    }
+    __shared__ double ss[256], sc[256];
+    __shared__ int sb[256];
+    ss[tid] = s;
+    sc[tid] = c;
+    sb[tid] = bad;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            double ns, e;
+            two_sum(ss[tid], ss[tid + w], ns, e);
+            ss[tid] = ns;
+            sc[tid] = __dadd_rn(__dadd_rn(sc[tid], sc[tid + w]), e);
+            sb[tid] |= sb[tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) total[t] = sb[0] ? __builtin_huge_val() : -__dadd_rn(ss[0], sc[0]);
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------
+template <typename T, bool SCALED>
+static hipError_t launch_mfma_nt(int nt, int grid, const PruneParams& p, hipStream_t st)
+{
+#define CAFE_MFMA_CASE(N)                                                                      \
+    if (nt <= N) {                                                                             \
+        hipLaunchKernelGGL((k_prune_mfma<T, N, SCALED>), dim3(grid), dim3(64), 0, st, p);      \
+        return hipGetLastError();                                                              \
+    }
+    CAFE_MFMA_CASE(2)
+    CAFE_MFMA_CASE(4)
+    CAFE_MFMA_CASE(8)
+    if constexpr (!SCALED) {
+        // FP64: larger vectors run in chunks of 8 tiles
+        hipLaunchKernelGGL((k_prune_mfma<T, 8, SCALED>), dim3(grid), dim3(64), 0, st, p);
+        return hipGetLastError();
+    } else {
+        CAFE_MFMA_CASE(13)
+        CAFE_MFMA_CASE(16)
+        CAFE_MFMA_CASE(24)
+        CAFE_MFMA_CASE(32)
+    }
+#undef CAFE_MFMA_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_prune(int path, int nt, int grid, const PruneParams& p, hipStream_t st)
+{
+    if (path == 0) {
+        const int nq = nt;   // rows per lane for the plain kernel
+        if (nq <= 1) hipLaunchKernelGGL(k_prune_plain<1>, dim3(grid), dim3(64), 0, st, p);
+        else if (nq <= 2) hipLaunchKernelGGL(k_prune_plain<2>, dim3(grid), dim3(64), 0, st, p);
+        else if (nq <= 4) hipLaunchKernelGGL(k_prune_plain<4>, dim3(grid), dim3(64), 0, st, p);
+        else if (nq <= 8) hipLaunchKernelGGL(k_prune_plain<8>, dim3(grid), dim3(64), 0, st, p);
+        else if (nq <= 16) hipLaunchKernelGGL(k_prune_plain<16>, dim3(grid), dim3(64), 0, st, p);
+        else return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
+    if (path == 1) return launch_mfma_nt<double, false>(nt, grid, p, st);
+    return launch_mfma_nt<float, true>(nt, grid, p, st);
+}
+
+hipError_t launch_compact(const TileDesc* tiles, const int* n_tiles_dev, int max_tiles,
+                          const BucketDev* buckets, const int* perm, const int* flags, int F, int mask,
+                          TileDesc* out_tiles, int* out_count, int* fam_count, hipStream_t st)
+{
+    int blocks = (max_tiles + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_compact_flags, dim3(blocks), dim3(256), 0, st, tiles, n_tiles_dev, buckets, perm,
+                       flags, F, mask, out_tiles, out_count, fam_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(int model, int T, int K, int F, const double* val, const int* flag,
+                           const double* cat_probs, double* lnl, double* cat, double* post, int* fail,
+                           double* total, hipStream_t st)
+{
+    long long n = (long long)T * F;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_finalize, dim3((unsigned)blocks), dim3(256), 0, st, model, T, K, F, val, flag,
+                       cat_probs, lnl, cat, post, fail);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_totals, dim3(T), dim3(256), 0, st, F, lnl, fail, total);
+    return hipGetLastError();
+}
+
+}  // namespace cafe

@@ -1,0 +1,250 @@
+"""ctypes binding of libcafe_gpu.so (include/cafe_gpu.h).
+
+The library is the product's compute path; this module only marshals numpy
+arrays.  Loading fails loudly when the library has not been built -- there is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .workloads import Tree
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcafe_gpu.so")
+
+BUILD_CLOSED_FORM = 0
+BUILD_RECURRENCE = 1
+PATH_PLAIN64 = 0
+PATH_BATCHED64 = 1
+PATH_BATCHED32 = 2
+MODEL_BASE = 0
+MODEL_GAMMA = 1
+PATH_NAMES = {PATH_PLAIN64: "plain64", PATH_BATCHED64: "batched64", PATH_BATCHED32: "batched32"}
+
+_i = ctypes.c_int
+_d = ctypes.c_double
+_p = ctypes.c_void_p
+_pi = ctypes.POINTER(ctypes.c_int)
+_pd = ctypes.POINTER(ctypes.c_double)
+_pll = ctypes.POINTER(ctypes.c_longlong)
+
+
+class EvalDesc(ctypes.Structure):
+    _fields_ = [("model", _i), ("path", _i), ("n_trials", _i), ("n_cat", _i), ("node_slot", _pi),
+                ("cat_probs", _pd), ("prior", _pd), ("prior_stride", _i)]
+
+
+class CafeGpuError(RuntimeError):
+    pass
+
+
+_LIB: Optional[ctypes.CDLL] = None
+
+
+def load_library() -> ctypes.CDLL:
+    """Load libcafe_gpu.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise CafeGpuError(f"{LIB_PATH} is missing: build it with `python -m cafe5_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "cafe_gpu_create": (_i, [_i, ctypes.POINTER(_p)]),
+        "cafe_gpu_destroy": (_i, [_p]),
+        "cafe_gpu_last_error": (ctypes.c_char_p, []),
+        "cafe_gpu_matrix_reset": (_i, [_p, _i, _i]),
+        "cafe_gpu_matrix_build": (_i, [_p, _i, _pd, _pd, _i, _i]),
+        "cafe_gpu_matrix_get": (_i, [_p, _i, _pd]),
+        "cafe_gpu_set_tree": (_i, [_p, _i, _pi, _pi]),
+        "cafe_gpu_set_families": (_i, [_p, _i, _i, _pi, _pi, _pi, _i, _i]),
+        "cafe_gpu_bucket_count": (_i, [_p, _pi]),
+        "cafe_gpu_bucket_info": (_i, [_p, _i, _pi, _pi, _pi]),
+        "cafe_gpu_set_error_model": (_i, [_p, _i, _i, _pd]),
+        "cafe_gpu_eval": (_i, [_p, ctypes.POINTER(EvalDesc), _pd, _pd, _pd, _pd, _pll]),
+        "cafe_gpu_root_vectors": (_i, [_p, ctypes.POINTER(EvalDesc), _i, _i, _i, _i, _pd]),
+        "cafe_gpu_last_timings": (_i, [_p, _pd]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+EXPORTED_SYMBOLS = ["cafe_gpu_create", "cafe_gpu_destroy", "cafe_gpu_last_error", "cafe_gpu_matrix_reset",
+                    "cafe_gpu_matrix_build", "cafe_gpu_matrix_get", "cafe_gpu_set_tree",
+                    "cafe_gpu_set_families", "cafe_gpu_bucket_count", "cafe_gpu_bucket_info",
+                    "cafe_gpu_set_error_model", "cafe_gpu_eval", "cafe_gpu_root_vectors",
+                    "cafe_gpu_last_timings"]
+
+
+def _ip(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(_pi)
+
+
+def _dp(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(_pd)
+
+
+def key_lambda(lam: float) -> float:
+    """matrix_cache_key lambda quantization (matrix_cache.h:50, :57-58)."""
+    return float(int(lam * 1000000000)) / 1000000000.0
+
+
+def key_branch_length(t: float) -> float:
+    """matrix_cache_key branch-length quantization (matrix_cache.h:51, :60-61)."""
+    return float(int(t * 1000)) / 1000.0
+
+
+class Engine:
+    """One libcafe_gpu context on one device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = _p()
+        self._check(self.lib.cafe_gpu_create(device, ctypes.byref(h)))
+        self.ctx = h
+        self.N = 0
+        self.F = 0
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.cafe_gpu_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise CafeGpuError(f"libcafe_gpu error {rc}: {self.lib.cafe_gpu_last_error().decode()}")
+
+    # ---- matrices ----
+    def matrix_reset(self, N: int, capacity: int):
+        self._check(self.lib.cafe_gpu_matrix_reset(self.ctx, N, capacity))
+        self.N = N
+
+    def matrix_build(self, lam_q: Sequence[float], t_q: Sequence[float], first_slot: int = 0,
+                     algo: int = BUILD_RECURRENCE):
+        lq = np.ascontiguousarray(lam_q, dtype=np.float64)
+        tq = np.ascontiguousarray(t_q, dtype=np.float64)
+        self._check(self.lib.cafe_gpu_matrix_build(self.ctx, len(lq), _dp(lq), _dp(tq), first_slot, algo))
+
+    def matrix_get(self, slot: int) -> np.ndarray:
+        out = np.zeros((self.N, self.N))
+        self._check(self.lib.cafe_gpu_matrix_get(self.ctx, slot, _dp(out)))
+        return out
+
+    # ---- inputs ----
+    def set_tree(self, tree: Tree):
+        par = np.ascontiguousarray(tree.parent, dtype=np.int32)
+        col = np.ascontiguousarray(tree.leaf_col, dtype=np.int32)
+        self._check(self.lib.cafe_gpu_set_tree(self.ctx, tree.n_nodes, _ip(par), _ip(col)))
+        self.tree = tree
+
+    def set_families(self, counts: np.ndarray, B: int = 0, R: int = 0, bound_B=None, bound_R=None):
+        c = np.ascontiguousarray(counts, dtype=np.int32)
+        bb = None if bound_B is None else np.ascontiguousarray(bound_B, dtype=np.int32)
+        br = None if bound_R is None else np.ascontiguousarray(bound_R, dtype=np.int32)
+        self._check(self.lib.cafe_gpu_set_families(self.ctx, c.shape[0], c.shape[1], _ip(c), _ip(bb), _ip(br),
+                                                   B, R))
+        self.F = c.shape[0]
+
+    def buckets(self) -> List[Tuple[int, int, int]]:
+        n = ctypes.c_int()
+        self._check(self.lib.cafe_gpu_bucket_count(self.ctx, ctypes.byref(n)))
+        out = []
+        for i in range(n.value):
+            b, r, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            self._check(self.lib.cafe_gpu_bucket_info(self.ctx, i, ctypes.byref(b), ctypes.byref(r),
+                                                      ctypes.byref(c)))
+            out.append((b.value, r.value, c.value))
+        return out
+
+    def set_error_model(self, rows: Optional[np.ndarray]):
+        if rows is None:
+            self._check(self.lib.cafe_gpu_set_error_model(self.ctx, 0, 0, None))
+            return
+        r = np.ascontiguousarray(rows, dtype=np.float64)
+        self._check(self.lib.cafe_gpu_set_error_model(self.ctx, r.shape[1], r.shape[0], _dp(r)))
+
+    # ---- evaluation ----
+    def _desc(self, model, path, node_slot, prior, cat_probs):
+        ns = np.ascontiguousarray(node_slot, dtype=np.int32)
+        pr = np.ascontiguousarray(np.atleast_2d(prior), dtype=np.float64)
+        cp = None if cat_probs is None else np.ascontiguousarray(cat_probs, dtype=np.float64)
+        K = 1 if cp is None else len(cp)
+        T = ns.shape[0] // K
+        d = EvalDesc(model, path, T, K, _ip(ns), _dp(cp), _dp(pr), pr.shape[1])
+        return d, (ns, pr, cp), T, K
+
+    def evaluate(self, node_slot: np.ndarray, prior: np.ndarray, path: int = PATH_BATCHED32,
+                 model: int = MODEL_BASE, cat_probs=None) -> Dict[str, np.ndarray]:
+        """node_slot: (T*K, n_nodes) matrix slots; prior: (n_buckets, stride)."""
+        d, keep, T, K = self._desc(model, path, node_slot, prior, cat_probs)
+        lnl = np.zeros((T, self.F))
+        cat = np.zeros((T, self.F, K)) if model == MODEL_GAMMA else None
+        post = np.zeros((T, self.F, K)) if model == MODEL_GAMMA else None
+        total = np.zeros(T)
+        nfb = ctypes.c_longlong(0)
+        self._check(self.lib.cafe_gpu_eval(self.ctx, ctypes.byref(d), _dp(lnl), _dp(cat), _dp(post), _dp(total),
+                                           ctypes.byref(nfb)))
+        return {"lnl": lnl, "cat": cat, "post": post, "total": total, "n_fallback": nfb.value}
+
+    def root_vectors(self, node_slot: np.ndarray, prior: np.ndarray, path: int, trial: int = 0, cat: int = 0,
+                     model: int = MODEL_BASE, cat_probs=None, R: Optional[int] = None) -> np.ndarray:
+        d, keep, T, K = self._desc(model, path, node_slot, prior, cat_probs)
+        R = R or int(np.atleast_2d(prior).shape[1])
+        out = np.zeros(self.F * R)
+        self._check(self.lib.cafe_gpu_root_vectors(self.ctx, ctypes.byref(d), trial, cat, 0, self.F, _dp(out)))
+        return out.reshape(self.F, R)
+
+    def timings(self) -> np.ndarray:
+        out = np.zeros(4)
+        self._check(self.lib.cafe_gpu_last_timings(self.ctx, _dp(out)))
+        return out
+
+
+class KeyTable:
+    """Matrix keys of a likelihood evaluation, as matrix_cache sees them.
+
+    Every (lambda, branch length) pair is quantized like matrix_cache_key
+    (truncation to 1e-9 / 1e-3) and de-duplicated; node_slot maps each branch
+    of each parameter point to its matrix slot.
+    """
+
+    def __init__(self):
+        self.keys: Dict[Tuple[float, float], int] = {}
+
+    def slot(self, lam: float, t: float) -> int:
+        k = (key_lambda(lam), key_branch_length(t))
+        if k not in self.keys:
+            self.keys[k] = len(self.keys)
+        return self.keys[k]
+
+    def node_slots(self, tree: Tree, node_lambdas: np.ndarray) -> np.ndarray:
+        """node_lambdas: (P, n_nodes) lambda of every branch at every point."""
+        P = node_lambdas.shape[0]
+        out = np.zeros((P, tree.n_nodes), dtype=np.int32)
+        for p in range(P):
+            for i in range(tree.n_nodes):
+                if i != tree.root:
+                    out[p, i] = self.slot(float(node_lambdas[p, i]), float(tree.length[i]))
+        return out
+
+    def arrays(self) -> Tuple[np.ndarray, np.ndarray]:
+        lam = np.zeros(len(self.keys))
+        t = np.zeros(len(self.keys))
+        for (lq, tq), s in self.keys.items():
+            lam[s] = lq
+            t[s] = tq
+        return lam, t

@@ -1,0 +1,176 @@
+/*
+ * cafe_gpu.h -- C ABI of the MI355X likelihood engine for CAFE5's hot path.
+ *
+ * The engine replaces, for one GPU:
+ *   - matrix_cache::precalculate_matrices / get_matrix
+ *       (src/matrix_cache.h:76-77, src/matrix_cache.cpp:88-163)
+ *   - inference_prune + compute_node_probability for every family
+ *       (src/core.cpp:134-145, src/probability.cpp:175-330)
+ *   - the per-family root reductions of base_model / gamma_model
+ *       (src/base_model.cpp:53-100, src/gamma_core.cpp:97-237)
+ *
+ * Conventions
+ *   - Every call returns an int status (CAFE_GPU_OK == 0).  On failure
+ *     cafe_gpu_last_error() returns a message for the calling thread.
+ *   - The caller owns all host buffers; the context owns device memory and
+ *     one HIP stream.  Calls on one context are not thread safe.
+ *   - Matrices follow matrix_cache semantics: keys are (lambda, t) already
+ *     quantized and de-quantized as matrix_cache_key does
+ *     (src/matrix_cache.h:44-63); entries are P(s -> c) of the linear
+ *     birth-death process, row 0 = delta_0, saturated keys (1-2a < 0) give a
+ *     zero matrix except (0,0).
+ *   - A tree has n_nodes nodes; parent[i] == -1 marks the root; leaf_col[i]
+ *     is a leaf's column in a family's count row (-1 for interior nodes);
+ *     a node's children are taken in increasing node index, which the caller
+ *     makes equal to the reference's descendant order.
+ */
+#ifndef CAFE_GPU_H
+#define CAFE_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CAFE_GPU_OK = 0,
+    CAFE_GPU_EINVAL = 1,     /* bad argument or shape */
+    CAFE_GPU_ENOMEM = 2,     /* device allocation failed */
+    CAFE_GPU_EDEVICE = 3,    /* HIP runtime / kernel launch error */
+    CAFE_GPU_ESTATE = 4,     /* call order (e.g. eval before set_tree) */
+    CAFE_GPU_ENOTFOUND = 5   /* matrix slot not built (get_matrix miss) */
+};
+
+/* Matrix build algorithms. */
+enum {
+    /* probability.cpp:104-148 term by term: log-binomials, exp, pow, summed
+     * in ascending j, clamped to [0,1].  O(N^3) per matrix. */
+    CAFE_GPU_BUILD_CLOSED_FORM = 0,
+    /* The same matrix from the generating function of the process,
+     * G_s(z) = G_1(z)^s with G_1 = (a + (1-2a) z)/(1 - a z):
+     * P_s(c) = a P_{s-1}(c) + (1-2a) P_{s-1}(c-1) + a P_s(c-1).
+     * All terms are positive, so the result agrees with the closed form to
+     * ~1e-13 relative.  O(N^2) per matrix. */
+    CAFE_GPU_BUILD_RECURRENCE = 1
+};
+
+/* Pruning paths. */
+enum {
+    /* per-family FP64 matrix-vector products in the reference's summation
+     * order (the drop-in); no rescaling */
+    CAFE_GPU_PATH_PLAIN64 = 0,
+    /* families as columns, FP64 MFMA contractions; no rescaling */
+    CAFE_GPU_PATH_BATCHED64 = 1,
+    /* families as columns, FP32 MFMA contractions with per-family
+     * power-of-two rescaling and an exponent carry; each family gets a bound
+     * on what FP32 flushing could have lost, and families whose bound is not
+     * below 1e-9 of their likelihood (or whose FP64 value would be
+     * subnormal) are recomputed on the BATCHED64 path */
+    CAFE_GPU_PATH_BATCHED32 = 2
+};
+
+enum {
+    CAFE_GPU_MODEL_BASE = 0,   /* base_model: lnL_f = max_j log L_j + log prior_j */
+    CAFE_GPU_MODEL_GAMMA = 1   /* gamma_model: K rate categories */
+};
+
+typedef struct cafe_gpu_ctx cafe_gpu_ctx;
+
+/* --- context --------------------------------------------------------- */
+
+/* Creates a context on HIP device `device` (no counterpart in the reference,
+ * which has no device state). */
+int cafe_gpu_create(int device, cafe_gpu_ctx **out);
+int cafe_gpu_destroy(cafe_gpu_ctx *ctx);
+const char *cafe_gpu_last_error(void);
+
+/* --- transition matrices (matrix_cache) --------------------------------- */
+
+/* Replaces matrix_cache(int N) (src/matrix_cache.h:91): sets the matrix
+ * dimension N = max(R, B) + 1 and the pool capacity in matrices; drops all
+ * built matrices. */
+int cafe_gpu_matrix_reset(cafe_gpu_ctx *ctx, int N, int capacity);
+
+/* Replaces matrix_cache::precalculate_matrices (src/matrix_cache.cpp:113-163):
+ * builds n matrices at the de-quantized keys (lambda_q[i], t_q[i]) into slots
+ * first_slot .. first_slot + n - 1 with the given algorithm. */
+int cafe_gpu_matrix_build(cafe_gpu_ctx *ctx, int n, const double *lambda_q,
+                          const double *t_q, int first_slot, int algo);
+
+/* Replaces matrix_cache::get_matrix (src/matrix_cache.cpp:88-105) for host
+ * consumers: copies slot's N x N matrix, row-major out[s*N + c] = P(s -> c). */
+int cafe_gpu_matrix_get(cafe_gpu_ctx *ctx, int slot, double *out);
+
+/* --- model inputs -------------------------------------------------------- */
+
+/* The species tree (clade, src/clade.h:25-121). */
+int cafe_gpu_set_tree(cafe_gpu_ctx *ctx, int n_nodes, const int *parent,
+                      const int *leaf_col);
+
+/* Gene families (gene_family, src/gene_family.h): counts is F x n_leaves
+ * row-major.  bound_B[f] / bound_R[f] are the family's max_family_size and
+ * max_root_family_size (NULL: B / R for all).  Families with equal (B, R)
+ * form one size bucket. */
+int cafe_gpu_set_families(cafe_gpu_ctx *ctx, int F, int n_leaves, const int *counts,
+                          const int *bound_B, const int *bound_R, int B, int R);
+
+/* Number of size buckets and, for bucket i, its (B, R) and family count.
+ * Buckets are numbered in ascending (B, R). */
+int cafe_gpu_bucket_count(cafe_gpu_ctx *ctx, int *n);
+int cafe_gpu_bucket_info(cafe_gpu_ctx *ctx, int i, int *B, int *R, int *count);
+
+/* Leaf error model (error_model::get_probs, src/error_model.cpp:52-57):
+ * rows[k * n_dev + i] = probability of deviation i - (n_dev-1)/2 for an
+ * observed count k, k = 0 .. n_rows-1 (n_rows must exceed every family's B).
+ * rows == NULL removes the error model. */
+int cafe_gpu_set_error_model(cafe_gpu_ctx *ctx, int n_dev, int n_rows, const double *rows);
+
+/* --- evaluation (infer_family_likelihoods) --------------------------------- */
+
+typedef struct {
+    int model;            /* CAFE_GPU_MODEL_* */
+    int path;             /* CAFE_GPU_PATH_* */
+    int n_trials;         /* T parameter points evaluated together */
+    int n_cat;            /* K categories (1 for the base model) */
+    /* node_slot[(t * K + k) * n_nodes + i]: matrix slot of the branch above
+     * node i at trial t, category k (the root's entry is ignored) */
+    const int *node_slot;
+    /* cat_probs[k] (gamma only) */
+    const double *cat_probs;
+    /* prior[b * prior_stride + j] = double(root_equilibrium_distribution::
+     * compute(j)) for bucket b, j < R_b (base_model.cpp:84, gamma_core.cpp:156) */
+    const double *prior;
+    int prior_stride;
+} cafe_gpu_eval_desc;
+
+/* One likelihood evaluation per trial point for every family
+ * (base_model.cpp:53-100 / gamma_core.cpp:168-237).  Outputs, all optional:
+ *   lnl[t * F + f]          per-family log-likelihood (base: the stash's
+ *                           posterior_probability; gamma: log of the family
+ *                           likelihood)
+ *   cat[(t * F + f) * K + k] gamma category likelihood (max_j L_j prior_j) p_k
+ *   post[(t * F + f) * K + k] gamma posterior (p_k applied twice, as
+ *                           gamma_core.cpp:102)
+ *   total[t]                -sum_f lnl (compensated sum), +inf when any
+ *                           gamma family has a zero-sum root vector
+ *   n_fallback              families recomputed on the FP64 path (BATCHED32)
+ */
+int cafe_gpu_eval(cafe_gpu_ctx *ctx, const cafe_gpu_eval_desc *desc, double *lnl,
+                  double *cat, double *post, double *total, long long *n_fallback);
+
+/* Root vectors of one trial/category for families f0 .. f0+nf-1 (FP64
+ * paths only): out[(f - f0) * R_f + j] = L_root[j] as inference_prune
+ * returns it (src/core.cpp:134-145).  Used by host consumers and tests. */
+int cafe_gpu_root_vectors(cafe_gpu_ctx *ctx, const cafe_gpu_eval_desc *desc, int trial,
+                          int cat, int f0, int nf, double *out);
+
+/* --- timing (measurement support; no reference counterpart) --------------- */
+
+/* Milliseconds of the last eval's phases, measured with HIP events on the
+ * context stream: [0] matrix build (last cafe_gpu_matrix_build), [1] main
+ * prune kernel, [2] FP64 recompute, [3] finalize + totals. */
+int cafe_gpu_last_timings(cafe_gpu_ctx *ctx, double *ms4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,74 @@
+"""CPU-side checks of the engine library: it is built, it exports every entry
+point include/cafe_gpu.h declares, and the host-side key quantization matches
+the oracle.  No compute call is made here (there is no GPU)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cafe5_amd import native
+from oracle_lib import ORC
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _ensure_built():
+    if not os.path.exists(native.LIB_PATH):
+        from cafe5_amd import build
+        build.build_native()
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "cafe_gpu.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cafe_gpu_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    _ensure_built()
+    lib = ctypes.CDLL(native.LIB_PATH)
+    names = header_functions()
+    assert len(names) >= 14
+    for name in names:
+        assert hasattr(lib, name), name
+    assert sorted(native.EXPORTED_SYMBOLS) == names
+
+
+def test_binding_signatures_load():
+    _ensure_built()
+    lib = native.load_library()
+    assert lib.cafe_gpu_last_error() is not None
+
+
+def test_engine_without_gpu_fails_loudly():
+    _ensure_built()
+    import torch
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises(native.CafeGpuError):
+        native.Engine(0)
+
+
+def test_key_quantization_matches_oracle():
+    rng = np.random.default_rng(5)
+    lams = np.concatenate([np.exp(rng.uniform(np.log(1e-6), np.log(0.5), 2000)),
+                           [1.57e-05, 0.0025, 0.3, 3.0, 0.1 * 3]])
+    ts = np.concatenate([rng.uniform(0.0, 120.0, 2000), [1.001, 0.29, 68.7105, 2.5, 0.0004]])
+    for lam in lams:
+        assert native.key_lambda(float(lam)) == ORC.orc_key_lambda(float(lam))
+    for t in ts:
+        assert native.key_branch_length(float(t)) == ORC.orc_key_branch_length(float(t))
+
+
+def test_key_table_dedups_quantized_keys():
+    from cafe5_amd import workloads as wl
+    tree = wl.parse_newick("((A:1.0004,B:1.0001):2,C:3.0009);")
+    kt = native.KeyTable()
+    slots = kt.node_slots(tree, np.full((2, tree.n_nodes), 0.01))
+    # A and B quantize to t = 1.000 and share a matrix; both points share keys
+    leaves = {tree.names[i]: i for i in range(tree.n_nodes) if tree.leaf_col[i] >= 0}
+    assert slots[0, leaves["A"]] == slots[0, leaves["B"]]
+    assert (slots[0] == slots[1]).all()
+    assert len(kt.keys) == 3
