@@ -33,6 +33,12 @@ __device__ __forceinline__ doublex4 mfma16(double a, double b, doublex4 c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Output row held by lane group g (= lane / 16), register r, of 16-row tile t.
+// v_mfma_f32_16x16x4f32 puts D rows 4g..4g+3 in one lane; v_mfma_f64_16x16x4f64
+// interleaves them (rows g, g+4, g+8, g+12), as measured on gfx950.
+__device__ __forceinline__ int drow(float*, int t, int g, int r) { return 16 * t + 4 * g + r; }
+__device__ __forceinline__ int drow(double*, int t, int g, int r) { return 16 * t + g + 4 * r; }
+
 __device__ __forceinline__ const float* pool_of(const PruneParams& p, float*) { return p.mt32; }
 __device__ __forceinline__ const double* pool_of(const PruneParams& p, double*) { return p.mt64; }
 __device__ __forceinline__ const float* err_of(const PruneParams& p, float*) { return p.err32; }
@@ -93,10 +99,10 @@ __device__ __forceinline__ RootBest root_combine(RootBest b)
 // ---------------------------------------------------------------------------
 // Batched MFMA pruning.  One wave = 16 families (columns) of one bucket at one
 // parameter point.  For v_mfma_*_16x16x4: lane l supplies A[s0 + l%16][k] and
-// B[k][f = l%16] with k = l/16, and holds D[s0 + 4(l/16) + r][f = l%16], r<4.
-// Within each 16-wide chunk of the contraction index, MFMA j maps lane l's k
-// to x = 16 kt + 4 (l/16) + j, so every lane reads four consecutive entries of
-// the partial vectors it stored itself (no cross-lane memory traffic).
+// B[k][f = l%16] with k = l/16, and holds four rows of D[.][f = l%16] (rows
+// given by drow()).  Within each 16-wide chunk of the contraction index, MFMA
+// j maps lane l's k to x = 16 kt + 4 (l/16) + j, so every lane reads four
+// consecutive entries of a partial vector of its own family.
 //
 // T = float, SCALED: each node's partial is renormalised per family to a max
 // in [0.5, 1) and the exponent is carried as an integer; a per-family bound on
@@ -164,12 +170,12 @@ __global__ __launch_bounds__(64) void k_prune_mfma(PruneParams p)
                     if (ch.leaf_col >= 0) {
                         const int cnt = valid ? p.counts_T[(long long)ch.leaf_col * p.F + fs] : 0;
                         if (p.n_dev == 0) {
-                            const T* col = M + (long long)cnt * Np + shift + 4 * g;
+                            const T* col = M + (long long)cnt * Np + shift;
 #pragma unroll
                             for (int t = 0; t < NT; ++t)
                                 if (t0 + t < rows) {
-                                    const T* q = col + kTile * (t0 + t);
-                                    x[t] = V4{q[0], q[1], q[2], q[3]};
+#pragma unroll
+                                    for (int r = 0; r < 4; ++r) x[t][r] = col[drow((T*)nullptr, t0 + t, g, r)];
                                 }
                         } else {
                             // error model: columns cnt-h .. cnt+h weighted by
@@ -179,12 +185,12 @@ __global__ __launch_bounds__(64) void k_prune_mfma(PruneParams p)
                                 const int cc = cnt - h + i;
                                 if (cc < 0) continue;
                                 const T w = err[(long long)cnt * p.n_dev + i];
-                                const T* col = M + (long long)cc * Np + shift + 4 * g;
+                                const T* col = M + (long long)cc * Np + shift;
 #pragma unroll
                                 for (int t = 0; t < NT; ++t)
                                     if (t0 + t < rows) {
-                                        const T* q = col + kTile * (t0 + t);
-                                        x[t] += w * V4{q[0], q[1], q[2], q[3]};
+#pragma unroll
+                                        for (int r = 0; r < 4; ++r) x[t][r] += w * col[drow((T*)nullptr, t0 + t, g, r)];
                                     }
                             }
                         }
@@ -209,7 +215,7 @@ __global__ __launch_bounds__(64) void k_prune_mfma(PruneParams p)
                 for (int t = 0; t < NT; ++t)
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        if (kTile * (t0 + t) + 4 * g + r >= limit || t0 + t >= rows) acc[t][r] = 0;
+                        if (drow((T*)nullptr, t0 + t, g, r) >= limit || t0 + t >= rows) acc[t][r] = 0;
 
                 if (!is_root) {
                     if (SCALED) {
@@ -233,17 +239,24 @@ __global__ __launch_bounds__(64) void k_prune_mfma(PruneParams p)
                         }
                         __syncthreads();
                     }
-                    T* out = slots + ((long long)o.out_slot * 16 + fl) * ldv + 4 * g;
+                    T* out = slots + ((long long)o.out_slot * 16 + fl) * ldv;
 #pragma unroll
                     for (int t = 0; t < NT; ++t)
-                        if (t0 + t < rows) *(V4*)(out + kTile * (t0 + t)) = acc[t];
+                        if (t0 + t < rows) {
+                            if (sizeof(T) == 4) {
+                                *(V4*)(out + drow((T*)nullptr, t0 + t, g, 0)) = acc[t];
+                            } else {
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) out[drow((T*)nullptr, t0 + t, g, r)] = acc[t][r];
+                            }
+                        }
                 } else {
                     const double* pr = p.prior64 + bk.prior_off;
 #pragma unroll
                     for (int t = 0; t < NT; ++t)
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            const int j = kTile * (t0 + t) + 4 * g + r;
+                            const int j = drow((T*)nullptr, t0 + t, g, r);
                             if (j >= limit || t0 + t >= rows) continue;
                             const double xv = (double)acc[t][r];
                             if (p.root_out && valid)
