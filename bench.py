@@ -1,0 +1,330 @@
+#!/usr/bin/env python3
+"""Family-likelihood throughput of the MI355X engine on the uneven-size
+lambda-grid workload.
+
+Workload (one "step" = one pass of the hot path over the whole batch):
+  seeded Yule tree with 34 taxa (height 100); 20,000 families with
+  heavy-tailed root sizes (85% <= 10, tail to 150) simulated by birth-death;
+  per-family size bounds 20..200 in 8 buckets; a 64-point lambda grid
+  1e-4 .. 1e-2.  A step builds every (lambda x branch) transition matrix
+  (4,224 matrices of 201 x 201), prunes all 20,000 x 64 family likelihoods,
+  reduces them and returns the 64 totals -- replayed as one HIP graph with all
+  inputs resident on the GPU.
+
+The headline path is BATCHED32 (FP32 MFMA contractions, per-family rescaling,
+FP64 recompute of families the flush bound cannot certify).  The plain FP64
+drop-in (per-family matrix-vector products) and the batched FP64 MFMA path are
+timed on the same inputs, the FP32 results are compared with FP64 ones, and the
+compiled reference (oracle/_ref, or the oracle port) is timed on a bounded
+sample on the host cores.
+
+    python bench.py [--steps K] [--warmup W]      # one JSON line on stdout
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from cafe5_amd import native, priors, workloads as wl  # noqa: E402
+
+METRIC = "family likelihoods/sec, batched FP32-MFMA vs plain FP64 on GPU at |ΔlnL|≤1e-6"
+PEAK_FP32_MFMA = 157.3e12     # MI355X dense FP32 matrix, FLOP/s
+PEAK_HBM = 8.0e12             # MI355X HBM3E, B/s
+
+
+def log(msg: str):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def tree_flop_factors(tree: wl.Tree):
+    """(n1, n2): interior non-root nodes whose parent is interior / the root.
+    A family likelihood at bound B (L = B+1 entries, R root sizes) costs
+    2 L (n1 L + n2 R) flops of contractions; leaf branches are column gathers."""
+    n1 = n2 = 0
+    for u in range(tree.n_nodes):
+        if tree.leaf_col[u] >= 0 or u == tree.root:
+            continue
+        if tree.parent[u] == tree.root:
+            n2 += 1
+        else:
+            n1 += 1
+    return n1, n2
+
+
+def setup(eng: native.Engine, w: wl.Workload):
+    eng.set_tree(w.tree)
+    eng.set_families(w.counts, bound_B=w.bounds, bound_R=w.bounds)
+    buckets = eng.buckets()
+    keys = native.KeyTable()
+    node_lams = np.repeat(w.lambdas[:, None], w.tree.n_nodes, axis=1)
+    slots = keys.node_slots(w.tree, node_lams)
+    lam_q, t_q = keys.arrays()
+    N = int(w.bounds.max()) + 1
+    eng.matrix_reset(N, len(lam_q))
+    stride = max(r for _, r, _ in buckets)
+    prior = np.zeros((len(buckets), stride))
+    for i, (_, r, _) in enumerate(buckets):
+        prior[i, :r] = priors.uniform(r, r)
+    return buckets, slots, lam_q, t_q, prior, N
+
+
+def time_plan(plan: native.Plan, steps: int, warmup: int, profile_steps: int):
+    """ms per step over `steps` graph replays (each waits for its totals, as
+    the optimizer does), then the phase times of `profile_steps` executions
+    with HIP events between the phases (same kernels, same stream)."""
+    for _ in range(warmup):
+        plan.execute()
+        plan.synchronize()
+    plan.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.execute()
+        plan.synchronize()
+    t1 = time.perf_counter()
+    phase = np.zeros(4)
+    graph_ok = False
+    for _ in range(profile_steps):
+        ms, graph_ok = plan.profile()
+        phase += ms
+    return (t1 - t0) * 1e3 / steps, phase / max(1, profile_steps), graph_ok
+
+
+# ----------------------------------------------------------------------------
+# CPU baseline (checker libraries under oracle/, loaded only here)
+# ----------------------------------------------------------------------------
+
+def cpu_baseline(w: wl.Workload, n_lambda: int, n_sample: int, threads: int):
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libcaferef.so")
+    if os.path.exists(ref_so):
+        return _cpu_reference(ref_so, w, n_lambda, n_sample, threads)
+    return _cpu_port(w, n_lambda, n_sample, threads)
+
+
+def _sample(w: wl.Workload, n_sample: int, seed: int = 11):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    F = w.counts.shape[0]
+    idx = np.sort(rng.choice(F, size=min(n_sample, F), replace=False))
+    return idx
+
+
+def _extrapolate(times_tiny, times_sample, n_tiny, n_sample, n_full):
+    """Matrix build + per-family cost, amortized over the full family count."""
+    if n_sample <= n_tiny:
+        return times_sample * (n_full / max(n_sample, 1))
+    per_fam = max(0.0, (times_sample - times_tiny) / (n_sample - n_tiny))
+    return times_tiny + per_fam * (n_full - n_tiny)
+
+
+def _cpu_reference(so, w, n_lambda, n_sample, threads):
+    lib = ctypes.CDLL(so)
+    lib.ref_init.restype = None
+    lib.ref_init()
+    lib.ref_set_threads.restype = ctypes.c_int
+    lib.ref_set_threads.argtypes = [ctypes.c_int]
+    used = lib.ref_set_threads(threads)
+    lib.ref_base_session_create.restype = ctypes.c_void_p
+    lib.ref_base_session_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    lib.ref_base_session_eval.restype = ctypes.c_double
+    lib.ref_base_session_eval.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]
+    lib.ref_base_session_free.argtypes = [ctypes.c_void_p]
+    newick = w.tree.newick().encode()
+    names = " ".join(w.tree.leaf_names()).encode()
+    idx = _sample(w, n_sample)
+    lams = w.lambdas[np.linspace(0, len(w.lambdas) - 1, n_lambda).round().astype(int)]
+    est_total = 0.0
+    wall = 0.0
+    for B in sorted(set(w.bounds.tolist())):
+        full = int((w.bounds == B).sum())
+        sel = idx[w.bounds[idx] == B]
+        if len(sel) == 0:
+            sel = np.flatnonzero(w.bounds == B)[:1]
+        n_tiny = min(8, len(sel))
+        sessions = []
+        for part in (sel[:n_tiny], sel):
+            c = np.ascontiguousarray(w.counts[part], dtype=np.int32)
+            h = lib.ref_base_session_create(newick, names, len(part), c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                            B, B, 0, B, 0.0)
+            sessions.append((h, c, len(part)))
+        for lam in lams:
+            t = []
+            for h, _, _ in sessions:
+                t0 = time.perf_counter()
+                lib.ref_base_session_eval(h, float(lam), None)
+                t.append(time.perf_counter() - t0)
+            wall += sum(t)
+            est_total += _extrapolate(t[0], t[1], sessions[0][2], sessions[1][2], full)
+        for h, _, _ in sessions:
+            lib.ref_base_session_free(h)
+    F = w.counts.shape[0]
+    return {"value": F * len(lams) / est_total, "unit": "family likelihoods/s", "cores": int(used),
+            "kind": "reference",
+            "sample": (f"reference base_model::infer_family_likelihoods (oracle/_ref, OpenMP) at {len(lams)} of the "
+                       f"64 grid lambdas on a seeded {len(idx)}-family subsample plus 8-family runs per bucket; "
+                       f"matrix build and per-family cost extrapolated to all {F} families "
+                       f"({wall:.1f} s of CPU wall time)")}
+
+
+def _cpu_port(w, n_lambda, n_sample, threads):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle  # checker only
+    idx = _sample(w, n_sample)
+    lams = w.lambdas[np.linspace(0, len(w.lambdas) - 1, n_lambda).round().astype(int)]
+    est_total = wall = 0.0
+    for B in sorted(set(w.bounds.tolist())):
+        full = int((w.bounds == B).sum())
+        sel = idx[w.bounds[idx] == B]
+        if len(sel) == 0:
+            sel = np.flatnonzero(w.bounds == B)[:1]
+        n_tiny = min(8, len(sel))
+        prior = priors.uniform(B, B)
+        for lam in lams:
+            t = []
+            for part in (sel[:n_tiny], sel):
+                t0 = time.perf_counter()
+                Oracle.evaluate(w.tree, np.full(w.tree.n_nodes, lam), w.counts[part], B, B, prior, threads=threads)
+                t.append(time.perf_counter() - t0)
+            wall += sum(t)
+            est_total += _extrapolate(t[0], t[1], n_tiny, len(sel), full)
+    F = w.counts.shape[0]
+    return {"value": F * len(lams) / est_total, "unit": "family likelihoods/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle port at {len(lams)} lambdas, {len(idx)}-family subsample, extrapolated "
+                      f"({wall:.1f} s)"}
+
+
+# ----------------------------------------------------------------------------
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--families", type=int, default=20000)
+    ap.add_argument("--trials", type=int, default=64)
+    ap.add_argument("--fp64-steps", type=int, default=3, help="timed steps of the FP64 comparison paths")
+    ap.add_argument("--cpu-lambdas", type=int, default=3)
+    ap.add_argument("--cpu-families", type=int, default=2000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
+              "family sharding with one all-reduce of per-trial totals and is not built", file=sys.stderr)
+        sys.exit(2)
+
+    w = wl.config_uneven(seed=2, F=args.families, T=args.trials)
+    F, T = w.counts.shape[0], len(w.lambdas)
+    log(f"workload: {w.name} F={F} T={T} taxa={w.tree.n_leaves}")
+    eng = native.Engine(0)
+    buckets, slots, lam_q, t_q, prior, N = setup(eng, w)
+    log(f"buckets (B,R,count): {buckets}; keys={len(lam_q)} N={N}")
+    use_graph = not args.no_graph
+
+    # ---- headline: BATCHED32 ----
+    p32 = eng.plan(slots, prior, lam_q, t_q, path=native.PATH_BATCHED32, use_graph=use_graph)
+    ms32, ph32, graph_ok = time_plan(p32, args.steps, args.warmup, args.steps)
+    r32 = p32.results()
+    rc64, rcplain = p32.recompute_counts()
+    log(f"batched32: {ms32:.3f} ms/step, phases(build, prune, recompute, finalize)={ph32.round(3).tolist()} "
+        f"graph={graph_ok} fallback={r32['n_fallback']}")
+
+    # ---- FP64 paths on the same inputs ----
+    p64 = eng.plan(slots, prior, lam_q, t_q, path=native.PATH_BATCHED64, use_graph=use_graph)
+    ms64, ph64, _ = time_plan(p64, args.fp64_steps, 1, 1)
+    r64 = p64.results()
+    log(f"batched64: {ms64:.3f} ms/step phases={ph64.round(3).tolist()}")
+    pplain = eng.plan(slots, prior, lam_q, t_q, path=native.PATH_PLAIN64, use_graph=use_graph)
+    msplain, phplain, _ = time_plan(pplain, args.fp64_steps, 1, 1)
+    rplain = pplain.results()
+    log(f"plain64: {msplain:.3f} ms/step phases={phplain.round(3).tolist()}")
+
+    # ---- accuracy of BATCHED32 against the plain FP64 drop-in ----
+    a = r32["lnl"]
+    b = rplain["lnl"]
+    fin = np.isfinite(b)
+    same_fin = bool(np.array_equal(np.isfinite(a), fin))
+    d = np.abs(a[fin] - b[fin])
+    tot_rel = np.abs(r32["total"] - rplain["total"]) / np.abs(rplain["total"])
+    tot_rel = tot_rel[np.isfinite(tot_rel)]
+    d64 = np.abs(r64["lnl"][fin] - b[fin]) / np.abs(b[fin])
+    accuracy = {
+        "reference_for_comparison": "plain64 GPU path (matches the FP64 oracle to 1e-9 in tests)",
+        "max_abs_dlnL_per_family": float(d.max()) if d.size else 0.0,
+        "max_rel_dlnL_per_family": float((d / np.abs(b[fin])).max()) if d.size else 0.0,
+        "max_rel_dlnL_total_over_lambdas": float(tot_rel.max()) if tot_rel.size else 0.0,
+        "nonfinite_families_match": same_fin,
+        "nonfinite_family_likelihoods": int((~fin).sum()),
+        "fp32_families_recomputed_batched64": int(rc64),
+        "families_recomputed_plain64_near_subnormal": int(rcplain),
+        "batched64_max_rel_dlnL": float(d64.max()) if d64.size else 0.0,
+    }
+    log(f"accuracy: {accuracy}")
+
+    # ---- roofline of the FP32 prune phase ----
+    n1, n2 = tree_flop_factors(w.tree)
+    flops = 0.0
+    for B, R, cnt in buckets:
+        L = B + 1
+        flops += cnt * T * 2.0 * L * (n1 * L + n2 * R)
+    prune_ms = ph32[1]
+    achieved = flops / (prune_ms * 1e-3) if prune_ms > 0 else None
+    build_bytes = len(lam_q) * N * N * (8 + 4)
+    roofline = {"bound": "mfma", "achieved": achieved / 1e12 if achieved else None, "peak": PEAK_FP32_MFMA / 1e12,
+                "unit": "TFLOP/s", "frac": (achieved / PEAK_FP32_MFMA) if achieved else None, "traffic": None,
+                "kernel": "k_prune_mfma<float,*,true> (main FP32 prune phase, all size classes)",
+                "algorithmic_flops_per_launch": flops, "avg_ms": prune_ms,
+                "flops_per_family_likelihood": "2 L (n1 L + n2 R), n1=%d n2=%d" % (n1, n2),
+                "matrix_build": {"ms": ph32[0], "bytes_written": build_bytes,
+                                 "GB_per_s": build_bytes / (ph32[0] * 1e-3) / 1e9 if ph32[0] > 0 else None,
+                                 "hbm_frac": build_bytes / (ph32[0] * 1e-3) / PEAK_HBM if ph32[0] > 0 else None}}
+
+    value = F * T / (ms32 * 1e-3)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "family likelihoods/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "uneven-34taxa lambda grid: seeded Yule tree (34 taxa, height 100), "
+                               f"{F} birth-death families, bounds 20-200 in {len(buckets)} buckets, "
+                               f"{T}-point lambda grid 1e-4..1e-2, {len(lam_q)} matrices of {N}x{N} per step",
+                   "path": "batched32 (FP32 MFMA + certified FP64 recompute)",
+                   "graph": bool(graph_ok), "families": F, "trials": T},
+        "plain64": {"value": F * T / (msplain * 1e-3), "ms_per_step": msplain},
+        "batched64": {"value": F * T / (ms64 * 1e-3), "ms_per_step": ms64},
+        "speedup_vs_plain64": msplain / ms32,
+        "accuracy": accuracy,
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads ...")
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_lambdas, args.cpu_families, threads)
+        log(f"cpu baseline: {out['cpu_baseline']}")
+    for p in (p32, p64, pplain):
+        p.close()
+    eng.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -19,9 +19,9 @@ hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const 
                         hipStream_t stream);
 hipError_t launch_fill_zero(void* p, long long bytes, hipStream_t stream);
 hipError_t launch_prune(int path, int nt, int grid, const PruneParams& p, hipStream_t st);
-hipError_t launch_compact(const TileDesc* tiles, const int* n_tiles_dev, int max_tiles,
-                          const BucketDev* buckets, const int* perm, const int* flags, int F, int mask,
-                          TileDesc* out_tiles, int* out_count, int* fam_count, hipStream_t st);
+hipError_t launch_pack(const SegDesc* segs, int n_segs, const BucketDev* buckets, const int* perm,
+                       const int* flags, int F, int mask, int* fam_list, TileDesc* out_tiles,
+                       const int* class_off, int* class_count, int* fam_count, int tile, hipStream_t st);
 hipError_t launch_finalize(int model, int T, int K, int F, const double* val, const int* flag,
                            const double* cat_probs, double* lnl, double* cat, double* post, int* fail,
                            double* total, hipStream_t st);
@@ -81,6 +81,19 @@ struct BucketHost {
     int B, R, count, fam_begin, max_count;
 };
 
+// Device buffers of one evaluation (eager calls or a plan).
+struct EvalBufs {
+    DevBuf node_slot, prior64, prior32, catp, buckets, tiles, ntiles, counters, fb_tiles, fb_count, val,
+        flag, lnl, cat, post, fail, total, scratch, root, segs, class_off, fam_list;
+    void release_all()
+    {
+        DevBuf* all[] = {&node_slot, &prior64, &prior32, &catp, &buckets, &tiles, &ntiles, &counters, &fb_tiles,
+                         &fb_count, &val, &flag, &lnl, &cat, &post, &fail, &total, &scratch, &root, &segs,
+                         &class_off, &fam_list};
+        for (DevBuf* b : all) b->release();
+    }
+};
+
 }  // namespace
 
 struct cafe_gpu_ctx {
@@ -99,7 +112,7 @@ struct cafe_gpu_ctx {
     std::vector<char> built;
 
     // tree program
-    int n_nodes = 0, tree_leaves = 0, n_slots = 0;
+    int n_nodes = 0, tree_leaves = 0, n_slots = 0, max_children = 2;
     std::vector<OpDev> ops;
     std::vector<ChildDev> kids;
     DevBuf d_ops, d_kids;
@@ -114,10 +127,9 @@ struct cafe_gpu_ctx {
     int n_dev = 0, err_rows = 0;
     DevBuf d_err64, d_err32;
 
-    // eval buffers
-    DevBuf d_node_slot, d_prior64, d_prior32, d_catp, d_buckets, d_tiles, d_ntiles, d_counters;
-    DevBuf d_fb_tiles, d_fb_count, d_val, d_flag, d_lnl, d_cat, d_post, d_fail, d_total, d_scratch,
-        d_root;
+    // buffers of eager evaluations (plans own their own)
+    EvalBufs eager;
+    long long pool_gen = 0;   // bumped by matrix_reset; plans check it
 };
 
 namespace {
@@ -245,12 +257,9 @@ int cafe_gpu_destroy(cafe_gpu_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf* bufs[] = {&ctx->lg, &ctx->mt64, &ctx->mt32, &ctx->key_lam, &ctx->key_t, &ctx->d_ops,
-                      &ctx->d_kids, &ctx->d_counts, &ctx->d_perm, &ctx->d_err64, &ctx->d_err32,
-                      &ctx->d_node_slot, &ctx->d_prior64, &ctx->d_prior32, &ctx->d_catp, &ctx->d_buckets,
-                      &ctx->d_tiles, &ctx->d_ntiles, &ctx->d_counters, &ctx->d_fb_tiles, &ctx->d_fb_count,
-                      &ctx->d_val, &ctx->d_flag, &ctx->d_lnl, &ctx->d_cat, &ctx->d_post, &ctx->d_fail,
-                      &ctx->d_total, &ctx->d_scratch, &ctx->d_root};
+                      &ctx->d_kids, &ctx->d_counts, &ctx->d_perm, &ctx->d_err64, &ctx->d_err32};
     for (DevBuf* b : bufs) b->release();
+    ctx->eager.release_all();
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -267,6 +276,7 @@ int cafe_gpu_matrix_reset(cafe_gpu_ctx* ctx, int N, int capacity)
     ctx->Np = round_up(N, kTile) + kTile;
     ctx->slot_elems = (long long)ctx->Np * ctx->Np;
     ctx->capacity = capacity;
+    ctx->pool_gen++;
     const size_t n64 = sizeof(double) * ctx->slot_elems * capacity;
     const size_t n32 = sizeof(float) * ctx->slot_elems * capacity;
     HIP_OR_FAIL(ctx->mt64.ensure(n64));
@@ -368,6 +378,8 @@ int cafe_gpu_set_tree(cafe_gpu_ctx* ctx, int n_nodes, const int* parent, const i
         return fail(CAFE_GPU_EINVAL, "tree needs %d live partial vectors (max %d)", pl.max_slot + 1, kMaxSlots);
     ctx->n_nodes = n_nodes;
     ctx->tree_leaves = n_leaves;
+    ctx->max_children = 1;
+    for (auto& k : kids) ctx->max_children = std::max(ctx->max_children, (int)k.size());
     ctx->n_slots = pl.max_slot + 1;
     ctx->ops = pl.ops;
     ctx->kids = pl.out_kids;
@@ -462,6 +474,7 @@ int cafe_gpu_set_error_model(cafe_gpu_ctx* ctx, int n_dev, int n_rows, const dou
 
 }  // extern "C"
 
+
 namespace {
 
 struct KernelClass {
@@ -469,187 +482,30 @@ struct KernelClass {
     int begin, end;
 };
 
-// Builds the work list for `points` parameter points and runs the main
-// pruning path plus the FP64 / plain recomputation passes.  Results land in
-// d_val / d_flag ([point][F]).
-int run_prune(cafe_gpu_ctx* ctx, int path, int model, int n_points, const int* point_ids,
-              double* root_out, int root_stride, long long* n_fallback)
-{
-    const int F = ctx->F;
-    // buckets in descending work order
-    std::vector<int> border(ctx->buckets.size());
-    std::iota(border.begin(), border.end(), 0);
-    std::stable_sort(border.begin(), border.end(), [&](int a, int b) {
-        return std::max(ctx->buckets[a].B + 1, ctx->buckets[a].R) > std::max(ctx->buckets[b].B + 1, ctx->buckets[b].R);
-    });
-    int maxLR = 0;
-    for (auto& b : ctx->buckets) maxLR = std::max(maxLR, std::max(round_up(b.B + 1, kTile), round_up(b.R, kTile)));
-
-    auto class_of = [&](const BucketHost& b) {
-        const int lp = round_up(b.B + 1, kTile), rp = round_up(b.R, kTile);
-        if (path == CAFE_GPU_PATH_PLAIN64) return (std::max(b.B + 1, b.R) + 63) / 64;
-        const int nt = std::max(lp, rp) / kTile;
-        static const int scaled_sizes[] = {2, 4, 8, 13, 16, 24, 32};
-        static const int f64_sizes[] = {2, 4, 8};
-        if (path == CAFE_GPU_PATH_BATCHED32) {
-            for (int s : scaled_sizes)
-                if (nt <= s) return s;
-            return -1;
-        }
-        for (int s : f64_sizes)
-            if (nt <= s) return s;
-        return 8;
-    };
-    // plain kernel row classes must be powers of two up to 16
-    auto plain_class = [](int nq) { int c = 1; while (c < nq) c <<= 1; return c; };
-
-    std::vector<TileDesc> tiles;
+// Host-side description of one pruning sequence over a tile list.
+struct PrunePlan {
+    int path = 0, model = 0;
     std::vector<KernelClass> classes;
-    std::vector<std::pair<int, int>> bucket_class;   // (class, bucket)
-    for (int bi : border) {
-        int c = class_of(ctx->buckets[bi]);
-        if (path == CAFE_GPU_PATH_PLAIN64) c = plain_class(c);
-        if (c < 0 || (path == CAFE_GPU_PATH_PLAIN64 && c > 16))
-            return fail(CAFE_GPU_EINVAL, "bucket B=%d R=%d exceeds the supported size", ctx->buckets[bi].B,
-                        ctx->buckets[bi].R);
-        bucket_class.push_back({c, bi});
-    }
-    std::stable_sort(bucket_class.begin(), bucket_class.end(),
-                     [](const std::pair<int, int>& a, const std::pair<int, int>& b) { return a.first > b.first; });
-    for (auto& bc : bucket_class) {
-        const BucketHost& b = ctx->buckets[bc.second];
-        if (classes.empty() || classes.back().nt != bc.first)
-            classes.push_back(KernelClass{bc.first, (int)tiles.size(), (int)tiles.size()});
-        for (int pi = 0; pi < n_points; ++pi)
-            for (int f0 = 0; f0 < b.count; f0 += kTile)
-                tiles.push_back(TileDesc{bc.second, point_ids[pi], f0, std::min(kTile, b.count - f0)});
-        classes.back().end = (int)tiles.size();
-    }
-    const int n_tiles = (int)tiles.size();
-    const int n_cls = (int)classes.size();
-    std::vector<int> counts(n_cls + 4, 0);
-    for (int i = 0; i < n_cls; ++i) counts[i] = classes[i].end - classes[i].begin;
+    std::vector<int> class_counts;   // main tiles per class (+ padding)
+    std::vector<int> plain_nq;       // PLAIN64 rows-per-lane class of each class
+    std::vector<int> plain_off;      // offsets of the one-family PLAIN64 recompute lists
+    int plain_list_size = 0;
+    int n_tiles = 0, n_segs = 0, max_grid = 0, maxLR = 0;
+    long long per_wg_mfma = 0, per_wg_plain = 0;
+    PruneParams base{};
+    // fb_count layout: [0, n) FP64 recompute tiles per class, [n, 2n) PLAIN64
+    // tiles per class, [2n] / [2n+1] families recomputed in each pass
+    int fam_count_off() const { return 2 * (int)classes.size(); }
+};
 
-    HIP_OR_FAIL(ctx->d_tiles.ensure(sizeof(TileDesc) * std::max(1, n_tiles)));
-    HIP_OR_FAIL(ctx->d_fb_tiles.ensure(sizeof(TileDesc) * 2 * std::max(1, n_tiles)));
-    HIP_OR_FAIL(ctx->d_ntiles.ensure(sizeof(int) * (n_cls + 4)));
-    HIP_OR_FAIL(ctx->d_counters.ensure(sizeof(int) * (n_cls + 8)));
-    HIP_OR_FAIL(ctx->d_fb_count.ensure(sizeof(int) * 4));
-    HIP_OR_FAIL(hipMemcpyAsync(ctx->d_tiles.p, tiles.data(), sizeof(TileDesc) * n_tiles, hipMemcpyHostToDevice,
-                               ctx->stream));
-    HIP_OR_FAIL(hipMemcpyAsync(ctx->d_ntiles.p, counts.data(), sizeof(int) * counts.size(), hipMemcpyHostToDevice,
-                               ctx->stream));
-    HIP_OR_FAIL(launch_fill_zero(ctx->d_counters.p, sizeof(int) * (n_cls + 8), ctx->stream));
-    HIP_OR_FAIL(launch_fill_zero(ctx->d_fb_count.p, sizeof(int) * 4, ctx->stream));
-
-    // per-workgroup scratch for partial vectors
-    const int ldv = round_up(maxLR, 4) + 4;
-    const int max_grid = std::max(64, ctx->n_cu * 16);
-    const long long per_wg_mfma = (long long)ctx->n_slots * 16 * ldv;
-    const long long per_wg_plain = (long long)ctx->n_slots * ldv;
-    const size_t scratch_bytes =
-        (size_t)max_grid * std::max(per_wg_mfma * (path == CAFE_GPU_PATH_BATCHED32 ? 8 : 8), per_wg_plain * 8);
-    HIP_OR_FAIL(ctx->d_scratch.ensure(scratch_bytes));
-
-    PruneParams p{};
-    p.buckets = ctx->d_buckets.as<BucketDev>();
-    p.ops = ctx->d_ops.as<OpDev>();
-    p.n_ops = (int)ctx->ops.size();
-    p.kids = ctx->d_kids.as<ChildDev>();
-    p.n_nodes = ctx->n_nodes;
-    p.node_slot = ctx->d_node_slot.as<int>();
-    p.counts_T = ctx->d_counts.as<int>();
-    p.F = F;
-    p.perm = ctx->d_perm.as<int>();
-    p.mt64 = ctx->mt64.as<double>();
-    p.mt32 = ctx->mt32.as<float>();
-    p.slot_elems = ctx->slot_elems;
-    p.Np = ctx->Np;
-    p.err64 = ctx->n_dev ? ctx->d_err64.as<double>() : nullptr;
-    p.err32 = ctx->n_dev ? ctx->d_err32.as<float>() : nullptr;
-    p.n_dev = ctx->n_dev;
-    p.prior64 = ctx->d_prior64.as<double>();
-    p.prior32 = ctx->d_prior32.as<float>();
-    p.model = model;
-    p.out_val = ctx->d_val.as<double>();
-    p.out_flag = ctx->d_flag.as<int>();
-    p.root_out = root_out;
-    p.root_stride = root_stride;
-    p.scratch = ctx->d_scratch.p;
-    p.ldv = ldv;
-    p.n_slots = ctx->n_slots;
-
-    int* d_cnt = ctx->d_ntiles.as<int>();
-    int* d_ctr = ctx->d_counters.as<int>();
-    HIP_OR_FAIL(hipEventRecord(ctx->ev[0], ctx->stream));
-    for (int ci = 0; ci < n_cls; ++ci) {
-        PruneParams q = p;
-        q.tiles = ctx->d_tiles.as<TileDesc>() + classes[ci].begin;
-        q.n_tiles_dev = d_cnt + ci;
-        q.tile_counter = d_ctr + ci;
-        q.scratch_per_wg = path == CAFE_GPU_PATH_PLAIN64 ? per_wg_plain : per_wg_mfma;
-        const int grid = std::max(1, std::min(counts[ci], max_grid));
-        HIP_OR_FAIL(launch_prune(path, classes[ci].nt, grid, q, ctx->stream));
-    }
-    HIP_OR_FAIL(hipEventRecord(ctx->ev[1], ctx->stream));
-
-    // recomputation passes (device-side lists, no host round trip)
-    TileDesc* fb64 = ctx->d_fb_tiles.as<TileDesc>();
-    TileDesc* fbplain = fb64 + std::max(1, n_tiles);
-    int* fbc = ctx->d_fb_count.as<int>();   // [0] fp64 tiles, [1] plain tiles
-    const TileDesc* plain_src = ctx->d_tiles.as<TileDesc>();
-    const int* plain_src_n = nullptr;
-    if (path == CAFE_GPU_PATH_BATCHED32) {
-        for (int ci = 0; ci < n_cls; ++ci)
-            HIP_OR_FAIL(launch_compact(ctx->d_tiles.as<TileDesc>() + classes[ci].begin, d_cnt + ci, counts[ci],
-                                       ctx->d_buckets.as<BucketDev>(), ctx->d_perm.as<int>(), ctx->d_flag.as<int>(),
-                                       F, kFlagRecompute64, fb64, fbc + 0, fbc + 2, ctx->stream));
-        PruneParams q = p;
-        q.tiles = fb64;
-        q.n_tiles_dev = fbc + 0;
-        q.tile_counter = d_ctr + n_cls;
-        q.scratch_per_wg = per_wg_mfma;
-        HIP_OR_FAIL(launch_prune(CAFE_GPU_PATH_BATCHED64, 8, std::min(max_grid, std::max(1, n_tiles)), q,
-                                 ctx->stream));
-        HIP_OR_FAIL(launch_compact(fb64, fbc + 0, n_tiles, ctx->d_buckets.as<BucketDev>(), ctx->d_perm.as<int>(),
-                                   ctx->d_flag.as<int>(), F, kFlagRecomputePlain, fbplain, fbc + 1, fbc + 3, ctx->stream));
-        plain_src_n = fbc + 1;
-    } else if (path == CAFE_GPU_PATH_BATCHED64) {
-        for (int ci = 0; ci < n_cls; ++ci)
-            HIP_OR_FAIL(launch_compact(ctx->d_tiles.as<TileDesc>() + classes[ci].begin, d_cnt + ci, counts[ci],
-                                       ctx->d_buckets.as<BucketDev>(), ctx->d_perm.as<int>(), ctx->d_flag.as<int>(),
-                                       F, kFlagRecomputePlain, fbplain, fbc + 1, fbc + 3, ctx->stream));
-        plain_src_n = fbc + 1;
-    }
-    (void)plain_src;
-    if (plain_src_n) {
-        PruneParams q = p;
-        q.tiles = fbplain;
-        q.n_tiles_dev = plain_src_n;
-        q.tile_counter = d_ctr + n_cls + 1;
-        q.scratch_per_wg = per_wg_plain;
-        int nq = 1;
-        while (nq * 64 < maxLR) nq <<= 1;
-        HIP_OR_FAIL(launch_prune(CAFE_GPU_PATH_PLAIN64, nq, std::min(max_grid, std::max(1, n_tiles)), q,
-                                 ctx->stream));
-    }
-    HIP_OR_FAIL(hipEventRecord(ctx->ev[2], ctx->stream));
-    if (n_fallback) {
-        int h[4] = {0, 0, 0, 0};
-        HIP_OR_FAIL(hipMemcpyAsync(h, fbc, sizeof(int) * 4, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
-        // families recomputed: FP64 pass + plain pass
-        *n_fallback = (long long)h[2] + (long long)h[3];
-    }
-    return CAFE_GPU_OK;
-}
-
-int prepare_eval(cafe_gpu_ctx* ctx, const cafe_gpu_eval_desc* d, int& P)
+// Validates a descriptor and uploads node slots, priors, bucket table and
+// category probabilities into b; sizes the per-point outputs.
+int upload_eval_inputs(cafe_gpu_ctx* ctx, EvalBufs& b, const cafe_gpu_eval_desc* d, int& P)
 {
     if (!d) return fail(CAFE_GPU_EINVAL, "null descriptor");
     if (ctx->n_nodes == 0) return fail(CAFE_GPU_ESTATE, "set_tree has not been called");
     if (ctx->F == 0) return fail(CAFE_GPU_ESTATE, "set_families has not been called");
-    if (ctx->N == 0) return fail(CAFE_GPU_ESTATE, "no matrices have been built");
+    if (ctx->N == 0) return fail(CAFE_GPU_ESTATE, "matrix_reset has not been called");
     if (ctx->n_leaves != ctx->tree_leaves)
         return fail(CAFE_GPU_EINVAL, "families have %d leaves, tree has %d", ctx->n_leaves, ctx->tree_leaves);
     if (d->model != CAFE_GPU_MODEL_BASE && d->model != CAFE_GPU_MODEL_GAMMA)
@@ -663,20 +519,20 @@ int prepare_eval(cafe_gpu_ctx* ctx, const cafe_gpu_eval_desc* d, int& P)
     if (d->model == CAFE_GPU_MODEL_GAMMA && !d->cat_probs) return fail(CAFE_GPU_EINVAL, "gamma needs cat_probs");
     const int h = ctx->n_dev ? (ctx->n_dev - 1) / 2 : 0;
     int maxR = 0;
-    for (auto& b : ctx->buckets) {
-        if (std::max(b.B, b.R) + 1 > ctx->N)
-            return fail(CAFE_GPU_EINVAL, "bucket B=%d R=%d needs matrices of size %d (built %d)", b.B, b.R,
-                        std::max(b.B, b.R) + 1, ctx->N);
-        if (ctx->n_dev && (b.B >= ctx->err_rows || b.max_count + h > b.B))
-            return fail(CAFE_GPU_EINVAL, "error model does not cover bucket B=%d (max count %d)", b.B, b.max_count);
-        maxR = std::max(maxR, b.R);
+    for (auto& bk : ctx->buckets) {
+        if (std::max(bk.B, bk.R) + 1 > ctx->N)
+            return fail(CAFE_GPU_EINVAL, "bucket B=%d R=%d needs matrices of size %d (pool has %d)", bk.B, bk.R,
+                        std::max(bk.B, bk.R) + 1, ctx->N);
+        if (ctx->n_dev && (bk.B >= ctx->err_rows || bk.max_count + h > bk.B))
+            return fail(CAFE_GPU_EINVAL, "error model does not cover bucket B=%d (max count %d)", bk.B,
+                        bk.max_count);
+        maxR = std::max(maxR, bk.R);
     }
     if (d->prior_stride < maxR) return fail(CAFE_GPU_EINVAL, "prior_stride %d < max R %d", d->prior_stride, maxR);
     P = d->n_trials * K;
     const long long ns = (long long)P * ctx->n_nodes;
     std::vector<int> slot(d->node_slot, d->node_slot + ns);
-    int root = -1;
-    for (auto& op : ctx->ops) root = op.node;   // root is the last op
+    const int root = ctx->ops.back().node;
     for (long long i = 0; i < ns; ++i) {
         const int node = (int)(i % ctx->n_nodes);
         if (node == root) {
@@ -691,39 +547,341 @@ int prepare_eval(cafe_gpu_ctx* ctx, const cafe_gpu_eval_desc* d, int& P)
     std::vector<double> pr64((size_t)nb * d->prior_stride, 0.0);
     std::vector<float> pr32(pr64.size());
     std::vector<BucketDev> bd(nb);
-    for (int b = 0; b < nb; ++b) {
-        const auto& bh = ctx->buckets[b];
-        for (int j = 0; j < bh.R; ++j) pr64[(size_t)b * d->prior_stride + j] = d->prior[(size_t)b * d->prior_stride + j];
-        bd[b].B = bh.B;
-        bd[b].R = bh.R;
-        bd[b].L = bh.B + 1;
-        bd[b].Lp = round_up(bh.B + 1, kTile);
-        bd[b].Rp = round_up(bh.R, kTile);
-        bd[b].fam_begin = bh.fam_begin;
-        bd[b].count = bh.count;
-        bd[b].prior_off = b * d->prior_stride;
+    for (int i = 0; i < nb; ++i) {
+        const auto& bh = ctx->buckets[i];
+        for (int j = 0; j < bh.R; ++j) pr64[(size_t)i * d->prior_stride + j] = d->prior[(size_t)i * d->prior_stride + j];
+        bd[i].B = bh.B;
+        bd[i].R = bh.R;
+        bd[i].L = bh.B + 1;
+        bd[i].Lp = round_up(bh.B + 1, kTile);
+        bd[i].Rp = round_up(bh.R, kTile);
+        bd[i].fam_begin = bh.fam_begin;
+        bd[i].count = bh.count;
+        bd[i].prior_off = i * d->prior_stride;
     }
     for (size_t i = 0; i < pr64.size(); ++i) pr32[i] = (float)pr64[i];
-    HIP_OR_FAIL(ctx->d_node_slot.ensure(sizeof(int) * ns));
-    HIP_OR_FAIL(ctx->d_prior64.ensure(sizeof(double) * pr64.size()));
-    HIP_OR_FAIL(ctx->d_prior32.ensure(sizeof(float) * pr32.size()));
-    HIP_OR_FAIL(ctx->d_buckets.ensure(sizeof(BucketDev) * nb));
-    HIP_OR_FAIL(ctx->d_catp.ensure(sizeof(double) * K));
-    HIP_OR_FAIL(hipMemcpyAsync(ctx->d_node_slot.p, slot.data(), sizeof(int) * ns, hipMemcpyHostToDevice, ctx->stream));
-    HIP_OR_FAIL(hipMemcpyAsync(ctx->d_prior64.p, pr64.data(), sizeof(double) * pr64.size(), hipMemcpyHostToDevice,
-                               ctx->stream));
-    HIP_OR_FAIL(hipMemcpyAsync(ctx->d_prior32.p, pr32.data(), sizeof(float) * pr32.size(), hipMemcpyHostToDevice,
-                               ctx->stream));
-    HIP_OR_FAIL(hipMemcpyAsync(ctx->d_buckets.p, bd.data(), sizeof(BucketDev) * nb, hipMemcpyHostToDevice,
-                               ctx->stream));
     std::vector<double> cp(K, 1.0);
     if (d->cat_probs) cp.assign(d->cat_probs, d->cat_probs + K);
-    HIP_OR_FAIL(hipMemcpyAsync(ctx->d_catp.p, cp.data(), sizeof(double) * K, hipMemcpyHostToDevice, ctx->stream));
+    HIP_OR_FAIL(b.node_slot.ensure(sizeof(int) * ns));
+    HIP_OR_FAIL(b.prior64.ensure(sizeof(double) * pr64.size()));
+    HIP_OR_FAIL(b.prior32.ensure(sizeof(float) * pr32.size()));
+    HIP_OR_FAIL(b.buckets.ensure(sizeof(BucketDev) * nb));
+    HIP_OR_FAIL(b.catp.ensure(sizeof(double) * K));
+    HIP_OR_FAIL(hipMemcpy(b.node_slot.p, slot.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+    HIP_OR_FAIL(hipMemcpy(b.prior64.p, pr64.data(), sizeof(double) * pr64.size(), hipMemcpyHostToDevice));
+    HIP_OR_FAIL(hipMemcpy(b.prior32.p, pr32.data(), sizeof(float) * pr32.size(), hipMemcpyHostToDevice));
+    HIP_OR_FAIL(hipMemcpy(b.buckets.p, bd.data(), sizeof(BucketDev) * nb, hipMemcpyHostToDevice));
+    HIP_OR_FAIL(hipMemcpy(b.catp.p, cp.data(), sizeof(double) * K, hipMemcpyHostToDevice));
     const size_t PF = (size_t)P * ctx->F;
-    HIP_OR_FAIL(ctx->d_val.ensure(sizeof(double) * PF));
-    HIP_OR_FAIL(ctx->d_flag.ensure(sizeof(int) * PF));
-    HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));   // host vectors above go out of scope
+    HIP_OR_FAIL(b.val.ensure(sizeof(double) * PF));
+    HIP_OR_FAIL(b.flag.ensure(sizeof(int) * PF));
+    const size_t TF = (size_t)d->n_trials * ctx->F;
+    HIP_OR_FAIL(b.lnl.ensure(sizeof(double) * TF));
+    HIP_OR_FAIL(b.cat.ensure(sizeof(double) * TF * K));
+    HIP_OR_FAIL(b.post.ensure(sizeof(double) * TF * K));
+    HIP_OR_FAIL(b.fail.ensure(sizeof(int) * TF));
+    HIP_OR_FAIL(b.total.ensure(sizeof(double) * d->n_trials));
     return CAFE_GPU_OK;
+}
+
+// Builds and uploads the tile list for `n_points` parameter points (heaviest
+// buckets first, grouped by kernel size class) and sizes the scratch.
+int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points, const int* point_ids,
+               double* root_out, int root_stride, PrunePlan& pp)
+{
+    pp = PrunePlan();
+    pp.path = path;
+    pp.model = model;
+    int maxLR = 0;
+    for (auto& bk : ctx->buckets)
+        maxLR = std::max(maxLR, std::max(round_up(bk.B + 1, kTile), round_up(bk.R, kTile)));
+    pp.maxLR = maxLR;
+    auto class_of = [&](const BucketHost& bk) {
+        const int lp = round_up(bk.B + 1, kTile), rp = round_up(bk.R, kTile);
+        if (std::max(bk.B + 1, bk.R) > 1024) return -1;   // PLAIN64 (and its recompute) covers 16 x 64 rows
+        if (path == CAFE_GPU_PATH_PLAIN64) {
+            int c = 1;
+            while (c * 64 < std::max(bk.B + 1, bk.R)) c <<= 1;
+            return c <= 16 ? c : -1;
+        }
+        const int nt = std::max(lp, rp) / kTile;
+        static const int scaled_sizes[] = {2, 4, 8, 13, 16, 24, 32};
+        static const int f64_sizes[] = {2, 4, 8};
+        if (path == CAFE_GPU_PATH_BATCHED32) {
+            for (int sz : scaled_sizes)
+                if (nt <= sz) return sz;
+            return -1;
+        }
+        for (int sz : f64_sizes)
+            if (nt <= sz) return sz;
+        return 8;   // FP64: chunks of 8 tiles
+    };
+    std::vector<std::pair<int, int>> bucket_class;   // (class, bucket)
+    for (int bi = 0; bi < (int)ctx->buckets.size(); ++bi) {
+        const int c = class_of(ctx->buckets[bi]);
+        if (c < 0)
+            return fail(CAFE_GPU_EINVAL, "bucket B=%d R=%d exceeds the sizes this path supports",
+                        ctx->buckets[bi].B, ctx->buckets[bi].R);
+        bucket_class.push_back({c, bi});
+    }
+    // heavy classes (and, within a class, larger buckets) first
+    std::stable_sort(bucket_class.begin(), bucket_class.end(), [&](const std::pair<int, int>& x,
+                                                                   const std::pair<int, int>& y) {
+        if (x.first != y.first) return x.first > y.first;
+        return ctx->buckets[x.second].B > ctx->buckets[y.second].B;
+    });
+    std::vector<TileDesc> tiles;
+    for (auto& bc : bucket_class) {
+        const BucketHost& bk = ctx->buckets[bc.second];
+        if (pp.classes.empty() || pp.classes.back().nt != bc.first)
+            pp.classes.push_back(KernelClass{bc.first, (int)tiles.size(), (int)tiles.size()});
+        for (int pi = 0; pi < n_points; ++pi)
+            for (int f0 = 0; f0 < bk.count; f0 += kTile)
+                tiles.push_back(TileDesc{bc.second, point_ids[pi], f0, std::min(kTile, bk.count - f0)});
+        pp.classes.back().end = (int)tiles.size();
+    }
+    pp.n_tiles = (int)tiles.size();
+    const int n_cls = (int)pp.classes.size();
+    pp.class_counts.assign(n_cls + 4, 0);
+    for (int i = 0; i < n_cls; ++i) pp.class_counts[i] = pp.classes[i].end - pp.classes[i].begin;
+    // (bucket, point) segments per class, recompute-list offsets, and the
+    // PLAIN64 row class that covers each class's buckets
+    std::vector<SegDesc> segs;
+    std::vector<int> class_off(2 * (n_cls + 1), 0), cls_max(n_cls, 1), cls_fams(n_cls, 0);
+    {
+        int ci = -1, prev = -1;
+        for (auto& bc : bucket_class) {
+            if (bc.first != prev) {
+                ++ci;
+                prev = bc.first;
+            }
+            const BucketHost& bk = ctx->buckets[bc.second];
+            cls_max[ci] = std::max(cls_max[ci], std::max(bk.B + 1, bk.R));
+            cls_fams[ci] += bk.count * n_points;
+            for (int pi = 0; pi < n_points; ++pi) segs.push_back(SegDesc{bc.second, point_ids[pi], ci, 0});
+        }
+    }
+    // [0, n_cls] offsets of the 16-family recompute lists (one slot per main
+    // tile), [n_cls + 1, 2 n_cls + 1] of the one-family PLAIN64 lists
+    for (int i = 0; i < n_cls; ++i) class_off[i + 1] = class_off[i] + pp.class_counts[i];
+    for (int i = 0; i < n_cls; ++i) class_off[n_cls + 1 + i + 1] = class_off[n_cls + 1 + i] + cls_fams[i];
+    pp.plain_list_size = class_off[2 * n_cls + 1];
+    pp.plain_off.assign(class_off.begin() + n_cls + 1, class_off.end());
+    pp.plain_nq.assign(n_cls, 1);
+    for (int i = 0; i < n_cls; ++i)
+        while (pp.plain_nq[i] * 64 < cls_max[i] && pp.plain_nq[i] < 16) pp.plain_nq[i] <<= 1;
+    pp.n_segs = (int)segs.size();
+
+    HIP_OR_FAIL(b.tiles.ensure(sizeof(TileDesc) * std::max(1, pp.n_tiles)));
+    HIP_OR_FAIL(b.fb_tiles.ensure(sizeof(TileDesc) * ((size_t)std::max(1, pp.n_tiles) + std::max(1, pp.plain_list_size))));
+    HIP_OR_FAIL(b.ntiles.ensure(sizeof(int) * (n_cls + 4)));
+    HIP_OR_FAIL(b.counters.ensure(sizeof(int) * (3 * n_cls + 8)));
+    HIP_OR_FAIL(b.fb_count.ensure(sizeof(int) * (2 * n_cls + 4)));
+    HIP_OR_FAIL(b.segs.ensure(sizeof(SegDesc) * std::max(1, pp.n_segs)));
+    HIP_OR_FAIL(b.class_off.ensure(sizeof(int) * 2 * (n_cls + 1)));
+    {
+        size_t maxp = 0;
+        for (int i = 0; i < n_points; ++i) maxp = std::max(maxp, (size_t)point_ids[i] + 1);
+        HIP_OR_FAIL(b.fam_list.ensure(sizeof(int) * maxp * (size_t)ctx->F));
+    }
+    HIP_OR_FAIL(hipMemcpy(b.tiles.p, tiles.data(), sizeof(TileDesc) * pp.n_tiles, hipMemcpyHostToDevice));
+    HIP_OR_FAIL(hipMemcpy(b.ntiles.p, pp.class_counts.data(), sizeof(int) * pp.class_counts.size(),
+                          hipMemcpyHostToDevice));
+    HIP_OR_FAIL(hipMemcpy(b.segs.p, segs.data(), sizeof(SegDesc) * pp.n_segs, hipMemcpyHostToDevice));
+    HIP_OR_FAIL(hipMemcpy(b.class_off.p, class_off.data(), sizeof(int) * 2 * (n_cls + 1), hipMemcpyHostToDevice));
+
+    const int ldv = round_up(maxLR, 4) + 4;
+    pp.max_grid = std::max(64, ctx->n_cu * 16);
+    pp.per_wg_mfma = (long long)ctx->n_slots * 16 * ldv;
+    pp.per_wg_plain = (long long)ctx->n_slots * ldv;
+    HIP_OR_FAIL(b.scratch.ensure((size_t)pp.max_grid * 8 * std::max(pp.per_wg_mfma, pp.per_wg_plain)));
+
+    PruneParams& p = pp.base;
+    p.buckets = b.buckets.as<BucketDev>();
+    p.ops = ctx->d_ops.as<OpDev>();
+    p.n_ops = (int)ctx->ops.size();
+    p.kids = ctx->d_kids.as<ChildDev>();
+    p.n_nodes = ctx->n_nodes;
+    p.node_slot = b.node_slot.as<int>();
+    p.counts_T = ctx->d_counts.as<int>();
+    p.F = ctx->F;
+    p.perm = ctx->d_perm.as<int>();
+    p.mt64 = ctx->mt64.as<double>();
+    p.mt32 = ctx->mt32.as<float>();
+    p.slot_elems = ctx->slot_elems;
+    p.Np = ctx->Np;
+    p.err64 = ctx->n_dev ? ctx->d_err64.as<double>() : nullptr;
+    p.err32 = ctx->n_dev ? ctx->d_err32.as<float>() : nullptr;
+    p.n_dev = ctx->n_dev;
+    p.prior64 = b.prior64.as<double>();
+    p.prior32 = b.prior32.as<float>();
+    p.model = model;
+    p.out_val = b.val.as<double>();
+    p.out_flag = b.flag.as<int>();
+    p.root_out = root_out;
+    p.root_stride = root_stride;
+    p.fam_list = b.fam_list.as<int>();
+    p.vec_exp = 120 / std::max(1, ctx->max_children) - kMatExp;
+    p.scratch = b.scratch.p;
+    p.ldv = ldv;
+    p.n_slots = ctx->n_slots;
+    return CAFE_GPU_OK;
+}
+
+// Enqueues the pruning sequence: the main path per size class, then
+// (device-side lists, no host round trip) the FP64 recompute of the FP32
+// families the flush bound did not certify and the PLAIN64 recompute of
+// families near the subnormal range, each packed 16 families to a tile.
+int enqueue_prune(EvalBufs& b, const PrunePlan& pp, hipStream_t st, hipEvent_t ev_main, hipEvent_t ev_fb)
+{
+    const int n_cls = (int)pp.classes.size();
+    int* d_cnt = b.ntiles.as<int>();
+    int* d_ctr = b.counters.as<int>();
+    int* fbc = b.fb_count.as<int>();
+    HIP_OR_FAIL(launch_fill_zero(b.counters.p, sizeof(int) * (3 * n_cls + 8), st));
+    HIP_OR_FAIL(launch_fill_zero(b.fb_count.p, sizeof(int) * (2 * n_cls + 4), st));
+    for (int ci = 0; ci < n_cls; ++ci) {
+        PruneParams q = pp.base;
+        q.tiles = b.tiles.as<TileDesc>() + pp.classes[ci].begin;
+        q.n_tiles_dev = d_cnt + ci;
+        q.tile_counter = d_ctr + ci;
+        q.scratch_per_wg = pp.path == CAFE_GPU_PATH_PLAIN64 ? pp.per_wg_plain : pp.per_wg_mfma;
+        const int grid = std::max(1, std::min(pp.class_counts[ci], pp.max_grid));
+        HIP_OR_FAIL(launch_prune(pp.path, pp.classes[ci].nt, grid, q, st));
+    }
+    if (ev_main) HIP_OR_FAIL(hipEventRecord(ev_main, st));
+    TileDesc* fb64 = b.fb_tiles.as<TileDesc>();
+    TileDesc* fbplain = fb64 + std::max(1, pp.n_tiles);
+    const int* coff = b.class_off.as<int>();
+    std::vector<int> hoff(n_cls + 1, 0);
+    for (int i = 0; i < n_cls; ++i) hoff[i + 1] = hoff[i] + pp.class_counts[i];
+    const int fam_off = pp.fam_count_off();
+    if (pp.path == CAFE_GPU_PATH_BATCHED32) {
+        HIP_OR_FAIL(launch_pack(b.segs.as<SegDesc>(), pp.n_segs, b.buckets.as<BucketDev>(), pp.base.perm,
+                                b.flag.as<int>(), pp.base.F, kFlagRecompute64, b.fam_list.as<int>(), fb64, coff,
+                                fbc, fbc + fam_off, kTile, st));
+        for (int ci = 0; ci < n_cls; ++ci) {
+            PruneParams q = pp.base;
+            q.tiles = fb64 + hoff[ci];
+            q.n_tiles_dev = fbc + ci;
+            q.tile_counter = d_ctr + n_cls + ci;
+            q.scratch_per_wg = pp.per_wg_mfma;
+            const int grid = std::max(1, std::min(pp.class_counts[ci], pp.max_grid));
+            HIP_OR_FAIL(launch_prune(CAFE_GPU_PATH_BATCHED64, pp.classes[ci].nt, grid, q, st));
+        }
+    }
+    if (pp.path != CAFE_GPU_PATH_PLAIN64) {
+        HIP_OR_FAIL(launch_pack(b.segs.as<SegDesc>(), pp.n_segs, b.buckets.as<BucketDev>(), pp.base.perm,
+                                b.flag.as<int>(), pp.base.F, kFlagRecomputePlain, b.fam_list.as<int>(), fbplain,
+                                coff + n_cls + 1, fbc + n_cls, fbc + fam_off + 1, 1, st));
+        for (int ci = 0; ci < n_cls; ++ci) {
+            PruneParams q = pp.base;
+            q.tiles = fbplain + pp.plain_off[ci];
+            q.n_tiles_dev = fbc + n_cls + ci;
+            q.tile_counter = d_ctr + 2 * n_cls + ci;
+            q.scratch_per_wg = pp.per_wg_plain;
+            const int grid = std::max(1, std::min(pp.plain_off[ci + 1] - pp.plain_off[ci], pp.max_grid));
+            HIP_OR_FAIL(launch_prune(CAFE_GPU_PATH_PLAIN64, pp.plain_nq[ci], grid, q, st));
+        }
+    }
+    if (ev_fb) HIP_OR_FAIL(hipEventRecord(ev_fb, st));
+    return CAFE_GPU_OK;
+}
+
+int enqueue_finalize(cafe_gpu_ctx* ctx, EvalBufs& b, int model, int T, int K, hipStream_t st)
+{
+    HIP_OR_FAIL(launch_finalize(model, T, K, ctx->F, b.val.as<double>(), b.flag.as<int>(), b.catp.as<double>(),
+                                b.lnl.as<double>(), b.cat.as<double>(), b.post.as<double>(), b.fail.as<int>(),
+                                b.total.as<double>(), st));
+    return CAFE_GPU_OK;
+}
+
+int copy_results(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, int model, int T, int K, double* lnl,
+                 double* cat, double* post, double* total, long long* n_fallback)
+{
+    const size_t TF = (size_t)T * ctx->F;
+    hipStream_t st = ctx->stream;
+    if (lnl) HIP_OR_FAIL(hipMemcpyAsync(lnl, b.lnl.p, sizeof(double) * TF, hipMemcpyDeviceToHost, st));
+    if (cat && model == CAFE_GPU_MODEL_GAMMA)
+        HIP_OR_FAIL(hipMemcpyAsync(cat, b.cat.p, sizeof(double) * TF * K, hipMemcpyDeviceToHost, st));
+    if (post && model == CAFE_GPU_MODEL_GAMMA)
+        HIP_OR_FAIL(hipMemcpyAsync(post, b.post.p, sizeof(double) * TF * K, hipMemcpyDeviceToHost, st));
+    if (total) HIP_OR_FAIL(hipMemcpyAsync(total, b.total.p, sizeof(double) * T, hipMemcpyDeviceToHost, st));
+    int h[2] = {0, 0};
+    if (n_fallback)
+        HIP_OR_FAIL(hipMemcpyAsync(h, b.fb_count.as<int>() + pp.fam_count_off(), sizeof(int) * 2,
+                                   hipMemcpyDeviceToHost, st));
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    if (n_fallback) *n_fallback = (long long)h[0] + (long long)h[1];
+    return CAFE_GPU_OK;
+}
+
+double elapsed_ms(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return ms;
+}
+
+}  // namespace
+
+// One device-resident evaluation (see cafe_gpu_plan_* in cafe_gpu.h).
+struct cafe_gpu_plan {
+    cafe_gpu_ctx* ctx = nullptr;
+    EvalBufs bufs;
+    PrunePlan pp;
+    int model = 0, path = 0, T = 0, K = 0, n_keys = 0, algo = 0;
+    DevBuf d_lam, d_t;
+    long long pool_gen = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool graph_ok = false;
+    hipEvent_t ev[5] = {};
+};
+
+namespace {
+
+// build -> prune -> recompute -> finalize; events bracket the phases when
+// `timed` (ev[0] start, ev[1] after build, ev[2] after the main prune,
+// ev[3] after the recompute passes, ev[4] after finalize)
+int plan_enqueue(cafe_gpu_plan* pl, bool timed)
+{
+    cafe_gpu_ctx* ctx = pl->ctx;
+    hipStream_t st = ctx->stream;
+    if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[0], st));
+    HIP_OR_FAIL(launch_build(pl->algo, pl->d_lam.as<double>(), pl->d_t.as<double>(), ctx->lg.as<double>(), pl->n_keys,
+                             0, ctx->N, ctx->Np, ctx->slot_elems, ctx->mt64.as<double>(), ctx->mt32.as<float>(), st));
+    if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[1], st));
+    if (int rc = enqueue_prune(pl->bufs, pl->pp, st, timed ? pl->ev[2] : nullptr, timed ? pl->ev[3] : nullptr))
+        return rc;
+    if (int rc = enqueue_finalize(ctx, pl->bufs, pl->model, pl->T, pl->K, st)) return rc;
+    if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[4], st));
+    return CAFE_GPU_OK;
+}
+
+// Captures plan_enqueue into a graph; returns false (stream left clean) if the
+// runtime refuses any step of the capture.
+bool plan_capture(cafe_gpu_plan* pl, bool timed)
+{
+    hipStream_t st = pl->ctx->stream;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) return false;
+    const int rc = plan_enqueue(pl, timed);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc != CAFE_GPU_OK || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return false;
+    }
+    hipGraphExec_t ex = nullptr;
+    if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return false;
+    }
+    pl->graph = g;
+    pl->exec = ex;
+    return true;
 }
 
 }  // namespace
@@ -734,37 +892,22 @@ int cafe_gpu_eval(cafe_gpu_ctx* ctx, const cafe_gpu_eval_desc* d, double* lnl, d
                   double* total, long long* n_fallback)
 {
     if (int rc = check_ctx(ctx)) return rc;
+    EvalBufs& b = ctx->eager;
     int P = 0;
-    if (int rc = prepare_eval(ctx, d, P)) return rc;
+    if (int rc = upload_eval_inputs(ctx, b, d, P)) return rc;
     std::vector<int> pts(P);
     std::iota(pts.begin(), pts.end(), 0);
-    if (int rc = run_prune(ctx, d->path, d->model, P, pts.data(), nullptr, 0, n_fallback)) return rc;
-    const int T = d->n_trials, K = d->n_cat, F = ctx->F;
-    const size_t TF = (size_t)T * F;
-    HIP_OR_FAIL(ctx->d_lnl.ensure(sizeof(double) * TF));
-    HIP_OR_FAIL(ctx->d_cat.ensure(sizeof(double) * TF * K));
-    HIP_OR_FAIL(ctx->d_post.ensure(sizeof(double) * TF * K));
-    HIP_OR_FAIL(ctx->d_fail.ensure(sizeof(int) * TF));
-    HIP_OR_FAIL(ctx->d_total.ensure(sizeof(double) * T));
-    HIP_OR_FAIL(launch_finalize(d->model, T, K, F, ctx->d_val.as<double>(), ctx->d_flag.as<int>(),
-                                ctx->d_catp.as<double>(), ctx->d_lnl.as<double>(), ctx->d_cat.as<double>(),
-                                ctx->d_post.as<double>(), ctx->d_fail.as<int>(), ctx->d_total.as<double>(),
-                                ctx->stream));
+    PrunePlan pp;
+    if (int rc = plan_prune(ctx, b, d->path, d->model, P, pts.data(), nullptr, 0, pp)) return rc;
+    HIP_OR_FAIL(hipEventRecord(ctx->ev[0], ctx->stream));
+    if (int rc = enqueue_prune(b, pp, ctx->stream, ctx->ev[1], ctx->ev[2])) return rc;
+    if (int rc = enqueue_finalize(ctx, b, d->model, d->n_trials, d->n_cat, ctx->stream)) return rc;
     HIP_OR_FAIL(hipEventRecord(ctx->ev[3], ctx->stream));
-    if (lnl) HIP_OR_FAIL(hipMemcpyAsync(lnl, ctx->d_lnl.p, sizeof(double) * TF, hipMemcpyDeviceToHost, ctx->stream));
-    if (cat && d->model == CAFE_GPU_MODEL_GAMMA)
-        HIP_OR_FAIL(hipMemcpyAsync(cat, ctx->d_cat.p, sizeof(double) * TF * K, hipMemcpyDeviceToHost, ctx->stream));
-    if (post && d->model == CAFE_GPU_MODEL_GAMMA)
-        HIP_OR_FAIL(hipMemcpyAsync(post, ctx->d_post.p, sizeof(double) * TF * K, hipMemcpyDeviceToHost, ctx->stream));
-    if (total) HIP_OR_FAIL(hipMemcpyAsync(total, ctx->d_total.p, sizeof(double) * T, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
-    ctx->timings[1] = ms;
-    (void)hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]);
-    ctx->timings[2] = ms;
-    (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
-    ctx->timings[3] = ms;
+    if (int rc = copy_results(ctx, b, pp, d->model, d->n_trials, d->n_cat, lnl, cat, post, total, n_fallback))
+        return rc;
+    ctx->timings[1] = elapsed_ms(ctx->ev[0], ctx->ev[1]);
+    ctx->timings[2] = elapsed_ms(ctx->ev[1], ctx->ev[2]);
+    ctx->timings[3] = elapsed_ms(ctx->ev[2], ctx->ev[3]);
     return CAFE_GPU_OK;
 }
 
@@ -772,23 +915,25 @@ int cafe_gpu_root_vectors(cafe_gpu_ctx* ctx, const cafe_gpu_eval_desc* d, int tr
                           double* out)
 {
     if (int rc = check_ctx(ctx)) return rc;
+    EvalBufs& b = ctx->eager;
     int P = 0;
-    if (int rc = prepare_eval(ctx, d, P)) return rc;
+    if (int rc = upload_eval_inputs(ctx, b, d, P)) return rc;
     if (trial < 0 || trial >= d->n_trials || cat_k < 0 || cat_k >= d->n_cat)
         return fail(CAFE_GPU_EINVAL, "trial/category out of range");
     if (f0 < 0 || nf < 0 || f0 + nf > ctx->F || !out) return fail(CAFE_GPU_EINVAL, "family range out of range");
     int maxR = 0;
-    for (auto& b : ctx->buckets) maxR = std::max(maxR, b.R);
-    HIP_OR_FAIL(ctx->d_root.ensure(sizeof(double) * (size_t)ctx->F * maxR));
+    for (auto& bk : ctx->buckets) maxR = std::max(maxR, bk.R);
+    HIP_OR_FAIL(b.root.ensure(sizeof(double) * (size_t)ctx->F * maxR));
     const int point = trial * d->n_cat + cat_k;
-    if (int rc = run_prune(ctx, d->path, d->model, 1, &point, ctx->d_root.as<double>(), maxR, nullptr)) return rc;
+    PrunePlan pp;
+    if (int rc = plan_prune(ctx, b, d->path, d->model, 1, &point, b.root.as<double>(), maxR, pp)) return rc;
+    if (int rc = enqueue_prune(b, pp, ctx->stream, nullptr, nullptr)) return rc;
     std::vector<double> h((size_t)ctx->F * maxR);
-    HIP_OR_FAIL(hipMemcpyAsync(h.data(), ctx->d_root.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OR_FAIL(hipMemcpyAsync(h.data(), b.root.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
     HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
-    // R of each family from its bucket
     std::vector<int> famR(ctx->F);
-    for (auto& b : ctx->buckets)
-        for (int k = 0; k < b.count; ++k) famR[ctx->perm[b.fam_begin + k]] = b.R;
+    for (auto& bk : ctx->buckets)
+        for (int k = 0; k < bk.count; ++k) famR[ctx->perm[bk.fam_begin + k]] = bk.R;
     size_t o = 0;
     for (int f = f0; f < f0 + nf; ++f)
         for (int j = 0; j < famR[f]; ++j) out[o++] = h[(size_t)f * maxR + j];
@@ -799,6 +944,136 @@ int cafe_gpu_last_timings(cafe_gpu_ctx* ctx, double* ms4)
 {
     if (!ctx || !ms4) return fail(CAFE_GPU_EINVAL, "null argument");
     for (int i = 0; i < 4; ++i) ms4[i] = ctx->timings[i];
+    return CAFE_GPU_OK;
+}
+
+int cafe_gpu_plan_create(cafe_gpu_ctx* ctx, const cafe_gpu_eval_desc* d, int n_keys, const double* lambda_q,
+                         const double* t_q, int algo, int use_graph, cafe_gpu_plan** out)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!out) return fail(CAFE_GPU_EINVAL, "null output pointer");
+    if (ctx->N == 0) return fail(CAFE_GPU_ESTATE, "matrix_reset has not been called");
+    if (n_keys < 1 || n_keys > ctx->capacity)
+        return fail(CAFE_GPU_EINVAL, "%d keys do not fit the pool capacity %d", n_keys, ctx->capacity);
+    if (algo != CAFE_GPU_BUILD_CLOSED_FORM && algo != CAFE_GPU_BUILD_RECURRENCE)
+        return fail(CAFE_GPU_EINVAL, "unknown build algorithm %d", algo);
+    if (!lambda_q || !t_q) return fail(CAFE_GPU_EINVAL, "null key arrays");
+    auto* pl = new cafe_gpu_plan();
+    pl->ctx = ctx;
+    // the plan builds its keys into slots 0 .. n_keys-1 at every execution
+    for (int i = 0; i < n_keys; ++i) ctx->built[i] = 1;
+    int P = 0;
+    int rc = upload_eval_inputs(ctx, pl->bufs, d, P);
+    std::vector<int> pts(P);
+    std::iota(pts.begin(), pts.end(), 0);
+    if (rc == CAFE_GPU_OK) rc = plan_prune(ctx, pl->bufs, d->path, d->model, P, pts.data(), nullptr, 0, pl->pp);
+    if (rc == CAFE_GPU_OK && (pl->d_lam.ensure(sizeof(double) * n_keys) != hipSuccess ||
+                              pl->d_t.ensure(sizeof(double) * n_keys) != hipSuccess))
+        rc = fail(CAFE_GPU_ENOMEM, "key upload buffers");
+    if (rc != CAFE_GPU_OK) {
+        pl->bufs.release_all();
+        delete pl;
+        return rc;
+    }
+    pl->model = d->model;
+    pl->path = d->path;
+    pl->T = d->n_trials;
+    pl->K = d->n_cat;
+    pl->n_keys = n_keys;
+    pl->algo = algo;
+    pl->pool_gen = ctx->pool_gen;
+    for (auto& e : pl->ev) (void)hipEventCreate(&e);
+    *out = pl;
+    if ((rc = cafe_gpu_plan_set_keys(pl, lambda_q, t_q)) != CAFE_GPU_OK) {
+        cafe_gpu_plan_destroy(pl);
+        *out = nullptr;
+        return rc;
+    }
+    if (use_graph) pl->graph_ok = plan_capture(pl, false);   // eager launches if the runtime refuses
+    return CAFE_GPU_OK;
+}
+
+int cafe_gpu_plan_set_keys(cafe_gpu_plan* pl, const double* lambda_q, const double* t_q)
+{
+    if (!pl || !lambda_q || !t_q) return fail(CAFE_GPU_EINVAL, "null argument");
+    for (int i = 0; i < pl->n_keys; ++i)
+        if (!(lambda_q[i] >= 0) || !(t_q[i] >= 0) || !std::isfinite(lambda_q[i]) || !std::isfinite(t_q[i]))
+            return fail(CAFE_GPU_EINVAL, "key %d has an invalid lambda or branch length", i);
+    HIP_OR_FAIL(hipSetDevice(pl->ctx->device));
+    HIP_OR_FAIL(hipStreamSynchronize(pl->ctx->stream));
+    HIP_OR_FAIL(hipMemcpy(pl->d_lam.p, lambda_q, sizeof(double) * pl->n_keys, hipMemcpyHostToDevice));
+    HIP_OR_FAIL(hipMemcpy(pl->d_t.p, t_q, sizeof(double) * pl->n_keys, hipMemcpyHostToDevice));
+    return CAFE_GPU_OK;
+}
+
+int cafe_gpu_plan_execute(cafe_gpu_plan* pl)
+{
+    if (!pl) return fail(CAFE_GPU_EINVAL, "null plan");
+    cafe_gpu_ctx* ctx = pl->ctx;
+    if (pl->pool_gen != ctx->pool_gen) return fail(CAFE_GPU_ESTATE, "matrix pool was reset after the plan was made");
+    if (pl->graph_ok) {
+        HIP_OR_FAIL(hipGraphLaunch(pl->exec, ctx->stream));
+        return CAFE_GPU_OK;
+    }
+    return plan_enqueue(pl, false);
+}
+
+int cafe_gpu_plan_synchronize(cafe_gpu_plan* pl)
+{
+    if (!pl) return fail(CAFE_GPU_EINVAL, "null plan");
+    HIP_OR_FAIL(hipStreamSynchronize(pl->ctx->stream));
+    return CAFE_GPU_OK;
+}
+
+int cafe_gpu_plan_results(cafe_gpu_plan* pl, double* lnl, double* cat, double* post, double* total,
+                          long long* n_fallback)
+{
+    if (!pl) return fail(CAFE_GPU_EINVAL, "null plan");
+    return copy_results(pl->ctx, pl->bufs, pl->pp, pl->model, pl->T, pl->K, lnl, cat, post, total, n_fallback);
+}
+
+int cafe_gpu_plan_profile(cafe_gpu_plan* pl, double* ms4, int* graph_ok)
+{
+    if (!pl || !ms4) return fail(CAFE_GPU_EINVAL, "null argument");
+    cafe_gpu_ctx* ctx = pl->ctx;
+    if (pl->pool_gen != ctx->pool_gen) return fail(CAFE_GPU_ESTATE, "matrix pool was reset after the plan was made");
+    HIP_OR_FAIL(hipSetDevice(ctx->device));
+    if (int rc = plan_enqueue(pl, true)) return rc;
+    HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
+    if (graph_ok) *graph_ok = pl->graph_ok ? 1 : 0;
+    ms4[0] = elapsed_ms(pl->ev[0], pl->ev[1]);
+    ms4[1] = elapsed_ms(pl->ev[1], pl->ev[2]);
+    ms4[2] = elapsed_ms(pl->ev[2], pl->ev[3]);
+    ms4[3] = elapsed_ms(pl->ev[3], pl->ev[4]);
+    return CAFE_GPU_OK;
+}
+
+int cafe_gpu_plan_recompute_counts(cafe_gpu_plan* pl, long long* fp64, long long* plain)
+{
+    if (!pl) return fail(CAFE_GPU_EINVAL, "null plan");
+    int h[2] = {0, 0};
+    HIP_OR_FAIL(hipSetDevice(pl->ctx->device));
+    HIP_OR_FAIL(hipMemcpyAsync(h, pl->bufs.fb_count.as<int>() + pl->pp.fam_count_off(), sizeof(int) * 2,
+                               hipMemcpyDeviceToHost, pl->ctx->stream));
+    HIP_OR_FAIL(hipStreamSynchronize(pl->ctx->stream));
+    if (fp64) *fp64 = h[0];
+    if (plain) *plain = h[1];
+    return CAFE_GPU_OK;
+}
+
+int cafe_gpu_plan_destroy(cafe_gpu_plan* pl)
+{
+    if (!pl) return CAFE_GPU_OK;
+    (void)hipSetDevice(pl->ctx->device);
+    (void)hipStreamSynchronize(pl->ctx->stream);
+    if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    for (auto& e : pl->ev)
+        if (e) (void)hipEventDestroy(e);
+    pl->bufs.release_all();
+    pl->d_lam.release();
+    pl->d_t.release();
+    delete pl;
     return CAFE_GPU_OK;
 }
 

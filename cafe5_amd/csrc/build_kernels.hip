@@ -2,7 +2,7 @@
 // (replaces matrix_cache::precalculate_matrices, src/matrix_cache.cpp:113-163).
 //
 // Both kernels write the c-major pool layout of engine.h: an FP64 copy and,
-// when requested, an FP32 copy of every matrix.  Entries are those of
+// when requested, an FP32 copy of every matrix scaled by 2^kMatExp.  Entries are those of
 // get_from_parent_fam_size_to_c (matrix_cache.cpp:78-85): row 0 is delta_0,
 // rows s >= 1 come from the birth-death process with a = lt/(1+lt) and
 // coeff = 1 - 2a; keys with coeff <= 0 or coeff == 1 (saturated, lt == 1, or
@@ -52,7 +52,7 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
                 if (s < N) {
                     const double v = (s == 0 && c == 0) ? 1.0 : 0.0;
                     M64[(long long)c * Np + s] = v;
-                    if (M32) M32[(long long)c * Np + s] = (float)v;
+                    if (M32) M32[(long long)c * Np + s] = (float)ldexp(v, kMatExp);
                 }
             }
         return;
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
             if (s < N) {
                 const double v = y < 1.0 ? y : 1.0;   // [0,1] clamp, probability.cpp:147
                 M64[(long long)c * Np + s] = v;
-                if (M32) M32[(long long)c * Np + s] = (float)v;
+                if (M32) M32[(long long)c * Np + s] = (float)ldexp(v, kMatExp);
             }
         }
     }
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void k_build_closed_form(const double* __restr
     }
     double* M64 = mt64 + (long long)(first_slot + key) * slot_elems;
     M64[(long long)c * Np + s] = v;
-    if (mt32) mt32[(long long)(first_slot + key) * slot_elems + (long long)c * Np + s] = (float)v;
+    if (mt32) mt32[(long long)(first_slot + key) * slot_elems + (long long)c * Np + s] = (float)ldexp(v, kMatExp);
 }
 
 // Zero-fill (used instead of hipMemsetAsync so the calls can be captured).

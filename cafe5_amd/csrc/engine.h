@@ -13,14 +13,30 @@ namespace cafe {
 constexpr int kTile = 16;
 constexpr int kMaxSlots = 24;         // live partial vectors per family tile
 constexpr int kLgammaN = 4096;        // log-gamma table (integers < 4096)
+// The FP32 copy of every matrix is stored multiplied by 2^kMatExp, and FP32
+// partial vectors are renormalised to a max just below 2^vec_exp, so that the
+// whole FP32 exponent range is used: products of k children stay below 2^127
+// when k (kMatExp + vec_exp) <= 126 (the host picks vec_exp from the tree's
+// largest child count) while entries far below each vector's max survive.
+constexpr int kMatExp = 27;
 
 // One unit of work of the pruning kernels: families fam0 .. fam0+nf-1 of a
-// size bucket (bucket-sorted order) at one parameter point.
+// size bucket (bucket-sorted order) at one parameter point.  nf < 0 marks a
+// packed tile: its -nf families are fam_list[fam0 .. fam0-nf-1].
 struct TileDesc {
     int bucket;
     int point;
     int fam0;
     int nf;
+};
+
+// (bucket, point) run of tiles, the unit in which flagged families are packed
+// into recompute tiles; cls is the size class whose lists receive them.
+struct SegDesc {
+    int bucket;
+    int point;
+    int cls;
+    int pad;
 };
 
 struct BucketDev {
@@ -89,11 +105,15 @@ struct PruneParams {
     // optional root vectors (FP64) [F][root_stride], one point only
     double* root_out;
     int root_stride;
+    // packed recompute tiles index families through this list
+    const int* fam_list;
+    int vec_exp;              // FP32 partial vectors are kept below 2^vec_exp
     // partial-vector scratch: per workgroup n_slots x 16 x ldv values
     void* scratch;
     long long scratch_per_wg;  // elements
     int ldv;
     int n_slots;
+    int plain_lds;            // PLAIN64: partial vectors in dynamic LDS
 };
 
 }  // namespace cafe

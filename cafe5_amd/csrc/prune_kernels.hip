@@ -52,9 +52,10 @@ __device__ __forceinline__ int fetch_tile(const PruneParams& p)
 }
 
 constexpr double kLn2 = 0.69314718055994530942;
-// FP32 flush threshold (smallest normal float): the certificate charges this
+// FP32 flush threshold (smallest normal float): the flush bound charges this
 // for every quantity FP32 could have flushed to zero.
 constexpr double kTau32 = 1.1754943508222875e-38;
+constexpr double kTwoPowMinusMatExp = 1.0 / (double)(1 << kMatExp);
 // an FP32 likelihood is accepted when the flush bound is below this fraction
 constexpr double kCertEta = 1e-9;
 // below this, FP64 values may lose bits to subnormals; such families are
@@ -129,10 +130,12 @@ __global__ __launch_bounds__(64) void k_prune_mfma(PruneParams p)
     for (int tile = fetch_tile(p); tile < n_tiles; tile = fetch_tile(p)) {
         const TileDesc td = p.tiles[tile];
         const BucketDev bk = p.buckets[td.bucket];
-        const bool valid = fl < td.nf;
-        const int fs = bk.fam_begin + td.fam0 + (valid ? fl : 0);
+        const bool packed = td.nf < 0;
+        const bool valid = fl < (packed ? -td.nf : td.nf);
+        const int fs = packed ? p.fam_list[td.fam0 + (valid ? fl : 0)] : bk.fam_begin + td.fam0 + (valid ? fl : 0);
         const int* __restrict__ nslot = p.node_slot + (long long)td.point * p.n_nodes;
         const int nts = bk.Lp / kTile;
+        const int a_exp = p.vec_exp;
 
         for (int op = 0; op < p.n_ops; ++op) {
             const OpDev o = p.ops[op];
@@ -141,17 +144,29 @@ __global__ __launch_bounds__(64) void k_prune_mfma(PruneParams p)
             const int limit = is_root ? bk.R : bk.L;
             const int shift = is_root ? 1 : 0;
 
-            // exponents and flush bounds inherited from the children
-            int e_sum = 0;
+            // SCALED bookkeeping from the children.  A stored partial v' of
+            // child c holds v_c = v' 2^E_c with max v' < 2^a; its factor
+            // X' = M' v' (M' = 2^kMatExp M) holds X_c = X' 2^(E_c - kMatExp)
+            // and is at most 2^(kMatExp + a) (leaf factors: 2^kMatExp).
+            //   e_sum   sum_c (E_c - kMatExp): true product = P' 2^e_sum
+            //   log2S   log2 of the product's nominal scale S = prod_c 2^(kMatExp + a_c)
+            //   cert    sum_c of the factors' relative flush errors (w.r.t. their
+            //           nominal scale): inherited bound + what flushing of M'
+            //           entries, MFMA products and stored v' entries can drop
+            int e_sum = 0, log2S = 0;
             double cert = 0.0;
             if (SCALED) {
                 for (int k = 0; k < o.n_child; ++k) {
                     const ChildDev ch = p.kids[o.child_begin + k];
                     if (ch.leaf_col >= 0) {
-                        cert += 2.0 * kTau32;
+                        e_sum -= kMatExp;
+                        log2S += kMatExp;
+                        cert += 3.0 * kTau32 * kTwoPowMinusMatExp;   // up to 3 mixed columns
                     } else {
-                        e_sum += s_exp[ch.in_slot][fl];
-                        cert += s_cert[ch.in_slot][fl] + 4.0 * kTau32 * bk.Lp;
+                        e_sum += s_exp[ch.in_slot][fl] - kMatExp;
+                        log2S += kMatExp + a_exp;
+                        cert += s_cert[ch.in_slot][fl] +
+                                kTau32 * (bk.Lp * kTwoPowMinusMatExp * (1.0 + ldexp(1.0, -a_exp)) + ldexp(1.0, -a_exp));
                     }
                 }
             }
@@ -226,16 +241,21 @@ __global__ __launch_bounds__(64) void k_prune_mfma(PruneParams p)
                             for (int r = 0; r < 4; ++r) m = m > (double)acc[t][r] ? m : (double)acc[t][r];
                         m = fmax(m, __shfl_xor(m, 16));
                         m = fmax(m, __shfl_xor(m, 32));
+                        // renormalise to a max in [2^(a-1), 2^a); the bound is
+                        // re-expressed relative to the new scale: it grows by
+                        // S / m, the factor by which the product fell short of
+                        // its nominal scale
                         int e = 0;
-                        if (m > 0) (void)frexp(m, &e);
+                        const double mant = m > 0 ? frexp(m, &e) : 1.0;
 #pragma unroll
                         for (int t = 0; t < NT; ++t)
 #pragma unroll
-                            for (int r = 0; r < 4; ++r) acc[t][r] = ldexpf(acc[t][r], -e);
+                            for (int r = 0; r < 4; ++r) acc[t][r] = ldexpf(acc[t][r], a_exp - e);
                         __syncthreads();   // every lane has read the child slots' s_exp/s_cert
                         if (g == 0) {
-                            s_exp[o.out_slot][fl] = e_sum + e;
-                            s_cert[o.out_slot][fl] = m > 0 ? cert * 1.01 / m + kTau32 : 1e300;
+                            s_exp[o.out_slot][fl] = e_sum + e - a_exp;
+                            s_cert[o.out_slot][fl] =
+                                m > 0 ? ldexp(cert * 1.01 / mant, log2S - e) + ldexp(kTau32 / mant, -e) : 1e300;
                         }
                         __syncthreads();
                     }
@@ -284,9 +304,10 @@ __global__ __launch_bounds__(64) void k_prune_mfma(PruneParams p)
                             val = log(rb.x) + log(p.prior64[bk.prior_off + rb.j]) + log_scale;
                         else
                             val = ldexp(rb.key, e_sum);
-                        // flush bound (absolute, scaled units) against the
-                        // likelihood, and FP64 range of the unscaled values
-                        if (!(cert <= kCertEta * rb.key)) flag |= kFlagRecompute64;
+                        // flush bound (absolute, in units of 2^e_sum) against
+                        // the likelihood, and FP64 range of the unscaled values
+                        const double bound = (ldexp(cert, log2S) + kTau32) * 1.01;
+                        if (!(bound <= kCertEta * rb.key)) flag |= kFlagRecompute64;
                         if (!(log(rb.xmax) + log_scale >= log(kFp64Floor))) flag |= kFlagRecompute64;
                     }
                 } else {
@@ -316,7 +337,10 @@ template <int NQ>
 __global__ __launch_bounds__(64) void k_prune_plain(PruneParams p)
 {
     const int lane = threadIdx.x;
-    double* slots = (double*)p.scratch + (long long)blockIdx.x * p.scratch_per_wg;
+    extern __shared__ double s_plain_slots[];
+    // partial vectors live in LDS when the launch provides it (n_slots x ldv
+    // doubles), else in this workgroup's region of the global scratch
+    double* slots = p.plain_lds ? s_plain_slots : (double*)p.scratch + (long long)blockIdx.x * p.scratch_per_wg;
     const int ldv = p.ldv;
     const int Np = p.Np;
     const int n_tiles = *p.n_tiles_dev;
@@ -326,8 +350,10 @@ __global__ __launch_bounds__(64) void k_prune_plain(PruneParams p)
         const BucketDev bk = p.buckets[td.bucket];
         const int* __restrict__ nslot = p.node_slot + (long long)td.point * p.n_nodes;
         const int B = bk.B;
-        for (int fi = 0; fi < td.nf; ++fi) {
-            const int fs = bk.fam_begin + td.fam0 + fi;
+        const bool packed = td.nf < 0;
+        const int nf = packed ? -td.nf : td.nf;
+        for (int fi = 0; fi < nf; ++fi) {
+            const int fs = packed ? p.fam_list[td.fam0 + fi] : bk.fam_begin + td.fam0 + fi;
             double best = -__builtin_huge_val();
             double xmax = 0.0;
             for (int op = 0; op < p.n_ops; ++op) {
@@ -423,26 +449,53 @@ __global__ __launch_bounds__(64) void k_prune_plain(PruneParams p)
     }
 }
 
-// Appends every tile with a family flagged `mask` to out_tiles.
-__global__ void k_compact_flags(const TileDesc* __restrict__ tiles, const int* __restrict__ n_tiles_dev,
-                                const BucketDev* __restrict__ buckets, const int* __restrict__ perm,
-                                const int* __restrict__ flags, int F, int mask,
-                                TileDesc* __restrict__ out_tiles, int* __restrict__ out_count,
-                                int* __restrict__ fam_count)
+// Packs the families of one (bucket, point) segment whose flag has `mask` set
+// into dense tiles of `tile` families (16 for MFMA passes, 1 for PLAIN64): their bucket-sorted indices go to the segment's
+// own region of fam_list (point * F + fam_begin, so no two segments collide),
+// and the tiles are appended to the recompute list of the segment's size class.
+__global__ __launch_bounds__(256) void k_pack_flags(const SegDesc* __restrict__ segs,
+                                                    const BucketDev* __restrict__ buckets,
+                                                    const int* __restrict__ perm, const int* __restrict__ flags,
+                                                    int F, int mask, int* __restrict__ fam_list,
+                                                    TileDesc* __restrict__ out_tiles,
+                                                    const int* __restrict__ class_off, int* __restrict__ class_count,
+                                                    int* __restrict__ fam_count, int tile)
 {
-    const int n = *n_tiles_dev;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const TileDesc td = tiles[i];
-        const BucketDev bk = buckets[td.bucket];
-        int hits = 0;
-        for (int k = 0; k < td.nf; ++k) {
-            const int fam = perm[bk.fam_begin + td.fam0 + k];
-            hits += (flags[(long long)td.point * F + fam] & mask) != 0;
-        }
-        if (hits) {
-            out_tiles[atomicAdd(out_count, 1)] = td;
-            atomicAdd(fam_count, hits);
-        }
+    const SegDesc sg = segs[blockIdx.x];
+    const BucketDev bk = buckets[sg.bucket];
+    const long long base = (long long)sg.point * F + bk.fam_begin;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ int wave_tot[4];
+    __shared__ int s_total;
+    if (tid == 0) s_total = 0;
+    __syncthreads();
+    for (int i0 = 0; i0 < bk.count; i0 += 256) {
+        const int i = i0 + tid;
+        const bool hit = i < bk.count && (flags[(long long)sg.point * F + perm[bk.fam_begin + i]] & mask) != 0;
+        const unsigned long long ball = __ballot(hit);
+        const int before = __popcll(ball & ((1ull << lane) - 1ull));
+        if (lane == 0) wave_tot[wave] = __popcll(ball);
+        __syncthreads();
+        int off = s_total;
+        for (int w = 0; w < wave; ++w) off += wave_tot[w];
+        if (hit) fam_list[base + off + before] = bk.fam_begin + i;
+        __syncthreads();
+        if (tid == 0) s_total += wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+        __syncthreads();
+    }
+    const int n = s_total;
+    if (n == 0) return;
+    const int nt = (n + tile - 1) / tile;
+    __shared__ int s_first;
+    if (tid == 0) {
+        s_first = atomicAdd(&class_count[sg.cls], nt);
+        atomicAdd(fam_count, n);
+    }
+    __syncthreads();
+    for (int k = tid; k < nt; k += 256) {
+        const int left = n - tile * k;
+        out_tiles[class_off[sg.cls] + s_first + k] =
+            TileDesc{sg.bucket, sg.point, (int)(base + tile * k), -(left < tile ? left : tile)};
     }
 }
 
@@ -498,14 +551,25 @@ __global__ __launch_bounds__(256) void k_totals(int F, const double* __restrict_
     const int tid = threadIdx.x;
     const int chunk = (F + 255) / 256;
     const int lo = tid * chunk, hi = lo + chunk < F ? lo + chunk : F;
+    // non-finite values are tallied apart (two-sum of an infinity gives NaN):
+    // bit 1 gamma failure, 2 NaN, 4 -inf, 8 +inf
     double s = 0.0, c = 0.0;
     int bad = 0;
     for (int i = lo; i < hi; ++i) {
-        bad |= fl[i];
-        double ns, e;
-        two_sum(s, x[i], ns, e);
-        s = ns;
-        c = __dadd_rn(c, e);
+        const double v = x[i];
+        if (fl[i]) bad |= 1;
+        if (v != v) {
+            bad |= 2;
+        } else if (v == -__builtin_huge_val()) {
+            bad |= 4;
+        } else if (v == __builtin_huge_val()) {
+            bad |= 8;
+        } else {
+            double ns, e;
+            two_sum(s, v, ns, e);
+            s = ns;
+            c = __dadd_rn(c, e);
+        }
     }
     __shared__ double ss[256], sc[256];
     __shared__ int sb[256];
@@ -523,7 +587,17 @@ __global__ __launch_bounds__(256) void k_totals(int F, const double* __restrict_
         }
         __syncthreads();
     }
-    if (tid == 0) total[t] = sb[0] ? __builtin_huge_val() : -__dadd_rn(ss[0], sc[0]);
+    if (tid == 0) {
+        const int b = sb[0];
+        const double inf = __builtin_huge_val();
+        double r;
+        if (b & 1) r = inf;                             // gamma_core.cpp:216-225
+        else if ((b & 2) || ((b & 4) && (b & 8))) r = __builtin_nan("");
+        else if (b & 4) r = inf;                        // -sum with a -inf term
+        else if (b & 8) r = -inf;
+        else r = -__dadd_rn(ss[0], sc[0]);
+        total[t] = r;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -558,11 +632,15 @@ hipError_t launch_prune(int path, int nt, int grid, const PruneParams& p, hipStr
 {
     if (path == 0) {
         const int nq = nt;   // rows per lane for the plain kernel
-        if (nq <= 1) hipLaunchKernelGGL(k_prune_plain<1>, dim3(grid), dim3(64), 0, st, p);
-        else if (nq <= 2) hipLaunchKernelGGL(k_prune_plain<2>, dim3(grid), dim3(64), 0, st, p);
-        else if (nq <= 4) hipLaunchKernelGGL(k_prune_plain<4>, dim3(grid), dim3(64), 0, st, p);
-        else if (nq <= 8) hipLaunchKernelGGL(k_prune_plain<8>, dim3(grid), dim3(64), 0, st, p);
-        else if (nq <= 16) hipLaunchKernelGGL(k_prune_plain<16>, dim3(grid), dim3(64), 0, st, p);
+        const size_t lds = sizeof(double) * (size_t)p.n_slots * p.ldv;
+        PruneParams q = p;
+        q.plain_lds = lds <= 48 * 1024 ? 1 : 0;
+        const size_t shm = q.plain_lds ? lds : 0;
+        if (nq <= 1) hipLaunchKernelGGL(k_prune_plain<1>, dim3(grid), dim3(64), shm, st, q);
+        else if (nq <= 2) hipLaunchKernelGGL(k_prune_plain<2>, dim3(grid), dim3(64), shm, st, q);
+        else if (nq <= 4) hipLaunchKernelGGL(k_prune_plain<4>, dim3(grid), dim3(64), shm, st, q);
+        else if (nq <= 8) hipLaunchKernelGGL(k_prune_plain<8>, dim3(grid), dim3(64), shm, st, q);
+        else if (nq <= 16) hipLaunchKernelGGL(k_prune_plain<16>, dim3(grid), dim3(64), shm, st, q);
         else return hipErrorInvalidValue;
         return hipGetLastError();
     }
@@ -570,15 +648,13 @@ hipError_t launch_prune(int path, int nt, int grid, const PruneParams& p, hipStr
     return launch_mfma_nt<float, true>(nt, grid, p, st);
 }
 
-hipError_t launch_compact(const TileDesc* tiles, const int* n_tiles_dev, int max_tiles,
-                          const BucketDev* buckets, const int* perm, const int* flags, int F, int mask,
-                          TileDesc* out_tiles, int* out_count, int* fam_count, hipStream_t st)
+hipError_t launch_pack(const SegDesc* segs, int n_segs, const BucketDev* buckets, const int* perm,
+                       const int* flags, int F, int mask, int* fam_list, TileDesc* out_tiles,
+                       const int* class_off, int* class_count, int* fam_count, int tile, hipStream_t st)
 {
-    int blocks = (max_tiles + 255) / 256;
-    if (blocks < 1) blocks = 1;
-    if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(k_compact_flags, dim3(blocks), dim3(256), 0, st, tiles, n_tiles_dev, buckets, perm,
-                       flags, F, mask, out_tiles, out_count, fam_count);
+    if (n_segs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_flags, dim3(n_segs), dim3(256), 0, st, segs, buckets, perm, flags, F, mask, fam_list,
+                       out_tiles, class_off, class_count, fam_count, tile);
     return hipGetLastError();
 }
 

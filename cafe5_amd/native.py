@@ -68,6 +68,14 @@ def load_library() -> ctypes.CDLL:
         "cafe_gpu_eval": (_i, [_p, ctypes.POINTER(EvalDesc), _pd, _pd, _pd, _pd, _pll]),
         "cafe_gpu_root_vectors": (_i, [_p, ctypes.POINTER(EvalDesc), _i, _i, _i, _i, _pd]),
         "cafe_gpu_last_timings": (_i, [_p, _pd]),
+        "cafe_gpu_plan_create": (_i, [_p, ctypes.POINTER(EvalDesc), _i, _pd, _pd, _i, _i, ctypes.POINTER(_p)]),
+        "cafe_gpu_plan_set_keys": (_i, [_p, _pd, _pd]),
+        "cafe_gpu_plan_execute": (_i, [_p]),
+        "cafe_gpu_plan_synchronize": (_i, [_p]),
+        "cafe_gpu_plan_results": (_i, [_p, _pd, _pd, _pd, _pd, _pll]),
+        "cafe_gpu_plan_profile": (_i, [_p, _pd, _pi]),
+        "cafe_gpu_plan_recompute_counts": (_i, [_p, _pll, _pll]),
+        "cafe_gpu_plan_destroy": (_i, [_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -81,7 +89,9 @@ EXPORTED_SYMBOLS = ["cafe_gpu_create", "cafe_gpu_destroy", "cafe_gpu_last_error"
                     "cafe_gpu_matrix_build", "cafe_gpu_matrix_get", "cafe_gpu_set_tree",
                     "cafe_gpu_set_families", "cafe_gpu_bucket_count", "cafe_gpu_bucket_info",
                     "cafe_gpu_set_error_model", "cafe_gpu_eval", "cafe_gpu_root_vectors",
-                    "cafe_gpu_last_timings"]
+                    "cafe_gpu_last_timings", "cafe_gpu_plan_create", "cafe_gpu_plan_set_keys",
+                    "cafe_gpu_plan_execute", "cafe_gpu_plan_synchronize", "cafe_gpu_plan_results",
+                    "cafe_gpu_plan_profile", "cafe_gpu_plan_recompute_counts", "cafe_gpu_plan_destroy"]
 
 
 def _ip(a: Optional[np.ndarray]):
@@ -212,6 +222,74 @@ class Engine:
         out = np.zeros(4)
         self._check(self.lib.cafe_gpu_last_timings(self.ctx, _dp(out)))
         return out
+
+    def plan(self, node_slot: np.ndarray, prior: np.ndarray, lam_q, t_q, path: int = PATH_BATCHED32,
+             model: int = MODEL_BASE, cat_probs=None, algo: int = BUILD_RECURRENCE, use_graph: bool = True):
+        return Plan(self, node_slot, prior, lam_q, t_q, path, model, cat_probs, algo, use_graph)
+
+
+class Plan:
+    """A device-resident evaluation replayed as one HIP graph per execute()."""
+
+    def __init__(self, eng: Engine, node_slot, prior, lam_q, t_q, path, model, cat_probs, algo, use_graph):
+        self.eng = eng
+        self.lib = eng.lib
+        d, self._keep, self.T, self.K = eng._desc(model, path, node_slot, prior, cat_probs)
+        self.model = model
+        lq = np.ascontiguousarray(lam_q, dtype=np.float64)
+        tq = np.ascontiguousarray(t_q, dtype=np.float64)
+        self.n_keys = len(lq)
+        h = _p()
+        eng._check(self.lib.cafe_gpu_plan_create(eng.ctx, ctypes.byref(d), self.n_keys, _dp(lq), _dp(tq), algo,
+                                                 1 if use_graph else 0, ctypes.byref(h)))
+        self.h = h
+
+    def set_keys(self, lam_q, t_q):
+        lq = np.ascontiguousarray(lam_q, dtype=np.float64)
+        tq = np.ascontiguousarray(t_q, dtype=np.float64)
+        self.eng._check(self.lib.cafe_gpu_plan_set_keys(self.h, _dp(lq), _dp(tq)))
+
+    def execute(self):
+        self.eng._check(self.lib.cafe_gpu_plan_execute(self.h))
+
+    def synchronize(self):
+        self.eng._check(self.lib.cafe_gpu_plan_synchronize(self.h))
+
+    def results(self) -> Dict[str, np.ndarray]:
+        F = self.eng.F
+        lnl = np.zeros((self.T, F))
+        cat = np.zeros((self.T, F, self.K)) if self.model == MODEL_GAMMA else None
+        post = np.zeros((self.T, F, self.K)) if self.model == MODEL_GAMMA else None
+        total = np.zeros(self.T)
+        nfb = ctypes.c_longlong(0)
+        self.eng._check(self.lib.cafe_gpu_plan_results(self.h, _dp(lnl), _dp(cat), _dp(post), _dp(total),
+                                                       ctypes.byref(nfb)))
+        return {"lnl": lnl, "cat": cat, "post": post, "total": total, "n_fallback": nfb.value}
+
+    def profile(self):
+        """One execution outside the graph with events between phases:
+        ([build, main prune, recompute, finalize] ms, graph_ok)."""
+        out = np.zeros(4)
+        g = ctypes.c_int(0)
+        self.eng._check(self.lib.cafe_gpu_plan_profile(self.h, _dp(out), ctypes.byref(g)))
+        return out, bool(g.value)
+
+    def recompute_counts(self):
+        """(families redone on BATCHED64, families redone on PLAIN64)."""
+        a, b = ctypes.c_longlong(0), ctypes.c_longlong(0)
+        self.eng._check(self.lib.cafe_gpu_plan_recompute_counts(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cafe_gpu_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class KeyTable:

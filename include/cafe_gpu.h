@@ -163,6 +163,39 @@ int cafe_gpu_eval(cafe_gpu_ctx *ctx, const cafe_gpu_eval_desc *desc, double *lnl
 int cafe_gpu_root_vectors(cafe_gpu_ctx *ctx, const cafe_gpu_eval_desc *desc, int trial,
                           int cat, int f0, int nf, double *out);
 
+/* --- device-resident plans (no reference counterpart) ----------------------
+ * A plan keeps everything an evaluation needs on the device and replays it
+ * as one HIP graph: matrix build for n_keys keys into pool slots
+ * 0 .. n_keys-1, pruning on the descriptor's path (with its recompute passes),
+ * the model reduction and the totals.  It serves the repeated calls of the
+ * optimizer scorer (src/optimizer_scorer.cpp:22-36) and lambda-grid scans:
+ * only the key values change between executions (plan_set_keys).  The node
+ * slots of the descriptor must refer to slots < n_keys.  use_graph = 0 (or a
+ * runtime that refuses the capture) runs the same launches eagerly. */
+typedef struct cafe_gpu_plan cafe_gpu_plan;
+
+int cafe_gpu_plan_create(cafe_gpu_ctx *ctx, const cafe_gpu_eval_desc *desc, int n_keys,
+                         const double *lambda_q, const double *t_q, int algo, int use_graph,
+                         cafe_gpu_plan **out);
+/* New key values (same count) for the next execution. */
+int cafe_gpu_plan_set_keys(cafe_gpu_plan *plan, const double *lambda_q, const double *t_q);
+/* Enqueues one execution on the context stream and returns. */
+int cafe_gpu_plan_execute(cafe_gpu_plan *plan);
+int cafe_gpu_plan_synchronize(cafe_gpu_plan *plan);
+/* Waits and copies the last execution's outputs (layouts as cafe_gpu_eval). */
+int cafe_gpu_plan_results(cafe_gpu_plan *plan, double *lnl, double *cat, double *post,
+                          double *total, long long *n_fallback);
+/* Runs one execution with the plan's launches issued directly (not from the
+ * graph) and HIP events between the phases; returns their times in ms:
+ * [0] matrix build, [1] main prune, [2] recompute passes, [3] finalize.
+ * *graph_ok = 1 when plan_execute replays a captured graph. */
+int cafe_gpu_plan_profile(cafe_gpu_plan *plan, double *ms4, int *graph_ok);
+/* Families of the last execution recomputed on BATCHED64 (FP32 results the
+ * flush bound did not certify) and on PLAIN64 (FP64 values near the
+ * subnormal range, redone in the reference's summation order). */
+int cafe_gpu_plan_recompute_counts(cafe_gpu_plan *plan, long long *fp64, long long *plain);
+int cafe_gpu_plan_destroy(cafe_gpu_plan *plan);
+
 /* --- timing (measurement support; no reference counterpart) --------------- */
 
 /* Milliseconds of the last eval's phases, measured with HIP events on the

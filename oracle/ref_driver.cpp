@@ -98,7 +98,16 @@ lambda* make_lambda(int n_lambdas, const double* lambdas, const char* lambda_tre
 
 }  // namespace
 
-REF_API void ref_init() { init_lgamma_cache(); }
+REF_API void ref_init()
+{
+    init_lgamma_cache();
+    // as the reference's test main does: no log output (test.cpp:3149-3168)
+    el::Configurations conf;
+    conf.setToDefault();
+    conf.setGlobally(el::ConfigurationType::Enabled, "false");
+    conf.setGlobally(el::ConfigurationType::ToFile, "false");
+    el::Loggers::setDefaultConfigurations(conf, true);
+}
 
 REF_API double ref_chooseln(double n, double r) { return chooseln(n, r); }
 
@@ -238,4 +247,68 @@ REF_API void ref_get_gamma(int K, double alpha, double* freq, double* rate)
     get_gamma(f, r, alpha);
     std::copy(f.begin(), f.end(), freq);
     std::copy(r.begin(), r.end(), rate);
+}
+
+// --- timing sessions: the base model is constructed once (its O(F^2)
+// build_reference_list runs in the constructor, core.cpp:69-70) and then
+// evaluated repeatedly, as the optimizer does.
+namespace {
+struct base_session {
+    clade* tree;
+    std::vector<gene_family> fams;
+    single_lambda lam{0.0};
+    root_equilibrium_distribution* prior;
+    base_probe* model;
+};
+}  // namespace
+
+REF_API void* ref_base_session_create(const char* newick, const char* leaf_names, int F, const int* counts,
+                                      int B, int R, int prior_kind, int prior_max, double poisson_lambda)
+{
+    try {
+        auto* s = new base_session();
+        s->tree = parse_newick(newick);
+        s->fams = make_families(F, leaf_names, counts);
+        s->prior = make_prior(prior_kind, prior_max, poisson_lambda, 0, nullptr, nullptr);
+        s->model = new base_probe(&s->lam, s->tree, &s->fams, B, R, nullptr);
+        return s;
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+REF_API double ref_base_session_eval(void* h, double lambda, double* out_lnl)
+{
+    auto* s = (base_session*)h;
+    s->lam.update(&lambda);
+    double v = s->model->infer_family_likelihoods(*s->prior, &s->lam);
+    if (out_lnl) {
+        const auto& st = s->model->stash();
+        for (size_t f = 0; f < st.size(); ++f) out_lnl[f] = st[f].posterior_probability;
+    }
+    return v;
+}
+
+REF_API void ref_base_session_free(void* h)
+{
+    auto* s = (base_session*)h;
+    if (!s) return;
+    delete s->model;
+    delete s->prior;
+    delete s->tree;
+    delete s;
+}
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+REF_API int ref_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }
