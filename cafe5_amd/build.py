@@ -44,5 +44,40 @@ def build_native(verbose: bool = False) -> str:
     return LIB
 
 
+HOST_LIB = os.path.join(HERE, "libcafe5_host.so")
+ADAPTER_TEST = os.path.join(ROOT, "tests", "cpp", "adapter_test")
+
+
+def _run(cmd, what):
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"failed to build {what}")
+
+
+def build_host(verbose: bool = False) -> str:
+    """The C++ host interface (cafe5_amd/host) over libcafe_gpu.so, and the
+    C++ test driver that exercises it (tests/cpp/adapter_test)."""
+    src = os.path.join(HERE, "host", "cafe5_gpu.cpp")
+    common = ["-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "host")]
+    tmp = HOST_LIB + ".tmp"
+    cmd = ["g++", *common, "-fPIC", "-shared", src, "-o", tmp, "-L", HERE, "-lcafe_gpu",
+           "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _run(cmd, "libcafe5_host.so")
+    os.replace(tmp, HOST_LIB)
+    test_src = os.path.join(ROOT, "tests", "cpp", "adapter_test.cpp")
+    tmp = ADAPTER_TEST + ".tmp"
+    cmd = ["g++", *common, test_src, "-o", tmp, "-L", HERE, "-lcafe5_host", "-lcafe_gpu",
+           "-Wl,-rpath," + os.path.relpath(HERE, os.path.dirname(ADAPTER_TEST)).join(["$ORIGIN/", ""])]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _run(cmd, "tests/cpp/adapter_test")
+    os.replace(tmp, ADAPTER_TEST)
+    return HOST_LIB
+
+
 if __name__ == "__main__":
     print(build_native(verbose=True))
+    print(build_host(verbose=True))

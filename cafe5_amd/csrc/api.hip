@@ -18,7 +18,7 @@ hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const 
                         int first_slot, int N, int Np, long long slot_elems, double* mt64, float* mt32,
                         hipStream_t stream);
 hipError_t launch_fill_zero(void* p, long long bytes, hipStream_t stream);
-hipError_t launch_prune(int path, int nt, int grid, const PruneParams& p, hipStream_t st);
+hipError_t launch_prune(int path, int nt, int max_tiles, int max_waves, const PruneParams& p, hipStream_t st);
 hipError_t launch_pack(const SegDesc* segs, int n_segs, const BucketDev* buckets, const int* perm,
                        const int* flags, int F, int mask, int* fam_list, TileDesc* out_tiles,
                        const int* class_off, int* class_count, int* fam_count, int tile, hipStream_t st);
@@ -604,17 +604,13 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
             while (c * 64 < std::max(bk.B + 1, bk.R)) c <<= 1;
             return c <= 16 ? c : -1;
         }
+        // 16-row tiles of the largest vector; kernels exist for <= 2, 4, 8,
+        // 16, 32 tiles (FP64: more in several passes)
         const int nt = std::max(lp, rp) / kTile;
-        static const int scaled_sizes[] = {2, 4, 8, 13, 16, 24, 32};
-        static const int f64_sizes[] = {2, 4, 8};
-        if (path == CAFE_GPU_PATH_BATCHED32) {
-            for (int sz : scaled_sizes)
-                if (nt <= sz) return sz;
-            return -1;
-        }
-        for (int sz : f64_sizes)
+        static const int sizes[] = {2, 4, 8, 16, 32};
+        for (int sz : sizes)
             if (nt <= sz) return sz;
-        return 8;   // FP64: chunks of 8 tiles
+        return path == CAFE_GPU_PATH_BATCHED32 ? -1 : 64;
     };
     std::vector<std::pair<int, int>> bucket_class;   // (class, bucket)
     for (int bi = 0; bi < (int)ctx->buckets.size(); ++bi) {
@@ -746,8 +742,7 @@ int enqueue_prune(EvalBufs& b, const PrunePlan& pp, hipStream_t st, hipEvent_t e
         q.n_tiles_dev = d_cnt + ci;
         q.tile_counter = d_ctr + ci;
         q.scratch_per_wg = pp.path == CAFE_GPU_PATH_PLAIN64 ? pp.per_wg_plain : pp.per_wg_mfma;
-        const int grid = std::max(1, std::min(pp.class_counts[ci], pp.max_grid));
-        HIP_OR_FAIL(launch_prune(pp.path, pp.classes[ci].nt, grid, q, st));
+        HIP_OR_FAIL(launch_prune(pp.path, pp.classes[ci].nt, pp.class_counts[ci], pp.max_grid, q, st));
     }
     if (ev_main) HIP_OR_FAIL(hipEventRecord(ev_main, st));
     TileDesc* fb64 = b.fb_tiles.as<TileDesc>();
@@ -766,8 +761,7 @@ int enqueue_prune(EvalBufs& b, const PrunePlan& pp, hipStream_t st, hipEvent_t e
             q.n_tiles_dev = fbc + ci;
             q.tile_counter = d_ctr + n_cls + ci;
             q.scratch_per_wg = pp.per_wg_mfma;
-            const int grid = std::max(1, std::min(pp.class_counts[ci], pp.max_grid));
-            HIP_OR_FAIL(launch_prune(CAFE_GPU_PATH_BATCHED64, pp.classes[ci].nt, grid, q, st));
+            HIP_OR_FAIL(launch_prune(CAFE_GPU_PATH_BATCHED64, pp.classes[ci].nt, pp.class_counts[ci], pp.max_grid, q, st));
         }
     }
     if (pp.path != CAFE_GPU_PATH_PLAIN64) {
@@ -780,8 +774,8 @@ int enqueue_prune(EvalBufs& b, const PrunePlan& pp, hipStream_t st, hipEvent_t e
             q.n_tiles_dev = fbc + n_cls + ci;
             q.tile_counter = d_ctr + 2 * n_cls + ci;
             q.scratch_per_wg = pp.per_wg_plain;
-            const int grid = std::max(1, std::min(pp.plain_off[ci + 1] - pp.plain_off[ci], pp.max_grid));
-            HIP_OR_FAIL(launch_prune(CAFE_GPU_PATH_PLAIN64, pp.plain_nq[ci], grid, q, st));
+            HIP_OR_FAIL(launch_prune(CAFE_GPU_PATH_PLAIN64, pp.plain_nq[ci], pp.plain_off[ci + 1] - pp.plain_off[ci],
+                                     pp.max_grid, q, st));
         }
     }
     if (ev_fb) HIP_OR_FAIL(hipEventRecord(ev_fb, st));

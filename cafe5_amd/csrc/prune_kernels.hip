@@ -98,24 +98,34 @@ __device__ __forceinline__ RootBest root_combine(RootBest b)
 }
 
 // ---------------------------------------------------------------------------
-// Batched MFMA pruning.  One wave = 16 families (columns) of one bucket at one
-// parameter point.  For v_mfma_*_16x16x4: lane l supplies A[s0 + l%16][k] and
+// Batched MFMA pruning.  A workgroup of W waves carries 16 families (the
+// columns of every contraction) of one bucket at one parameter point through
+// the whole tree.  For v_mfma_*_16x16x4: lane l supplies A[s0 + l%16][k] and
 // B[k][f = l%16] with k = l/16, and holds four rows of D[.][f = l%16] (rows
 // given by drow()).  Within each 16-wide chunk of the contraction index, MFMA
-// j maps lane l's k to x = 16 kt + 4 (l/16) + j, so every lane reads four
+// j maps lane l's k to x = 16 kt + 4 (l/16) + j, so a lane reads four
 // consecutive entries of a partial vector of its own family.
 //
+// The output rows of a node are 16-row tiles; wave w computes tiles
+// w, w + W, w + 2W, ... (NTW tiles per pass; the FP64 kernel makes more
+// passes when a vector has more tiles).  Partial vectors live in the
+// workgroup's scratch region; a barrier after each node makes them visible to
+// every wave.  W > 1 shortens the critical path of large buckets, whose
+// tiles are few but long.
+//
 // T = float, SCALED: each node's partial is renormalised per family to a max
-// in [0.5, 1) and the exponent is carried as an integer; a per-family bound on
-// what FP32 flushing could have dropped decides whether the result is kept.
+// just below 2^vec_exp (FP32 matrices are stored times 2^kMatExp) and the
+// exponent is carried as an integer; a per-family bound on what FP32 flushing
+// could have dropped decides whether the result is kept.  SCALED needs all of
+// a vector's tiles in one pass (W * NTW >= tiles).
 // T = double: unscaled, so values (and underflow) follow the reference.
-// NT: 16-row tiles held in registers; SCALED needs every tile in one pass.
 // ---------------------------------------------------------------------------
-template <typename T, int NT, bool SCALED>
-__global__ __launch_bounds__(64) void k_prune_mfma(PruneParams p)
+template <typename T, int NTW, int W, bool SCALED>
+__global__ __launch_bounds__(64 * W) void k_prune_mfma(PruneParams p)
 {
     typedef typename Vec4<T>::type V4;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
     const int fl = lane & 15;
     const int g = lane >> 4;
     const T* __restrict__ pool = pool_of(p, (T*)nullptr);
@@ -126,8 +136,17 @@ __global__ __launch_bounds__(64) void k_prune_mfma(PruneParams p)
     const int n_tiles = *p.n_tiles_dev;
     __shared__ int s_exp[kMaxSlots][16];       // SCALED: exponent per slot/family
     __shared__ double s_cert[kMaxSlots][16];   // SCALED: flush bound per slot/family
+    __shared__ double s_max[W][16];            // per-wave family maxima
+    __shared__ double s_rkey[W][16], s_rx[W][16], s_rxmax[W][16];
+    __shared__ int s_rj[W][16];
+    __shared__ int s_tile;
 
-    for (int tile = fetch_tile(p); tile < n_tiles; tile = fetch_tile(p)) {
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(p.tile_counter, 1);
+        __syncthreads();
+        const int tile = s_tile;
+        __syncthreads();
+        if (tile >= n_tiles) break;
         const TileDesc td = p.tiles[tile];
         const BucketDev bk = p.buckets[td.bucket];
         const bool packed = td.nf < 0;
@@ -171,42 +190,48 @@ __global__ __launch_bounds__(64) void k_prune_mfma(PruneParams p)
                 }
             }
             RootBest rb{-__builtin_huge_val(), 0.0, -1, 0.0};
+            double fmax_local = 0.0;
+            V4 acc[NTW];
 
-            for (int t0 = 0; t0 < rows; t0 += NT) {
-                V4 acc[NT];
+            for (int t0 = 0; t0 < rows; t0 += W * NTW) {
+                // this wave's tiles of the pass: t0 + wv + W i
 #pragma unroll
-                for (int t = 0; t < NT; ++t) acc[t] = V4{1, 1, 1, 1};
+                for (int i = 0; i < NTW; ++i) acc[i] = V4{1, 1, 1, 1};
                 for (int k = 0; k < o.n_child; ++k) {
                     const ChildDev ch = p.kids[o.child_begin + k];
                     const T* __restrict__ M = pool + (long long)nslot[ch.node] * p.slot_elems;
-                    V4 x[NT];
+                    V4 x[NTW];
 #pragma unroll
-                    for (int t = 0; t < NT; ++t) x[t] = V4{0, 0, 0, 0};
+                    for (int i = 0; i < NTW; ++i) x[i] = V4{0, 0, 0, 0};
                     if (ch.leaf_col >= 0) {
                         const int cnt = valid ? p.counts_T[(long long)ch.leaf_col * p.F + fs] : 0;
                         if (p.n_dev == 0) {
                             const T* col = M + (long long)cnt * Np + shift;
 #pragma unroll
-                            for (int t = 0; t < NT; ++t)
-                                if (t0 + t < rows) {
+                            for (int i = 0; i < NTW; ++i) {
+                                const int tt = t0 + wv + W * i;
+                                if (tt < rows) {
 #pragma unroll
-                                    for (int r = 0; r < 4; ++r) x[t][r] = col[drow((T*)nullptr, t0 + t, g, r)];
+                                    for (int r = 0; r < 4; ++r) x[i][r] = col[drow((T*)nullptr, tt, g, r)];
                                 }
+                            }
                         } else {
                             // error model: columns cnt-h .. cnt+h weighted by
                             // get_probs(cnt), negative columns skipped
                             const int h = (p.n_dev - 1) / 2;
-                            for (int i = 0; i < p.n_dev; ++i) {
-                                const int cc = cnt - h + i;
+                            for (int d = 0; d < p.n_dev; ++d) {
+                                const int cc = cnt - h + d;
                                 if (cc < 0) continue;
-                                const T w = err[(long long)cnt * p.n_dev + i];
+                                const T w = err[(long long)cnt * p.n_dev + d];
                                 const T* col = M + (long long)cc * Np + shift;
 #pragma unroll
-                                for (int t = 0; t < NT; ++t)
-                                    if (t0 + t < rows) {
+                                for (int i = 0; i < NTW; ++i) {
+                                    const int tt = t0 + wv + W * i;
+                                    if (tt < rows) {
 #pragma unroll
-                                        for (int r = 0; r < 4; ++r) x[t][r] += w * col[drow((T*)nullptr, t0 + t, g, r)];
+                                        for (int r = 0; r < 4; ++r) x[i][r] += w * col[drow((T*)nullptr, tt, g, r)];
                                     }
+                                }
                             }
                         }
                     } else {
@@ -217,107 +242,145 @@ __global__ __launch_bounds__(64) void k_prune_mfma(PruneParams p)
                             for (int j = 0; j < 4; ++j) {
                                 const T* a = M + (long long)(kTile * kt + 4 * g + j) * Np + shift + fl;
 #pragma unroll
-                                for (int t = 0; t < NT; ++t)
-                                    if (t0 + t < rows) x[t] = mfma16(a[kTile * (t0 + t)], b[j], x[t]);
+                                for (int i = 0; i < NTW; ++i) {
+                                    const int tt = t0 + wv + W * i;
+                                    if (tt < rows) x[i] = mfma16(a[kTile * tt], b[j], x[i]);
+                                }
                             }
                         }
                     }
 #pragma unroll
-                    for (int t = 0; t < NT; ++t) acc[t] *= x[t];
+                    for (int i = 0; i < NTW; ++i) acc[i] *= x[i];
                 }
                 // rows past the bound hold real matrix rows: clear them
 #pragma unroll
-                for (int t = 0; t < NT; ++t)
+                for (int i = 0; i < NTW; ++i) {
+                    const int tt = t0 + wv + W * i;
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        if (drow((T*)nullptr, t0 + t, g, r) >= limit || t0 + t >= rows) acc[t][r] = 0;
-
+                        if (tt >= rows || drow((T*)nullptr, tt, g, r) >= limit) acc[i][r] = 0;
+                }
                 if (!is_root) {
                     if (SCALED) {
-                        double m = 0.0;
 #pragma unroll
-                        for (int t = 0; t < NT; ++t)
+                        for (int i = 0; i < NTW; ++i)
 #pragma unroll
-                            for (int r = 0; r < 4; ++r) m = m > (double)acc[t][r] ? m : (double)acc[t][r];
-                        m = fmax(m, __shfl_xor(m, 16));
-                        m = fmax(m, __shfl_xor(m, 32));
-                        // renormalise to a max in [2^(a-1), 2^a); the bound is
-                        // re-expressed relative to the new scale: it grows by
-                        // S / m, the factor by which the product fell short of
-                        // its nominal scale
-                        int e = 0;
-                        const double mant = m > 0 ? frexp(m, &e) : 1.0;
+                            for (int r = 0; r < 4; ++r)
+                                fmax_local = fmax_local > (double)acc[i][r] ? fmax_local : (double)acc[i][r];
+                    } else {
+                        T* out = slots + ((long long)o.out_slot * 16 + fl) * ldv;
 #pragma unroll
-                        for (int t = 0; t < NT; ++t)
+                        for (int i = 0; i < NTW; ++i) {
+                            const int tt = t0 + wv + W * i;
+                            if (tt < rows)
 #pragma unroll
-                            for (int r = 0; r < 4; ++r) acc[t][r] = ldexpf(acc[t][r], a_exp - e);
-                        __syncthreads();   // every lane has read the child slots' s_exp/s_cert
-                        if (g == 0) {
-                            s_exp[o.out_slot][fl] = e_sum + e - a_exp;
-                            s_cert[o.out_slot][fl] =
-                                m > 0 ? ldexp(cert * 1.01 / mant, log2S - e) + ldexp(kTau32 / mant, -e) : 1e300;
+                                for (int r = 0; r < 4; ++r) out[drow((T*)nullptr, tt, g, r)] = acc[i][r];
                         }
-                        __syncthreads();
                     }
-                    T* out = slots + ((long long)o.out_slot * 16 + fl) * ldv;
-#pragma unroll
-                    for (int t = 0; t < NT; ++t)
-                        if (t0 + t < rows) {
-                            if (sizeof(T) == 4) {
-                                *(V4*)(out + drow((T*)nullptr, t0 + t, g, 0)) = acc[t];
-                            } else {
-#pragma unroll
-                                for (int r = 0; r < 4; ++r) out[drow((T*)nullptr, t0 + t, g, r)] = acc[t][r];
-                            }
-                        }
                 } else {
                     const double* pr = p.prior64 + bk.prior_off;
 #pragma unroll
-                    for (int t = 0; t < NT; ++t)
+                    for (int i = 0; i < NTW; ++i) {
+                        const int tt = t0 + wv + W * i;
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            const int j = drow((T*)nullptr, t0 + t, g, r);
-                            if (j >= limit || t0 + t >= rows) continue;
-                            const double xv = (double)acc[t][r];
+                            const int j = drow((T*)nullptr, tt, g, r);
+                            if (tt >= rows || j >= limit) continue;
+                            const double xv = (double)acc[i][r];
                             if (p.root_out && valid)
                                 p.root_out[(long long)p.perm[fs] * p.root_stride + j] =
                                     SCALED ? ldexp(xv, e_sum) : xv;
-                            const double key = (!SCALED && p.model == 0) ? log(xv) + log(pr[j])
-                                                                         : xv * pr[j];
+                            const double key = (!SCALED && p.model == 0) ? log(xv) + log(pr[j]) : xv * pr[j];
                             root_take(rb, key, xv, j);
                         }
+                    }
                 }
             }
 
-            if (is_root) {
-                rb = root_combine(rb);
-                double val;
-                int flag = 0;
+            if (!is_root) {
                 if (SCALED) {
-                    // rb.key = x_j * prior_j in units of 2^e_sum
-                    if (!(rb.key > 0)) {
-                        flag = kFlagRecompute64;
-                        val = 0.0;
-                    } else {
-                        const double log_scale = (double)e_sum * kLn2;
-                        if (p.model == 0)
-                            val = log(rb.x) + log(p.prior64[bk.prior_off + rb.j]) + log_scale;
-                        else
-                            val = ldexp(rb.key, e_sum);
-                        // flush bound (absolute, in units of 2^e_sum) against
-                        // the likelihood, and FP64 range of the unscaled values
-                        const double bound = (ldexp(cert, log2S) + kTau32) * 1.01;
-                        if (!(bound <= kCertEta * rb.key)) flag |= kFlagRecompute64;
-                        if (!(log(rb.xmax) + log_scale >= log(kFp64Floor))) flag |= kFlagRecompute64;
+                    // per-family max over all waves, then renormalise to a max in
+                    // [2^(a-1), 2^a); the bound is re-expressed relative to the
+                    // new scale: it grows by S / m, the factor by which the
+                    // product fell short of its nominal scale
+                    double m = fmax_local;
+                    m = fmax(m, __shfl_xor(m, 16));
+                    m = fmax(m, __shfl_xor(m, 32));
+                    if (g == 0) s_max[wv][fl] = m;
+                    __syncthreads();   // also: every wave has read the children's s_exp/s_cert
+                    m = 0.0;
+#pragma unroll
+                    for (int w2 = 0; w2 < W; ++w2) m = fmax(m, s_max[w2][fl]);
+                    int e = 0;
+                    const double mant = m > 0 ? frexp(m, &e) : 1.0;
+                    T* out = slots + ((long long)o.out_slot * 16 + fl) * ldv;
+#pragma unroll
+                    for (int i = 0; i < NTW; ++i) {
+                        const int tt = wv + W * i;
+                        if (tt < rows) {
+                            V4 v;
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) v[r] = ldexpf(acc[i][r], a_exp - e);
+                            *(V4*)(out + drow((T*)nullptr, tt, g, 0)) = v;
+                        }
                     }
-                } else {
-                    val = rb.key;
-                    if (rb.xmax == 0.0) flag |= kFlagZeroSum;
-                    if (rb.xmax < kFp64Floor) flag |= kFlagRecomputePlain;
+                    if (wv == 0 && g == 0) {
+                        s_exp[o.out_slot][fl] = e_sum + e - a_exp;
+                        s_cert[o.out_slot][fl] =
+                            m > 0 ? ldexp(cert * 1.01 / mant, log2S - e) + ldexp(kTau32 / mant, -e) : 1e300;
+                    }
                 }
-                if (valid && g == 0) {
-                    p.out_val[(long long)td.point * p.F + p.perm[fs]] = val;
-                    p.out_flag[(long long)td.point * p.F + p.perm[fs]] = flag;
+                __syncthreads();   // the node's vector is complete before anyone reads it
+            } else {
+                rb = root_combine(rb);
+                if (W > 1) {
+                    if (g == 0) {
+                        s_rkey[wv][fl] = rb.key;
+                        s_rx[wv][fl] = rb.x;
+                        s_rxmax[wv][fl] = rb.xmax;
+                        s_rj[wv][fl] = rb.j;
+                    }
+                    __syncthreads();
+                    if (wv == 0) {
+                        for (int w2 = 1; w2 < W; ++w2) {
+                            const int oj = s_rj[w2][fl];
+                            if (oj >= 0) {
+                                const double xm = rb.xmax > s_rxmax[w2][fl] ? rb.xmax : s_rxmax[w2][fl];
+                                root_take(rb, s_rkey[w2][fl], s_rx[w2][fl], oj);
+                                rb.xmax = xm;
+                            }
+                        }
+                    }
+                }
+                if (wv == 0) {
+                    double val;
+                    int flag = 0;
+                    if (SCALED) {
+                        // rb.key = x_j * prior_j in units of 2^e_sum
+                        if (!(rb.key > 0)) {
+                            flag = kFlagRecompute64;
+                            val = 0.0;
+                        } else {
+                            const double log_scale = (double)e_sum * kLn2;
+                            if (p.model == 0)
+                                val = log(rb.x) + log(p.prior64[bk.prior_off + rb.j]) + log_scale;
+                            else
+                                val = ldexp(rb.key, e_sum);
+                            // flush bound (absolute, in units of 2^e_sum) against
+                            // the likelihood, and FP64 range of the unscaled values
+                            const double bound = (ldexp(cert, log2S) + kTau32) * 1.01;
+                            if (!(bound <= kCertEta * rb.key)) flag |= kFlagRecompute64;
+                            if (!(log(rb.xmax) + log_scale >= log(kFp64Floor))) flag |= kFlagRecompute64;
+                        }
+                    } else {
+                        val = rb.key;
+                        if (rb.xmax == 0.0) flag |= kFlagZeroSum;
+                        if (rb.xmax < kFp64Floor) flag |= kFlagRecomputePlain;
+                    }
+                    if (valid && g == 0) {
+                        p.out_val[(long long)td.point * p.F + p.perm[fs]] = val;
+                        p.out_flag[(long long)td.point * p.F + p.perm[fs]] = flag;
+                    }
                 }
             }
         }
@@ -603,33 +666,36 @@ __global__ __launch_bounds__(256) void k_totals(int F, const double* __restrict_
 // ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
+// Size class nt = 16-row tiles of the largest vector: (tiles per wave per
+// pass, waves per workgroup).  Small vectors run one wave per tile (many
+// tiles, throughput); large ones split rows over waves (few, long tiles).
 template <typename T, bool SCALED>
-static hipError_t launch_mfma_nt(int nt, int grid, const PruneParams& p, hipStream_t st)
+static hipError_t launch_mfma_nt(int nt, int max_tiles, int max_waves, const PruneParams& p, hipStream_t st)
 {
-#define CAFE_MFMA_CASE(N)                                                                      \
-    if (nt <= N) {                                                                             \
-        hipLaunchKernelGGL((k_prune_mfma<T, N, SCALED>), dim3(grid), dim3(64), 0, st, p);      \
-        return hipGetLastError();                                                              \
+    // workgroups: one per tile in flight, within the wave budget
+#define CAFE_MFMA_CASE(NTW, W)                                                                          \
+    {                                                                                                   \
+        int grid = max_waves / W < max_tiles ? max_waves / W : max_tiles;                               \
+        if (grid < 1) grid = 1;                                                                         \
+        hipLaunchKernelGGL((k_prune_mfma<T, NTW, W, SCALED>), dim3(grid), dim3(64 * W), 0, st, p);      \
+        return hipGetLastError();                                                                       \
     }
-    CAFE_MFMA_CASE(2)
-    CAFE_MFMA_CASE(4)
-    CAFE_MFMA_CASE(8)
-    if constexpr (!SCALED) {
-        // FP64: larger vectors run in chunks of 8 tiles
-        hipLaunchKernelGGL((k_prune_mfma<T, 8, SCALED>), dim3(grid), dim3(64), 0, st, p);
-        return hipGetLastError();
-    } else {
-        CAFE_MFMA_CASE(13)
-        CAFE_MFMA_CASE(16)
-        CAFE_MFMA_CASE(24)
-        CAFE_MFMA_CASE(32)
-    }
+    if (nt <= 2) CAFE_MFMA_CASE(2, 1)
+    if (nt <= 4) CAFE_MFMA_CASE(4, 1)
+    if (nt <= 8) CAFE_MFMA_CASE(4, 2)
+    if (nt <= 16) CAFE_MFMA_CASE(4, 4)
+    if (nt <= 32) CAFE_MFMA_CASE(4, 8)
+    if (!SCALED) CAFE_MFMA_CASE(4, 8)   // FP64 makes several passes
 #undef CAFE_MFMA_CASE
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_prune(int path, int nt, int grid, const PruneParams& p, hipStream_t st)
+// max_tiles: upper bound of the work list; max_waves: waves to keep resident
+// (the scratch holds one region per workgroup, at most max_waves of them).
+hipError_t launch_prune(int path, int nt, int max_tiles, int max_waves, const PruneParams& p, hipStream_t st)
 {
+    int grid = max_tiles < max_waves ? max_tiles : max_waves;
+    if (grid < 1) grid = 1;
     if (path == 0) {
         const int nq = nt;   // rows per lane for the plain kernel
         const size_t lds = sizeof(double) * (size_t)p.n_slots * p.ldv;
@@ -644,8 +710,8 @@ hipError_t launch_prune(int path, int nt, int grid, const PruneParams& p, hipStr
         else return hipErrorInvalidValue;
         return hipGetLastError();
     }
-    if (path == 1) return launch_mfma_nt<double, false>(nt, grid, p, st);
-    return launch_mfma_nt<float, true>(nt, grid, p, st);
+    if (path == 1) return launch_mfma_nt<double, false>(nt, max_tiles, max_waves, p, st);
+    return launch_mfma_nt<float, true>(nt, max_tiles, max_waves, p, st);
 }
 
 hipError_t launch_pack(const SegDesc* segs, int n_segs, const BucketDev* buckets, const int* perm,

@@ -1,0 +1,329 @@
+// cafe5_gpu.hpp -- C++ host interface of the MI355X likelihood engine.
+//
+// Mirrors the public interface of CAFE5's likelihood hot path -- the classes a
+// caller of base_model / gamma_model::infer_family_likelihoods and of
+// matrix_cache touches -- with the same names, argument meaning and error
+// behaviour, and runs the work through the C ABI of include/cafe_gpu.h:
+//
+//   clade                       src/clade.h:25-121 (tree; Newick reader)
+//   gene_family                 src/gene_family.h:28-86 (case-insensitive counts)
+//   lambda / single_lambda /
+//   multiple_lambda             src/lambda.h:21-100
+//   error_model                 src/error_model.h:27-60
+//   root_equilibrium_distribution  src/root_equilibrium_distribution.h:13-53
+//   matrix, matrix_cache        src/matrix_cache.h:13-96
+//   family_info_stash           src/core.h:19-31
+//   base_model                  src/base_model.h:17-43 (infer_family_likelihoods)
+//   gamma_model                 src/gamma_core.h:38-101 (infer_family_likelihoods,
+//                               prune, set_alpha / get_gamma)
+//
+// Errors: std::runtime_error, as the reference; invalid parameters and
+// saturated gamma families give +inf from infer_family_likelihoods.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <ostream>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+struct cafe_gpu_ctx;
+
+namespace cafe5_gpu {
+
+// ---------------------------------------------------------------------------
+// device context (no reference counterpart)
+// ---------------------------------------------------------------------------
+class gpu_context {
+public:
+    explicit gpu_context(int device = 0);
+    ~gpu_context();
+    gpu_context(const gpu_context&) = delete;
+    gpu_context& operator=(const gpu_context&) = delete;
+    cafe_gpu_ctx* get() const { return _ctx; }
+    // pruning path: 0 plain FP64, 1 batched FP64 MFMA, 2 batched FP32 MFMA
+    int path = 2;
+    // matrix build: 0 closed form (reference term order), 1 recurrence
+    int build_algo = 1;
+
+private:
+    cafe_gpu_ctx* _ctx = nullptr;
+};
+
+// throws std::runtime_error carrying cafe_gpu_last_error() when rc != 0
+void check(int rc, const char* what);
+
+// ---------------------------------------------------------------------------
+// tree
+// ---------------------------------------------------------------------------
+class clade {
+public:
+    clade() = default;
+    clade(std::string taxon_name, double length) : _taxon_name(std::move(taxon_name)), _branch_length(length) {}
+    ~clade();
+    clade(const clade&) = delete;
+    clade& operator=(const clade&) = delete;
+
+    const clade* get_parent() const { return _p_parent; }
+    void add_descendant(clade* c);
+    bool is_leaf() const { return _descendants.empty(); }
+    bool is_root() const { return _p_parent == nullptr; }
+    double get_branch_length() const;
+    int get_lambda_index() const;
+    std::string get_taxon_name() const { return _taxon_name; }
+    std::vector<clade*>::const_iterator descendant_begin() const { return _descendants.begin(); }
+    std::vector<clade*>::const_iterator descendant_end() const { return _descendants.end(); }
+    // children before parents (clade.cpp:69-100)
+    std::vector<const clade*> reverse_level_order() const;
+    // distinct positive branch lengths (clade.cpp:236-245)
+    std::set<double> get_branch_lengths() const;
+    // node name -> lambda index - 1 (clade.cpp:194-204)
+    std::map<std::string, int> get_lambda_index_map() const;
+    const clade* find_descendant(const std::string& name) const;
+
+    friend clade* parse_newick(const std::string& text, bool parse_to_lambdas);
+
+private:
+    void name_interior();
+    clade* _p_parent = nullptr;
+    std::string _taxon_name;
+    double _branch_length = 0.0;
+    int _lambda_index = 0;
+    bool _is_lambda_clade = false;
+    std::vector<clade*> _descendants;
+};
+
+// Newick text -> tree (clade.cpp:293-419 semantics: interior nodes named by
+// their sorted leaf names, branch lengths (or lambda indices) after ':').
+clade* parse_newick(const std::string& text, bool parse_to_lambdas = false);
+
+// ---------------------------------------------------------------------------
+// families, lambdas, error model, prior
+// ---------------------------------------------------------------------------
+class gene_family {
+public:
+    void set_id(std::string id) { _id = std::move(id); }
+    std::string id() const { return _id; }
+    void set_species_size(const std::string& species, int count);
+    // throws if the species is missing (gene_family.cpp:38-45)
+    int get_species_size(const std::string& species) const;
+    int get_max_size() const;
+
+private:
+    std::string _id;
+    std::map<std::string, int> _sizes;   // lower-cased species -> count
+};
+
+class lambda {
+public:
+    virtual ~lambda() {}
+    virtual lambda* multiply(double factor) const = 0;
+    virtual double get_value_for_clade(const clade* c) const = 0;
+    virtual bool is_valid() const = 0;
+    virtual std::vector<double> values() const = 0;
+    virtual lambda* clone() const = 0;
+};
+
+class single_lambda : public lambda {
+public:
+    explicit single_lambda(double lam) : _lambda(lam) {}
+    double get_single_lambda() const { return _lambda; }
+    lambda* multiply(double factor) const override { return new single_lambda(_lambda * factor); }
+    double get_value_for_clade(const clade*) const override { return _lambda; }
+    bool is_valid() const override { return _lambda > 0; }   // lambda.h:58-60
+    std::vector<double> values() const override { return {_lambda}; }
+    lambda* clone() const override { return new single_lambda(_lambda); }
+    void update(double v) { _lambda = v; }
+
+private:
+    double _lambda;
+};
+
+class multiple_lambda : public lambda {
+public:
+    multiple_lambda(std::map<std::string, int> name_to_index, std::vector<double> lambdas)
+        : _index(std::move(name_to_index)), _lambdas(std::move(lambdas)) {}
+    lambda* multiply(double factor) const override;
+    double get_value_for_clade(const clade* c) const override;   // by clade name (lambda.cpp:32-40)
+    bool is_valid() const override;                               // none negative (lambda.cpp:58-61)
+    std::vector<double> values() const override { return _lambdas; }
+    lambda* clone() const override { return new multiple_lambda(_index, _lambdas); }
+
+private:
+    std::map<std::string, int> _index;
+    std::vector<double> _lambdas;
+};
+
+// lambda.cpp:69-83
+std::vector<double> get_lambda_values(const lambda* p);
+
+class error_model {
+public:
+    error_model() : _deviations{-1, 0, 1} {}
+    void set_deviations(const std::vector<int>& d) { _deviations = d; }
+    void set_max_family_size(size_t m) { _max_family_size = m; }
+    // error_model.cpp:31-50 (validation and forward fill)
+    void set_probabilities(size_t fam_size, const std::vector<double>& probs);
+    // error_model.cpp:52-57
+    std::vector<double> get_probs(size_t fam_size) const;
+    size_t n_deviations() const { return _deviations.size(); }
+
+private:
+    size_t _max_family_size = 0;
+    std::vector<int> _deviations;
+    std::vector<std::vector<double>> _dists;
+};
+
+class root_equilibrium_distribution {
+public:
+    explicit root_equilibrium_distribution(const std::map<int, int>& root_distribution);
+    explicit root_equilibrium_distribution(size_t max_size);               // uniform
+    root_equilibrium_distribution(double poisson_lambda, size_t num_values);
+    // float valued (root_equilibrium_distribution.h:40)
+    float compute(size_t n) const;
+
+private:
+    std::vector<double> _pct;
+};
+
+// ---------------------------------------------------------------------------
+// transition matrices
+// ---------------------------------------------------------------------------
+class matrix {
+public:
+    explicit matrix(int size) : _size(size), _values((size_t)size * size) {}
+    double get(int x, int y) const { return _values[(size_t)x * _size + y]; }
+    void set(int x, int y, double v) { _values[(size_t)x * _size + y] = v; }
+    int size() const { return _size; }
+    // matrix_cache.cpp:32-58
+    void multiply(const std::vector<double>& v, int s_min, int s_max, int c_min, int c_max, double* result) const;
+    double* data() { return _values.data(); }
+
+private:
+    int _size;
+    std::vector<double> _values;
+};
+
+class matrix_cache {
+public:
+    matrix_cache(int matrix_size, gpu_context& gpu);
+    ~matrix_cache();
+    // builds every missing (lambda, t) key on the GPU (matrix_cache.cpp:113-163)
+    void precalculate_matrices(const std::vector<double>& lambdas, const std::set<double>& branch_lengths);
+    // host copy of a built matrix; throws on a miss (matrix_cache.cpp:88-105)
+    const matrix* get_matrix(double branch_length, double lambda) const;
+    int get_cache_size() const { return (int)_slots.size(); }
+    int get_matrix_size() const { return _matrix_size; }
+    static bool is_saturated(double branch_length, double lambda);
+    double get_from_parent_fam_size_to_c(double lambda, double branch_length, int parent_size, int child_size) const;
+    void warn_on_saturation(std::ostream& ost) const;
+
+    // engine slot of a key (quantized as matrix_cache_key does), -1 if absent
+    int slot_of(double lambda, double branch_length) const;
+    gpu_context& gpu() const { return _gpu; }
+
+private:
+    typedef std::pair<long, long> key_t;   // (branch_length * 1e3, lambda * 1e9), truncated
+    static key_t make_key(double lambda, double branch_length);
+    int _matrix_size;
+    gpu_context& _gpu;
+    int _capacity = 0;
+    std::map<key_t, int> _slots;
+    mutable std::map<int, std::unique_ptr<matrix>> _host;
+};
+
+// ---------------------------------------------------------------------------
+// models
+// ---------------------------------------------------------------------------
+struct family_info_stash {
+    family_info_stash() {}
+    family_info_stash(std::string fam, double lam, double cat_lh, double fam_lh, double pp, bool signif)
+        : family_id(std::move(fam)), lambda_multiplier(lam), category_likelihood(cat_lh),
+          family_likelihood(fam_lh), posterior_probability(pp), significant(signif) {}
+    std::string family_id;
+    double lambda_multiplier = 0.0;
+    double category_likelihood = 0.0;
+    double family_likelihood = 0.0;
+    double posterior_probability = 0.0;
+    bool significant = false;
+};
+std::ostream& operator<<(std::ostream& ost, const family_info_stash& r);
+
+class model {
+public:
+    model(lambda* p_lambda, const clade* p_tree, const std::vector<gene_family>* p_gene_families,
+          int max_family_size, int max_root_family_size, error_model* p_error_model, gpu_context& gpu);
+    virtual ~model() {}
+    virtual double infer_family_likelihoods(const root_equilibrium_distribution& prior, const lambda* p_lambda) = 0;
+    virtual void write_family_likelihoods(std::ostream& ost) = 0;
+    const std::vector<family_info_stash>& get_results() const { return results; }
+    lambda* get_lambda() const { return _p_lambda; }
+
+    // engine plumbing: upload tree, families and error model; map every
+    // branch of every (lambda x multiplier) point to its matrix slot
+    void upload();
+    std::vector<int> node_slots(matrix_cache& cache, const lambda* lam, const std::vector<double>& multipliers);
+
+protected:
+    std::vector<double> prior_row(const root_equilibrium_distribution& prior) const;
+
+    lambda* _p_lambda;
+    const clade* _p_tree;
+    const std::vector<gene_family>* _p_gene_families;
+    int _max_family_size;
+    int _max_root_family_size;
+    error_model* _p_error_model;
+    gpu_context& _gpu;
+    std::vector<const clade*> _nodes;     // engine node order (children before parents)
+    std::vector<family_info_stash> results;
+};
+
+class base_model : public model {
+public:
+    base_model(lambda* p_lambda, const clade* p_tree, const std::vector<gene_family>* p_gene_families,
+               int max_family_size, int max_root_family_size, error_model* p_error_model, gpu_context& gpu)
+        : model(p_lambda, p_tree, p_gene_families, max_family_size, max_root_family_size, p_error_model, gpu) {}
+    // base_model.cpp:53-100: -lnL, +inf for an invalid lambda; results hold
+    // lnL_f in posterior_probability
+    double infer_family_likelihoods(const root_equilibrium_distribution& prior, const lambda* p_lambda) override;
+    void write_family_likelihoods(std::ostream& ost) override;   // base_model.cpp:102-109
+};
+
+class gamma_model : public model {
+public:
+    // explicit categories (gamma_core.h constructor 2)
+    gamma_model(lambda* p_lambda, const clade* p_tree, const std::vector<gene_family>* p_gene_families,
+                int max_family_size, int max_root_family_size, std::vector<double> gamma_categories,
+                std::vector<double> multipliers, error_model* p_error_model, gpu_context& gpu);
+    // n categories at shape alpha (gamma_core.cpp:24-34)
+    gamma_model(lambda* p_lambda, const clade* p_tree, const std::vector<gene_family>* p_gene_families,
+                int max_family_size, int max_root_family_size, int n_gamma_cats, double fixed_alpha,
+                error_model* p_error_model, gpu_context& gpu);
+    void set_alpha(double alpha);                 // gamma_core.cpp:61-67
+    double get_alpha() const { return _alpha; }
+    std::vector<double> get_lambda_multipliers() const { return _lambda_multipliers; }
+    bool can_infer() const;                       // gamma_core.cpp:123-141
+    // gamma_core.cpp:143-165 for one family: false on a zero-sum category
+    bool prune(const gene_family& family, const root_equilibrium_distribution& prior, matrix_cache& calc,
+               const lambda* p_lambda, std::vector<double>& category_likelihoods);
+    double infer_family_likelihoods(const root_equilibrium_distribution& prior, const lambda* p_lambda) override;
+    void write_family_likelihoods(std::ostream& ost) override;   // gamma_core.cpp:52-58
+
+private:
+    std::vector<double> _gamma_cat_probs;
+    std::vector<double> _lambda_multipliers;
+    double _alpha = 0.0;
+};
+
+// PAML discrete gamma rates (gamma.cpp:15-52, 225-241)
+void get_gamma(std::vector<double>& v_freq, std::vector<double>& v_rate, double alpha);
+
+// inference_prune (core.cpp:134-145) for one family on the engine: the root
+// vector (max_root_family_size entries, root size j+1 at index j)
+std::vector<double> inference_prune(const gene_family& gf, matrix_cache& calc, const lambda* p_lambda,
+                                    const error_model* p_error_model, const clade* p_tree,
+                                    double lambda_multiplier, int max_root_family_size, int max_family_size);
+
+}  // namespace cafe5_gpu

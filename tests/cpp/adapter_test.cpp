@@ -1,0 +1,279 @@
+// adapter_test.cpp -- drives the C++ host interface (cafe5_amd/host) on a GPU.
+//
+//   adapter_test kats            the reference's known-answer tests, run
+//                                through the mirrored classes; prints
+//                                "PASS name" / "FAIL name ..." lines
+//   adapter_test base FILE       evaluates the base or gamma model described
+//                                in FILE and prints per-family values
+//
+// FILE (written by tests/test_gpu_cpp_adapter.py):
+//   newick <text>
+//   lambda_tree <text>|-          (multi-lambda classes, "-" for one lambda)
+//   lambdas v1 v2 ...
+//   bounds B R
+//   prior uniform N | poisson mu N
+//   gamma K alpha | none
+//   error eps | none
+//   species name1 name2 ...
+//   family id c1 c2 ...          (one line per family)
+#include <cmath>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "cafe5_gpu.hpp"
+
+using namespace cafe5_gpu;
+
+static int failures = 0;
+
+static void report(const std::string& name, bool ok, const std::string& detail = "")
+{
+    std::cout << (ok ? "PASS " : "FAIL ") << name << (detail.empty() ? "" : " " + detail) << std::endl;
+    if (!ok) ++failures;
+}
+
+static bool close_to(double v, double want, double unit)
+{
+    return std::fabs(v - want) <= unit * (1 + 1e-9);
+}
+
+static int run_kats(gpu_context& gpu)
+{
+    // test.cpp:595-617
+    {
+        matrix_cache calc(5, gpu);
+        single_lambda lam(0.05);
+        calc.precalculate_matrices(get_lambda_values(&lam), std::set<double>{5});
+        const matrix* m = calc.get_matrix(5, 0.05);
+        const double want[5][5] = {{1, 0, 0, 0, 0},
+                                   {0.2, 0.64, 0.128, 0.0256, 0.00512},
+                                   {0.04, 0.256, 0.4608, 0.17408, 0.0512},
+                                   {0.008, 0.0768, 0.26112, 0.36352, 0.187392},
+                                   {0.0016, 0.02048, 0.1024, 0.249856, 0.305562}};
+        bool ok = true;
+        for (int i = 0; i < 5; ++i)
+            for (int j = 0; j < 5; ++j) ok &= std::fabs(m->get(i, j) - want[i][j]) < 0.00001;
+        report("probability_of_matrix", ok);
+        report("matrix_cache_get_matrix_miss_throws", [&] {
+            try {
+                calc.get_matrix(6, 0.05);
+                return false;
+            } catch (std::runtime_error&) {
+                return true;
+            }
+        }());
+        report("get_from_parent_fam_size_to_c", close_to(calc.get_from_parent_fam_size_to_c(0.05, 5, 5, 9), 0.0152237, 1e-7),
+               std::to_string(calc.get_from_parent_fam_size_to_c(0.05, 5, 5, 9)));
+    }
+    // test.cpp:856-863
+    {
+        matrix_cache calc(5, gpu);
+        multiple_lambda lam(std::map<std::string, int>(), std::vector<double>{.1, .2, .3, .4});
+        calc.precalculate_matrices(get_lambda_values(&lam), std::set<double>{1, 2, 3});
+        report("precalculate_matrices_calculates_all_lambdas_all_branchlengths", calc.get_cache_size() == 12);
+    }
+    // test.cpp:1630-1644
+    report("matrix_is_saturated", matrix_cache::is_saturated(25, 0.05) && !matrix_cache::is_saturated(25, 0.01));
+    // test.cpp:1662-1683
+    {
+        std::unique_ptr<clade> tree(parse_newick("(A:1,B:3):7"));
+        gene_family fam;
+        fam.set_species_size("A", 3);
+        fam.set_species_size("B", 6);
+        single_lambda lam(0.03);
+        matrix_cache cache(21, gpu);
+        cache.precalculate_matrices({0.045}, std::set<double>{1.0, 3.0, 7.0});
+        auto actual = inference_prune(fam, cache, &lam, nullptr, tree.get(), 1.5, 20, 20);
+        const double want[] = {-17.2771, -10.0323, -5.0695, -4.91426, -5.86062, -7.75163, -10.7347, -14.2334,
+                               -18.0458, -22.073, -26.2579, -30.5639, -34.9663, -39.4472, -43.9935, -48.595,
+                               -53.2439, -57.9338, -62.6597, -67.4173};
+        bool ok = actual.size() == 20;
+        for (int i = 0; ok && i < 20; ++i) ok &= std::fabs(std::log(actual[i]) - want[i]) <= 1.2e-5 * (1 + std::fabs(want[i]));
+        report("inference_prune", ok);
+    }
+    // test.cpp:461-488
+    {
+        std::unique_ptr<clade> tree(parse_newick("(A:1,B:1);"));
+        std::vector<gene_family> families(4);
+        const int counts[4][2] = {{1, 2}, {2, 1}, {3, 6}, {6, 3}};
+        for (int f = 0; f < 4; ++f) {
+            families[f].set_species_size("A", counts[f][0]);
+            families[f].set_species_size("B", counts[f][1]);
+        }
+        single_lambda lam(0.01);
+        root_equilibrium_distribution prior(100);
+        for (int path = 0; path < 3; ++path) {
+            gpu.path = path;
+            base_model core(&lam, tree.get(), &families, 56, 8, nullptr, gpu);
+            const double v = core.infer_family_likelihoods(prior, &lam);
+            report("infer_processes_path" + std::to_string(path), std::fabs(v - 46.56632) < 1e-5, std::to_string(v));
+        }
+        gpu.path = 2;
+        single_lambda bad(-0.01);
+        base_model core(&bad, tree.get(), &families, 56, 8, nullptr, gpu);
+        report("base_model_invalid_lambda_is_inf", std::isinf(core.infer_family_likelihoods(prior, &bad)));
+    }
+    // test.cpp:1269-1294
+    {
+        std::vector<gene_family> families(1);
+        families[0].set_species_size("A", 3);
+        families[0].set_species_size("B", 6);
+        std::unique_ptr<clade> tree(parse_newick("(A:1,B:3):7"));
+        single_lambda lam(0.005);
+        std::map<int, int> rd{{1, 2}, {2, 2}, {3, 2}, {4, 2}, {5, 1}};
+        root_equilibrium_distribution dist(rd);
+        matrix_cache cache(11, gpu);
+        cache.precalculate_matrices({0.0005, 0.0025}, std::set<double>{1, 3, 7});
+        gamma_model model(&lam, tree.get(), &families, 10, 8, {0.01, 0.05}, {0.1, 0.5}, nullptr, gpu);
+        std::vector<double> cat;
+        const bool ok = model.prune(families[0], dist, cache, &lam, cat);
+        report("gamma_model_prune", ok && cat.size() == 2 && std::fabs(std::log(cat[0]) + 23.04433) < 1e-5 &&
+                                        std::fabs(std::log(cat[1]) + 16.68005) < 1e-5);
+    }
+    // test.cpp:1296-1311
+    {
+        std::vector<gene_family> families(1);
+        families[0].set_species_size("A", 3);
+        families[0].set_species_size("B", 6);
+        std::unique_ptr<clade> tree(parse_newick("(A:1,B:3):7"));
+        single_lambda lam(0.9);
+        matrix_cache cache(11, gpu);
+        cache.precalculate_matrices({0.09, 0.45}, std::set<double>{1, 3, 7});
+        gamma_model model(&lam, tree.get(), &families, 10, 8, {1.0, 1.0}, {0.1, 0.5}, nullptr, gpu);
+        std::vector<double> cat;
+        root_equilibrium_distribution prior(100);
+        report("gamma_model_prune_returns_false_if_saturated", !model.prune(families[0], prior, cache, &lam, cat));
+        report("gamma_model_infer_saturated_is_inf", std::isinf(model.infer_family_likelihoods(prior, &lam)));
+    }
+    // test.cpp:490-505, 2690-2738
+    {
+        root_equilibrium_distribution u(10);
+        std::map<int, int> m{{1, 3}, {2, 5}};
+        root_equilibrium_distribution md(m);
+        root_equilibrium_distribution p(0.75, 100);
+        report("root_prior_values", std::fabs(u.compute(5) - 0.1) < 1e-7 && md.compute(1) == 0.375f &&
+                                        std::fabs(p.compute(0) - 0.47237) < 1e-5 && p.compute(11) == 0.0f);
+    }
+    // test.cpp:1765-1804 (error model rows and forward fill)
+    {
+        error_model em;
+        em.set_max_family_size(20);
+        em.set_probabilities(0, {0.0, 0.8, 0.2});
+        em.set_probabilities(1, {0.2, 0.6, 0.2});
+        em.set_probabilities(20, {0.2, 0.6, 0.2});
+        const auto r = em.get_probs(7);
+        bool threw = false;
+        try {
+            error_model e2;
+            e2.set_probabilities(0, {0.1, 0.8, 0.1});
+        } catch (std::runtime_error&) {
+            threw = true;
+        }
+        report("error_model_probabilities", r.size() == 3 && r[0] == 0.2 && r[1] == 0.6 && threw);
+    }
+    // gamma.cpp (no reference test pins the values; printed for the oracle)
+    {
+        std::vector<double> f(4), r(4);
+        get_gamma(f, r, 0.5);
+        std::cout << std::setprecision(17) << "GAMMA 4 0.5";
+        for (double x : r) std::cout << " " << x;
+        std::cout << std::endl;
+    }
+    return failures ? 1 : 0;
+}
+
+static int run_model(gpu_context& gpu, const std::string& file)
+{
+    std::ifstream in(file);
+    if (!in) throw std::runtime_error("cannot open " + file);
+    std::string line, newick, lambda_tree = "-", prior_kind = "uniform";
+    std::vector<double> lambdas;
+    int B = 0, R = 0, K = 0, prior_n = 0;
+    double alpha = 0, eps = -1, mu = 0;
+    std::vector<std::string> species;
+    std::vector<gene_family> families;
+    while (std::getline(in, line)) {
+        std::istringstream ist(line);
+        std::string tag;
+        ist >> tag;
+        if (tag == "newick") ist >> newick;
+        else if (tag == "lambda_tree") ist >> lambda_tree;
+        else if (tag == "lambdas") { double v; while (ist >> v) lambdas.push_back(v); }
+        else if (tag == "bounds") ist >> B >> R;
+        else if (tag == "prior") {
+            ist >> prior_kind;
+            if (prior_kind == "poisson") ist >> mu;
+            ist >> prior_n;
+        } else if (tag == "gamma") {
+            std::string k;
+            ist >> k;
+            if (k != "none") { K = std::stoi(k); ist >> alpha; }
+        } else if (tag == "error") {
+            std::string e;
+            ist >> e;
+            if (e != "none") eps = std::stod(e);
+        } else if (tag == "species") { std::string s; while (ist >> s) species.push_back(s); }
+        else if (tag == "path") ist >> gpu.path;
+        else if (tag == "family") {
+            gene_family f;
+            std::string id;
+            ist >> id;
+            f.set_id(id);
+            for (auto& s : species) { int c; ist >> c; f.set_species_size(s, c); }
+            families.push_back(f);
+        }
+    }
+    std::unique_ptr<clade> tree(parse_newick(newick));
+    std::unique_ptr<lambda> lam;
+    if (lambda_tree != "-") {
+        std::unique_ptr<clade> lt(parse_newick(lambda_tree, true));
+        lam.reset(new multiple_lambda(lt->get_lambda_index_map(), lambdas));
+    } else {
+        lam.reset(new single_lambda(lambdas.at(0)));
+    }
+    std::unique_ptr<error_model> em;
+    if (eps >= 0) {
+        em.reset(new error_model());
+        em->set_max_family_size(B);
+        em->set_probabilities(0, {0, 1 - eps, eps});
+        em->set_probabilities(1, {eps, 1 - 2 * eps, eps});
+    }
+    std::unique_ptr<root_equilibrium_distribution> prior(
+        prior_kind == "poisson" ? new root_equilibrium_distribution(mu, (size_t)prior_n)
+                                : new root_equilibrium_distribution((size_t)prior_n));
+    std::cout << std::setprecision(17);
+    if (K > 0) {
+        gamma_model m(lam.get(), tree.get(), &families, B, R, K, alpha, em.get(), gpu);
+        const double total = m.infer_family_likelihoods(*prior, lam.get());
+        std::cout << "TOTAL " << total << std::endl;
+        for (auto& r : m.get_results())
+            std::cout << "STASH " << r.family_id << " " << r.lambda_multiplier << " " << r.category_likelihood << " "
+                      << r.family_likelihood << " " << r.posterior_probability << std::endl;
+    } else {
+        base_model m(lam.get(), tree.get(), &families, B, R, em.get(), gpu);
+        const double total = m.infer_family_likelihoods(*prior, lam.get());
+        std::cout << "TOTAL " << total << std::endl;
+        for (auto& r : m.get_results()) std::cout << "LNL " << r.family_id << " " << r.posterior_probability << std::endl;
+    }
+    return 0;
+}
+
+int main(int argc, char** argv)
+{
+    try {
+        gpu_context gpu(0);
+        const std::string mode = argc > 1 ? argv[1] : "kats";
+        if (mode == "kats") return run_kats(gpu);
+        if (mode == "base" && argc > 2) return run_model(gpu, argv[2]);
+        std::cerr << "usage: adapter_test kats | base FILE" << std::endl;
+        return 2;
+    } catch (std::exception& e) {
+        std::cerr << "error: " << e.what() << std::endl;
+        return 3;
+    }
+}

@@ -72,3 +72,17 @@ def test_key_table_dedups_quantized_keys():
     assert slots[0, leaves["A"]] == slots[0, leaves["B"]]
     assert (slots[0] == slots[1]).all()
     assert len(kt.keys) == 3
+
+
+def test_cpp_interface_built_and_fails_loudly_without_gpu():
+    import subprocess
+    import torch
+    driver = os.path.join(ROOT, "tests", "cpp", "adapter_test")
+    if not os.path.exists(driver) or not os.path.exists(os.path.join(ROOT, "cafe5_amd", "libcafe5_host.so")):
+        from cafe5_amd import build
+        build.build_native()
+        build.build_host()
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is present")
+    res = subprocess.run([driver, "kats"], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 3 and "no HIP device" in res.stderr
