@@ -18,7 +18,8 @@ hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const 
                         int first_slot, int N, int Np, long long slot_elems, double* mt64, float* mt32,
                         hipStream_t stream);
 hipError_t launch_fill_zero(void* p, long long bytes, hipStream_t stream);
-hipError_t launch_prune(int path, int nt, int max_tiles, int max_waves, const PruneParams& p, hipStream_t st);
+hipError_t launch_prune_plain(int nq, int grid, const PruneParams& p, hipStream_t st);
+hipError_t launch_prune_mfma(bool fp64, const MfmaShape& s, int grid, const PruneParams& p, hipStream_t st);
 hipError_t launch_pack(const SegDesc* segs, int n_segs, const BucketDev* buckets, const int* perm,
                        const int* flags, int F, int mask, int* fam_list, TileDesc* out_tiles,
                        const int* class_off, int* class_count, int* fam_count, int tile, hipStream_t st);
@@ -477,9 +478,14 @@ int cafe_gpu_set_error_model(cafe_gpu_ctx* ctx, int n_dev, int n_rows, const dou
 
 namespace {
 
+// Kernel size class: buckets whose vectors have the same number of 16-row
+// tiles (MFMA paths) or the same PLAIN64 rows per lane (PLAIN64 path).
 struct KernelClass {
-    int nt;       // template size class
-    int begin, end;
+    int nt;              // 16-row tiles of the class's largest vector
+    int nq;              // PLAIN64 rows per lane (row s = lane + 64 q)
+    int slot_tiles;      // 16-row tiles of its largest stored (non-root) vector
+    MfmaShape s32, s64;  // launch shapes of the FP32 and FP64 MFMA kernels
+    int begin, end;      // its main tiles
 };
 
 // Host-side description of one pruning sequence over a tile list.
@@ -487,16 +493,55 @@ struct PrunePlan {
     int path = 0, model = 0;
     std::vector<KernelClass> classes;
     std::vector<int> class_counts;   // main tiles per class (+ padding)
-    std::vector<int> plain_nq;       // PLAIN64 rows-per-lane class of each class
     std::vector<int> plain_off;      // offsets of the one-family PLAIN64 recompute lists
     int plain_list_size = 0;
-    int n_tiles = 0, n_segs = 0, max_grid = 0, maxLR = 0;
-    long long per_wg_mfma = 0, per_wg_plain = 0;
+    int n_tiles = 0, n_segs = 0, maxLR = 0;
+    int waves_per_cu = 16;           // persistent grid: resident waves per CU
+    int n_cu = 256;
+    long long per_wg_plain = 0;
     PruneParams base{};
     // fb_count layout: [0, n) FP64 recompute tiles per class, [n, 2n) PLAIN64
     // tiles per class, [2n] / [2n+1] families recomputed in each pass
     int fam_count_off() const { return 2 * (int)classes.size(); }
+    int grid(int w, int max_tiles) const
+    {
+        const int g = std::max(1, n_cu * waves_per_cu / std::max(1, w));
+        return std::max(1, std::min(g, max_tiles));
+    }
 };
+
+// MFMA launch shape for vectors of nt 16-row tiles: W waves per workgroup
+// split the output tiles, NTW per wave, so that W NTW >= nt (one pass; the
+// scaled FP32 kernel needs the whole vector in one pass).  Small vectors run
+// one wave per family tile (throughput), large ones several (latency).
+MfmaShape mfma_shape(int nt, bool fp64)
+{
+    MfmaShape s{1, 1, false, 0};
+    if (nt <= 4) s = {std::max(1, nt), 1, false, 0};
+    else if (nt <= 8) s = {(nt + 1) / 2, 2, false, 0};
+    else if (nt <= 16) s = {(nt + 3) / 4, 4, false, 0};
+    else if (nt <= 32) s = {(nt + 7) / 8, 8, false, 0};
+    else if (!fp64) s = {(nt + 15) / 16, 16, false, 0};
+    else s = {4, 8, false, 0};   // FP64 above 512 rows: several passes
+    return s;
+}
+
+// Partial vectors go to LDS when the workgroup's share stays small enough for
+// several workgroups per CU; the rest use the global scratch.
+void place_partials(MfmaShape& s, bool fp64, int n_slots, int slot_tiles)
+{
+    const size_t meta = (size_t)mfma_meta_bytes(n_slots, s.w, !fp64);
+    const size_t frag = (size_t)n_slots * slot_tiles * 64 * 4 * (fp64 ? 8 : 4);
+    s.lds = s.w <= 4 && meta + frag + 4096 <= 64 * 1024 && (meta + frag) / s.w <= 20 * 1024;
+    s.shm = meta + (s.lds ? frag : 0);
+}
+
+// Global scratch of an MFMA launch (partials not in LDS), bytes.
+size_t mfma_scratch_bytes(const MfmaShape& s, bool fp64, int n_slots, int slot_tiles, int grid)
+{
+    if (s.lds) return 0;
+    return (size_t)grid * n_slots * slot_tiles * 64 * 4 * (fp64 ? 8 : 4);
+}
 
 // Validates a descriptor and uploads node slots, priors, bucket table and
 // category probabilities into b; sizes the per-point outputs.
@@ -585,40 +630,35 @@ int upload_eval_inputs(cafe_gpu_ctx* ctx, EvalBufs& b, const cafe_gpu_eval_desc*
 }
 
 // Builds and uploads the tile list for `n_points` parameter points (heaviest
-// buckets first, grouped by kernel size class) and sizes the scratch.
+// classes and buckets first, grouped by kernel size class) and sizes the
+// scratch.
 int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points, const int* point_ids,
                double* root_out, int root_stride, PrunePlan& pp)
 {
     pp = PrunePlan();
     pp.path = path;
     pp.model = model;
+    pp.n_cu = ctx->n_cu;
     int maxLR = 0;
     for (auto& bk : ctx->buckets)
         maxLR = std::max(maxLR, std::max(round_up(bk.B + 1, kTile), round_up(bk.R, kTile)));
     pp.maxLR = maxLR;
-    auto class_of = [&](const BucketHost& bk) {
-        const int lp = round_up(bk.B + 1, kTile), rp = round_up(bk.R, kTile);
-        if (std::max(bk.B + 1, bk.R) > 1024) return -1;   // PLAIN64 (and its recompute) covers 16 x 64 rows
-        if (path == CAFE_GPU_PATH_PLAIN64) {
-            int c = 1;
-            while (c * 64 < std::max(bk.B + 1, bk.R)) c <<= 1;
-            return c <= 16 ? c : -1;
-        }
-        // 16-row tiles of the largest vector; kernels exist for <= 2, 4, 8,
-        // 16, 32 tiles (FP64: more in several passes)
-        const int nt = std::max(lp, rp) / kTile;
-        static const int sizes[] = {2, 4, 8, 16, 32};
-        for (int sz : sizes)
-            if (nt <= sz) return sz;
-        return path == CAFE_GPU_PATH_BATCHED32 ? -1 : 64;
+    auto nq_of = [](const BucketHost& bk) {
+        int c = 1;
+        while (c * 64 < std::max(bk.B + 1, bk.R)) c <<= 1;
+        return c;
     };
-    std::vector<std::pair<int, int>> bucket_class;   // (class, bucket)
+    auto nt_of = [](const BucketHost& bk) {
+        return std::max(round_up(bk.B + 1, kTile), round_up(bk.R, kTile)) / kTile;
+    };
+    std::vector<std::pair<int, int>> bucket_class;   // (class key, bucket)
     for (int bi = 0; bi < (int)ctx->buckets.size(); ++bi) {
-        const int c = class_of(ctx->buckets[bi]);
-        if (c < 0)
-            return fail(CAFE_GPU_EINVAL, "bucket B=%d R=%d exceeds the sizes this path supports",
-                        ctx->buckets[bi].B, ctx->buckets[bi].R);
-        bucket_class.push_back({c, bi});
+        const BucketHost& bk = ctx->buckets[bi];
+        // PLAIN64 (which also recomputes the other paths' hard families)
+        // covers 16 x 64 rows
+        if (std::max(bk.B + 1, bk.R) > 1024)
+            return fail(CAFE_GPU_EINVAL, "bucket B=%d R=%d exceeds the sizes this path supports", bk.B, bk.R);
+        bucket_class.push_back({path == CAFE_GPU_PATH_PLAIN64 ? nq_of(bk) : nt_of(bk), bi});
     }
     // heavy classes (and, within a class, larger buckets) first
     std::stable_sort(bucket_class.begin(), bucket_class.end(), [&](const std::pair<int, int>& x,
@@ -627,45 +667,56 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         return ctx->buckets[x.second].B > ctx->buckets[y.second].B;
     });
     std::vector<TileDesc> tiles;
+    std::vector<SegDesc> segs;
+    std::vector<int> cls_fams;
+    int prev_key = -1;
     for (auto& bc : bucket_class) {
         const BucketHost& bk = ctx->buckets[bc.second];
-        if (pp.classes.empty() || pp.classes.back().nt != bc.first)
-            pp.classes.push_back(KernelClass{bc.first, (int)tiles.size(), (int)tiles.size()});
-        for (int pi = 0; pi < n_points; ++pi)
+        if (pp.classes.empty() || bc.first != prev_key) {
+            KernelClass kc{};
+            kc.begin = kc.end = (int)tiles.size();
+            pp.classes.push_back(kc);
+            cls_fams.push_back(0);
+            prev_key = bc.first;
+        }
+        KernelClass& kc = pp.classes.back();
+        const int ci = (int)pp.classes.size() - 1;
+        kc.nt = std::max(kc.nt, nt_of(bk));
+        kc.nq = std::max(kc.nq, nq_of(bk));
+        kc.slot_tiles = std::max(kc.slot_tiles, round_up(bk.B + 1, kTile) / kTile);
+        for (int pi = 0; pi < n_points; ++pi) {
             for (int f0 = 0; f0 < bk.count; f0 += kTile)
                 tiles.push_back(TileDesc{bc.second, point_ids[pi], f0, std::min(kTile, bk.count - f0)});
-        pp.classes.back().end = (int)tiles.size();
+            segs.push_back(SegDesc{bc.second, point_ids[pi], ci, 0});
+        }
+        cls_fams[ci] += bk.count * n_points;
+        kc.end = (int)tiles.size();
+    }
+    const int n_cls = (int)pp.classes.size();
+    size_t scratch = 0;
+    for (auto& kc : pp.classes) {
+        kc.s32 = mfma_shape(kc.nt, false);
+        kc.s64 = mfma_shape(kc.nt, true);
+        place_partials(kc.s32, false, ctx->n_slots, kc.slot_tiles);
+        place_partials(kc.s64, true, ctx->n_slots, kc.slot_tiles);
+        const int n = kc.end - kc.begin;
+        if (path == CAFE_GPU_PATH_BATCHED32)
+            scratch = std::max(scratch, mfma_scratch_bytes(kc.s32, false, ctx->n_slots, kc.slot_tiles,
+                                                           pp.grid(kc.s32.w, n)));
+        if (path != CAFE_GPU_PATH_PLAIN64)   // BATCHED64 main, or the FP32 path's FP64 recompute
+            scratch = std::max(scratch, mfma_scratch_bytes(kc.s64, true, ctx->n_slots, kc.slot_tiles,
+                                                           pp.grid(kc.s64.w, n)));
     }
     pp.n_tiles = (int)tiles.size();
-    const int n_cls = (int)pp.classes.size();
     pp.class_counts.assign(n_cls + 4, 0);
     for (int i = 0; i < n_cls; ++i) pp.class_counts[i] = pp.classes[i].end - pp.classes[i].begin;
-    // (bucket, point) segments per class, recompute-list offsets, and the
-    // PLAIN64 row class that covers each class's buckets
-    std::vector<SegDesc> segs;
-    std::vector<int> class_off(2 * (n_cls + 1), 0), cls_max(n_cls, 1), cls_fams(n_cls, 0);
-    {
-        int ci = -1, prev = -1;
-        for (auto& bc : bucket_class) {
-            if (bc.first != prev) {
-                ++ci;
-                prev = bc.first;
-            }
-            const BucketHost& bk = ctx->buckets[bc.second];
-            cls_max[ci] = std::max(cls_max[ci], std::max(bk.B + 1, bk.R));
-            cls_fams[ci] += bk.count * n_points;
-            for (int pi = 0; pi < n_points; ++pi) segs.push_back(SegDesc{bc.second, point_ids[pi], ci, 0});
-        }
-    }
     // [0, n_cls] offsets of the 16-family recompute lists (one slot per main
     // tile), [n_cls + 1, 2 n_cls + 1] of the one-family PLAIN64 lists
+    std::vector<int> class_off(2 * (n_cls + 1), 0);
     for (int i = 0; i < n_cls; ++i) class_off[i + 1] = class_off[i] + pp.class_counts[i];
     for (int i = 0; i < n_cls; ++i) class_off[n_cls + 1 + i + 1] = class_off[n_cls + 1 + i] + cls_fams[i];
     pp.plain_list_size = class_off[2 * n_cls + 1];
     pp.plain_off.assign(class_off.begin() + n_cls + 1, class_off.end());
-    pp.plain_nq.assign(n_cls, 1);
-    for (int i = 0; i < n_cls; ++i)
-        while (pp.plain_nq[i] * 64 < cls_max[i] && pp.plain_nq[i] < 16) pp.plain_nq[i] <<= 1;
     pp.n_segs = (int)segs.size();
 
     HIP_OR_FAIL(b.tiles.ensure(sizeof(TileDesc) * std::max(1, pp.n_tiles)));
@@ -686,11 +737,12 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     HIP_OR_FAIL(hipMemcpy(b.segs.p, segs.data(), sizeof(SegDesc) * pp.n_segs, hipMemcpyHostToDevice));
     HIP_OR_FAIL(hipMemcpy(b.class_off.p, class_off.data(), sizeof(int) * 2 * (n_cls + 1), hipMemcpyHostToDevice));
 
+    // PLAIN64 partials: one region of n_slots x ldv doubles per workgroup (LDS
+    // when it fits; the global region is sized for the worst case)
     const int ldv = round_up(maxLR, 4) + 4;
-    pp.max_grid = std::max(64, ctx->n_cu * 16);
-    pp.per_wg_mfma = (long long)ctx->n_slots * 16 * ldv;
     pp.per_wg_plain = (long long)ctx->n_slots * ldv;
-    HIP_OR_FAIL(b.scratch.ensure((size_t)pp.max_grid * 8 * std::max(pp.per_wg_mfma, pp.per_wg_plain)));
+    scratch = std::max(scratch, (size_t)pp.grid(1, 1 << 30) * 8 * (size_t)pp.per_wg_plain);
+    HIP_OR_FAIL(b.scratch.ensure(std::max<size_t>(scratch, 256)));
 
     PruneParams& p = pp.base;
     p.buckets = b.buckets.as<BucketDev>();
@@ -727,7 +779,7 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
 // Enqueues the pruning sequence: the main path per size class, then
 // (device-side lists, no host round trip) the FP64 recompute of the FP32
 // families the flush bound did not certify and the PLAIN64 recompute of
-// families near the subnormal range, each packed 16 families to a tile.
+// families near the subnormal range.
 int enqueue_prune(EvalBufs& b, const PrunePlan& pp, hipStream_t st, hipEvent_t ev_main, hipEvent_t ev_fb)
 {
     const int n_cls = (int)pp.classes.size();
@@ -737,31 +789,39 @@ int enqueue_prune(EvalBufs& b, const PrunePlan& pp, hipStream_t st, hipEvent_t e
     HIP_OR_FAIL(launch_fill_zero(b.counters.p, sizeof(int) * (3 * n_cls + 8), st));
     HIP_OR_FAIL(launch_fill_zero(b.fb_count.p, sizeof(int) * (2 * n_cls + 4), st));
     for (int ci = 0; ci < n_cls; ++ci) {
+        const KernelClass& kc = pp.classes[ci];
+        const int n = pp.class_counts[ci];
         PruneParams q = pp.base;
-        q.tiles = b.tiles.as<TileDesc>() + pp.classes[ci].begin;
+        q.tiles = b.tiles.as<TileDesc>() + kc.begin;
         q.n_tiles_dev = d_cnt + ci;
         q.tile_counter = d_ctr + ci;
-        q.scratch_per_wg = pp.path == CAFE_GPU_PATH_PLAIN64 ? pp.per_wg_plain : pp.per_wg_mfma;
-        HIP_OR_FAIL(launch_prune(pp.path, pp.classes[ci].nt, pp.class_counts[ci], pp.max_grid, q, st));
+        q.slot_tiles = kc.slot_tiles;
+        if (pp.path == CAFE_GPU_PATH_PLAIN64) {
+            q.scratch_per_wg = pp.per_wg_plain;
+            HIP_OR_FAIL(launch_prune_plain(kc.nq, pp.grid(1, n), q, st));
+        } else if (pp.path == CAFE_GPU_PATH_BATCHED64) {
+            HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64.w, n), q, st));
+        } else {
+            HIP_OR_FAIL(launch_prune_mfma(false, kc.s32, pp.grid(kc.s32.w, n), q, st));
+        }
     }
     if (ev_main) HIP_OR_FAIL(hipEventRecord(ev_main, st));
     TileDesc* fb64 = b.fb_tiles.as<TileDesc>();
     TileDesc* fbplain = fb64 + std::max(1, pp.n_tiles);
     const int* coff = b.class_off.as<int>();
-    std::vector<int> hoff(n_cls + 1, 0);
-    for (int i = 0; i < n_cls; ++i) hoff[i + 1] = hoff[i] + pp.class_counts[i];
     const int fam_off = pp.fam_count_off();
     if (pp.path == CAFE_GPU_PATH_BATCHED32) {
         HIP_OR_FAIL(launch_pack(b.segs.as<SegDesc>(), pp.n_segs, b.buckets.as<BucketDev>(), pp.base.perm,
                                 b.flag.as<int>(), pp.base.F, kFlagRecompute64, b.fam_list.as<int>(), fb64, coff,
                                 fbc, fbc + fam_off, kTile, st));
         for (int ci = 0; ci < n_cls; ++ci) {
+            const KernelClass& kc = pp.classes[ci];
             PruneParams q = pp.base;
-            q.tiles = fb64 + hoff[ci];
+            q.tiles = fb64 + kc.begin;
             q.n_tiles_dev = fbc + ci;
             q.tile_counter = d_ctr + n_cls + ci;
-            q.scratch_per_wg = pp.per_wg_mfma;
-            HIP_OR_FAIL(launch_prune(CAFE_GPU_PATH_BATCHED64, pp.classes[ci].nt, pp.class_counts[ci], pp.max_grid, q, st));
+            q.slot_tiles = kc.slot_tiles;
+            HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64.w, pp.class_counts[ci]), q, st));
         }
     }
     if (pp.path != CAFE_GPU_PATH_PLAIN64) {
@@ -769,13 +829,13 @@ int enqueue_prune(EvalBufs& b, const PrunePlan& pp, hipStream_t st, hipEvent_t e
                                 b.flag.as<int>(), pp.base.F, kFlagRecomputePlain, b.fam_list.as<int>(), fbplain,
                                 coff + n_cls + 1, fbc + n_cls, fbc + fam_off + 1, 1, st));
         for (int ci = 0; ci < n_cls; ++ci) {
+            const KernelClass& kc = pp.classes[ci];
             PruneParams q = pp.base;
             q.tiles = fbplain + pp.plain_off[ci];
             q.n_tiles_dev = fbc + n_cls + ci;
             q.tile_counter = d_ctr + 2 * n_cls + ci;
             q.scratch_per_wg = pp.per_wg_plain;
-            HIP_OR_FAIL(launch_prune(CAFE_GPU_PATH_PLAIN64, pp.plain_nq[ci], pp.plain_off[ci + 1] - pp.plain_off[ci],
-                                     pp.max_grid, q, st));
+            HIP_OR_FAIL(launch_prune_plain(kc.nq, pp.grid(1, pp.plain_off[ci + 1] - pp.plain_off[ci]), q, st));
         }
     }
     if (ev_fb) HIP_OR_FAIL(hipEventRecord(ev_fb, st));

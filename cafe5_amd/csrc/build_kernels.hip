@@ -1,8 +1,9 @@
 // build_kernels.hip -- transition-matrix construction on the GPU
 // (replaces matrix_cache::precalculate_matrices, src/matrix_cache.cpp:113-163).
 //
-// Both kernels write the c-major pool layout of engine.h: an FP64 copy and,
-// when requested, an FP32 copy of every matrix scaled by 2^kMatExp.  Entries are those of
+// The kernels write the pool layouts of engine.h: an FP64 copy (c-major) and,
+// when requested, an FP32 copy (row-major) of every matrix scaled by
+// 2^kMatExp.  Entries are those of
 // get_from_parent_fam_size_to_c (matrix_cache.cpp:78-85): row 0 is delta_0,
 // rows s >= 1 come from the birth-death process with a = lt/(1+lt) and
 // coeff = 1 - 2a; keys with coeff <= 0 or coeff == 1 (saturated, lt == 1, or
@@ -34,8 +35,7 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
                                                           const double* __restrict__ t_q,
                                                           int first_slot, int N, int Np,
                                                           long long slot_elems,
-                                                          double* __restrict__ mt64,
-                                                          float* __restrict__ mt32)
+                                                          double* __restrict__ mt64)
 {
     const int key = blockIdx.x;
     const int lane = threadIdx.x;
@@ -43,17 +43,12 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
     const double alpha = lt / (1 + lt);
     const double coeff = 1 - 2 * alpha;
     double* M64 = mt64 + (long long)(first_slot + key) * slot_elems;
-    float* M32 = mt32 ? mt32 + (long long)(first_slot + key) * slot_elems : nullptr;
 
     if (degenerate_key(alpha, coeff)) {
         for (int c = 0; c < N; ++c)
             for (int b = 0; b < NB; ++b) {
                 const int s = lane + 64 * b;
-                if (s < N) {
-                    const double v = (s == 0 && c == 0) ? 1.0 : 0.0;
-                    M64[(long long)c * Np + s] = v;
-                    if (M32) M32[(long long)c * Np + s] = (float)ldexp(v, kMatExp);
-                }
+                if (s < N) M64[(long long)c * Np + s] = (s == 0 && c == 0) ? 1.0 : 0.0;
             }
         return;
     }
@@ -103,11 +98,92 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
             carry = __shfl(y, 63);
             prev[b] = y;
             const int s = lane + 64 * b;
-            if (s < N) {
-                const double v = y < 1.0 ? y : 1.0;   // [0,1] clamp, probability.cpp:147
-                M64[(long long)c * Np + s] = v;
-                if (M32) M32[(long long)c * Np + s] = (float)ldexp(v, kMatExp);
+            if (s < N) M64[(long long)c * Np + s] = y < 1.0 ? y : 1.0;   // [0,1] clamp, probability.cpp:147
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The same recurrence read along a row: for fixed s,
+//     P_s(c) = a P_s(c-1) + K_c,   K_c = a P_{s-1}(c) + (1-2a) P_{s-1}(c-1),
+// a first-order scan over c from row s-1 (row 0 is delta_0).  One wavefront per
+// matrix with lanes owning columns c = l + 64 b writes the row-major FP32 pool
+// with contiguous stores (values computed in FP64, clamped to [0,1], scaled by
+// 2^kMatExp and rounded once).
+// ---------------------------------------------------------------------------
+template <int NB>
+__global__ __launch_bounds__(64) void k_build_recurrence_rows(const double* __restrict__ lam_q,
+                                                               const double* __restrict__ t_q,
+                                                               int first_slot, int N, int Np,
+                                                               long long slot_elems,
+                                                               float* __restrict__ mt32)
+{
+    const int key = blockIdx.x;
+    const int lane = threadIdx.x;
+    const double lt = lam_q[key] * t_q[key];
+    const double alpha = lt / (1 + lt);
+    const double coeff = 1 - 2 * alpha;
+    float* M32 = mt32 + (long long)(first_slot + key) * slot_elems;
+    const float one = (float)ldexp(1.0, kMatExp);
+
+    // row 0: delta_0 (also the whole matrix of a degenerate key)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int c = lane + 64 * b;
+        if (c < N) M32[c] = c == 0 ? one : 0.f;
+    }
+    if (degenerate_key(alpha, coeff)) {
+        for (int s = 1; s < N; ++s)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int c = lane + 64 * b;
+                if (c < N) M32[(long long)s * Np + c] = 0.f;
             }
+        return;
+    }
+
+    double apow[6];
+    apow[0] = alpha;
+    for (int i = 1; i < 6; ++i) apow[i] = apow[i - 1] * apow[i - 1];
+    double a_lane = 1.0;
+    {
+        int e = lane + 1;
+        double base = alpha;
+        while (e) {
+            if (e & 1) a_lane *= base;
+            base *= base;
+            e >>= 1;
+        }
+    }
+
+    double prev[NB];   // P_{s-1}(c) for the lane's columns
+#pragma unroll
+    for (int b = 0; b < NB; ++b) prev[b] = (lane + 64 * b == 0) ? 1.0 : 0.0;
+
+    for (int s = 1; s < N; ++s) {
+        double K[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            double left = __shfl_up(prev[b], 1);   // P_{s-1}(c-1)
+            const double from_prev_block = b > 0 ? __shfl(prev[b > 0 ? b - 1 : 0], 63) : 0.0;
+            if (lane == 0) left = from_prev_block;
+            K[b] = alpha * prev[b] + coeff * left;
+        }
+        double carry = 0.0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            double y = K[b];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                const int d = 1 << i;
+                const double up = __shfl_up(y, d);
+                if (lane >= d) y += apow[i] * up;
+            }
+            if (b > 0) y += a_lane * carry;
+            carry = __shfl(y, 63);
+            prev[b] = y;
+            const int c = lane + 64 * b;
+            if (c < N) M32[(long long)s * Np + c] = (float)ldexp(y < 1.0 ? y : 1.0, kMatExp);
         }
     }
 }
@@ -160,7 +236,7 @@ __global__ __launch_bounds__(256) void k_build_closed_form(const double* __restr
     }
     double* M64 = mt64 + (long long)(first_slot + key) * slot_elems;
     M64[(long long)c * Np + s] = v;
-    if (mt32) mt32[(long long)(first_slot + key) * slot_elems + (long long)c * Np + s] = (float)ldexp(v, kMatExp);
+    if (mt32) mt32[(long long)(first_slot + key) * slot_elems + (long long)s * Np + c] = (float)ldexp(v, kMatExp);
 }
 
 // Zero-fill (used instead of hipMemsetAsync so the calls can be captured).
@@ -191,26 +267,23 @@ hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const 
         }
     } else {
         const int nb = (N + 63) / 64;
-        if (nb <= 1)
-            hipLaunchKernelGGL(k_build_recurrence<1>, dim3(n), dim3(64), 0, stream, d_lam, d_t,
-                               first_slot, N, Np, slot_elems, mt64, mt32);
-        else if (nb <= 2)
-            hipLaunchKernelGGL(k_build_recurrence<2>, dim3(n), dim3(64), 0, stream, d_lam, d_t,
-                               first_slot, N, Np, slot_elems, mt64, mt32);
-        else if (nb <= 4)
-            hipLaunchKernelGGL(k_build_recurrence<4>, dim3(n), dim3(64), 0, stream, d_lam, d_t,
-                               first_slot, N, Np, slot_elems, mt64, mt32);
-        else if (nb <= 8)
-            hipLaunchKernelGGL(k_build_recurrence<8>, dim3(n), dim3(64), 0, stream, d_lam, d_t,
-                               first_slot, N, Np, slot_elems, mt64, mt32);
-        else if (nb <= 16)
-            hipLaunchKernelGGL(k_build_recurrence<16>, dim3(n), dim3(64), 0, stream, d_lam, d_t,
-                               first_slot, N, Np, slot_elems, mt64, mt32);
-        else if (nb <= 32)
-            hipLaunchKernelGGL(k_build_recurrence<32>, dim3(n), dim3(64), 0, stream, d_lam, d_t,
-                               first_slot, N, Np, slot_elems, mt64, mt32);
-        else
-            return hipErrorInvalidValue;
+#define CAFE_BUILD_NB(NB)                                                                                 \
+    if (nb <= NB) {                                                                                       \
+        hipLaunchKernelGGL(k_build_recurrence<NB>, dim3(n), dim3(64), 0, stream, d_lam, d_t, first_slot, N, \
+                           Np, slot_elems, mt64);                                                         \
+        if (mt32)                                                                                         \
+            hipLaunchKernelGGL(k_build_recurrence_rows<NB>, dim3(n), dim3(64), 0, stream, d_lam, d_t,      \
+                               first_slot, N, Np, slot_elems, mt32);                                     \
+        return hipGetLastError();                                                                         \
+    }
+        CAFE_BUILD_NB(1)
+        CAFE_BUILD_NB(2)
+        CAFE_BUILD_NB(4)
+        CAFE_BUILD_NB(8)
+        CAFE_BUILD_NB(16)
+        CAFE_BUILD_NB(32)
+#undef CAFE_BUILD_NB
+        return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -7,9 +7,13 @@
 
 namespace cafe {
 
-// Matrices are stored column-major ("c-major"): element (s, c) = P(s -> c) at
-// pool[slot * slot_elems + c * Np + s].  Np = round_up(N, 16) + 16 leaves room
-// for 16-row tiles and the one-row shift of the root contraction; padding is 0.
+// Matrix pools: element (s, c) = P(s -> c) of slot k is, in the FP64 pool,
+// at mt64[k * slot_elems + c * Np + s] ("c-major", so a wave reading rows
+// s = lane.. of one column is coalesced) and, in the FP32 pool, at
+// mt32[k * slot_elems + s * Np + c] (row-major, so one 16-byte load gives an
+// MFMA lane four consecutive contraction entries).  Np = round_up(N, 16) + 16
+// leaves room for 16-row tiles and the one-row shift of the root contraction;
+// padding is 0.
 constexpr int kTile = 16;
 constexpr int kMaxSlots = 24;         // live partial vectors per family tile
 constexpr int kLgammaN = 4096;        // log-gamma table (integers < 4096)
@@ -113,7 +117,28 @@ struct PruneParams {
     long long scratch_per_wg;  // elements
     int ldv;
     int n_slots;
+    int slot_tiles;           // MFMA: 16-row tiles per stored partial vector
     int plain_lds;            // PLAIN64: partial vectors in dynamic LDS
 };
+
+// Launch shape of the MFMA pruning kernel: ntw 16-row output tiles per wave,
+// w waves per workgroup, partial vectors in LDS or global scratch, dynamic
+// LDS bytes.
+struct MfmaShape {
+    int ntw;
+    int w;
+    bool lds;
+    size_t shm;
+};
+
+// Dynamic-LDS bookkeeping of the scaled FP32 kernel for n_slots partial
+// vectors and w waves: per slot a flush bound (double) and exponent (int) per
+// family, per-wave maxima (float) per family, and one nominal-scale exponent.
+__host__ __device__ inline int mfma_meta_bytes(int n_slots, int w, bool scaled)
+{
+    if (!scaled) return 0;
+    const int b = n_slots * 16 * 8 + n_slots * w * 16 * 4 + n_slots * 16 * 4 + n_slots * 4;
+    return (b + 31) & ~31;
+}
 
 }  // namespace cafe
