@@ -7,7 +7,7 @@ Workload (one "step" = one pass of the hot path over the whole batch):
   heavy-tailed root sizes (85% <= 10, tail to 150) simulated by birth-death;
   per-family size bounds 20..200 in 8 buckets; a 64-point lambda grid
   1e-4 .. 1e-2.  A step builds every (lambda x branch) transition matrix
-  (4,224 matrices of 201 x 201), prunes all 20,000 x 64 family likelihoods,
+  (3,648 matrices of 201 x 201), prunes all 20,000 x 64 family likelihoods,
   reduces them and returns the 64 totals -- replayed as one HIP graph with all
   inputs resident on the GPU.
 
@@ -93,10 +93,59 @@ def time_plan(plan: native.Plan, steps: int, warmup: int, profile_steps: int):
     t1 = time.perf_counter()
     phase = np.zeros(4)
     graph_ok = False
+    cls = None
     for _ in range(profile_steps):
         ms, graph_ok = plan.profile()
         phase += ms
-    return (t1 - t0) * 1e3 / steps, phase / max(1, profile_steps), graph_ok
+        nt, tiles, cms = plan.class_times()
+        cls = (nt, tiles, cms if cls is None else cls[2] + cms)
+    n = max(1, profile_steps)
+    if cls is not None:
+        cls = (cls[0], cls[1], cls[2] / n)
+    return (t1 - t0) * 1e3 / steps, phase / n, graph_ok, cls
+
+
+def read_traffic(nt: int):
+    """HBM bytes per launch of the FP32 main-pass kernel of the nt-tile size
+    class, from the committed rocprofv3 counter summary (FETCH_SIZE and
+    WRITE_SIZE of that kernel, each scaled by what the counter reads for a
+    device copy of known size); None when the summary does not cover it."""
+    path = os.path.join(ROOT, "profiles", "prune_fp32_uneven34_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    k = t.get("kernels", {}).get(str(nt))
+    if not k:
+        return None
+    return {"bytes": k["fetch_kib"] * 1024 * t["fetch_scale"] + k["write_kib"] * 1024 * t["write_scale"],
+            "source": f"profiles/prune_fp32_uneven34_traffic.json ({t['command']})"}
+
+
+def kernels_only(w: wl.Workload, steps: int):
+    """The FP32 plan launched eagerly `steps` times, then two device copies
+    of 256 MiB (the known-size reference for counter scaling): the program
+    that rocprofv3 counter passes run."""
+    eng = native.Engine(0)
+    buckets, slots, lam_q, t_q, prior, N = setup(eng, w)
+    p = eng.plan(slots, prior, lam_q, t_q, path=native.PATH_BATCHED32, use_graph=False)
+    for _ in range(steps):
+        p.execute()
+        p.synchronize()
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = ctypes.c_size_t(256 << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), nbytes) != 0 or hip.hipMalloc(ctypes.byref(b), nbytes) != 0:
+        raise RuntimeError("hipMalloc failed")
+    for _ in range(2):
+        if hip.hipMemcpy(b, a, nbytes, 3) != 0:   # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    p.close()
+    eng.close()
+    log(f"kernels-only: {steps} eager FP32 executions, 2 x 256 MiB device copies")
 
 
 # ----------------------------------------------------------------------------
@@ -213,10 +262,12 @@ def main():
     ap.add_argument("--families", type=int, default=20000)
     ap.add_argument("--trials", type=int, default=64)
     ap.add_argument("--fp64-steps", type=int, default=3, help="timed steps of the FP64 comparison paths")
-    ap.add_argument("--cpu-lambdas", type=int, default=3)
-    ap.add_argument("--cpu-families", type=int, default=2000)
+    ap.add_argument("--cpu-lambdas", type=int, default=8)
+    ap.add_argument("--cpu-families", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="run the FP32 plan eagerly --steps times plus a 256 MiB copy (for counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
@@ -224,6 +275,9 @@ def main():
         sys.exit(2)
 
     w = wl.config_uneven(seed=2, F=args.families, T=args.trials)
+    if args.kernels_only:
+        kernels_only(w, args.steps)
+        return
     F, T = w.counts.shape[0], len(w.lambdas)
     log(f"workload: {w.name} F={F} T={T} taxa={w.tree.n_leaves}")
     eng = native.Engine(0)
@@ -233,7 +287,7 @@ def main():
 
     # ---- headline: BATCHED32 ----
     p32 = eng.plan(slots, prior, lam_q, t_q, path=native.PATH_BATCHED32, use_graph=use_graph)
-    ms32, ph32, graph_ok = time_plan(p32, args.steps, args.warmup, args.steps)
+    ms32, ph32, graph_ok, cls32 = time_plan(p32, args.steps, args.warmup, args.steps)
     r32 = p32.results()
     rc64, rcplain = p32.recompute_counts()
     log(f"batched32: {ms32:.3f} ms/step, phases(build, prune, recompute, finalize)={ph32.round(3).tolist()} "
@@ -241,11 +295,11 @@ def main():
 
     # ---- FP64 paths on the same inputs ----
     p64 = eng.plan(slots, prior, lam_q, t_q, path=native.PATH_BATCHED64, use_graph=use_graph)
-    ms64, ph64, _ = time_plan(p64, args.fp64_steps, 1, 1)
+    ms64, ph64, _, _ = time_plan(p64, args.fp64_steps, 1, 1)
     r64 = p64.results()
     log(f"batched64: {ms64:.3f} ms/step phases={ph64.round(3).tolist()}")
     pplain = eng.plan(slots, prior, lam_q, t_q, path=native.PATH_PLAIN64, use_graph=use_graph)
-    msplain, phplain, _ = time_plan(pplain, args.fp64_steps, 1, 1)
+    msplain, phplain, _, _ = time_plan(pplain, args.fp64_steps, 1, 1)
     rplain = pplain.results()
     log(f"plain64: {msplain:.3f} ms/step phases={phplain.round(3).tolist()}")
 
@@ -271,20 +325,36 @@ def main():
     }
     log(f"accuracy: {accuracy}")
 
-    # ---- roofline of the FP32 prune phase ----
+    # ---- roofline of the dominant kernel: the FP32 main-pass launch of the
+    # size class that takes longest (per-class HIP events, same stream) ----
     n1, n2 = tree_flop_factors(w.tree)
-    flops = 0.0
+    nt_cls, tiles_cls, ms_cls = cls32
+    dom = int(np.argmax(ms_cls))
+    dom_nt = int(nt_cls[dom])
+    dom_flops = 0.0
+    dom_buckets = []
     for B, R, cnt in buckets:
         L = B + 1
-        flops += cnt * T * 2.0 * L * (n1 * L + n2 * R)
-    prune_ms = ph32[1]
-    achieved = flops / (prune_ms * 1e-3) if prune_ms > 0 else None
-    build_bytes = len(lam_q) * N * N * (8 + 4)
+        if max(-(-L // 16), -(-(R + 1) // 16)) == dom_nt:
+            dom_flops += cnt * T * 2.0 * L * (n1 * L + n2 * R)
+            dom_buckets.append(B)
+    dom_ms = float(ms_cls[dom])
+    achieved = dom_flops / (dom_ms * 1e-3) if dom_ms > 0 else None
+    phase_flops = sum(cnt * T * 2.0 * (B + 1) * (n1 * (B + 1) + n2 * R) for B, R, cnt in buckets)
+    traffic = read_traffic(dom_nt)
+    build_bytes = len(lam_q) * N * N * (8 + 4 + 4)
     roofline = {"bound": "mfma", "achieved": achieved / 1e12 if achieved else None, "peak": PEAK_FP32_MFMA / 1e12,
-                "unit": "TFLOP/s", "frac": (achieved / PEAK_FP32_MFMA) if achieved else None, "traffic": None,
-                "kernel": "k_prune_mfma<float,*,true> (main FP32 prune phase, all size classes)",
-                "algorithmic_flops_per_launch": flops, "avg_ms": prune_ms,
+                "unit": "TFLOP/s", "frac": (achieved / PEAK_FP32_MFMA) if achieved else None,
+                "traffic": traffic["bytes"] if traffic else None,
+                "kernel": f"k_prune_mfma<float,*,*,true,*> main-pass launch of the {dom_nt}-tile size class "
+                          f"(buckets B={dom_buckets}, {int(tiles_cls[dom])} tiles of 16 families)",
+                "algorithmic_flops_per_launch": dom_flops, "avg_ms": dom_ms,
                 "flops_per_family_likelihood": "2 L (n1 L + n2 R), n1=%d n2=%d" % (n1, n2),
+                "traffic_source": traffic["source"] if traffic else None,
+                "classes": [{"tiles_per_vector": int(a_), "tiles": int(b_), "ms": float(c_)}
+                            for a_, b_, c_ in zip(nt_cls, tiles_cls, ms_cls)],
+                "prune_phase": {"ms": ph32[1], "algorithmic_flops": phase_flops,
+                                "TFLOP_per_s": phase_flops / (ph32[1] * 1e-3) / 1e12 if ph32[1] > 0 else None},
                 "matrix_build": {"ms": ph32[0], "bytes_written": build_bytes,
                                  "GB_per_s": build_bytes / (ph32[0] * 1e-3) / 1e9 if ph32[0] > 0 else None,
                                  "hbm_frac": build_bytes / (ph32[0] * 1e-3) / PEAK_HBM if ph32[0] > 0 else None}}

@@ -16,8 +16,11 @@
 namespace cafe {
 hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const double* d_lg, int n,
                         int first_slot, int N, int Np, long long slot_elems, double* mt64, float* mt32,
-                        hipStream_t stream);
+                        float* lf32, hipStream_t stream);
 hipError_t launch_fill_zero(void* p, long long bytes, hipStream_t stream);
+hipError_t launch_fold(const double* mt64, const float* lf32, double* out64, float* out32, const double* err64,
+                       const float* err32, int n_dev, int n_rows, int N, int Np, long long slot_elems,
+                       int first_slot, int n, hipStream_t stream);
 hipError_t launch_prune_plain(int nq, int grid, const PruneParams& p, hipStream_t st);
 hipError_t launch_prune_mfma(bool fp64, const MfmaShape& s, int grid, const PruneParams& p, hipStream_t st);
 hipError_t launch_pack(const SegDesc* segs, int n_segs, const BucketDev* buckets, const int* perm,
@@ -109,14 +112,15 @@ struct cafe_gpu_ctx {
     // matrix pool
     int N = 0, Np = 0, capacity = 0;
     long long slot_elems = 0;
-    DevBuf mt64, mt32, key_lam, key_t;
+    DevBuf mt64, mt32, lf32, key_lam, key_t;
     std::vector<char> built;
 
     // tree program
     int n_nodes = 0, tree_leaves = 0, n_slots = 0, max_children = 2;
     std::vector<OpDev> ops;
     std::vector<ChildDev> kids;
-    DevBuf d_ops, d_kids;
+    DevBuf d_ops, d_kids, d_steps;
+    int n_steps = 0;
 
     // families
     int F = 0, n_leaves = 0;
@@ -124,9 +128,10 @@ struct cafe_gpu_ctx {
     std::vector<BucketHost> buckets;
     DevBuf d_counts, d_perm;
 
-    // error model
+    // error model, and the pools with it folded into every matrix (leaf
+    // branches read those)
     int n_dev = 0, err_rows = 0;
-    DevBuf d_err64, d_err32;
+    DevBuf d_err64, d_err32, mt64e, mt32e;
 
     // buffers of eager evaluations (plans own their own)
     EvalBufs eager;
@@ -139,6 +144,27 @@ int check_ctx(cafe_gpu_ctx* ctx)
 {
     if (!ctx) return fail(CAFE_GPU_EINVAL, "null context");
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(CAFE_GPU_EDEVICE, "hipSetDevice failed");
+    return CAFE_GPU_OK;
+}
+
+// Pools with the error model folded in: sized with the matrix pool (outside
+// any graph capture).
+int ensure_fold_pools(cafe_gpu_ctx* ctx)
+{
+    if (!ctx->n_dev || !ctx->capacity) return CAFE_GPU_OK;
+    HIP_OR_FAIL(ctx->mt64e.ensure(sizeof(double) * ctx->slot_elems * ctx->capacity));
+    HIP_OR_FAIL(ctx->mt32e.ensure(sizeof(float) * ctx->slot_elems * ctx->capacity));
+    return CAFE_GPU_OK;
+}
+
+// Enqueues the error-model fold of slots first .. first+n-1 (nothing without
+// an error model).
+int enqueue_fold(cafe_gpu_ctx* ctx, int first, int n, hipStream_t st)
+{
+    if (!ctx->n_dev || n <= 0) return CAFE_GPU_OK;
+    HIP_OR_FAIL(launch_fold(ctx->mt64.as<double>(), ctx->lf32.as<float>(), ctx->mt64e.as<double>(),
+                            ctx->mt32e.as<float>(), ctx->d_err64.as<double>(), ctx->d_err32.as<float>(), ctx->n_dev,
+                            ctx->err_rows, ctx->N, ctx->Np, ctx->slot_elems, first, n, st));
     return CAFE_GPU_OK;
 }
 
@@ -257,8 +283,9 @@ int cafe_gpu_destroy(cafe_gpu_ctx* ctx)
     if (!ctx) return CAFE_GPU_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DevBuf* bufs[] = {&ctx->lg, &ctx->mt64, &ctx->mt32, &ctx->key_lam, &ctx->key_t, &ctx->d_ops,
-                      &ctx->d_kids, &ctx->d_counts, &ctx->d_perm, &ctx->d_err64, &ctx->d_err32};
+    DevBuf* bufs[] = {&ctx->lg, &ctx->mt64, &ctx->mt32, &ctx->lf32, &ctx->key_lam, &ctx->key_t, &ctx->d_ops,
+                      &ctx->d_kids, &ctx->d_steps, &ctx->d_counts, &ctx->d_perm, &ctx->d_err64,
+                      &ctx->d_err32, &ctx->mt64e, &ctx->mt32e};
     for (DevBuf* b : bufs) b->release();
     ctx->eager.release_all();
     for (auto& e : ctx->ev)
@@ -282,9 +309,13 @@ int cafe_gpu_matrix_reset(cafe_gpu_ctx* ctx, int N, int capacity)
     const size_t n32 = sizeof(float) * ctx->slot_elems * capacity;
     HIP_OR_FAIL(ctx->mt64.ensure(n64));
     HIP_OR_FAIL(ctx->mt32.ensure(n32));
+    HIP_OR_FAIL(ctx->lf32.ensure(n32));
     // padding rows/columns must read as zero
     HIP_OR_FAIL(launch_fill_zero(ctx->mt64.p, (long long)n64, ctx->stream));
     HIP_OR_FAIL(launch_fill_zero(ctx->mt32.p, (long long)n32, ctx->stream));
+    HIP_OR_FAIL(launch_fill_zero(ctx->lf32.p, (long long)n32, ctx->stream));
+    if (int rc = ensure_fold_pools(ctx)) return rc;
+    if (int rc = enqueue_fold(ctx, 0, capacity, ctx->stream)) return rc;
     HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
     ctx->built.assign(capacity, 0);
     return CAFE_GPU_OK;
@@ -312,7 +343,8 @@ int cafe_gpu_matrix_build(cafe_gpu_ctx* ctx, int n, const double* lambda_q, cons
     HIP_OR_FAIL(hipEventRecord(ctx->ev[4], ctx->stream));
     HIP_OR_FAIL(launch_build(algo, ctx->key_lam.as<double>(), ctx->key_t.as<double>(), ctx->lg.as<double>(), n,
                              first_slot, ctx->N, ctx->Np, ctx->slot_elems, ctx->mt64.as<double>(),
-                             ctx->mt32.as<float>(), ctx->stream));
+                             ctx->mt32.as<float>(), ctx->lf32.as<float>(), ctx->stream));
+    if (int rc = enqueue_fold(ctx, first_slot, n, ctx->stream)) return rc;
     HIP_OR_FAIL(hipEventRecord(ctx->ev[5], ctx->stream));
     HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
     float ms = 0;
@@ -389,6 +421,22 @@ int cafe_gpu_set_tree(cafe_gpu_ctx* ctx, int n_nodes, const int* parent, const i
     HIP_OR_FAIL(hipMemcpy(ctx->d_ops.p, ctx->ops.data(), sizeof(OpDev) * ctx->ops.size(), hipMemcpyHostToDevice));
     HIP_OR_FAIL(hipMemcpy(ctx->d_kids.p, ctx->kids.data(), sizeof(ChildDev) * ctx->kids.size(),
                           hipMemcpyHostToDevice));
+    // the same program as a flat list of child steps
+    std::vector<StepDev> steps;
+    for (size_t o = 0; o < ctx->ops.size(); ++o) {
+        const OpDev& op = ctx->ops[o];
+        for (int k = 0; k < op.n_child; ++k) {
+            const ChildDev& ch = ctx->kids[op.child_begin + k];
+            int ctl = op.out_slot;
+            if (k == 0) ctl |= kStepFirst;
+            if (k == op.n_child - 1) ctl |= kStepLast;
+            if (o + 1 == ctx->ops.size()) ctl |= kStepRoot;
+            steps.push_back(StepDev{ch.node, ch.leaf_col, ch.in_slot, ctl});
+        }
+    }
+    ctx->n_steps = (int)steps.size();
+    HIP_OR_FAIL(ctx->d_steps.ensure(sizeof(StepDev) * steps.size()));
+    HIP_OR_FAIL(hipMemcpy(ctx->d_steps.p, steps.data(), sizeof(StepDev) * steps.size(), hipMemcpyHostToDevice));
     return CAFE_GPU_OK;
 }
 
@@ -470,6 +518,10 @@ int cafe_gpu_set_error_model(cafe_gpu_ctx* ctx, int n_dev, int n_rows, const dou
     HIP_OR_FAIL(hipMemcpy(ctx->d_err32.p, r32.data(), sizeof(float) * n, hipMemcpyHostToDevice));
     ctx->n_dev = n_dev;
     ctx->err_rows = n_rows;
+    // fold into every slot of an existing pool (later builds fold their own slots)
+    if (int rc = ensure_fold_pools(ctx)) return rc;
+    if (int rc = enqueue_fold(ctx, 0, ctx->capacity, ctx->stream)) return rc;
+    HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
     return CAFE_GPU_OK;
 }
 
@@ -503,10 +555,13 @@ struct PrunePlan {
     // fb_count layout: [0, n) FP64 recompute tiles per class, [n, 2n) PLAIN64
     // tiles per class, [2n] / [2n+1] families recomputed in each pass
     int fam_count_off() const { return 2 * (int)classes.size(); }
+    // workgroups of a launch: resident waves per CU over the machine, no more
+    // than the tiles, and a multiple of 8 (the MFMA kernels give each XCD a
+    // contiguous share of the tiles) when there are at least 8
     int grid(int w, int max_tiles) const
     {
-        const int g = std::max(1, n_cu * waves_per_cu / std::max(1, w));
-        return std::max(1, std::min(g, max_tiles));
+        const int g = std::max(1, std::min(n_cu * waves_per_cu / std::max(1, w), max_tiles));
+        return g >= 8 ? g / 8 * 8 : g;
     }
 };
 
@@ -528,9 +583,9 @@ MfmaShape mfma_shape(int nt, bool fp64)
 
 // Partial vectors go to LDS when the workgroup's share stays small enough for
 // several workgroups per CU; the rest use the global scratch.
-void place_partials(MfmaShape& s, bool fp64, int n_slots, int slot_tiles)
+void place_partials(MfmaShape& s, bool fp64, int n_slots, int slot_tiles, int n_leaves)
 {
-    const size_t meta = (size_t)mfma_meta_bytes(n_slots, s.w, !fp64);
+    const size_t meta = (size_t)mfma_meta_bytes(n_slots, s.w, !fp64) + (size_t)mfma_count_bytes(n_leaves);
     const size_t frag = (size_t)n_slots * slot_tiles * 64 * 4 * (fp64 ? 8 : 4);
     s.lds = s.w <= 4 && meta + frag + 4096 <= 64 * 1024 && (meta + frag) / s.w <= 20 * 1024;
     s.shm = meta + (s.lds ? frag : 0);
@@ -599,7 +654,7 @@ int upload_eval_inputs(cafe_gpu_ctx* ctx, EvalBufs& b, const cafe_gpu_eval_desc*
         bd[i].R = bh.R;
         bd[i].L = bh.B + 1;
         bd[i].Lp = round_up(bh.B + 1, kTile);
-        bd[i].Rp = round_up(bh.R, kTile);
+        bd[i].Rp = round_up(bh.R + 1, kTile);
         bd[i].fam_begin = bh.fam_begin;
         bd[i].count = bh.count;
         bd[i].prior_off = i * d->prior_stride;
@@ -641,15 +696,15 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     pp.n_cu = ctx->n_cu;
     int maxLR = 0;
     for (auto& bk : ctx->buckets)
-        maxLR = std::max(maxLR, std::max(round_up(bk.B + 1, kTile), round_up(bk.R, kTile)));
+        maxLR = std::max(maxLR, std::max(round_up(bk.B + 1, kTile), round_up(bk.R + 1, kTile)));
     pp.maxLR = maxLR;
     auto nq_of = [](const BucketHost& bk) {
         int c = 1;
         while (c * 64 < std::max(bk.B + 1, bk.R)) c <<= 1;
         return c;
     };
-    auto nt_of = [](const BucketHost& bk) {
-        return std::max(round_up(bk.B + 1, kTile), round_up(bk.R, kTile)) / kTile;
+    auto nt_of = [](const BucketHost& bk) {   // root rows run 0..R
+        return std::max(round_up(bk.B + 1, kTile), round_up(bk.R + 1, kTile)) / kTile;
     };
     std::vector<std::pair<int, int>> bucket_class;   // (class key, bucket)
     for (int bi = 0; bi < (int)ctx->buckets.size(); ++bi) {
@@ -697,8 +752,8 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     for (auto& kc : pp.classes) {
         kc.s32 = mfma_shape(kc.nt, false);
         kc.s64 = mfma_shape(kc.nt, true);
-        place_partials(kc.s32, false, ctx->n_slots, kc.slot_tiles);
-        place_partials(kc.s64, true, ctx->n_slots, kc.slot_tiles);
+        place_partials(kc.s32, false, ctx->n_slots, kc.slot_tiles, ctx->n_leaves);
+        place_partials(kc.s64, true, ctx->n_slots, kc.slot_tiles, ctx->n_leaves);
         const int n = kc.end - kc.begin;
         if (path == CAFE_GPU_PATH_BATCHED32)
             scratch = std::max(scratch, mfma_scratch_bytes(kc.s32, false, ctx->n_slots, kc.slot_tiles,
@@ -749,6 +804,9 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     p.ops = ctx->d_ops.as<OpDev>();
     p.n_ops = (int)ctx->ops.size();
     p.kids = ctx->d_kids.as<ChildDev>();
+    p.steps = ctx->d_steps.as<StepDev>();
+    p.n_steps = ctx->n_steps;
+    p.n_leaves = ctx->n_leaves;
     p.n_nodes = ctx->n_nodes;
     p.node_slot = b.node_slot.as<int>();
     p.counts_T = ctx->d_counts.as<int>();
@@ -758,8 +816,8 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     p.mt32 = ctx->mt32.as<float>();
     p.slot_elems = ctx->slot_elems;
     p.Np = ctx->Np;
-    p.err64 = ctx->n_dev ? ctx->d_err64.as<double>() : nullptr;
-    p.err32 = ctx->n_dev ? ctx->d_err32.as<float>() : nullptr;
+    p.leaf64 = ctx->n_dev ? ctx->mt64e.as<double>() : ctx->mt64.as<double>();
+    p.leaf32 = ctx->n_dev ? ctx->mt32e.as<float>() : ctx->lf32.as<float>();
     p.n_dev = ctx->n_dev;
     p.prior64 = b.prior64.as<double>();
     p.prior32 = b.prior32.as<float>();
@@ -773,6 +831,7 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     p.scratch = b.scratch.p;
     p.ldv = ldv;
     p.n_slots = ctx->n_slots;
+    p.zero = 0;
     return CAFE_GPU_OK;
 }
 
@@ -780,7 +839,8 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
 // (device-side lists, no host round trip) the FP64 recompute of the FP32
 // families the flush bound did not certify and the PLAIN64 recompute of
 // families near the subnormal range.
-int enqueue_prune(EvalBufs& b, const PrunePlan& pp, hipStream_t st, hipEvent_t ev_main, hipEvent_t ev_fb)
+int enqueue_prune(EvalBufs& b, const PrunePlan& pp, hipStream_t st, hipEvent_t ev_main, hipEvent_t ev_fb,
+                  const std::vector<hipEvent_t>* cls_ev = nullptr)
 {
     const int n_cls = (int)pp.classes.size();
     int* d_cnt = b.ntiles.as<int>();
@@ -791,6 +851,7 @@ int enqueue_prune(EvalBufs& b, const PrunePlan& pp, hipStream_t st, hipEvent_t e
     for (int ci = 0; ci < n_cls; ++ci) {
         const KernelClass& kc = pp.classes[ci];
         const int n = pp.class_counts[ci];
+        if (cls_ev) HIP_OR_FAIL(hipEventRecord((*cls_ev)[ci], st));
         PruneParams q = pp.base;
         q.tiles = b.tiles.as<TileDesc>() + kc.begin;
         q.n_tiles_dev = d_cnt + ci;
@@ -805,6 +866,7 @@ int enqueue_prune(EvalBufs& b, const PrunePlan& pp, hipStream_t st, hipEvent_t e
             HIP_OR_FAIL(launch_prune_mfma(false, kc.s32, pp.grid(kc.s32.w, n), q, st));
         }
     }
+    if (cls_ev) HIP_OR_FAIL(hipEventRecord((*cls_ev)[n_cls], st));
     if (ev_main) HIP_OR_FAIL(hipEventRecord(ev_main, st));
     TileDesc* fb64 = b.fb_tiles.as<TileDesc>();
     TileDesc* fbplain = fb64 + std::max(1, pp.n_tiles);
@@ -891,6 +953,8 @@ struct cafe_gpu_plan {
     hipGraphExec_t exec = nullptr;
     bool graph_ok = false;
     hipEvent_t ev[5] = {};
+    std::vector<hipEvent_t> cls_ev;   // around each main-pass class launch (profile runs)
+    std::vector<double> cls_ms;       // their times in the last profile run
 };
 
 namespace {
@@ -904,9 +968,12 @@ int plan_enqueue(cafe_gpu_plan* pl, bool timed)
     hipStream_t st = ctx->stream;
     if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[0], st));
     HIP_OR_FAIL(launch_build(pl->algo, pl->d_lam.as<double>(), pl->d_t.as<double>(), ctx->lg.as<double>(), pl->n_keys,
-                             0, ctx->N, ctx->Np, ctx->slot_elems, ctx->mt64.as<double>(), ctx->mt32.as<float>(), st));
+                             0, ctx->N, ctx->Np, ctx->slot_elems, ctx->mt64.as<double>(), ctx->mt32.as<float>(),
+                             ctx->lf32.as<float>(), st));
+    if (int rc = enqueue_fold(ctx, 0, pl->n_keys, st)) return rc;
     if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[1], st));
-    if (int rc = enqueue_prune(pl->bufs, pl->pp, st, timed ? pl->ev[2] : nullptr, timed ? pl->ev[3] : nullptr))
+    if (int rc = enqueue_prune(pl->bufs, pl->pp, st, timed ? pl->ev[2] : nullptr, timed ? pl->ev[3] : nullptr,
+                               timed ? &pl->cls_ev : nullptr))
         return rc;
     if (int rc = enqueue_finalize(ctx, pl->bufs, pl->model, pl->T, pl->K, st)) return rc;
     if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[4], st));
@@ -1037,6 +1104,8 @@ int cafe_gpu_plan_create(cafe_gpu_ctx* ctx, const cafe_gpu_eval_desc* d, int n_k
     pl->algo = algo;
     pl->pool_gen = ctx->pool_gen;
     for (auto& e : pl->ev) (void)hipEventCreate(&e);
+    pl->cls_ev.assign(pl->pp.classes.size() + 1, nullptr);
+    for (auto& e : pl->cls_ev) (void)hipEventCreate(&e);
     *out = pl;
     if ((rc = cafe_gpu_plan_set_keys(pl, lambda_q, t_q)) != CAFE_GPU_OK) {
         cafe_gpu_plan_destroy(pl);
@@ -1099,6 +1168,23 @@ int cafe_gpu_plan_profile(cafe_gpu_plan* pl, double* ms4, int* graph_ok)
     ms4[1] = elapsed_ms(pl->ev[1], pl->ev[2]);
     ms4[2] = elapsed_ms(pl->ev[2], pl->ev[3]);
     ms4[3] = elapsed_ms(pl->ev[3], pl->ev[4]);
+    pl->cls_ms.assign(pl->pp.classes.size(), 0.0);
+    for (size_t i = 0; i < pl->pp.classes.size(); ++i) pl->cls_ms[i] = elapsed_ms(pl->cls_ev[i], pl->cls_ev[i + 1]);
+    return CAFE_GPU_OK;
+}
+
+int cafe_gpu_plan_class_times(cafe_gpu_plan* pl, int cap, int* n_classes, int* class_nt, long long* class_tiles,
+                              double* ms)
+{
+    if (!pl || !n_classes) return fail(CAFE_GPU_EINVAL, "null argument");
+    const int n = (int)pl->pp.classes.size();
+    *n_classes = n;
+    if (pl->cls_ms.size() != (size_t)n) return fail(CAFE_GPU_ESTATE, "cafe_gpu_plan_profile has not been run");
+    for (int i = 0; i < n && i < cap; ++i) {
+        if (class_nt) class_nt[i] = pl->pp.classes[i].nt;
+        if (class_tiles) class_tiles[i] = pl->pp.class_counts[i];
+        if (ms) ms[i] = pl->cls_ms[i];
+    }
     return CAFE_GPU_OK;
 }
 
@@ -1123,6 +1209,8 @@ int cafe_gpu_plan_destroy(cafe_gpu_plan* pl)
     if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
     if (pl->graph) (void)hipGraphDestroy(pl->graph);
     for (auto& e : pl->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : pl->cls_ev)
         if (e) (void)hipEventDestroy(e);
     pl->bufs.release_all();
     pl->d_lam.release();

@@ -35,7 +35,8 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
                                                           const double* __restrict__ t_q,
                                                           int first_slot, int N, int Np,
                                                           long long slot_elems,
-                                                          double* __restrict__ mt64)
+                                                          double* __restrict__ mt64,
+                                                          float* __restrict__ lf32)
 {
     const int key = blockIdx.x;
     const int lane = threadIdx.x;
@@ -43,12 +44,17 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
     const double alpha = lt / (1 + lt);
     const double coeff = 1 - 2 * alpha;
     double* M64 = mt64 + (long long)(first_slot + key) * slot_elems;
+    float* L32 = lf32 ? lf32 + (long long)(first_slot + key) * slot_elems : nullptr;
 
     if (degenerate_key(alpha, coeff)) {
         for (int c = 0; c < N; ++c)
             for (int b = 0; b < NB; ++b) {
                 const int s = lane + 64 * b;
-                if (s < N) M64[(long long)c * Np + s] = (s == 0 && c == 0) ? 1.0 : 0.0;
+                if (s < N) {
+                    const double v = (s == 0 && c == 0) ? 1.0 : 0.0;
+                    M64[(long long)c * Np + s] = v;
+                    if (L32) L32[(long long)c * Np + s] = (float)ldexp(v, kMatExp);
+                }
             }
         return;
     }
@@ -98,7 +104,11 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
             carry = __shfl(y, 63);
             prev[b] = y;
             const int s = lane + 64 * b;
-            if (s < N) M64[(long long)c * Np + s] = y < 1.0 ? y : 1.0;   // [0,1] clamp, probability.cpp:147
+            if (s < N) {
+                const double v = y < 1.0 ? y : 1.0;   // [0,1] clamp, probability.cpp:147
+                M64[(long long)c * Np + s] = v;
+                if (L32) L32[(long long)c * Np + s] = (float)ldexp(v, kMatExp);
+            }
         }
     }
 }
@@ -207,7 +217,8 @@ __global__ __launch_bounds__(256) void k_build_closed_form(const double* __restr
                                                            int first_slot, int N, int Np,
                                                            long long slot_elems,
                                                            double* __restrict__ mt64,
-                                                           float* __restrict__ mt32)
+                                                           float* __restrict__ mt32,
+                                                           float* __restrict__ lf32)
 {
     const int key = blockIdx.z;
     const int c = blockIdx.y;
@@ -237,6 +248,59 @@ __global__ __launch_bounds__(256) void k_build_closed_form(const double* __restr
     double* M64 = mt64 + (long long)(first_slot + key) * slot_elems;
     M64[(long long)c * Np + s] = v;
     if (mt32) mt32[(long long)(first_slot + key) * slot_elems + (long long)s * Np + c] = (float)ldexp(v, kMatExp);
+    if (lf32) lf32[(long long)(first_slot + key) * slot_elems + (long long)c * Np + s] = (float)ldexp(v, kMatExp);
+}
+
+// ---------------------------------------------------------------------------
+// Leaf error model folded into the matrices.  A leaf with observed count k
+// contributes sum_d w_d(k) M[s][k-h+d] (probability.cpp:187-198 builds the
+// leaf vector, matrix::multiply sums it in ascending column order, and the
+// columns outside k-h..k+h add exact zeros), so column k of the folded matrix
+//     M'[s][k] = sum_{d, k-h+d >= 0} w_d(k) M[s][k-h+d]      (ascending d)
+// is that leaf factor.  FP64: c-major, one block per (key, column), separately
+// rounded products and sums in the reference's order; FP32: the same from the
+// c-major FP32 copy.  Columns without an error-model row are zero.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fold_err64(const double* __restrict__ mt64, double* __restrict__ out,
+                                                    const double* __restrict__ err, int n_dev, int n_rows, int N,
+                                                    int Np, long long slot_elems, int first_slot)
+{
+    const int c = blockIdx.x;
+    const long long base = (long long)(first_slot + blockIdx.y) * slot_elems;
+    const double* M = mt64 + base;
+    double* O = out + base;
+    const int h = (n_dev - 1) / 2;
+    for (int s = threadIdx.x; s < Np; s += blockDim.x) {
+        double x = 0.0;
+        if (c < n_rows && c < N)
+            for (int d = 0; d < n_dev; ++d) {
+                const int cc = c - h + d;
+                if (cc < 0 || cc >= N) continue;
+                x = __dadd_rn(x, __dmul_rn(M[(long long)cc * Np + s], err[(long long)c * n_dev + d]));
+            }
+        O[(long long)c * Np + s] = x;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fold_err32(const float* __restrict__ lf32, float* __restrict__ out,
+                                                    const float* __restrict__ err, int n_dev, int n_rows, int N,
+                                                    int Np, long long slot_elems, int first_slot)
+{
+    const int c = blockIdx.x;
+    const long long base = (long long)(first_slot + blockIdx.y) * slot_elems;
+    const float* M = lf32 + base;
+    float* O = out + base;
+    const int h = (n_dev - 1) / 2;
+    for (int s = threadIdx.x; s < Np; s += blockDim.x) {
+        float x = 0.f;
+        if (c < n_rows && c < N)
+            for (int d = 0; d < n_dev; ++d) {
+                const int cc = c - h + d;
+                if (cc < 0 || cc >= N) continue;
+                x += M[(long long)cc * Np + s] * err[(long long)c * n_dev + d];
+            }
+        O[(long long)c * Np + s] = x;
+    }
 }
 
 // Zero-fill (used instead of hipMemsetAsync so the calls can be captured).
@@ -254,7 +318,7 @@ namespace cafe {
 
 hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const double* d_lg,
                         int n, int first_slot, int N, int Np, long long slot_elems, double* mt64,
-                        float* mt32, hipStream_t stream)
+                        float* mt32, float* lf32, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
     if (algo == 0) {
@@ -263,14 +327,14 @@ hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const 
             const int nk = n - k0 < 65535 ? n - k0 : 65535;
             dim3 grid((N + 255) / 256, N, nk);
             hipLaunchKernelGGL(k_build_closed_form, grid, dim3(256), 0, stream, d_lam + k0, d_t + k0,
-                               d_lg, first_slot + k0, N, Np, slot_elems, mt64, mt32);
+                               d_lg, first_slot + k0, N, Np, slot_elems, mt64, mt32, lf32);
         }
     } else {
         const int nb = (N + 63) / 64;
 #define CAFE_BUILD_NB(NB)                                                                                 \
     if (nb <= NB) {                                                                                       \
         hipLaunchKernelGGL(k_build_recurrence<NB>, dim3(n), dim3(64), 0, stream, d_lam, d_t, first_slot, N, \
-                           Np, slot_elems, mt64);                                                         \
+                           Np, slot_elems, mt64, lf32);                                                   \
         if (mt32)                                                                                         \
             hipLaunchKernelGGL(k_build_recurrence_rows<NB>, dim3(n), dim3(64), 0, stream, d_lam, d_t,      \
                                first_slot, N, Np, slot_elems, mt32);                                     \
@@ -284,6 +348,22 @@ hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const 
         CAFE_BUILD_NB(32)
 #undef CAFE_BUILD_NB
         return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Folds the error model into slots first_slot .. first_slot+n-1 of the FP64
+// and c-major FP32 pools (every column; padding stays zero).
+hipError_t launch_fold(const double* mt64, const float* lf32, double* out64, float* out32, const double* err64,
+                       const float* err32, int n_dev, int n_rows, int N, int Np, long long slot_elems,
+                       int first_slot, int n, hipStream_t stream)
+{
+    for (int k0 = 0; k0 < n; k0 += 65535) {
+        const int nk = n - k0 < 65535 ? n - k0 : 65535;
+        hipLaunchKernelGGL(k_fold_err64, dim3(Np, nk), dim3(256), 0, stream, mt64, out64, err64, n_dev, n_rows, N,
+                           Np, slot_elems, first_slot + k0);
+        hipLaunchKernelGGL(k_fold_err32, dim3(Np, nk), dim3(256), 0, stream, lf32, out32, err32, n_dev, n_rows, N,
+                           Np, slot_elems, first_slot + k0);
     }
     return hipGetLastError();
 }

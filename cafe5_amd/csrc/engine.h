@@ -9,11 +9,12 @@ namespace cafe {
 
 // Matrix pools: element (s, c) = P(s -> c) of slot k is, in the FP64 pool,
 // at mt64[k * slot_elems + c * Np + s] ("c-major", so a wave reading rows
-// s = lane.. of one column is coalesced) and, in the FP32 pool, at
+// s = lane.. of one column is coalesced); in the FP32 pool at
 // mt32[k * slot_elems + s * Np + c] (row-major, so one 16-byte load gives an
-// MFMA lane four consecutive contraction entries).  Np = round_up(N, 16) + 16
-// leaves room for 16-row tiles and the one-row shift of the root contraction;
-// padding is 0.
+// MFMA lane four consecutive contraction entries); and in the FP32 leaf pool
+// at lf32[k * slot_elems + c * Np + s] (c-major: one 16-byte load gives a lane
+// four consecutive rows of a leaf column).  Np = round_up(N, 16) + 16 leaves
+// room for 16-row tiles; padding is 0.
 constexpr int kTile = 16;
 constexpr int kMaxSlots = 24;         // live partial vectors per family tile
 constexpr int kLgammaN = 4096;        // log-gamma table (integers < 4096)
@@ -46,7 +47,7 @@ struct SegDesc {
 struct BucketDev {
     int B, R;          // max_family_size, max_root_family_size
     int L, Lp;         // B+1 and its 16-multiple
-    int Rp;            // R rounded up to 16
+    int Rp;            // root rows 0..R (size 0 unused) rounded up to 16
     int fam_begin;     // first family (bucket-sorted order)
     int count;
     int prior_off;     // offset of this bucket's prior in the prior tables
@@ -67,6 +68,21 @@ struct ChildDev {
     int pad;
 };
 
+// One child contraction of the evaluation program, flattened in op order
+// (the MFMA kernel walks these and prefetches the next one's operands).
+struct StepDev {
+    int node;       // child node (branch matrix node_slot[point][node])
+    int leaf_col;   // >= 0: leaf column; -1: interior child
+    int in_slot;    // interior: slot holding its partial vector
+    int ctl;        // out_slot | kStepFirst | kStepLast | kStepRoot
+};
+enum : int {
+    kStepFirst = 1 << 8,   // first child of its op: the product starts at 1
+    kStepLast = 1 << 9,    // last child: the op's vector is complete
+    kStepRoot = 1 << 10,   // the op is the root
+    kStepSlotMask = 0xff
+};
+
 // Output flags per (point, family).
 enum : int {
     kFlagZeroSum = 1,     // root vector sums to 0 (gamma_core.cpp:151-152)
@@ -84,21 +100,25 @@ struct PruneParams {
     const OpDev* ops;
     int n_ops;
     const ChildDev* kids;
+    const StepDev* steps;     // the same program as child steps
+    int n_steps;
+    int n_leaves;
     int n_nodes;
     const int* node_slot;     // [point][n_nodes] matrix slot per branch
     // families
     const int* counts_T;      // [leaf][F] counts, bucket-sorted family order
     int F;                    // number of families
     const int* perm;          // bucket-sorted index -> caller's family index
-    // matrices
+    // matrices; leaf branches read leaf64 / leaf32 (both c-major), which are
+    // the matrices with the leaf error model folded in (without one: mt64 and
+    // the c-major FP32 copy)
     const double* mt64;
     const float* mt32;
+    const double* leaf64;
+    const float* leaf32;
     long long slot_elems;     // Np * Np
     int Np;
-    // error model (n_dev == 0: none); rows[k * n_dev + i]
-    const double* err64;
-    const float* err32;
-    int n_dev;
+    int n_dev;                // error-model deviations (0: none); folded into leaf64/leaf32
     // root prior per bucket: prior[bucket.prior_off + j]
     const double* prior64;
     const float* prior32;
@@ -118,6 +138,7 @@ struct PruneParams {
     int ldv;
     int n_slots;
     int slot_tiles;           // MFMA: 16-row tiles per stored partial vector
+    int zero;                 // always 0 (an offset the compiler cannot fold)
     int plain_lds;            // PLAIN64: partial vectors in dynamic LDS
 };
 
@@ -132,13 +153,20 @@ struct MfmaShape {
 };
 
 // Dynamic-LDS bookkeeping of the scaled FP32 kernel for n_slots partial
-// vectors and w waves: per slot a flush bound (double) and exponent (int) per
-// family, per-wave maxima (float) per family, and one nominal-scale exponent.
+// vectors and w waves: per slot a flush bound (double), an exponent and a
+// renormalisation shift (int) per family, per-wave maxima (float) per family,
+// and one nominal-scale exponent.
 __host__ __device__ inline int mfma_meta_bytes(int n_slots, int w, bool scaled)
 {
     if (!scaled) return 0;
-    const int b = n_slots * 16 * 8 + n_slots * w * 16 * 4 + n_slots * 16 * 4 + n_slots * 4;
+    const int b = n_slots * 16 * 8 + n_slots * w * 16 * 4 + 2 * n_slots * 16 * 4 + n_slots * 4;
     return (b + 31) & ~31;
+}
+
+// Dynamic-LDS copy of a tile's leaf counts: n_leaves x 16 ints.
+__host__ __device__ inline int mfma_count_bytes(int n_leaves)
+{
+    return (n_leaves * 16 * 4 + 31) & ~31;
 }
 
 }  // namespace cafe

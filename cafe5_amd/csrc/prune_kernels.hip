@@ -70,25 +70,15 @@ __global__ __launch_bounds__(64) void k_prune_plain(PruneParams p)
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) x[q] = 0.0;
                     if (ch.leaf_col >= 0) {
+                        // column count of the leaf matrix (error model folded in:
+                        // the same sum in the same order as the leaf vector's
+                        // matrix-vector product)
                         const int cnt = p.counts_T[(long long)ch.leaf_col * p.F + fs];
-                        if (p.n_dev == 0) {
+                        const double* __restrict__ Ml = p.leaf64 + (long long)nslot[ch.node] * p.slot_elems + shift;
 #pragma unroll
-                            for (int q = 0; q < NQ; ++q) {
-                                const int s = lane + 64 * q;
-                                if (s < len) x[q] = M[(long long)cnt * Np + s];
-                            }
-                        } else {
-                            const int h = (p.n_dev - 1) / 2;
-                            for (int i = 0; i < p.n_dev; ++i) {
-                                const int cc = cnt - h + i;
-                                if (cc < 0) continue;
-                                const double w = p.err64[(long long)cnt * p.n_dev + i];
-#pragma unroll
-                                for (int q = 0; q < NQ; ++q) {
-                                    const int s = lane + 64 * q;
-                                    if (s < len) x[q] = __dadd_rn(x[q], __dmul_rn(M[(long long)cc * Np + s], w));
-                                }
-                            }
+                        for (int q = 0; q < NQ; ++q) {
+                            const int s = lane + 64 * q;
+                            if (s < len) x[q] = Ml[(long long)cnt * Np + s];
                         }
                     } else {
                         const double* V = slots + (long long)ch.in_slot * ldv;

@@ -21,9 +21,9 @@
 // g + 4 r in register r; the FP64 kernel feeds A row m from size
 // 4 (m % 4) + m / 4, which lands size 4 g + r in that register as well.
 //
-// Persistent kernels: a fixed grid pulls tiles from a device counter until the
-// device-resident tile count is reached; every wave of a workgroup sees the
-// same tile index (broadcast through LDS), so all waves leave together.
+// Persistent kernels: a fixed grid walks the class's tile list (its length is
+// device-resident, so recompute lists built on the device need no host round
+// trip); every wave of a workgroup works on the same tile.
 #include "engine.h"
 
 namespace cafe {
@@ -46,8 +46,41 @@ __device__ __forceinline__ doublex4 mfma16(double a, double b, doublex4 c)
 
 __device__ __forceinline__ const float* pool_of(const PruneParams& p, float*) { return p.mt32; }
 __device__ __forceinline__ const double* pool_of(const PruneParams& p, double*) { return p.mt64; }
-__device__ __forceinline__ const float* err_of(const PruneParams& p, float*) { return p.err32; }
-__device__ __forceinline__ const double* err_of(const PruneParams& p, double*) { return p.err64; }
+__device__ __forceinline__ const float* leaf_of(const PruneParams& p, float*) { return p.leaf32; }
+__device__ __forceinline__ const double* leaf_of(const PruneParams& p, double*) { return p.leaf64; }
+
+// Waits for this wave's LDS traffic and meets the workgroup at a barrier,
+// leaving global loads in flight (the next step's prefetched operands).
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Read-only tables of a launch (program steps, tiles, buckets, node -> matrix
+// slots) read through the constant address space: uniform loads of them go
+// through the scalar cache and do not count against the vector-memory
+// counter that the prefetched operands are waited on.  The kernels never
+// write them; earlier kernels that do (the recompute-list packing) have
+// finished before a launch starts.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(const T* q)
+{
+    return (const __attribute__((address_space(4))) T*)q;
+}
+
+// Element i of a read-only table of 16-byte-multiple records, via as_const.
+template <typename T>
+__device__ __forceinline__ T ld_const(const T* q, long long i)
+{
+    typedef int int4v __attribute__((ext_vector_type(4)));
+    static_assert(sizeof(T) % sizeof(int4v) == 0, "records are whole 16-byte units");
+    constexpr int n = sizeof(T) / sizeof(int4v);
+    const __attribute__((address_space(4))) int4v* c = as_const((const int4v*)(q + i));
+    struct Words { int4v w[n]; } r;
+#pragma unroll
+    for (int k = 0; k < n; ++k) r.w[k] = c[k];
+    return __builtin_bit_cast(T, r);
+}
 
 // Size that A row m of a 16-row tile carries (see the fragment layout above).
 __device__ __forceinline__ int a_row(float*, int m) { return m; }
@@ -73,6 +106,10 @@ __device__ __forceinline__ doublex4 load_a(const double* M, int off, int kt, int
     return doublex4{M[o], M[o + Np], M[o + 2 * Np], M[o + 3 * Np]};
 }
 
+// Operand prefetch depth: FP32 prefetches k-tiles 0 and 1 of an internal
+// step (two 16-byte loads per output tile), FP64 k-tile 0 (four 8-byte loads).
+template <typename T> struct Prefetch { static constexpr int PK = sizeof(T) == 4 ? 2 : 1; };
+
 constexpr double kLn2 = 0.69314718055994530942;
 // FP32 flush threshold (smallest normal float): the flush bound charges this
 // for every quantity FP32 could have flushed to zero.
@@ -83,6 +120,13 @@ constexpr double kCertEta = 1e-9;
 // below this, FP64 values may lose bits to subnormals; such families are
 // recomputed in the reference's summation order (PLAIN64)
 constexpr double kFp64Floor = 1e-290;
+
+// 1/mant rounded up (mant in [0.5, 1)): the FP32 reciprocal is within one
+// ulp, and the factor covers that and the rounding of mant to FP32.
+__device__ __forceinline__ double rcp_up(float mant)
+{
+    return (double)__builtin_amdgcn_rcpf(mant) * (1.0 + 0x1p-20);
+}
 
 // Running root reduction of one lane: best key over its rows.
 struct RootBest {
@@ -138,25 +182,35 @@ __device__ __forceinline__ void mma_step(typename Vec4<T>::type (&x)[NTW], const
     }
 }
 
-// x[i] = sum_x M(srow_i, x) v(x) over the child's nts k-tiles, two k-tiles
-// per iteration in ping-pong buffers: the loads of one buffer are in flight
-// while the other is multiplied.  The body has no branches (a k-tile past the
-// end re-reads the last one with its B fragment zeroed, which adds exactly 0),
-// so each wait covers only the older buffer's loads.
+// x[i] = sum_x M(srow_i, x) v(x) over the child's nts k-tiles.  The first
+// PK k-tiles' A operands arrive prefetched (pf, loaded during the previous
+// step); the remaining ones (vectors of more than PK 16-row tiles) run two
+// k-tiles per iteration in ping-pong buffers, one buffer's loads in flight
+// while the other is multiplied.  Those loads are unconditional: a k-tile
+// past the end re-reads the last one with its B fragment zeroed, which adds
+// exactly 0, so each wait covers only older loads.
 template <typename T, int NTW, bool SCALED>
 __device__ __forceinline__ void child_gemm(typename Vec4<T>::type (&x)[NTW], const T* __restrict__ M, int Np,
-                                           const int (&srow)[NTW], int g, const typename Vec4<T>::type* bsrc,
-                                           int nts, int bsh)
+                                           const int (&ao)[NTW],
+                                           const typename Vec4<T>::type (&pf)[Prefetch<T>::PK][NTW],
+                                           const typename Vec4<T>::type* bsrc, int nts, int bsh)
 {
     typedef typename Vec4<T>::type V4;
-    int ao[NTW];
-#pragma unroll
-    for (int i = 0; i < NTW; ++i) ao[i] = a_off((T*)nullptr, Np, srow[i], g);
+    constexpr int PK = Prefetch<T>::PK;
+    {
+        const V4 b0 = bsrc[0];
+        mma_step<T, NTW, SCALED>(x, pf[0], b0, bsh);
+        if (PK > 1 && nts > 1) {
+            const V4 b1 = bsrc[64];
+            mma_step<T, NTW, SCALED>(x, pf[PK - 1], b1, bsh);
+        }
+    }
+    if (nts <= PK) return;
     V4 a0[NTW], a1[NTW], b0, b1;
 #pragma unroll
-    for (int i = 0; i < NTW; ++i) a0[i] = load_a(M, ao[i], 0, Np);
-    b0 = bsrc[0];
-    for (int kt = 0; kt < nts; kt += 2) {
+    for (int i = 0; i < NTW; ++i) a0[i] = load_a(M, ao[i], PK, Np);
+    b0 = bsrc[PK * 64];
+    for (int kt = PK; kt < nts; kt += 2) {
         const bool has1 = kt + 1 < nts;
         const int k1 = has1 ? kt + 1 : kt;
 #pragma unroll
@@ -173,267 +227,361 @@ __device__ __forceinline__ void child_gemm(typename Vec4<T>::type (&x)[NTW], con
 }
 
 // ---------------------------------------------------------------------------
-// The pruning kernel.  Wave wv computes output tiles t0 + wv + W i, i < NTW
-// (FP64 makes several passes when a vector has more than W NTW tiles; tiles
-// past the vector's end are computed on a clamped, in-bounds tile and
-// dropped).  Partial vectors are kept in fragment layout, in LDS (LDSP) or in
-// the workgroup's region of the global scratch; one barrier per node makes a
-// node's output visible to every wave.
+// The pruning kernel.  The evaluation program is walked as a list of child
+// steps (StepDev, op order); while step s is multiplied, the loads of step
+// s+1 -- its leaf column, or the first PK k-tiles of its A operand -- are in
+// flight in the other half of a ping-pong register buffer (both kinds issue
+// the same loads from per-lane offsets computed once per tile, so the waits
+// count the younger loads exactly).  Wave wv computes output tiles wv + W i,
+// i < NTW (W NTW covers the largest vector of the size class; tiles past a
+// vector's end are computed on a clamped, in-bounds tile and dropped).
+// Partial vectors are kept in fragment layout, in LDS (LDSP) or in the
+// workgroup's region of the global scratch; the leaf counts of the tile's
+// families are staged in LDS.
 //
 // T = float, SCALED: FP32 matrices are stored times 2^kMatExp; a node's raw
-// product is stored as computed, and its per-family maximum m (over all waves)
-// renormalises it when it is read: v' = raw 2^(a - e), e = exponent of m, so
-// max v' is just below 2^a (a = vec_exp).  The exponent is carried as an
-// integer, and a per-family bound on what FP32 flushing could have dropped
-// decides at the root whether the family is kept or recomputed in FP64.
+// product is stored as computed, and its per-family maximum m renormalises it
+// when it is read: v' = raw 2^(a - e), e = exponent of m, so max v' is just
+// below 2^a (a = vec_exp).  The exponent is carried as an integer, and a
+// per-family bound on what FP32 flushing could have dropped decides at the
+// root whether the family is kept or recomputed in FP64.  A single-wave
+// workgroup finishes a node's exponent, shift and bound when it stores the
+// node; with several waves the reader combines the per-wave maxima.
 // T = double: unscaled, so values (and underflow) follow the reference.
 // ---------------------------------------------------------------------------
+// Single-wave kernels with small per-wave tiles ask for 4 waves per SIMD (the
+// register budget that still holds both prefetch buffers without spilling).
 template <typename T, int NTW, int W, bool SCALED, bool LDSP>
-__global__ __launch_bounds__(64 * W) void k_prune_mfma(PruneParams p)
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu((W == 1 && NTW * sizeof(T) <= 8) ? 4 : 1)))
+void k_prune_mfma(PruneParams p)
 {
     typedef typename Vec4<T>::type V4;
+    constexpr int PK = Prefetch<T>::PK;
+    constexpr bool kF64 = sizeof(T) == 8;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int fl = lane & 15;
     const int g = lane >> 4;
     const int my_arow = a_row((T*)nullptr, fl);
     const T* __restrict__ pool = pool_of(p, (T*)nullptr);
-    const T* __restrict__ err = err_of(p, (T*)nullptr);
+    const T* __restrict__ lpool = leaf_of(p, (T*)nullptr);
     const int Np = p.Np;
     const int n_slots = p.n_slots;
     const int slot_tiles = p.slot_tiles;
     const int a_exp = p.vec_exp;
+    const int n_steps = p.n_steps;
+    const int zero = __builtin_amdgcn_readfirstlane(p.zero);   // 0, opaque to the compiler
+    // FP32 leaf factors: up to n_dev flushed terms (one without an error model)
+    const double leaf_cert = (p.n_dev > 1 ? p.n_dev : 1) * kTau32 * kTwoPowMinusMatExp;
+    const double two_m_a = ldexp(1.0, -a_exp);
 
-    // dynamic LDS: scaled-path bookkeeping per slot, then LDS partials
+    // dynamic LDS: scaled-path bookkeeping per slot, the tile's leaf counts,
+    // then LDS partials.  Per slot and family:
+    //   m_cert  W == 1: the stored vector's flush bound; W > 1: before renormalising
+    //   m_esum  W == 1: its exponent E (true = stored 2^(E + bsh)); W > 1: e_sum of the raw product
+    //   m_bsh   W == 1: its renormalisation shift a - e
+    //   m_wmax  W > 1: per-wave maxima;  m_l2s  W > 1: log2 of the nominal scale (per slot)
     extern __shared__ __align__(32) unsigned char s_dyn[];
-    double* m_cpre = (double*)s_dyn;                  // [slot][16] flush bound before renormalising
-    float* m_wmax = (float*)(m_cpre + n_slots * 16);  // [slot][W][16] per-wave maxima
-    int* m_esum = (int*)(m_wmax + n_slots * W * 16);  // [slot][16] exponent of the raw product
-    int* m_l2s = m_esum + n_slots * 16;               // [slot] log2 of the nominal scale
-    V4* frag = LDSP ? (V4*)(s_dyn + mfma_meta_bytes(n_slots, W, SCALED))
+    double* m_cert = (double*)s_dyn;
+    float* m_wmax = (float*)(m_cert + n_slots * 16);
+    int* m_esum = (int*)(m_wmax + n_slots * W * 16);
+    int* m_bsh = m_esum + n_slots * 16;
+    int* m_l2s = m_bsh + n_slots * 16;
+    const int meta = mfma_meta_bytes(n_slots, W, SCALED);
+    int* s_cnt = (int*)(s_dyn + meta);                // [leaf][16]
+    V4* frag = LDSP ? (V4*)(s_dyn + meta + mfma_count_bytes(p.n_leaves))
                     : (V4*)p.scratch + (long long)blockIdx.x * n_slots * slot_tiles * 64;
     __shared__ double s_rkey[W][16], s_rx[W][16], s_rxmax[W][16];
     __shared__ int s_rj[W][16];
-    __shared__ int s_tile;
 
-    const int n_tiles = __builtin_amdgcn_readfirstlane(*p.n_tiles_dev);
-    for (;;) {
-        if (threadIdx.x == 0) {
-            // a plain read first: once the list is drained, workgroups leave
-            // without queueing on the counter
-            int t = *(volatile int*)p.tile_counter;
-            s_tile = t < n_tiles ? atomicAdd(p.tile_counter, 1) : n_tiles;
-        }
-        __syncthreads();
-        const int tile = __builtin_amdgcn_readfirstlane(s_tile);
-        __syncthreads();   // s_tile is rewritten only after every wave has read it
-        if (tile >= n_tiles) break;
-        const TileDesc td = p.tiles[tile];
-        const BucketDev bk = p.buckets[td.bucket];
+    // Static, XCD-aware split of the class's tiles: workgroup b runs on XCD
+    // b % 8 (dispatch order), each XCD takes one contiguous range of the list
+    // (consecutive tiles share a parameter point, hence their matrices, in
+    // that XCD's L2) and its workgroups stride through it.  Tiles of one class
+    // cost the same, so no work counter is needed.
+    const int n_tiles = *as_const(p.n_tiles_dev);
+    const int groups = (gridDim.x % 8 == 0) ? 8 : 1;
+    const int per_group = gridDim.x / groups;
+    const int grp = blockIdx.x % groups;
+    const int t_lo = (int)((long long)n_tiles * grp / groups);
+    const int t_hi = (int)((long long)n_tiles * (grp + 1) / groups);
+    for (int tile = t_lo + (int)blockIdx.x / groups; tile < t_hi; tile += per_group) {
+        const TileDesc td = ld_const(p.tiles, tile);
+        const BucketDev bk = ld_const(p.buckets, td.bucket);
         const int nf = td.nf < 0 ? -td.nf : td.nf;
+        // the tile's leaf counts -> LDS (lane f of each 16 holds family f)
+        for (int i = threadIdx.x; i < p.n_leaves * 16; i += 64 * W) {
+            const int f = i & 15;
+            const int ff = f < nf ? f : 0;
+            const int fam = td.nf < 0 ? p.fam_list[td.fam0 + ff] : bk.fam_begin + td.fam0 + ff;
+            s_cnt[i] = p.counts_T[(long long)(i >> 4) * p.F + fam];
+        }
+        __syncthreads();   // counts visible
         const bool valid = fl < nf;
         const int fi = valid ? fl : 0;
         const int fs = td.nf < 0 ? p.fam_list[td.fam0 + fi] : bk.fam_begin + td.fam0 + fi;
         const int dst = p.perm[fs];   // the caller's index of this lane's family
-        const int* __restrict__ nslot = p.node_slot + (long long)td.point * p.n_nodes;
-        const int nts = bk.Lp / kTile;   // k-tiles of every child vector
+        const __attribute__((address_space(4))) int* nslot = as_const(p.node_slot + (long long)td.point * p.n_nodes);
+        const int nts = bk.Lp / kTile;        // k-tiles of every child vector
+        const int rows_root = bk.Rp / kTile;  // root rows are sizes 0..R
+        // flush-bound terms of an interior factor (FP32 matrix entries, MFMA
+        // products and stored vector entries that may flush)
+        const double cint = SCALED ? kTau32 * (bk.Lp * kTwoPowMinusMatExp * (1.0 + two_m_a) + two_m_a) : 0.0;
 
-        for (int op = 0; op < p.n_ops; ++op) {
-            const OpDev o = p.ops[op];
-            const bool is_root = op == p.n_ops - 1;
-            const int rows = is_root ? bk.Rp / kTile : nts;
-            const int limit = is_root ? bk.R : bk.L;
-            const int shift = is_root ? 1 : 0;
+        // per-lane offsets of this wave's output tiles, non-root [0] / root [1]:
+        // A operand row (FP32 row-major) or column block (FP64 c-major), and
+        // leaf rows within a c-major column
+        // (separate arrays, chosen with selects: an array indexed by the
+        // root flag would be placed in scratch memory)
+        int offA0[NTW], offA1[NTW], offL0[NTW], offL1[NTW];
+#pragma unroll
+        for (int i = 0; i < NTW; ++i) {
+            const int t0 = wv + W * i < nts ? wv + W * i : nts - 1;
+            const int t1 = wv + W * i < rows_root ? wv + W * i : rows_root - 1;
+            offA0[i] = kF64 ? 4 * g * Np + kTile * t0 + my_arow : (kTile * t0 + my_arow) * Np + 4 * g;
+            offA1[i] = kF64 ? 4 * g * Np + kTile * t1 + my_arow : (kTile * t1 + my_arow) * Np + 4 * g;
+            offL0[i] = kTile * t0 + 4 * g;
+            offL1[i] = kTile * t1 + 4 * g;
+        }
 
-            // SCALED bookkeeping.  A renormalised child v' holds v_c = v' 2^E_c
-            // with max v' < 2^a; its factor X' = M' v' (M' = 2^kMatExp M) holds
-            // X_c = X' 2^(E_c - kMatExp) and is at most 2^(kMatExp + a) (leaf
-            // factors: 2^kMatExp).
-            //   e_sum  sum_c (E_c - kMatExp): true product = raw 2^e_sum
-            //   log2S  log2 of the product's nominal scale prod_c 2^(kMatExp + a_c)
-            //   cert   sum_c of the factors' relative flush errors (w.r.t. their
-            //          nominal scale): inherited bound + what flushing of M'
-            //          entries, MFMA products and stored v' entries can drop
-            int e_sum = 0, log2S = 0;
-            double cert = 0.0;
-            RootBest rb{-__builtin_huge_val(), 0.0, -1, 0.0};
-            T wmax = 0;
-
-            for (int t0 = 0; t0 < rows; t0 += W * NTW) {
-                int tt[NTW], srow[NTW];
-                bool live[NTW];
+        // loads of step st (clamped to the last step) into pf; its descriptor into sd
+        auto prefetch = [&](V4 (&pf)[PK][NTW], StepDev& sd, int st) {
+            // wave-uniform index: the descriptor comes through the scalar
+            // cache, so waiting for it does not wait for vector loads
+            st = __builtin_amdgcn_readfirstlane(st < n_steps ? st : n_steps - 1);
+            sd = ld_const(p.steps, st);
+            const bool root = (sd.ctl & kStepRoot) != 0;
+            const bool leaf = sd.leaf_col >= 0;
+            const T* __restrict__ M = (leaf ? lpool : pool) + (long long)nslot[sd.node] * p.slot_elems;
+            const int cn = s_cnt[(leaf ? sd.leaf_col : 0) * 16 + fl] * Np;
+#pragma unroll
+            for (int k = 0; k < PK; ++k)
 #pragma unroll
                 for (int i = 0; i < NTW; ++i) {
-                    const int t = t0 + wv + W * i;
-                    live[i] = t < rows;
-                    tt[i] = live[i] ? t : rows - 1;
-                    srow[i] = kTile * tt[i] + my_arow + shift;
+                    const int oa = root ? offA1[i] : offA0[i];
+                    const int ol = cn + (root ? offL1[i] : offL0[i]);
+                    const int kk = k < nts ? k : nts - 1;
+                    if constexpr (!kF64) {
+                        const int off = leaf ? ol + 4 * k * zero : oa + kTile * kk;
+                        pf[k][i] = *(const floatx4*)(M + off);
+                    } else {
+                        const int base = leaf ? ol : oa + kTile * kk * Np;
+                        const int step = leaf ? 1 + zero : Np;
+                        pf[k][i] = V4{M[base], M[base + step], M[base + 2 * step], M[base + 3 * step]};
+                    }
                 }
-                V4 acc[NTW];
+        };
+
+        // per-op state (SCALED bookkeeping: a renormalised child v' holds
+        // v_c = v' 2^E_c with max v' < 2^a; its factor X' = M' v'
+        // (M' = 2^kMatExp M) holds X_c = X' 2^(E_c - kMatExp) and is at most
+        // 2^(kMatExp + a), leaf factors 2^kMatExp.
+        //   e_sum  sum_c (E_c - kMatExp): true product = raw 2^e_sum
+        //   log2S  log2 of the product's nominal scale prod_c 2^(kMatExp + a_c)
+        //   cert   sum_c of the factors' relative flush errors (w.r.t. their
+        //          nominal scale): inherited bound + what flushing can drop)
+        V4 acc[NTW];
+        int e_sum = 0, log2S = 0;
+        double cert = 0.0;
+
+        auto consume = [&](const V4 (&pf)[PK][NTW], const StepDev& sd) {
+            const bool root = (sd.ctl & kStepRoot) != 0;
+            const int rows = root ? rows_root : nts;
+            const int out_slot = sd.ctl & kStepSlotMask;
+            int tt[NTW];
+            bool live[NTW];
+#pragma unroll
+            for (int i = 0; i < NTW; ++i) {
+                live[i] = wv + W * i < rows;
+                tt[i] = live[i] ? wv + W * i : rows - 1;
+            }
+            if (sd.ctl & kStepFirst) {
 #pragma unroll
                 for (int i = 0; i < NTW; ++i) acc[i] = V4{1, 1, 1, 1};
-                for (int k = 0; k < o.n_child; ++k) {
-                    const ChildDev ch = p.kids[o.child_begin + k];
-                    const T* __restrict__ M = pool + (long long)nslot[ch.node] * p.slot_elems;
-                    V4 x[NTW];
-#pragma unroll
-                    for (int i = 0; i < NTW; ++i) x[i] = V4{0, 0, 0, 0};
-                    if (ch.leaf_col >= 0) {
-                        if (SCALED && t0 == 0) {
-                            e_sum -= kMatExp;
-                            log2S += kMatExp;
-                            cert += 3.0 * kTau32 * kTwoPowMinusMatExp;   // up to 3 mixed columns
-                        }
-                        const int cnt = p.counts_T[(long long)ch.leaf_col * p.F + fs];
-                        if (p.n_dev == 0) {
-#pragma unroll
-                            for (int i = 0; i < NTW; ++i)
-#pragma unroll
-                                for (int r = 0; r < 4; ++r)
-                                    x[i][r] = mat(M, Np, kTile * tt[i] + 4 * g + r + shift, cnt);
-                        } else {
-                            // error model: columns cnt-h .. cnt+h weighted by
-                            // get_probs(cnt); columns below 0 get weight 0
-                            const int h = (p.n_dev - 1) / 2;
-                            for (int d = 0; d < p.n_dev; ++d) {
-                                const int cc = cnt - h + d;
-                                const T w = cc >= 0 ? err[(long long)cnt * p.n_dev + d] : T(0);
-                                const int c2 = cc >= 0 ? cc : 0;
-#pragma unroll
-                                for (int i = 0; i < NTW; ++i)
-#pragma unroll
-                                    for (int r = 0; r < 4; ++r)
-                                        x[i][r] += w * mat(M, Np, kTile * tt[i] + 4 * g + r + shift, c2);
-                            }
-                        }
-                    } else {
-                        int bsh = 0;
-                        if (SCALED) {
-                            float m = 0.f;
-                            for (int w2 = 0; w2 < W; ++w2) m = fmaxf(m, m_wmax[(ch.in_slot * W + w2) * 16 + fl]);
-                            int e = 0;
-                            const double mant = m > 0.f ? frexp((double)m, &e) : 1.0;
-                            bsh = a_exp - e;
-                            if (t0 == 0) {
-                                e_sum += m_esum[ch.in_slot * 16 + fl] + e - a_exp - kMatExp;
-                                log2S += kMatExp + a_exp;
-                                const double c_in = m > 0.f ? ldexp(m_cpre[ch.in_slot * 16 + fl] * 1.01 / mant,
-                                                                    m_l2s[ch.in_slot] - e) +
-                                                                  ldexp(kTau32 / mant, -e)
-                                                            : 1e300;
-                                cert += c_in + kTau32 * (bk.Lp * kTwoPowMinusMatExp * (1.0 + ldexp(1.0, -a_exp)) +
-                                                         ldexp(1.0, -a_exp));
-                            }
-                        }
-                        const V4* bsrc = frag + (long long)ch.in_slot * slot_tiles * 64 + lane;
-                        child_gemm<T, NTW, SCALED>(x, M, Np, srow, g, bsrc, nts, bsh);
-                    }
-#pragma unroll
-                    for (int i = 0; i < NTW; ++i) acc[i] *= x[i];
-                }
-
-                if (!is_root) {
-#pragma unroll
-                    for (int i = 0; i < NTW; ++i) {
-                        if (!live[i]) continue;
-                        V4 v = acc[i];
-#pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            if (kTile * tt[i] + 4 * g + r >= limit) v[r] = 0;   // rows past the bound
-                        frag[((long long)o.out_slot * slot_tiles + tt[i]) * 64 + lane] = v;
-                        if (SCALED)
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) wmax = wmax > v[r] ? wmax : v[r];
-                    }
-                } else {
-                    // prior values first (unconditional, clamped), then the reduction
-                    const double* __restrict__ pr = p.prior64 + bk.prior_off;
-                    double pv[NTW][4];
-#pragma unroll
-                    for (int i = 0; i < NTW; ++i)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int j = kTile * tt[i] + 4 * g + r;
-                            pv[i][r] = pr[j < limit ? j : limit - 1];
-                        }
-#pragma unroll
-                    for (int i = 0; i < NTW; ++i) {
-                        if (!live[i]) continue;
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int j = kTile * tt[i] + 4 * g + r;
-                            if (j >= limit) continue;
-                            const double xv = (double)acc[i][r];
-                            if (p.root_out && valid)
-                                p.root_out[(long long)dst * p.root_stride + j] = SCALED ? ldexp(xv, e_sum) : xv;
-                            root_take(rb, xv * pv[i][r], xv, j);
-                        }
-                    }
-                }
+                e_sum = 0;
+                log2S = 0;
+                cert = 0.0;
             }
+            V4 x[NTW];
+            if (sd.leaf_col >= 0) {
+                if (SCALED) {
+                    e_sum -= kMatExp;
+                    log2S += kMatExp;
+                    cert += leaf_cert;
+                }
+#pragma unroll
+                for (int i = 0; i < NTW; ++i) x[i] = pf[0][i];
+            } else {
+                int bsh = 0;
+                if (SCALED) {
+                    const int q = sd.in_slot * 16 + fl;
+                    log2S += kMatExp + a_exp;
+                    if (W == 1) {
+                        bsh = m_bsh[q];
+                        e_sum += m_esum[q] - kMatExp;
+                        cert += m_cert[q] + cint;
+                    } else {
+                        float m = 0.f;
+                        for (int w2 = 0; w2 < W; ++w2) m = fmaxf(m, m_wmax[(sd.in_slot * W + w2) * 16 + fl]);
+                        int e = 0;
+                        const float mant = m > 0.f ? frexpf(m, &e) : 1.f;
+                        bsh = a_exp - e;
+                        e_sum += m_esum[q] + e - a_exp - kMatExp;
+                        cert += (m > 0.f ? (ldexp(m_cert[q] * 1.01, m_l2s[sd.in_slot] - e) + ldexp(kTau32, -e)) *
+                                               rcp_up(mant)
+                                         : 1e300) +
+                                cint;
+                    }
+                }
+                const T* __restrict__ M = pool + (long long)nslot[sd.node] * p.slot_elems;
+                int ao[NTW];
+#pragma unroll
+                for (int i = 0; i < NTW; ++i) {
+                    x[i] = V4{0, 0, 0, 0};
+                    ao[i] = root ? offA1[i] : offA0[i];
+                }
+                const V4* bsrc = frag + (long long)sd.in_slot * slot_tiles * 64 + lane;
+                child_gemm<T, NTW, SCALED>(x, M, Np, ao, pf, bsrc, nts, bsh);
+            }
+#pragma unroll
+            for (int i = 0; i < NTW; ++i) acc[i] *= x[i];
+            if (!(sd.ctl & kStepLast)) return;
 
-            if (!is_root) {
+            if (!root) {
+                T wmax = 0;
+#pragma unroll
+                for (int i = 0; i < NTW; ++i) {
+                    if (!live[i]) continue;
+                    V4 v = acc[i];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if (kTile * tt[i] + 4 * g + r >= bk.L) v[r] = 0;   // rows past the bound
+                    frag[((long long)out_slot * slot_tiles + tt[i]) * 64 + lane] = v;
+                    if (SCALED)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) wmax = wmax > v[r] ? wmax : v[r];
+                }
                 if (SCALED) {
                     float m = (float)wmax;
                     m = fmaxf(m, __shfl_xor(m, 16));
                     m = fmaxf(m, __shfl_xor(m, 32));
-                    if (g == 0) m_wmax[(o.out_slot * W + wv) * 16 + fl] = m;
-                    if (wv == 0 && g == 0) {
-                        m_esum[o.out_slot * 16 + fl] = e_sum;
-                        m_cpre[o.out_slot * 16 + fl] = cert;
-                    }
-                    if (wv == 0 && lane == 0) m_l2s[o.out_slot] = log2S;
-                }
-                __syncthreads();   // the node's vector is complete before anyone reads it
-            } else {
-                rb = root_combine(rb);
-                if (W > 1) {
-                    if (g == 0) {
-                        s_rkey[wv][fl] = rb.key;
-                        s_rx[wv][fl] = rb.x;
-                        s_rxmax[wv][fl] = rb.xmax;
-                        s_rj[wv][fl] = rb.j;
-                    }
-                    __syncthreads();
-                    if (wv == 0)
-                        for (int w2 = 1; w2 < W; ++w2)
-                            root_merge(rb, s_rkey[w2][fl], s_rx[w2][fl], s_rj[w2][fl], s_rxmax[w2][fl]);
-                }
-                if (wv == 0) {
-                    double val;
-                    int flag = 0;
-                    if (SCALED) {
-                        // rb.key = x_j * prior_j in units of 2^e_sum
-                        if (!(rb.key > 0)) {
-                            flag = kFlagRecompute64;
-                            val = 0.0;
-                        } else {
-                            const double log_scale = (double)e_sum * kLn2;
-                            if (p.model == 0)
-                                val = log(rb.x) + log(p.prior64[bk.prior_off + rb.j]) + log_scale;
-                            else
-                                val = ldexp(rb.key, e_sum);
-                            // flush bound (absolute, in units of 2^e_sum) against
-                            // the likelihood, and FP64 range of the unscaled values
-                            const double bound = (ldexp(cert, log2S) + kTau32) * 1.01;
-                            if (!(bound <= kCertEta * rb.key)) flag |= kFlagRecompute64;
-                            if (!(log(rb.xmax) + log_scale >= log(kFp64Floor))) flag |= kFlagRecompute64;
-                            if (!(log(rb.key) + log_scale >= log(kFp64Floor))) flag |= kFlagRecompute64;
+                    const int q = out_slot * 16 + fl;
+                    if (W == 1) {
+                        // finish the stored vector: shift, exponent, flush bound
+                        // re-expressed relative to the new scale (it grows by
+                        // S / m, the factor by which the product fell short of
+                        // its nominal scale)
+                        int e = 0;
+                        const float mant = m > 0.f ? frexpf(m, &e) : 1.f;
+                        if (g == 0) {
+                            m_bsh[q] = a_exp - e;
+                            m_esum[q] = e_sum + e - a_exp;
+                            m_cert[q] = m > 0.f ? (ldexp(cert * 1.01, log2S - e) + ldexp(kTau32, -e)) * rcp_up(mant)
+                                                : 1e300;
                         }
                     } else {
-                        // base: log(x_j prior_j) at the best j (base_model.cpp:86
-                        // takes log x_j + log prior_j); gamma: x_j prior_j
-                        val = p.model == 0 ? log(rb.key) : rb.key;
-                        if (rb.xmax == 0.0) flag |= kFlagZeroSum;
-                        if (rb.xmax < kFp64Floor || rb.key < kFp64Floor) flag |= kFlagRecomputePlain;
-                    }
-                    if (valid && g == 0) {
-                        p.out_val[(long long)td.point * p.F + dst] = val;
-                        p.out_flag[(long long)td.point * p.F + dst] = flag;
+                        if (g == 0) m_wmax[(out_slot * W + wv) * 16 + fl] = m;
+                        if (wv == 0 && g == 0) {
+                            m_esum[q] = e_sum;
+                            m_cert[q] = cert;
+                        }
+                        if (wv == 0 && lane == 0) m_l2s[out_slot] = log2S;
                     }
                 }
+                // the node's vector is complete before anyone reads it (one
+                // wave: LDS keeps its own order, only the compiler must not
+                // move accesses across)
+                if (!LDSP) __syncthreads();
+                else if (W > 1) lds_barrier();
+                else asm volatile("" ::: "memory");
+                return;
+            }
+
+            // root: best x_j * prior_j over this lane's rows (row = root size
+            // j + 1, j = 0..R-1), then across lanes and waves
+            const double* __restrict__ pr = p.prior64 + bk.prior_off;
+            double pv[NTW][4];
+#pragma unroll
+            for (int i = 0; i < NTW; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = kTile * tt[i] + 4 * g + r - 1;
+                    pv[i][r] = pr[j < 0 ? 0 : (j < bk.R ? j : bk.R - 1)];
+                }
+            RootBest rb{-__builtin_huge_val(), 0.0, -1, 0.0};
+#pragma unroll
+            for (int i = 0; i < NTW; ++i) {
+                if (!live[i]) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = kTile * tt[i] + 4 * g + r - 1;
+                    if (j < 0 || j >= bk.R) continue;
+                    const double xv = (double)acc[i][r];
+                    if (p.root_out && valid)
+                        p.root_out[(long long)dst * p.root_stride + j] = SCALED ? ldexp(xv, e_sum) : xv;
+                    root_take(rb, xv * pv[i][r], xv, j);
+                }
+            }
+            rb = root_combine(rb);
+            if (W > 1) {
+                if (g == 0) {
+                    s_rkey[wv][fl] = rb.key;
+                    s_rx[wv][fl] = rb.x;
+                    s_rxmax[wv][fl] = rb.xmax;
+                    s_rj[wv][fl] = rb.j;
+                }
+                lds_barrier();
+                if (wv == 0)
+                    for (int w2 = 1; w2 < W; ++w2)
+                        root_merge(rb, s_rkey[w2][fl], s_rx[w2][fl], s_rj[w2][fl], s_rxmax[w2][fl]);
+            }
+            if (wv == 0) {
+                double val;
+                int flag = 0;
+                if (SCALED) {
+                    // rb.key = x_j * prior_j in units of 2^e_sum
+                    if (!(rb.key > 0)) {
+                        flag = kFlagRecompute64;
+                        val = 0.0;
+                    } else {
+                        const double log_scale = (double)e_sum * kLn2;
+                        if (p.model == 0)
+                            val = log(rb.x) + log(p.prior64[bk.prior_off + rb.j]) + log_scale;
+                        else
+                            val = ldexp(rb.key, e_sum);
+                        // flush bound (absolute, in units of 2^e_sum) against
+                        // the likelihood, and FP64 range of the unscaled values
+                        const double bound = (ldexp(cert, log2S) + kTau32) * 1.01;
+                        if (!(bound <= kCertEta * rb.key)) flag |= kFlagRecompute64;
+                        if (!(log(rb.xmax) + log_scale >= log(kFp64Floor))) flag |= kFlagRecompute64;
+                        if (!(log(rb.key) + log_scale >= log(kFp64Floor))) flag |= kFlagRecompute64;
+                    }
+                } else {
+                    // base: log(x_j prior_j) at the best j (base_model.cpp:86
+                    // takes log x_j + log prior_j); gamma: x_j prior_j
+                    val = p.model == 0 ? log(rb.key) : rb.key;
+                    if (rb.xmax == 0.0) flag |= kFlagZeroSum;
+                    if (rb.xmax < kFp64Floor || rb.key < kFp64Floor) flag |= kFlagRecomputePlain;
+                }
+                if (valid && g == 0) {
+                    p.out_val[(long long)td.point * p.F + dst] = val;
+                    p.out_flag[(long long)td.point * p.F + dst] = flag;
+                }
+            }
+        };
+
+        V4 pfa[PK][NTW], pfb[PK][NTW];
+        StepDev sda, sdb;
+        prefetch(pfa, sda, 0);
+        for (int st = 0; st < n_steps; st += 2) {
+            prefetch(pfb, sdb, st + 1);
+            consume(pfa, sda);
+            if (st + 1 < n_steps) {
+                prefetch(pfa, sda, st + 2);
+                consume(pfb, sdb);
             }
         }
+        __syncthreads();   // every wave is done with this tile's LDS before the next one
     }
 }
 

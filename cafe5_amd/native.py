@@ -75,6 +75,7 @@ def load_library() -> ctypes.CDLL:
         "cafe_gpu_plan_results": (_i, [_p, _pd, _pd, _pd, _pd, _pll]),
         "cafe_gpu_plan_profile": (_i, [_p, _pd, _pi]),
         "cafe_gpu_plan_recompute_counts": (_i, [_p, _pll, _pll]),
+        "cafe_gpu_plan_class_times": (_i, [_p, _i, _pi, _pi, _pll, _pd]),
         "cafe_gpu_plan_destroy": (_i, [_p]),
     }
     for name, (res, args) in sig.items():
@@ -91,7 +92,8 @@ EXPORTED_SYMBOLS = ["cafe_gpu_create", "cafe_gpu_destroy", "cafe_gpu_last_error"
                     "cafe_gpu_set_error_model", "cafe_gpu_eval", "cafe_gpu_root_vectors",
                     "cafe_gpu_last_timings", "cafe_gpu_plan_create", "cafe_gpu_plan_set_keys",
                     "cafe_gpu_plan_execute", "cafe_gpu_plan_synchronize", "cafe_gpu_plan_results",
-                    "cafe_gpu_plan_profile", "cafe_gpu_plan_recompute_counts", "cafe_gpu_plan_destroy"]
+                    "cafe_gpu_plan_profile", "cafe_gpu_plan_recompute_counts", "cafe_gpu_plan_class_times",
+                    "cafe_gpu_plan_destroy"]
 
 
 def _ip(a: Optional[np.ndarray]):
@@ -273,6 +275,18 @@ class Plan:
         g = ctypes.c_int(0)
         self.eng._check(self.lib.cafe_gpu_plan_profile(self.h, _dp(out), ctypes.byref(g)))
         return out, bool(g.value)
+
+    def class_times(self):
+        """Per size class of the main pass, from the last profile() run:
+        (16-row tiles per vector, 16-family tiles, kernel ms) arrays."""
+        n = ctypes.c_int(0)
+        self.eng._check(self.lib.cafe_gpu_plan_class_times(self.h, 0, ctypes.byref(n), None, None, None))
+        nt = np.zeros(n.value, dtype=np.int32)
+        tiles = np.zeros(n.value, dtype=np.int64)
+        ms = np.zeros(n.value)
+        self.eng._check(self.lib.cafe_gpu_plan_class_times(self.h, n.value, ctypes.byref(n), _ip(nt),
+                                                           tiles.ctypes.data_as(_pll), _dp(ms)))
+        return nt, tiles, ms
 
     def recompute_counts(self):
         """(families redone on BATCHED64, families redone on PLAIN64)."""

@@ -194,6 +194,13 @@ int cafe_gpu_plan_profile(cafe_gpu_plan *plan, double *ms4, int *graph_ok);
  * flush bound did not certify) and on PLAIN64 (FP64 values near the
  * subnormal range, redone in the reference's summation order). */
 int cafe_gpu_plan_recompute_counts(cafe_gpu_plan *plan, long long *fp64, long long *plain);
+/* Per size class of the main pruning pass, from the last
+ * cafe_gpu_plan_profile run (no reference counterpart; measurement): the
+ * class's 16-row tile count per vector, its number of 16-family tiles and
+ * the milliseconds of its kernel launch (HIP events on the context stream).
+ * Writes at most cap entries; *n_classes is the number of classes. */
+int cafe_gpu_plan_class_times(cafe_gpu_plan *plan, int cap, int *n_classes, int *class_nt,
+                              long long *class_tiles, double *ms);
 int cafe_gpu_plan_destroy(cafe_gpu_plan *plan);
 
 /* --- timing (measurement support; no reference counterpart) --------------- */
