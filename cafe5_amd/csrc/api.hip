@@ -33,6 +33,9 @@ hipError_t launch_finalize(int model, int T, int K, int F, const double* val, co
 
 using namespace cafe;
 
+// side streams per context: the size classes of one pruning phase
+constexpr int kSideStreams = 8;
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -57,6 +60,7 @@ int fail(int code, const char* fmt, ...)
     } while (0)
 
 int round_up(int x, int m) { return (x + m - 1) / m * m; }
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 // Grow-only device buffer.
 struct DevBuf {
@@ -104,6 +108,10 @@ struct cafe_gpu_ctx {
     int device = 0;
     int n_cu = 256;
     hipStream_t stream = nullptr;
+    // size classes of one pruning phase run concurrently on these (default
+    // priority), forked from and joined back to `stream` with join_ev
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> join_ev;
     hipEvent_t ev[6] = {};
     double timings[4] = {0, 0, 0, 0};
     bool build_timed = false;
@@ -265,6 +273,14 @@ int cafe_gpu_create(int device, cafe_gpu_ctx** out)
         return fail(CAFE_GPU_EDEVICE, "hipStreamCreate failed");
     }
     for (auto& e : ctx->ev) (void)hipEventCreate(&e);
+    ctx->side.assign(kSideStreams, nullptr);
+    for (auto& st : ctx->side)
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            cafe_gpu_destroy(ctx);
+            return fail(CAFE_GPU_EDEVICE, "hipStreamCreate failed");
+        }
+    ctx->join_ev.assign(3 * (kSideStreams + 1), nullptr);
+    for (auto& e : ctx->join_ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
     // log-gamma of integers from the host libm: the reference's
     // lgamma cache (probability.cpp:69-75)
     std::vector<double> lg(kLgammaN);
@@ -290,6 +306,13 @@ int cafe_gpu_destroy(cafe_gpu_ctx* ctx)
     ctx->eager.release_all();
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : ctx->join_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& st : ctx->side)
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return CAFE_GPU_OK;
@@ -538,6 +561,9 @@ struct KernelClass {
     int slot_tiles;      // 16-row tiles of its largest stored (non-root) vector
     MfmaShape s32, s64;  // launch shapes of the FP32 and FP64 MFMA kernels
     int begin, end;      // its main tiles
+    // byte offsets of its global scratch in the main pass, the FP64 and the
+    // PLAIN64 recompute (classes of one pass run concurrently)
+    size_t off_main = 0, off_r64 = 0, off_rp = 0;
 };
 
 // Host-side description of one pruning sequence over a tile list.
@@ -748,19 +774,11 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         kc.end = (int)tiles.size();
     }
     const int n_cls = (int)pp.classes.size();
-    size_t scratch = 0;
     for (auto& kc : pp.classes) {
         kc.s32 = mfma_shape(kc.nt, false);
         kc.s64 = mfma_shape(kc.nt, true);
         place_partials(kc.s32, false, ctx->n_slots, kc.slot_tiles, ctx->n_leaves);
         place_partials(kc.s64, true, ctx->n_slots, kc.slot_tiles, ctx->n_leaves);
-        const int n = kc.end - kc.begin;
-        if (path == CAFE_GPU_PATH_BATCHED32)
-            scratch = std::max(scratch, mfma_scratch_bytes(kc.s32, false, ctx->n_slots, kc.slot_tiles,
-                                                           pp.grid(kc.s32.w, n)));
-        if (path != CAFE_GPU_PATH_PLAIN64)   // BATCHED64 main, or the FP32 path's FP64 recompute
-            scratch = std::max(scratch, mfma_scratch_bytes(kc.s64, true, ctx->n_slots, kc.slot_tiles,
-                                                           pp.grid(kc.s64.w, n)));
     }
     pp.n_tiles = (int)tiles.size();
     pp.class_counts.assign(n_cls + 4, 0);
@@ -792,12 +810,36 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     HIP_OR_FAIL(hipMemcpy(b.segs.p, segs.data(), sizeof(SegDesc) * pp.n_segs, hipMemcpyHostToDevice));
     HIP_OR_FAIL(hipMemcpy(b.class_off.p, class_off.data(), sizeof(int) * 2 * (n_cls + 1), hipMemcpyHostToDevice));
 
-    // PLAIN64 partials: one region of n_slots x ldv doubles per workgroup (LDS
-    // when it fits; the global region is sized for the worst case)
+    // Global scratch: per class and pass (classes of one pass run
+    // concurrently, passes one after another).  PLAIN64 partials take one
+    // region of n_slots x ldv doubles per workgroup unless they fit in LDS.
     const int ldv = round_up(maxLR, 4) + 4;
     pp.per_wg_plain = (long long)ctx->n_slots * ldv;
-    scratch = std::max(scratch, (size_t)pp.grid(1, 1 << 30) * 8 * (size_t)pp.per_wg_plain);
-    HIP_OR_FAIL(b.scratch.ensure(std::max<size_t>(scratch, 256)));
+    const bool plain_in_lds = sizeof(double) * (size_t)ctx->n_slots * ldv <= 48 * 1024;
+    auto plain_bytes = [&](int n) {
+        return plain_in_lds || n <= 0 ? (size_t)0 : (size_t)pp.grid(1, n) * 8 * (size_t)pp.per_wg_plain;
+    };
+    size_t main_sz = 0, r64_sz = 0, rp_sz = 0;
+    for (int ci = 0; ci < n_cls; ++ci) {
+        KernelClass& kc = pp.classes[ci];
+        const int n = pp.class_counts[ci];
+        size_t mb = 0;
+        if (path == CAFE_GPU_PATH_PLAIN64) mb = plain_bytes(n);
+        else if (path == CAFE_GPU_PATH_BATCHED64)
+            mb = mfma_scratch_bytes(kc.s64, true, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s64.w, n));
+        else mb = mfma_scratch_bytes(kc.s32, false, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s32.w, n));
+        kc.off_main = main_sz;
+        main_sz += align256(mb);
+        if (path == CAFE_GPU_PATH_BATCHED32) {
+            kc.off_r64 = r64_sz;
+            r64_sz += align256(mfma_scratch_bytes(kc.s64, true, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s64.w, n)));
+        }
+        if (path != CAFE_GPU_PATH_PLAIN64) {
+            kc.off_rp = rp_sz;
+            rp_sz += align256(plain_bytes(pp.plain_off[ci + 1] - pp.plain_off[ci]));
+        }
+    }
+    HIP_OR_FAIL(b.scratch.ensure(std::max<size_t>(std::max(main_sz, std::max(r64_sz, rp_sz)), 256)));
 
     PruneParams& p = pp.base;
     p.buckets = b.buckets.as<BucketDev>();
@@ -839,34 +881,62 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
 // (device-side lists, no host round trip) the FP64 recompute of the FP32
 // families the flush bound did not certify and the PLAIN64 recompute of
 // families near the subnormal range.
-int enqueue_prune(EvalBufs& b, const PrunePlan& pp, hipStream_t st, hipEvent_t ev_main, hipEvent_t ev_fb,
-                  const std::vector<hipEvent_t>* cls_ev = nullptr)
+int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream_t st, hipEvent_t ev_main,
+                  hipEvent_t ev_fb, bool concurrent, const std::vector<hipEvent_t>* cls_ev = nullptr)
 {
     const int n_cls = (int)pp.classes.size();
     int* d_cnt = b.ntiles.as<int>();
     int* d_ctr = b.counters.as<int>();
     int* fbc = b.fb_count.as<int>();
+    char* scratch = (char*)pp.base.scratch;
     HIP_OR_FAIL(launch_fill_zero(b.counters.p, sizeof(int) * (3 * n_cls + 8), st));
     HIP_OR_FAIL(launch_fill_zero(b.fb_count.p, sizeof(int) * (2 * n_cls + 4), st));
-    for (int ci = 0; ci < n_cls; ++ci) {
-        const KernelClass& kc = pp.classes[ci];
-        const int n = pp.class_counts[ci];
-        if (cls_ev) HIP_OR_FAIL(hipEventRecord((*cls_ev)[ci], st));
-        PruneParams q = pp.base;
-        q.tiles = b.tiles.as<TileDesc>() + kc.begin;
-        q.n_tiles_dev = d_cnt + ci;
-        q.tile_counter = d_ctr + ci;
-        q.slot_tiles = kc.slot_tiles;
-        if (pp.path == CAFE_GPU_PATH_PLAIN64) {
-            q.scratch_per_wg = pp.per_wg_plain;
-            HIP_OR_FAIL(launch_prune_plain(kc.nq, pp.grid(1, n), q, st));
-        } else if (pp.path == CAFE_GPU_PATH_BATCHED64) {
-            HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64.w, n), q, st));
-        } else {
-            HIP_OR_FAIL(launch_prune_mfma(false, kc.s32, pp.grid(kc.s32.w, n), q, st));
+
+    // One pass over the size classes: launch(ci, stream) for each.  When
+    // concurrent, class ci runs on side stream ci % S after the work already
+    // on st, and st waits for every side stream before going on.
+    const int S = concurrent ? std::min((int)ctx->side.size(), n_cls) : 0;
+    auto pass = [&](int pass_idx, auto&& launch) -> int {
+        if (S <= 1) {
+            for (int ci = 0; ci < n_cls; ++ci)
+                if (int rc = launch(ci, st)) return rc;
+            return CAFE_GPU_OK;
         }
-    }
-    if (cls_ev) HIP_OR_FAIL(hipEventRecord((*cls_ev)[n_cls], st));
+        hipEvent_t* ev = ctx->join_ev.data() + pass_idx * (kSideStreams + 1);
+        HIP_OR_FAIL(hipEventRecord(ev[0], st));
+        for (int k = 0; k < S; ++k) HIP_OR_FAIL(hipStreamWaitEvent(ctx->side[k], ev[0], 0));
+        for (int ci = 0; ci < n_cls; ++ci)
+            if (int rc = launch(ci, ctx->side[ci % S])) return rc;
+        for (int k = 0; k < S; ++k) {
+            HIP_OR_FAIL(hipEventRecord(ev[1 + k], ctx->side[k]));
+            HIP_OR_FAIL(hipStreamWaitEvent(st, ev[1 + k], 0));
+        }
+        return CAFE_GPU_OK;
+    };
+
+    // main pass
+    if (int rc = pass(0, [&](int ci, hipStream_t s) -> int {
+            const KernelClass& kc = pp.classes[ci];
+            const int n = pp.class_counts[ci];
+            if (cls_ev && S <= 1) HIP_OR_FAIL(hipEventRecord((*cls_ev)[ci], s));
+            PruneParams q = pp.base;
+            q.tiles = b.tiles.as<TileDesc>() + kc.begin;
+            q.n_tiles_dev = d_cnt + ci;
+            q.tile_counter = d_ctr + ci;
+            q.slot_tiles = kc.slot_tiles;
+            q.scratch = scratch + kc.off_main;
+            if (pp.path == CAFE_GPU_PATH_PLAIN64) {
+                q.scratch_per_wg = pp.per_wg_plain;
+                HIP_OR_FAIL(launch_prune_plain(kc.nq, pp.grid(1, n), q, s));
+            } else if (pp.path == CAFE_GPU_PATH_BATCHED64) {
+                HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64.w, n), q, s));
+            } else {
+                HIP_OR_FAIL(launch_prune_mfma(false, kc.s32, pp.grid(kc.s32.w, n), q, s));
+            }
+            return CAFE_GPU_OK;
+        }))
+        return rc;
+    if (cls_ev && S <= 1) HIP_OR_FAIL(hipEventRecord((*cls_ev)[n_cls], st));
     if (ev_main) HIP_OR_FAIL(hipEventRecord(ev_main, st));
     TileDesc* fb64 = b.fb_tiles.as<TileDesc>();
     TileDesc* fbplain = fb64 + std::max(1, pp.n_tiles);
@@ -876,29 +946,35 @@ int enqueue_prune(EvalBufs& b, const PrunePlan& pp, hipStream_t st, hipEvent_t e
         HIP_OR_FAIL(launch_pack(b.segs.as<SegDesc>(), pp.n_segs, b.buckets.as<BucketDev>(), pp.base.perm,
                                 b.flag.as<int>(), pp.base.F, kFlagRecompute64, b.fam_list.as<int>(), fb64, coff,
                                 fbc, fbc + fam_off, kTile, st));
-        for (int ci = 0; ci < n_cls; ++ci) {
-            const KernelClass& kc = pp.classes[ci];
-            PruneParams q = pp.base;
-            q.tiles = fb64 + kc.begin;
-            q.n_tiles_dev = fbc + ci;
-            q.tile_counter = d_ctr + n_cls + ci;
-            q.slot_tiles = kc.slot_tiles;
-            HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64.w, pp.class_counts[ci]), q, st));
-        }
+        if (int rc = pass(1, [&](int ci, hipStream_t s) -> int {
+                const KernelClass& kc = pp.classes[ci];
+                PruneParams q = pp.base;
+                q.tiles = fb64 + kc.begin;
+                q.n_tiles_dev = fbc + ci;
+                q.tile_counter = d_ctr + n_cls + ci;
+                q.slot_tiles = kc.slot_tiles;
+                q.scratch = scratch + kc.off_r64;
+                HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64.w, pp.class_counts[ci]), q, s));
+                return CAFE_GPU_OK;
+            }))
+            return rc;
     }
     if (pp.path != CAFE_GPU_PATH_PLAIN64) {
         HIP_OR_FAIL(launch_pack(b.segs.as<SegDesc>(), pp.n_segs, b.buckets.as<BucketDev>(), pp.base.perm,
                                 b.flag.as<int>(), pp.base.F, kFlagRecomputePlain, b.fam_list.as<int>(), fbplain,
                                 coff + n_cls + 1, fbc + n_cls, fbc + fam_off + 1, 1, st));
-        for (int ci = 0; ci < n_cls; ++ci) {
-            const KernelClass& kc = pp.classes[ci];
-            PruneParams q = pp.base;
-            q.tiles = fbplain + pp.plain_off[ci];
-            q.n_tiles_dev = fbc + n_cls + ci;
-            q.tile_counter = d_ctr + 2 * n_cls + ci;
-            q.scratch_per_wg = pp.per_wg_plain;
-            HIP_OR_FAIL(launch_prune_plain(kc.nq, pp.grid(1, pp.plain_off[ci + 1] - pp.plain_off[ci]), q, st));
-        }
+        if (int rc = pass(2, [&](int ci, hipStream_t s) -> int {
+                const KernelClass& kc = pp.classes[ci];
+                PruneParams q = pp.base;
+                q.tiles = fbplain + pp.plain_off[ci];
+                q.n_tiles_dev = fbc + n_cls + ci;
+                q.tile_counter = d_ctr + 2 * n_cls + ci;
+                q.scratch = scratch + kc.off_rp;
+                q.scratch_per_wg = pp.per_wg_plain;
+                HIP_OR_FAIL(launch_prune_plain(kc.nq, pp.grid(1, pp.plain_off[ci + 1] - pp.plain_off[ci]), q, s));
+                return CAFE_GPU_OK;
+            }))
+            return rc;
     }
     if (ev_fb) HIP_OR_FAIL(hipEventRecord(ev_fb, st));
     return CAFE_GPU_OK;
@@ -972,8 +1048,10 @@ int plan_enqueue(cafe_gpu_plan* pl, bool timed)
                              ctx->lf32.as<float>(), st));
     if (int rc = enqueue_fold(ctx, 0, pl->n_keys, st)) return rc;
     if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[1], st));
-    if (int rc = enqueue_prune(pl->bufs, pl->pp, st, timed ? pl->ev[2] : nullptr, timed ? pl->ev[3] : nullptr,
-                               timed ? &pl->cls_ev : nullptr))
+    // profile runs issue the size classes one after another (standalone kernel
+    // times); executions and the captured graph run them concurrently
+    if (int rc = enqueue_prune(ctx, pl->bufs, pl->pp, st, timed ? pl->ev[2] : nullptr, timed ? pl->ev[3] : nullptr,
+                               !timed, timed ? &pl->cls_ev : nullptr))
         return rc;
     if (int rc = enqueue_finalize(ctx, pl->bufs, pl->model, pl->T, pl->K, st)) return rc;
     if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[4], st));
@@ -1021,7 +1099,7 @@ int cafe_gpu_eval(cafe_gpu_ctx* ctx, const cafe_gpu_eval_desc* d, double* lnl, d
     PrunePlan pp;
     if (int rc = plan_prune(ctx, b, d->path, d->model, P, pts.data(), nullptr, 0, pp)) return rc;
     HIP_OR_FAIL(hipEventRecord(ctx->ev[0], ctx->stream));
-    if (int rc = enqueue_prune(b, pp, ctx->stream, ctx->ev[1], ctx->ev[2])) return rc;
+    if (int rc = enqueue_prune(ctx, b, pp, ctx->stream, ctx->ev[1], ctx->ev[2], true)) return rc;
     if (int rc = enqueue_finalize(ctx, b, d->model, d->n_trials, d->n_cat, ctx->stream)) return rc;
     HIP_OR_FAIL(hipEventRecord(ctx->ev[3], ctx->stream));
     if (int rc = copy_results(ctx, b, pp, d->model, d->n_trials, d->n_cat, lnl, cat, post, total, n_fallback))
@@ -1048,7 +1126,7 @@ int cafe_gpu_root_vectors(cafe_gpu_ctx* ctx, const cafe_gpu_eval_desc* d, int tr
     const int point = trial * d->n_cat + cat_k;
     PrunePlan pp;
     if (int rc = plan_prune(ctx, b, d->path, d->model, 1, &point, b.root.as<double>(), maxR, pp)) return rc;
-    if (int rc = enqueue_prune(b, pp, ctx->stream, nullptr, nullptr)) return rc;
+    if (int rc = enqueue_prune(ctx, b, pp, ctx->stream, nullptr, nullptr, true)) return rc;
     std::vector<double> h((size_t)ctx->F * maxR);
     HIP_OR_FAIL(hipMemcpyAsync(h.data(), b.root.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
     HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
