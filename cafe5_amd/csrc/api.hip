@@ -33,8 +33,9 @@ hipError_t launch_finalize(int model, int T, int K, int F, const double* val, co
 
 using namespace cafe;
 
-// side streams per context: the size classes of one pruning phase
-constexpr int kSideStreams = 8;
+// side streams per context: the size classes of one pruning pass (as many as
+// the runtime's default hardware queues per process)
+constexpr int kSideStreams = 4;
 
 namespace {
 
@@ -324,7 +325,7 @@ int cafe_gpu_matrix_reset(cafe_gpu_ctx* ctx, int N, int capacity)
     if (N < 2 || N * 2 + 2 > kLgammaN) return fail(CAFE_GPU_EINVAL, "matrix size %d out of range", N);
     if (capacity < 1) return fail(CAFE_GPU_EINVAL, "capacity must be positive");
     ctx->N = N;
-    ctx->Np = round_up(N, kTile) + kTile;
+    ctx->Np = round_up(N + kTile, 32);   // >= round_up(N, 16) + 16, whole 32-wide transpose tiles
     ctx->slot_elems = (long long)ctx->Np * ctx->Np;
     ctx->capacity = capacity;
     ctx->pool_gen++;
@@ -564,6 +565,7 @@ struct KernelClass {
     // byte offsets of its global scratch in the main pass, the FP64 and the
     // PLAIN64 recompute (classes of one pass run concurrently)
     size_t off_main = 0, off_r64 = 0, off_rp = 0;
+    int stream = 0;      // side stream of its main-pass launch
 };
 
 // Host-side description of one pruning sequence over a tile list.
@@ -572,6 +574,7 @@ struct PrunePlan {
     std::vector<KernelClass> classes;
     std::vector<int> class_counts;   // main tiles per class (+ padding)
     std::vector<int> plain_off;      // offsets of the one-family PLAIN64 recompute lists
+    std::vector<int> main_order;     // main-pass launch order (longest first)
     int plain_list_size = 0;
     int n_tiles = 0, n_segs = 0, maxLR = 0;
     int waves_per_cu = 16;           // persistent grid: resident waves per CU
@@ -780,6 +783,27 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         place_partials(kc.s32, false, ctx->n_slots, kc.slot_tiles, ctx->n_leaves);
         place_partials(kc.s64, true, ctx->n_slots, kc.slot_tiles, ctx->n_leaves);
     }
+    // Main-pass classes to side streams, longest first onto the least loaded
+    // stream.  Estimated cost: contraction work (tiles x nt^2, plus a per-tile
+    // overhead) and a floor for the serial chain of one large tile.
+    {
+        std::vector<double> cost(n_cls);
+        std::vector<int> order(n_cls);
+        for (int ci = 0; ci < n_cls; ++ci) {
+            const KernelClass& kc = pp.classes[ci];
+            const double tiles = kc.end - kc.begin;
+            cost[ci] = tiles * (kc.nt * kc.nt + 8.0) + 2500.0 * kc.nt * kc.nt;
+            order[ci] = ci;
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return cost[a] > cost[b2]; });
+        std::vector<double> load(kSideStreams, 0.0);
+        for (int ci : order) {
+            const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            pp.classes[ci].stream = k;
+            load[k] += cost[ci];
+        }
+        pp.main_order = order;
+    }
     pp.n_tiles = (int)tiles.size();
     pp.class_counts.assign(n_cls + 4, 0);
     for (int i = 0; i < n_cls; ++i) pp.class_counts[i] = pp.classes[i].end - pp.classes[i].begin;
@@ -895,6 +919,9 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
     // One pass over the size classes: launch(ci, stream) for each.  When
     // concurrent, class ci runs on side stream ci % S after the work already
     // on st, and st waits for every side stream before going on.
+    // The main pass launches in pp.main_order on each class's assigned stream;
+    // the recompute passes (lists sized on the device) take classes in order,
+    // largest vectors first, round robin.
     const int S = concurrent ? std::min((int)ctx->side.size(), n_cls) : 0;
     auto pass = [&](int pass_idx, auto&& launch) -> int {
         if (S <= 1) {
@@ -905,8 +932,11 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
         hipEvent_t* ev = ctx->join_ev.data() + pass_idx * (kSideStreams + 1);
         HIP_OR_FAIL(hipEventRecord(ev[0], st));
         for (int k = 0; k < S; ++k) HIP_OR_FAIL(hipStreamWaitEvent(ctx->side[k], ev[0], 0));
-        for (int ci = 0; ci < n_cls; ++ci)
-            if (int rc = launch(ci, ctx->side[ci % S])) return rc;
+        for (int i = 0; i < n_cls; ++i) {
+            const int ci = pass_idx == 0 ? pp.main_order[i] : i;
+            const int k = pass_idx == 0 ? pp.classes[ci].stream % S : i % S;
+            if (int rc = launch(ci, ctx->side[k])) return rc;
+        }
         for (int k = 0; k < S; ++k) {
             HIP_OR_FAIL(hipEventRecord(ev[1 + k], ctx->side[k]));
             HIP_OR_FAIL(hipStreamWaitEvent(st, ev[1 + k], 0));

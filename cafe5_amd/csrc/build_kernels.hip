@@ -114,88 +114,21 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
 }
 
 // ---------------------------------------------------------------------------
-// The same recurrence read along a row: for fixed s,
-//     P_s(c) = a P_s(c-1) + K_c,   K_c = a P_{s-1}(c) + (1-2a) P_{s-1}(c-1),
-// a first-order scan over c from row s-1 (row 0 is delta_0).  One wavefront per
-// matrix with lanes owning columns c = l + 64 b writes the row-major FP32 pool
-// with contiguous stores (values computed in FP64, clamped to [0,1], scaled by
-// 2^kMatExp and rounded once).
+// Row-major FP32 pool from the c-major FP32 copy: 32 x 32 tiles through LDS
+// (reads contiguous along s, writes contiguous along c), padding included.
 // ---------------------------------------------------------------------------
-template <int NB>
-__global__ __launch_bounds__(64) void k_build_recurrence_rows(const double* __restrict__ lam_q,
-                                                               const double* __restrict__ t_q,
-                                                               int first_slot, int N, int Np,
-                                                               long long slot_elems,
-                                                               float* __restrict__ mt32)
+__global__ __launch_bounds__(256) void k_transpose32(const float* __restrict__ lf32, float* __restrict__ mt32,
+                                                     int first_slot, int Np, long long slot_elems)
 {
-    const int key = blockIdx.x;
-    const int lane = threadIdx.x;
-    const double lt = lam_q[key] * t_q[key];
-    const double alpha = lt / (1 + lt);
-    const double coeff = 1 - 2 * alpha;
-    float* M32 = mt32 + (long long)(first_slot + key) * slot_elems;
-    const float one = (float)ldexp(1.0, kMatExp);
-
-    // row 0: delta_0 (also the whole matrix of a degenerate key)
+    __shared__ float tile[32][33];
+    const long long base = (long long)(first_slot + blockIdx.z) * slot_elems;
+    const int c0 = blockIdx.y * 32, s0 = blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        const int c = lane + 64 * b;
-        if (c < N) M32[c] = c == 0 ? one : 0.f;
-    }
-    if (degenerate_key(alpha, coeff)) {
-        for (int s = 1; s < N; ++s)
+    for (int k = 0; k < 32; k += 8) tile[ty + k][tx] = lf32[base + (long long)(c0 + ty + k) * Np + s0 + tx];
+    __syncthreads();
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const int c = lane + 64 * b;
-                if (c < N) M32[(long long)s * Np + c] = 0.f;
-            }
-        return;
-    }
-
-    double apow[6];
-    apow[0] = alpha;
-    for (int i = 1; i < 6; ++i) apow[i] = apow[i - 1] * apow[i - 1];
-    double a_lane = 1.0;
-    {
-        int e = lane + 1;
-        double base = alpha;
-        while (e) {
-            if (e & 1) a_lane *= base;
-            base *= base;
-            e >>= 1;
-        }
-    }
-
-    double prev[NB];   // P_{s-1}(c) for the lane's columns
-#pragma unroll
-    for (int b = 0; b < NB; ++b) prev[b] = (lane + 64 * b == 0) ? 1.0 : 0.0;
-
-    for (int s = 1; s < N; ++s) {
-        double K[NB];
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            double left = __shfl_up(prev[b], 1);   // P_{s-1}(c-1)
-            const double from_prev_block = b > 0 ? __shfl(prev[b > 0 ? b - 1 : 0], 63) : 0.0;
-            if (lane == 0) left = from_prev_block;
-            K[b] = alpha * prev[b] + coeff * left;
-        }
-        double carry = 0.0;
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            double y = K[b];
-#pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                const int d = 1 << i;
-                const double up = __shfl_up(y, d);
-                if (lane >= d) y += apow[i] * up;
-            }
-            if (b > 0) y += a_lane * carry;
-            carry = __shfl(y, 63);
-            prev[b] = y;
-            const int c = lane + 64 * b;
-            if (c < N) M32[(long long)s * Np + c] = (float)ldexp(y < 1.0 ? y : 1.0, kMatExp);
-        }
-    }
+    for (int k = 0; k < 32; k += 8) mt32[base + (long long)(s0 + ty + k) * Np + c0 + tx] = tile[tx][ty + k];
 }
 
 // ---------------------------------------------------------------------------
@@ -316,6 +249,21 @@ __global__ void k_fill_zero_u32(uint32_t* __restrict__ p, long long n)
 // Host-side launchers (called from api.cpp).
 namespace cafe {
 
+// Row-major FP32 copy of slots first_slot .. first_slot+n-1 (Np is a
+// multiple of 32).
+static hipError_t launch_transpose32(const float* lf32, float* mt32, int first_slot, int n, int Np,
+                                     long long slot_elems, hipStream_t stream)
+{
+    if (!mt32 || !lf32) return hipSuccess;
+    if (Np % 32 != 0) return hipErrorInvalidValue;
+    for (int k0 = 0; k0 < n; k0 += 65535) {
+        const int nk = n - k0 < 65535 ? n - k0 : 65535;
+        hipLaunchKernelGGL(k_transpose32, dim3(Np / 32, Np / 32, nk), dim3(256), 0, stream, lf32, mt32,
+                           first_slot + k0, Np, slot_elems);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const double* d_lg,
                         int n, int first_slot, int N, int Np, long long slot_elems, double* mt64,
                         float* mt32, float* lf32, hipStream_t stream)
@@ -335,10 +283,7 @@ hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const 
     if (nb <= NB) {                                                                                       \
         hipLaunchKernelGGL(k_build_recurrence<NB>, dim3(n), dim3(64), 0, stream, d_lam, d_t, first_slot, N, \
                            Np, slot_elems, mt64, lf32);                                                   \
-        if (mt32)                                                                                         \
-            hipLaunchKernelGGL(k_build_recurrence_rows<NB>, dim3(n), dim3(64), 0, stream, d_lam, d_t,      \
-                               first_slot, N, Np, slot_elems, mt32);                                     \
-        return hipGetLastError();                                                                         \
+        return launch_transpose32(lf32, mt32, first_slot, n, Np, slot_elems, stream);                     \
     }
         CAFE_BUILD_NB(1)
         CAFE_BUILD_NB(2)

@@ -13,8 +13,8 @@ namespace cafe {
 // mt32[k * slot_elems + s * Np + c] (row-major, so one 16-byte load gives an
 // MFMA lane four consecutive contraction entries); and in the FP32 leaf pool
 // at lf32[k * slot_elems + c * Np + s] (c-major: one 16-byte load gives a lane
-// four consecutive rows of a leaf column).  Np = round_up(N, 16) + 16 leaves
-// room for 16-row tiles; padding is 0.
+// four consecutive rows of a leaf column).  Np = round_up(N + 16, 32) leaves
+// room for 16-row tiles (Np >= round_up(N, 16) + 16); padding is 0.
 constexpr int kTile = 16;
 constexpr int kMaxSlots = 24;         // live partial vectors per family tile
 constexpr int kLgammaN = 4096;        // log-gamma table (integers < 4096)
