@@ -97,11 +97,11 @@ def time_plan(plan: native.Plan, steps: int, warmup: int, profile_steps: int):
     for _ in range(profile_steps):
         ms, graph_ok = plan.profile()
         phase += ms
-        nt, tiles, cms = plan.class_times()
-        cls = (nt, tiles, cms if cls is None else cls[2] + cms)
+        nt, tiles, cms, launch = plan.class_times()
+        cls = (nt, tiles, cms if cls is None else cls[2] + cms, launch)
     n = max(1, profile_steps)
     if cls is not None:
-        cls = (cls[0], cls[1], cls[2] / n)
+        cls = (cls[0], cls[1], cls[2] / n, cls[3])
     return (t1 - t0) * 1e3 / steps, phase / n, graph_ok, cls
 
 
@@ -328,7 +328,7 @@ def main():
     # ---- roofline of the dominant kernel: the FP32 main-pass launch of the
     # size class that takes longest (per-class HIP events, same stream) ----
     n1, n2 = tree_flop_factors(w.tree)
-    nt_cls, tiles_cls, ms_cls = cls32
+    nt_cls, tiles_cls, ms_cls, launch_cls = cls32
     dom = int(np.argmax(ms_cls))
     dom_nt = int(nt_cls[dom])
     dom_flops = 0.0
@@ -351,8 +351,10 @@ def main():
                 "algorithmic_flops_per_launch": dom_flops, "avg_ms": dom_ms,
                 "flops_per_family_likelihood": "2 L (n1 L + n2 R), n1=%d n2=%d" % (n1, n2),
                 "traffic_source": traffic["source"] if traffic else None,
-                "classes": [{"tiles_per_vector": int(a_), "tiles": int(b_), "ms": float(c_)}
-                            for a_, b_, c_ in zip(nt_cls, tiles_cls, ms_cls)],
+                "classes": [{"tiles_per_vector": int(a_), "tiles": int(b_), "ms": float(c_),
+                             "launch": {"tiles_per_wave": int(l_[0]), "waves": int(l_[1]),
+                                        "partials_in_lds": bool(l_[2]), "workgroups": int(l_[3])}}
+                            for a_, b_, c_, l_ in zip(nt_cls, tiles_cls, ms_cls, launch_cls)],
                 "prune_phase": {"ms": ph32[1], "algorithmic_flops": phase_flops,
                                 "TFLOP_per_s": phase_flops / (ph32[1] * 1e-3) / 1e12 if ph32[1] > 0 else None},
                 "matrix_build": {"ms": ph32[0], "bytes_written": build_bytes,

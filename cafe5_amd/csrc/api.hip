@@ -23,6 +23,7 @@ hipError_t launch_fold(const double* mt64, const float* lf32, double* out64, flo
                        int first_slot, int n, hipStream_t stream);
 hipError_t launch_prune_plain(int nq, int grid, const PruneParams& p, hipStream_t st);
 hipError_t launch_prune_mfma(bool fp64, const MfmaShape& s, int grid, const PruneParams& p, hipStream_t st);
+hipError_t mfma_occupancy(bool fp64, const MfmaShape& s, int* per_cu);
 hipError_t launch_pack(const SegDesc* segs, int n_segs, const BucketDev* buckets, const int* perm,
                        const int* flags, int F, int mask, int* fam_list, TileDesc* out_tiles,
                        const int* class_off, int* class_count, int* fam_count, int tile, hipStream_t st);
@@ -592,6 +593,13 @@ struct PrunePlan {
         const int g = std::max(1, std::min(n_cu * waves_per_cu / std::max(1, w), max_tiles));
         return g >= 8 ? g / 8 * 8 : g;
     }
+    // an MFMA launch: the workgroups of its shape that are resident at once
+    int grid(const MfmaShape& s, int max_tiles) const
+    {
+        const int resident = s.per_cu > 0 ? n_cu * s.per_cu : n_cu * waves_per_cu / std::max(1, s.w);
+        const int g = std::max(1, std::min(resident, max_tiles));
+        return g >= 8 ? g / 8 * 8 : g;
+    }
 };
 
 // MFMA launch shape for vectors of nt 16-row tiles: W waves per workgroup
@@ -605,8 +613,7 @@ MfmaShape mfma_shape(int nt, bool fp64)
     else if (nt <= 8) s = {(nt + 1) / 2, 2, false, 0};
     else if (nt <= 16) s = {(nt + 3) / 4, 4, false, 0};
     else if (nt <= 32) s = {(nt + 7) / 8, 8, false, 0};
-    else if (!fp64) s = {(nt + 15) / 16, 16, false, 0};
-    else s = {4, 8, false, 0};   // FP64 above 512 rows: several passes
+    else s = {(nt + 15) / 16, 16, false, 0};   // up to 64 tiles
     return s;
 }
 
@@ -738,9 +745,9 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     std::vector<std::pair<int, int>> bucket_class;   // (class key, bucket)
     for (int bi = 0; bi < (int)ctx->buckets.size(); ++bi) {
         const BucketHost& bk = ctx->buckets[bi];
-        // PLAIN64 (which also recomputes the other paths' hard families)
-        // covers 16 x 64 rows
-        if (std::max(bk.B + 1, bk.R) > 1024)
+        // vectors (sizes 0..B) and root rows (sizes 0..R) up to 1024 rows:
+        // PLAIN64 covers 16 x 64, the MFMA shapes 64 tiles of 16
+        if (std::max(bk.B + 1, bk.R + 1) > 1024)
             return fail(CAFE_GPU_EINVAL, "bucket B=%d R=%d exceeds the sizes this path supports", bk.B, bk.R);
         bucket_class.push_back({path == CAFE_GPU_PATH_PLAIN64 ? nq_of(bk) : nt_of(bk), bi});
     }
@@ -782,6 +789,8 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         kc.s64 = mfma_shape(kc.nt, true);
         place_partials(kc.s32, false, ctx->n_slots, kc.slot_tiles, ctx->n_leaves);
         place_partials(kc.s64, true, ctx->n_slots, kc.slot_tiles, ctx->n_leaves);
+        HIP_OR_FAIL(mfma_occupancy(false, kc.s32, &kc.s32.per_cu));
+        HIP_OR_FAIL(mfma_occupancy(true, kc.s64, &kc.s64.per_cu));
     }
     // Main-pass classes to side streams, longest first onto the least loaded
     // stream.  Estimated cost: contraction work (tiles x nt^2, plus a per-tile
@@ -850,13 +859,13 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         size_t mb = 0;
         if (path == CAFE_GPU_PATH_PLAIN64) mb = plain_bytes(n);
         else if (path == CAFE_GPU_PATH_BATCHED64)
-            mb = mfma_scratch_bytes(kc.s64, true, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s64.w, n));
-        else mb = mfma_scratch_bytes(kc.s32, false, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s32.w, n));
+            mb = mfma_scratch_bytes(kc.s64, true, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s64, n));
+        else mb = mfma_scratch_bytes(kc.s32, false, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s32, n));
         kc.off_main = main_sz;
         main_sz += align256(mb);
         if (path == CAFE_GPU_PATH_BATCHED32) {
             kc.off_r64 = r64_sz;
-            r64_sz += align256(mfma_scratch_bytes(kc.s64, true, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s64.w, n)));
+            r64_sz += align256(mfma_scratch_bytes(kc.s64, true, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s64, n)));
         }
         if (path != CAFE_GPU_PATH_PLAIN64) {
             kc.off_rp = rp_sz;
@@ -959,9 +968,9 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
                 q.scratch_per_wg = pp.per_wg_plain;
                 HIP_OR_FAIL(launch_prune_plain(kc.nq, pp.grid(1, n), q, s));
             } else if (pp.path == CAFE_GPU_PATH_BATCHED64) {
-                HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64.w, n), q, s));
+                HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64, n), q, s));
             } else {
-                HIP_OR_FAIL(launch_prune_mfma(false, kc.s32, pp.grid(kc.s32.w, n), q, s));
+                HIP_OR_FAIL(launch_prune_mfma(false, kc.s32, pp.grid(kc.s32, n), q, s));
             }
             return CAFE_GPU_OK;
         }))
@@ -984,7 +993,7 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
                 q.tile_counter = d_ctr + n_cls + ci;
                 q.slot_tiles = kc.slot_tiles;
                 q.scratch = scratch + kc.off_r64;
-                HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64.w, pp.class_counts[ci]), q, s));
+                HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64, pp.class_counts[ci]), q, s));
                 return CAFE_GPU_OK;
             }))
             return rc;
@@ -1282,7 +1291,7 @@ int cafe_gpu_plan_profile(cafe_gpu_plan* pl, double* ms4, int* graph_ok)
 }
 
 int cafe_gpu_plan_class_times(cafe_gpu_plan* pl, int cap, int* n_classes, int* class_nt, long long* class_tiles,
-                              double* ms)
+                              double* ms, int* launch)
 {
     if (!pl || !n_classes) return fail(CAFE_GPU_EINVAL, "null argument");
     const int n = (int)pl->pp.classes.size();
@@ -1292,6 +1301,23 @@ int cafe_gpu_plan_class_times(cafe_gpu_plan* pl, int cap, int* n_classes, int* c
         if (class_nt) class_nt[i] = pl->pp.classes[i].nt;
         if (class_tiles) class_tiles[i] = pl->pp.class_counts[i];
         if (ms) ms[i] = pl->cls_ms[i];
+        if (launch) {
+            const KernelClass& kc = pl->pp.classes[i];
+            const int n = pl->pp.class_counts[i];
+            int* l = launch + 4 * i;
+            if (pl->path == CAFE_GPU_PATH_PLAIN64) {
+                l[0] = kc.nq;
+                l[1] = 1;
+                l[2] = 0;
+                l[3] = pl->pp.grid(1, n);
+            } else {
+                const MfmaShape& sh = pl->path == CAFE_GPU_PATH_BATCHED64 ? kc.s64 : kc.s32;
+                l[0] = sh.ntw;
+                l[1] = sh.w;
+                l[2] = sh.lds ? 1 : 0;
+                l[3] = pl->pp.grid(sh, n);
+            }
+        }
     }
     return CAFE_GPU_OK;
 }

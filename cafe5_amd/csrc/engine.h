@@ -150,6 +150,7 @@ struct MfmaShape {
     int w;
     bool lds;
     size_t shm;
+    int per_cu;   // workgroups resident on one CU (registers and LDS)
 };
 
 // Dynamic-LDS bookkeeping of the scaled FP32 kernel for n_slots partial

@@ -108,6 +108,15 @@ __device__ __forceinline__ doublex4 load_a(const double* M, int off, int kt, int
 
 // Operand prefetch depth: FP32 prefetches k-tiles 0 and 1 of an internal
 // step (two 16-byte loads per output tile), FP64 k-tile 0 (four 8-byte loads).
+// Makes the wave wait for a loaded value without using it (an empty asm that
+// reads each element), so that no load of it is left in flight.
+template <typename V>
+__device__ __forceinline__ void await_loaded(const V& v)
+{
+#pragma unroll
+    for (int j = 0; j < 4; ++j) asm volatile("" ::"v"(v[j]));
+}
+
 template <typename T> struct Prefetch { static constexpr int PK = sizeof(T) == 4 ? 2 : 1; };
 
 constexpr double kLn2 = 0.69314718055994530942;
@@ -184,12 +193,21 @@ __device__ __forceinline__ void mma_step(typename Vec4<T>::type (&x)[NTW], const
 
 // x[i] = sum_x M(srow_i, x) v(x) over the child's nts k-tiles.  The first
 // PK k-tiles' A operands arrive prefetched (pf, loaded during the previous
-// step); the remaining ones (vectors of more than PK 16-row tiles) run two
-// k-tiles per iteration in ping-pong buffers, one buffer's loads in flight
-// while the other is multiplied.  Those loads are unconditional: a k-tile
-// past the end re-reads the last one with its B fragment zeroed, which adds
-// exactly 0, so each wait covers only older loads.
-template <typename T, int NTW, bool SCALED>
+// step); the remaining ones (vectors of more than PK 16-row tiles) stream
+// through a ring of D buffers, D k-tiles' loads in flight while one is
+// multiplied (the large vectors read their matrices from HBM).  Loads are
+// unconditional: a k-tile past the end re-reads the last one with its B
+// fragment zeroed, which adds exactly 0, so each wait covers only the oldest
+// buffer.  A one-wave kernel holds whole vectors (nts <= NTW), so its ring
+// only needs the NTW - PK k-tiles past the prefetched ones (none for NTW <= PK).
+template <typename T, int NTW, int W> struct Ring {
+    // wider workgroups leave fewer registers per wave (W = 16: 128 VGPRs)
+    static constexpr int deep = W >= 16 ? 1 : (sizeof(T) == 4 && W < 8 ? 4 : 2);
+    static constexpr int tail = NTW > Prefetch<T>::PK ? NTW - Prefetch<T>::PK : 0;
+    static constexpr int D = W == 1 ? tail : deep;
+};
+
+template <typename T, int NTW, int D, bool SCALED>
 __device__ __forceinline__ void child_gemm(typename Vec4<T>::type (&x)[NTW], const T* __restrict__ M, int Np,
                                            const int (&ao)[NTW],
                                            const typename Vec4<T>::type (&pf)[Prefetch<T>::PK][NTW],
@@ -203,26 +221,32 @@ __device__ __forceinline__ void child_gemm(typename Vec4<T>::type (&x)[NTW], con
         if (PK > 1 && nts > 1) {
             const V4 b1 = bsrc[64];
             mma_step<T, NTW, SCALED>(x, pf[PK - 1], b1, bsh);
+        } else if (PK > 1) {
+            // unused second k-tile: still wait for it (see the leaf step)
+#pragma unroll
+            for (int i = 0; i < NTW; ++i) await_loaded(pf[PK - 1][i]);
         }
     }
+    if constexpr (D == 0) return;
     if (nts <= PK) return;
-    V4 a0[NTW], a1[NTW], b0, b1;
+    V4 a[D > 0 ? D : 1][NTW], bb[D > 0 ? D : 1];
 #pragma unroll
-    for (int i = 0; i < NTW; ++i) a0[i] = load_a(M, ao[i], PK, Np);
-    b0 = bsrc[PK * 64];
-    for (int kt = PK; kt < nts; kt += 2) {
-        const bool has1 = kt + 1 < nts;
-        const int k1 = has1 ? kt + 1 : kt;
+    for (int d = 0; d < D; ++d) {
+        const int k = PK + d < nts ? PK + d : nts - 1;
 #pragma unroll
-        for (int i = 0; i < NTW; ++i) a1[i] = load_a(M, ao[i], k1, Np);
-        b1 = bsrc[k1 * 64];
-        mma_step<T, NTW, SCALED>(x, a0, b0, bsh);
-        const int k2 = kt + 2 < nts ? kt + 2 : nts - 1;
+        for (int i = 0; i < NTW; ++i) a[d][i] = load_a(M, ao[i], k, Np);
+        bb[d] = PK + d < nts ? bsrc[k * 64] : V4{0, 0, 0, 0};
+    }
+    for (int kt = PK; kt < nts; kt += D) {
 #pragma unroll
-        for (int i = 0; i < NTW; ++i) a0[i] = load_a(M, ao[i], k2, Np);
-        b0 = bsrc[k2 * 64];
-        if (!has1) b1 = V4{0, 0, 0, 0};
-        mma_step<T, NTW, SCALED>(x, a1, b1, bsh);
+        for (int d = 0; d < D; ++d) {
+            mma_step<T, NTW, SCALED>(x, a[d], bb[d], bsh);
+            const int kn = kt + D + d;   // the k-tile this buffer holds next
+            const int k = kn < nts ? kn : nts - 1;
+#pragma unroll
+            for (int i = 0; i < NTW; ++i) a[d][i] = load_a(M, ao[i], k, Np);
+            bb[d] = kn < nts ? bsrc[k * 64] : V4{0, 0, 0, 0};
+        }
     }
 }
 
@@ -251,8 +275,12 @@ __device__ __forceinline__ void child_gemm(typename Vec4<T>::type (&x)[NTW], con
 // ---------------------------------------------------------------------------
 // Single-wave kernels with small per-wave tiles ask for 4 waves per SIMD (the
 // register budget that still holds both prefetch buffers without spilling).
+template <int NTW, int W, int BYTES> struct WavesHint {
+    static constexpr int v = (W == 1 && NTW * BYTES <= 8) ? 4 : 1;
+};
+
 template <typename T, int NTW, int W, bool SCALED, bool LDSP>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu((W == 1 && NTW * sizeof(T) <= 8) ? 4 : 1)))
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WavesHint<NTW, W, (int)sizeof(T)>::v)))
 void k_prune_mfma(PruneParams p)
 {
     typedef typename Vec4<T>::type V4;
@@ -411,6 +439,13 @@ void k_prune_mfma(PruneParams p)
                 }
 #pragma unroll
                 for (int i = 0; i < NTW; ++i) x[i] = pf[0][i];
+                // The leaf step ignores the other prefetched k-tiles; taking
+                // them here makes both paths leave no load of this buffer in
+                // flight, so its next prefetch needs no wait on the other one.
+#pragma unroll
+                for (int k = 1; k < PK; ++k)
+#pragma unroll
+                    for (int i = 0; i < NTW; ++i) await_loaded(pf[k][i]);
             } else {
                 int bsh = 0;
                 if (SCALED) {
@@ -441,7 +476,7 @@ void k_prune_mfma(PruneParams p)
                     ao[i] = root ? offA1[i] : offA0[i];
                 }
                 const V4* bsrc = frag + (long long)sd.in_slot * slot_tiles * 64 + lane;
-                child_gemm<T, NTW, SCALED>(x, M, Np, ao, pf, bsrc, nts, bsh);
+                child_gemm<T, NTW, Ring<T, NTW, W>::D, SCALED>(x, M, Np, ao, pf, bsrc, nts, bsh);
             }
 #pragma unroll
             for (int i = 0; i < NTW; ++i) acc[i] *= x[i];
@@ -508,6 +543,13 @@ void k_prune_mfma(PruneParams p)
                     const int j = kTile * tt[i] + 4 * g + r - 1;
                     pv[i][r] = pr[j < 0 ? 0 : (j < bk.R ? j : bk.R - 1)];
                 }
+            // every prior arrives here, whichever rows use it, so no load of
+            // them is left in flight when the next tile's prefetch reuses
+            // their registers (that would stall every step's prefetch)
+#pragma unroll
+            for (int i = 0; i < NTW; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) asm volatile("" ::"v"(pv[i][r]));
             RootBest rb{-__builtin_huge_val(), 0.0, -1, 0.0};
 #pragma unroll
             for (int i = 0; i < NTW; ++i) {
@@ -581,6 +623,15 @@ void k_prune_mfma(PruneParams p)
                 consume(pfb, sdb);
             }
         }
+        // the last prefetch (past the end) is never consumed: take it here, so
+        // the next tile's step loop starts with no load in flight
+#pragma unroll
+        for (int k = 0; k < PK; ++k)
+#pragma unroll
+            for (int i = 0; i < NTW; ++i) {
+                await_loaded(pfa[k][i]);
+                await_loaded(pfb[k][i]);
+            }
         __syncthreads();   // every wave is done with this tile's LDS before the next one
     }
 }
@@ -589,38 +640,52 @@ void k_prune_mfma(PruneParams p)
 // Launcher: one kernel per (tiles per wave, waves per workgroup, partials in
 // LDS) shape; the host picks the shape per size class.
 // ---------------------------------------------------------------------------
+// The kernel shapes (tiles per wave, waves per workgroup, partials in LDS);
+// (3|4, 16) cover vectors of up to 64 tiles (1024 rows) in one pass.
+#define CAFE_MFMA_SHAPES(X, T, S) \
+    X(T, 1, 1, S, true) X(T, 1, 1, S, false) X(T, 2, 1, S, true) X(T, 2, 1, S, false) \
+    X(T, 3, 1, S, true) X(T, 3, 1, S, false) X(T, 4, 1, S, true) X(T, 4, 1, S, false) \
+    X(T, 3, 2, S, true) X(T, 3, 2, S, false) X(T, 4, 2, S, true) X(T, 4, 2, S, false) \
+    X(T, 3, 4, S, true) X(T, 3, 4, S, false) X(T, 4, 4, S, true) X(T, 4, 4, S, false) \
+    X(T, 3, 8, S, false) X(T, 4, 8, S, false)
+#define CAFE_MFMA_WIDE(X, T, S) X(T, 3, 16, S, false) X(T, 4, 16, S, false)
+
+// One instance per shape; with per_cu set, reports how many workgroups of it
+// fit on one CU instead of launching.
 template <typename T, bool SCALED>
-static hipError_t launch_shape(const MfmaShape& s, int grid, const PruneParams& p, hipStream_t st)
+static hipError_t shape_dispatch(const MfmaShape& s, int grid, const PruneParams& p, hipStream_t st, int* per_cu)
 {
-#define CAFE_MFMA_CASE(NTW, W, LDSP)                                                                    \
-    if (s.ntw == NTW && s.w == W && s.lds == LDSP) {                                                    \
-        hipLaunchKernelGGL((k_prune_mfma<T, NTW, W, SCALED, LDSP>), dim3(grid), dim3(64 * W), s.shm, st, p); \
-        return hipGetLastError();                                                                       \
+#define CAFE_MFMA_CASE(T_, NTW, W, S_, LDSP)                                                                 \
+    if (s.ntw == NTW && s.w == W && s.lds == LDSP) {                                                        \
+        if (per_cu)                                                                                         \
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_prune_mfma<T, NTW, W, SCALED, LDSP>, \
+                                                                64 * W, s.shm);                             \
+        hipLaunchKernelGGL((k_prune_mfma<T, NTW, W, SCALED, LDSP>), dim3(grid), dim3(64 * W), s.shm, st, p);  \
+        return hipGetLastError();                                                                           \
     }
-#define CAFE_MFMA_BOTH(NTW, W) CAFE_MFMA_CASE(NTW, W, true) CAFE_MFMA_CASE(NTW, W, false)
-    CAFE_MFMA_BOTH(1, 1)
-    CAFE_MFMA_BOTH(2, 1)
-    CAFE_MFMA_BOTH(3, 1)
-    CAFE_MFMA_BOTH(4, 1)
-    CAFE_MFMA_BOTH(3, 2)
-    CAFE_MFMA_BOTH(4, 2)
-    CAFE_MFMA_BOTH(3, 4)
-    CAFE_MFMA_BOTH(4, 4)
-    CAFE_MFMA_CASE(3, 8, false)
-    CAFE_MFMA_CASE(4, 8, false)
-    if constexpr (SCALED) {   // FP64 covers large vectors in several passes of (4, 8)
-        CAFE_MFMA_CASE(3, 16, false)
-        CAFE_MFMA_CASE(4, 16, false)
-    }
-#undef CAFE_MFMA_BOTH
+    CAFE_MFMA_SHAPES(CAFE_MFMA_CASE, T, SCALED)
+    CAFE_MFMA_WIDE(CAFE_MFMA_CASE, T, SCALED)
 #undef CAFE_MFMA_CASE
     return hipErrorInvalidValue;
+}
+
+// Workgroups of a shape that fit on one CU at once (registers and LDS), so a
+// launch can size its grid to what is resident: the static tile split gives
+// every workgroup an equal share, and a workgroup that has to wait for a
+// slot would finish its share after all the others.
+hipError_t mfma_occupancy(bool fp64, const MfmaShape& s, int* per_cu)
+{
+    *per_cu = 0;
+    const PruneParams none{};
+    return fp64 ? shape_dispatch<double, false>(s, 0, none, nullptr, per_cu)
+                : shape_dispatch<float, true>(s, 0, none, nullptr, per_cu);
 }
 
 hipError_t launch_prune_mfma(bool fp64, const MfmaShape& s, int grid, const PruneParams& p, hipStream_t st)
 {
     if (grid < 1) grid = 1;
-    return fp64 ? launch_shape<double, false>(s, grid, p, st) : launch_shape<float, true>(s, grid, p, st);
+    return fp64 ? shape_dispatch<double, false>(s, grid, p, st, nullptr)
+                : shape_dispatch<float, true>(s, grid, p, st, nullptr);
 }
 
 }  // namespace cafe

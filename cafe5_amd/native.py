@@ -75,7 +75,7 @@ def load_library() -> ctypes.CDLL:
         "cafe_gpu_plan_results": (_i, [_p, _pd, _pd, _pd, _pd, _pll]),
         "cafe_gpu_plan_profile": (_i, [_p, _pd, _pi]),
         "cafe_gpu_plan_recompute_counts": (_i, [_p, _pll, _pll]),
-        "cafe_gpu_plan_class_times": (_i, [_p, _i, _pi, _pi, _pll, _pd]),
+        "cafe_gpu_plan_class_times": (_i, [_p, _i, _pi, _pi, _pll, _pd, _pi]),
         "cafe_gpu_plan_destroy": (_i, [_p]),
     }
     for name, (res, args) in sig.items():
@@ -278,15 +278,18 @@ class Plan:
 
     def class_times(self):
         """Per size class of the main pass, from the last profile() run:
-        (16-row tiles per vector, 16-family tiles, kernel ms) arrays."""
+        (16-row tiles per vector, 16-family tiles, kernel ms, launch) arrays;
+        launch rows are (tiles per wave, waves per workgroup, partials in
+        LDS, workgroups)."""
         n = ctypes.c_int(0)
-        self.eng._check(self.lib.cafe_gpu_plan_class_times(self.h, 0, ctypes.byref(n), None, None, None))
+        self.eng._check(self.lib.cafe_gpu_plan_class_times(self.h, 0, ctypes.byref(n), None, None, None, None))
         nt = np.zeros(n.value, dtype=np.int32)
         tiles = np.zeros(n.value, dtype=np.int64)
         ms = np.zeros(n.value)
+        launch = np.zeros((n.value, 4), dtype=np.int32)
         self.eng._check(self.lib.cafe_gpu_plan_class_times(self.h, n.value, ctypes.byref(n), _ip(nt),
-                                                           tiles.ctypes.data_as(_pll), _dp(ms)))
-        return nt, tiles, ms
+                                                           tiles.ctypes.data_as(_pll), _dp(ms), _ip(launch)))
+        return nt, tiles, ms, launch
 
     def recompute_counts(self):
         """(families redone on BATCHED64, families redone on PLAIN64)."""

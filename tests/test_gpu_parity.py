@@ -244,3 +244,52 @@ def test_tiny_likelihood_semantics(eng, path):
         ref = Oracle.evaluate(tree, np.full(tree.n_nodes, lam), counts, B, R, prior)
         assert np.array_equal(np.isfinite(m.results), np.isfinite(ref["lnl"])), (lam, m.results, ref["lnl"])
         assert_lnl(m.results, ref["lnl"], path, f"lambda {lam}")
+
+
+# --------------------------------------------------------------------------
+# large bounds: vectors of 32 (B = 500) and 38 (B = 600) 16-row tiles, the
+# widest kernel shapes (8 and 16 waves per workgroup)
+# --------------------------------------------------------------------------
+
+_LARGE = {}
+
+
+def _large_case(B, K):
+    """10-taxon tree, 40 families with root sizes spread up to 0.85 B, lambda
+    0.002; oracle results cached per (B, K) across the three paths."""
+    if (B, K) not in _LARGE:
+        tree = wl.yule_tree(10, 11)
+        roots = np.linspace(1, int(0.85 * B), 40).astype(np.int64)
+        counts = wl.simulate_families(tree, roots, np.full(tree.n_nodes, 0.002), 13, cap=B - 1)
+        prior = Oracle.prior_uniform(B, B)
+        if K:
+            f, r = Oracle.get_gamma(K, 1.0)
+            ref = Oracle.evaluate(tree, np.full(tree.n_nodes, 0.002), counts, B, B, prior, n_cat=K,
+                                  multipliers=r, cat_probs=f, threads=16)
+        else:
+            f = r = None
+            ref = Oracle.evaluate(tree, np.full(tree.n_nodes, 0.002), counts, B, B, prior, threads=16)
+        _LARGE[(B, K)] = (tree, counts, prior, f, r, ref)
+    return _LARGE[(B, K)]
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_large_bound_gamma_500(eng, path):
+    tree, counts, prior, f, r, ref = _large_case(500, 2)
+    assert counts.max() > 400   # rows of the last tiles carry data
+    m = GammaModel(eng, tree, counts, 500, 500, f, r, path=path, alpha=1.0)
+    total = m.infer_family_likelihoods(prior, 0.002)
+    assert_lnl(m.results["lnl"], ref["lnl"], path)
+    tol = 1e-6 if path == native.PATH_BATCHED32 else 1e-9
+    assert abs(total - ref["total"]) <= tol * abs(ref["total"])
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_large_bound_base_600(eng, path):
+    tree, counts, prior, _, _, ref = _large_case(600, 0)
+    assert counts.max() > 512   # rows beyond the first 32 tiles carry data
+    m = BaseModel(eng, tree, counts, 600, 600, path=path)
+    total = m.infer_family_likelihoods(prior, 0.002)
+    assert_lnl(m.results, ref["lnl"], path)
+    tol = 1e-6 if path == native.PATH_BATCHED32 else 1e-9
+    assert abs(total - ref["total"]) <= tol * abs(ref["total"])
