@@ -563,10 +563,12 @@ struct KernelClass {
     int slot_tiles;      // 16-row tiles of its largest stored (non-root) vector
     MfmaShape s32, s64;  // launch shapes of the FP32 and FP64 MFMA kernels
     int begin, end;      // its main tiles
+    int seg_begin = 0, seg_end = 0;   // its (bucket, point) segments
     // byte offsets of its global scratch in the main pass, the FP64 and the
-    // PLAIN64 recompute (classes of one pass run concurrently)
+    // PLAIN64 recompute: disjoint, since one class's recompute can run
+    // beside another class's main pass
     size_t off_main = 0, off_r64 = 0, off_rp = 0;
-    int stream = 0;      // side stream of its main-pass launch
+    int stream = 0;      // side stream of its launches
 };
 
 // Host-side description of one pruning sequence over a tile list.
@@ -593,11 +595,17 @@ struct PrunePlan {
         const int g = std::max(1, std::min(n_cu * waves_per_cu / std::max(1, w), max_tiles));
         return g >= 8 ? g / 8 * 8 : g;
     }
-    // an MFMA launch: the workgroups of its shape that are resident at once
+    // an MFMA launch: a multiple of the workgroups of its shape that are
+    // resident at once.  Each workgroup takes an equal static share of the
+    // tiles, so with many more workgroups than slots the dispatcher evens
+    // out the per-CU load as slots free up (16x: the small-vector classes run
+    // ~25% faster than with exactly one resident round).  Kernels with
+    // partials in global scratch, which grows with the grid, take 2x.
     int grid(const MfmaShape& s, int max_tiles) const
     {
-        const int resident = s.per_cu > 0 ? n_cu * s.per_cu : n_cu * waves_per_cu / std::max(1, s.w);
-        const int g = std::max(1, std::min(resident, max_tiles));
+        const int per_cu = s.per_cu > 0 ? s.per_cu : std::max(1, waves_per_cu / std::max(1, s.w));
+        const long long want = (long long)n_cu * per_cu * (s.lds ? 16 : 2);
+        const int g = (int)std::max(1LL, std::min(want, (long long)max_tiles));
         return g >= 8 ? g / 8 * 8 : g;
     }
 };
@@ -766,6 +774,7 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         if (pp.classes.empty() || bc.first != prev_key) {
             KernelClass kc{};
             kc.begin = kc.end = (int)tiles.size();
+            kc.seg_begin = kc.seg_end = (int)segs.size();
             pp.classes.push_back(kc);
             cls_fams.push_back(0);
             prev_key = bc.first;
@@ -778,10 +787,12 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         for (int pi = 0; pi < n_points; ++pi) {
             for (int f0 = 0; f0 < bk.count; f0 += kTile)
                 tiles.push_back(TileDesc{bc.second, point_ids[pi], f0, std::min(kTile, bk.count - f0)});
-            segs.push_back(SegDesc{bc.second, point_ids[pi], ci, 0});
+            for (int f0 = 0; f0 < bk.count; f0 += kSegChunk)
+                segs.push_back(SegDesc{bc.second, point_ids[pi], ci, f0});
         }
         cls_fams[ci] += bk.count * n_points;
         kc.end = (int)tiles.size();
+        kc.seg_end = (int)segs.size();
     }
     const int n_cls = (int)pp.classes.size();
     for (auto& kc : pp.classes) {
@@ -872,7 +883,11 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
             rp_sz += align256(plain_bytes(pp.plain_off[ci + 1] - pp.plain_off[ci]));
         }
     }
-    HIP_OR_FAIL(b.scratch.ensure(std::max<size_t>(std::max(main_sz, std::max(r64_sz, rp_sz)), 256)));
+    for (auto& kc : pp.classes) {
+        kc.off_r64 += main_sz;
+        kc.off_rp += main_sz + r64_sz;
+    }
+    HIP_OR_FAIL(b.scratch.ensure(std::max<size_t>(main_sz + r64_sz + rp_sz, 256)));
 
     PruneParams& p = pp.base;
     p.buckets = b.buckets.as<BucketDev>();
@@ -925,25 +940,89 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
     HIP_OR_FAIL(launch_fill_zero(b.counters.p, sizeof(int) * (3 * n_cls + 8), st));
     HIP_OR_FAIL(launch_fill_zero(b.fb_count.p, sizeof(int) * (2 * n_cls + 4), st));
 
-    // One pass over the size classes: launch(ci, stream) for each.  When
-    // concurrent, class ci runs on side stream ci % S after the work already
-    // on st, and st waits for every side stream before going on.
-    // The main pass launches in pp.main_order on each class's assigned stream;
-    // the recompute passes (lists sized on the device) take classes in order,
-    // largest vectors first, round robin.
+    // Three stages per size class: the main pass, then (BATCHED32) the FP64
+    // recompute of the families the FP32 bound did not certify, then
+    // (BATCHED32/64) the PLAIN64 recompute of those near the subnormal range.
+    // Each recompute stage packs its class's flagged families into a list
+    // sized on the device and prunes it.
+    TileDesc* fb64 = b.fb_tiles.as<TileDesc>();
+    TileDesc* fbplain = fb64 + std::max(1, pp.n_tiles);
+    const int* coff = b.class_off.as<int>();
+    const int fam_off = pp.fam_count_off();
+    const SegDesc* segs = b.segs.as<SegDesc>();
+    auto stage_main = [&](int ci, hipStream_t s) -> int {
+        const KernelClass& kc = pp.classes[ci];
+        const int n = pp.class_counts[ci];
+        PruneParams q = pp.base;
+        q.tiles = b.tiles.as<TileDesc>() + kc.begin;
+        q.n_tiles_dev = d_cnt + ci;
+        q.tile_counter = d_ctr + ci;
+        q.slot_tiles = kc.slot_tiles;
+        q.scratch = scratch + kc.off_main;
+        if (pp.path == CAFE_GPU_PATH_PLAIN64) {
+            q.scratch_per_wg = pp.per_wg_plain;
+            HIP_OR_FAIL(launch_prune_plain(kc.nq, pp.grid(1, n), q, s));
+        } else if (pp.path == CAFE_GPU_PATH_BATCHED64) {
+            HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64, n), q, s));
+        } else {
+            HIP_OR_FAIL(launch_prune_mfma(false, kc.s32, pp.grid(kc.s32, n), q, s));
+        }
+        return CAFE_GPU_OK;
+    };
+    auto stage_r64 = [&](int ci, hipStream_t s) -> int {
+        if (pp.path != CAFE_GPU_PATH_BATCHED32) return CAFE_GPU_OK;
+        const KernelClass& kc = pp.classes[ci];
+        HIP_OR_FAIL(launch_pack(segs + kc.seg_begin, kc.seg_end - kc.seg_begin, b.buckets.as<BucketDev>(),
+                                pp.base.perm, b.flag.as<int>(), pp.base.F, kFlagRecompute64, b.fam_list.as<int>(),
+                                fb64, coff, fbc, fbc + fam_off, kTile, s));
+        PruneParams q = pp.base;
+        q.tiles = fb64 + kc.begin;
+        q.n_tiles_dev = fbc + ci;
+        q.tile_counter = d_ctr + n_cls + ci;
+        q.slot_tiles = kc.slot_tiles;
+        q.scratch = scratch + kc.off_r64;
+        HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64, pp.class_counts[ci]), q, s));
+        return CAFE_GPU_OK;
+    };
+    auto stage_plain = [&](int ci, hipStream_t s) -> int {
+        if (pp.path == CAFE_GPU_PATH_PLAIN64) return CAFE_GPU_OK;
+        const KernelClass& kc = pp.classes[ci];
+        HIP_OR_FAIL(launch_pack(segs + kc.seg_begin, kc.seg_end - kc.seg_begin, b.buckets.as<BucketDev>(),
+                                pp.base.perm, b.flag.as<int>(), pp.base.F, kFlagRecomputePlain,
+                                b.fam_list.as<int>(), fbplain, coff + n_cls + 1, fbc + n_cls, fbc + fam_off + 1, 1,
+                                s));
+        PruneParams q = pp.base;
+        q.tiles = fbplain + pp.plain_off[ci];
+        q.n_tiles_dev = fbc + n_cls + ci;
+        q.tile_counter = d_ctr + 2 * n_cls + ci;
+        q.scratch = scratch + kc.off_rp;
+        q.scratch_per_wg = pp.per_wg_plain;
+        HIP_OR_FAIL(launch_prune_plain(kc.nq, pp.grid(1, pp.plain_off[ci + 1] - pp.plain_off[ci]), q, s));
+        return CAFE_GPU_OK;
+    };
+
+    // Stage by stage over the classes: every main pass, then every FP64
+    // recompute, then every PLAIN64 recompute.  When concurrent, the classes
+    // of one stage run on the side streams (the main passes on their
+    // longest-first assignment, the recomputes round robin) and st waits for
+    // all of them before the next stage: a recompute is a latency-bound chain
+    // on a few tiles, and beside a main pass it would slow that pass down.
     const int S = concurrent ? std::min((int)ctx->side.size(), n_cls) : 0;
-    auto pass = [&](int pass_idx, auto&& launch) -> int {
+    auto stage = [&](int idx, auto&& launch) -> int {
         if (S <= 1) {
-            for (int ci = 0; ci < n_cls; ++ci)
+            for (int ci = 0; ci < n_cls; ++ci) {
+                if (cls_ev && idx == 0) HIP_OR_FAIL(hipEventRecord((*cls_ev)[ci], st));
                 if (int rc = launch(ci, st)) return rc;
+            }
+            if (cls_ev && idx == 0) HIP_OR_FAIL(hipEventRecord((*cls_ev)[n_cls], st));
             return CAFE_GPU_OK;
         }
-        hipEvent_t* ev = ctx->join_ev.data() + pass_idx * (kSideStreams + 1);
+        hipEvent_t* ev = ctx->join_ev.data() + idx * (kSideStreams + 1);
         HIP_OR_FAIL(hipEventRecord(ev[0], st));
         for (int k = 0; k < S; ++k) HIP_OR_FAIL(hipStreamWaitEvent(ctx->side[k], ev[0], 0));
         for (int i = 0; i < n_cls; ++i) {
-            const int ci = pass_idx == 0 ? pp.main_order[i] : i;
-            const int k = pass_idx == 0 ? pp.classes[ci].stream % S : i % S;
+            const int ci = pp.main_order[i];
+            const int k = idx == 0 ? pp.classes[ci].stream % S : i % S;
             if (int rc = launch(ci, ctx->side[k])) return rc;
         }
         for (int k = 0; k < S; ++k) {
@@ -952,69 +1031,12 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
         }
         return CAFE_GPU_OK;
     };
-
-    // main pass
-    if (int rc = pass(0, [&](int ci, hipStream_t s) -> int {
-            const KernelClass& kc = pp.classes[ci];
-            const int n = pp.class_counts[ci];
-            if (cls_ev && S <= 1) HIP_OR_FAIL(hipEventRecord((*cls_ev)[ci], s));
-            PruneParams q = pp.base;
-            q.tiles = b.tiles.as<TileDesc>() + kc.begin;
-            q.n_tiles_dev = d_cnt + ci;
-            q.tile_counter = d_ctr + ci;
-            q.slot_tiles = kc.slot_tiles;
-            q.scratch = scratch + kc.off_main;
-            if (pp.path == CAFE_GPU_PATH_PLAIN64) {
-                q.scratch_per_wg = pp.per_wg_plain;
-                HIP_OR_FAIL(launch_prune_plain(kc.nq, pp.grid(1, n), q, s));
-            } else if (pp.path == CAFE_GPU_PATH_BATCHED64) {
-                HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64, n), q, s));
-            } else {
-                HIP_OR_FAIL(launch_prune_mfma(false, kc.s32, pp.grid(kc.s32, n), q, s));
-            }
-            return CAFE_GPU_OK;
-        }))
-        return rc;
-    if (cls_ev && S <= 1) HIP_OR_FAIL(hipEventRecord((*cls_ev)[n_cls], st));
+    if (int rc = stage(0, stage_main)) return rc;
     if (ev_main) HIP_OR_FAIL(hipEventRecord(ev_main, st));
-    TileDesc* fb64 = b.fb_tiles.as<TileDesc>();
-    TileDesc* fbplain = fb64 + std::max(1, pp.n_tiles);
-    const int* coff = b.class_off.as<int>();
-    const int fam_off = pp.fam_count_off();
-    if (pp.path == CAFE_GPU_PATH_BATCHED32) {
-        HIP_OR_FAIL(launch_pack(b.segs.as<SegDesc>(), pp.n_segs, b.buckets.as<BucketDev>(), pp.base.perm,
-                                b.flag.as<int>(), pp.base.F, kFlagRecompute64, b.fam_list.as<int>(), fb64, coff,
-                                fbc, fbc + fam_off, kTile, st));
-        if (int rc = pass(1, [&](int ci, hipStream_t s) -> int {
-                const KernelClass& kc = pp.classes[ci];
-                PruneParams q = pp.base;
-                q.tiles = fb64 + kc.begin;
-                q.n_tiles_dev = fbc + ci;
-                q.tile_counter = d_ctr + n_cls + ci;
-                q.slot_tiles = kc.slot_tiles;
-                q.scratch = scratch + kc.off_r64;
-                HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64, pp.class_counts[ci]), q, s));
-                return CAFE_GPU_OK;
-            }))
-            return rc;
-    }
-    if (pp.path != CAFE_GPU_PATH_PLAIN64) {
-        HIP_OR_FAIL(launch_pack(b.segs.as<SegDesc>(), pp.n_segs, b.buckets.as<BucketDev>(), pp.base.perm,
-                                b.flag.as<int>(), pp.base.F, kFlagRecomputePlain, b.fam_list.as<int>(), fbplain,
-                                coff + n_cls + 1, fbc + n_cls, fbc + fam_off + 1, 1, st));
-        if (int rc = pass(2, [&](int ci, hipStream_t s) -> int {
-                const KernelClass& kc = pp.classes[ci];
-                PruneParams q = pp.base;
-                q.tiles = fbplain + pp.plain_off[ci];
-                q.n_tiles_dev = fbc + n_cls + ci;
-                q.tile_counter = d_ctr + 2 * n_cls + ci;
-                q.scratch = scratch + kc.off_rp;
-                q.scratch_per_wg = pp.per_wg_plain;
-                HIP_OR_FAIL(launch_prune_plain(kc.nq, pp.grid(1, pp.plain_off[ci + 1] - pp.plain_off[ci]), q, s));
-                return CAFE_GPU_OK;
-            }))
-            return rc;
-    }
+    if (pp.path == CAFE_GPU_PATH_BATCHED32)
+        if (int rc = stage(1, stage_r64)) return rc;
+    if (pp.path != CAFE_GPU_PATH_PLAIN64)
+        if (int rc = stage(2, stage_plain)) return rc;
     if (ev_fb) HIP_OR_FAIL(hipEventRecord(ev_fb, st));
     return CAFE_GPU_OK;
 }

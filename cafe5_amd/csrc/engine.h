@@ -37,11 +37,14 @@ struct TileDesc {
 
 // (bucket, point) run of tiles, the unit in which flagged families are packed
 // into recompute tiles; cls is the size class whose lists receive them.
+// A chunk of one (bucket, point) segment for the recompute packing:
+// families [first, first + kSegChunk) of the bucket at that point.
+constexpr int kSegChunk = 1024;
 struct SegDesc {
     int bucket;
     int point;
     int cls;
-    int pad;
+    int first;
 };
 
 struct BucketDev {

@@ -138,10 +138,12 @@ __global__ __launch_bounds__(64) void k_prune_plain(PruneParams p)
     }
 }
 
-// Packs the families of one (bucket, point) segment whose flag has `mask` set
-// into dense tiles of `tile` families (16 for MFMA passes, 1 for PLAIN64): their bucket-sorted indices go to the segment's
-// own region of fam_list (point * F + fam_begin, so no two segments collide),
-// and the tiles are appended to the recompute list of the segment's size class.
+// Packs the families of one segment chunk (up to kSegChunk families of a
+// bucket at one point) whose flag has `mask` set into dense tiles of `tile`
+// families (16 for MFMA passes, 1 for PLAIN64): their bucket-sorted indices
+// go to the chunk's own region of fam_list (point * F + fam_begin + first, so
+// no two chunks collide), and the tiles are appended to the recompute list of
+// the chunk's size class.
 __global__ __launch_bounds__(256) void k_pack_flags(const SegDesc* __restrict__ segs,
                                                     const BucketDev* __restrict__ buckets,
                                                     const int* __restrict__ perm, const int* __restrict__ flags,
@@ -152,22 +154,23 @@ __global__ __launch_bounds__(256) void k_pack_flags(const SegDesc* __restrict__ 
 {
     const SegDesc sg = segs[blockIdx.x];
     const BucketDev bk = buckets[sg.bucket];
-    const long long base = (long long)sg.point * F + bk.fam_begin;
+    const long long base = (long long)sg.point * F + bk.fam_begin + sg.first;
+    const int count = bk.count - sg.first < kSegChunk ? bk.count - sg.first : kSegChunk;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     __shared__ int wave_tot[4];
     __shared__ int s_total;
     if (tid == 0) s_total = 0;
     __syncthreads();
-    for (int i0 = 0; i0 < bk.count; i0 += 256) {
+    for (int i0 = 0; i0 < count; i0 += 256) {
         const int i = i0 + tid;
-        const bool hit = i < bk.count && (flags[(long long)sg.point * F + perm[bk.fam_begin + i]] & mask) != 0;
+        const bool hit = i < count && (flags[(long long)sg.point * F + perm[bk.fam_begin + sg.first + i]] & mask) != 0;
         const unsigned long long ball = __ballot(hit);
         const int before = __popcll(ball & ((1ull << lane) - 1ull));
         if (lane == 0) wave_tot[wave] = __popcll(ball);
         __syncthreads();
         int off = s_total;
         for (int w = 0; w < wave; ++w) off += wave_tot[w];
-        if (hit) fam_list[base + off + before] = bk.fam_begin + i;
+        if (hit) fam_list[base + off + before] = bk.fam_begin + sg.first + i;
         __syncthreads();
         if (tid == 0) s_total += wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
         __syncthreads();

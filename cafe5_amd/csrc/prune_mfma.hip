@@ -201,8 +201,10 @@ __device__ __forceinline__ void mma_step(typename Vec4<T>::type (&x)[NTW], const
 // buffer.  A one-wave kernel holds whole vectors (nts <= NTW), so its ring
 // only needs the NTW - PK k-tiles past the prefetched ones (none for NTW <= PK).
 template <typename T, int NTW, int W> struct Ring {
-    // wider workgroups leave fewer registers per wave (W = 16: 128 VGPRs)
-    static constexpr int deep = W >= 16 ? 1 : (sizeof(T) == 4 && W < 8 ? 4 : 2);
+    // two k-tiles in flight; a deeper ring costs registers (the 4-tile
+    // shapes would drop to one wave per SIMD) without shortening the
+    // large vectors' step time.  W = 16 leaves 128 VGPRs per wave: one.
+    static constexpr int deep = W >= 16 ? 1 : 2;
     static constexpr int tail = NTW > Prefetch<T>::PK ? NTW - Prefetch<T>::PK : 0;
     static constexpr int D = W == 1 ? tail : deep;
 };
