@@ -562,6 +562,7 @@ struct KernelClass {
     int nq;              // PLAIN64 rows per lane (row s = lane + 64 q)
     int slot_tiles;      // 16-row tiles of its largest stored (non-root) vector
     MfmaShape s32, s64;  // launch shapes of the FP32 and FP64 MFMA kernels
+    MfmaShape r64;       // FP64 recompute of the FP32 pass (a few tiles: latency)
     int begin, end;      // its main tiles
     int seg_begin = 0, seg_end = 0;   // its (bucket, point) segments
     // byte offsets of its global scratch in the main pass, the FP64 and the
@@ -625,13 +626,32 @@ MfmaShape mfma_shape(int nt, bool fp64)
     return s;
 }
 
+// FP64 recompute shape: its lists hold a few tiles, so the time is one tile's
+// chain of steps; one or two output tiles per wave (four above 32 tiles) over
+// up to 8 waves (16 above 16 tiles) keep each step's k-loop short, with room
+// for four k-tiles of loads in flight.
+MfmaShape mfma_shape_chain(int nt)
+{
+    if (nt <= 2) return {std::max(1, nt), 1, false, 0};
+    if (nt <= 4) return {1, 4, false, 0};
+    if (nt <= 8) return {1, 8, false, 0};
+    if (nt <= 16) return {2, 8, false, 0};
+    if (nt <= 32) return {2, 16, false, 0};
+    return {4, 16, false, 0};
+}
+
 // Partial vectors go to LDS when the workgroup's share stays small enough for
-// several workgroups per CU; the rest use the global scratch.
-void place_partials(MfmaShape& s, bool fp64, int n_slots, int slot_tiles, int n_leaves)
+// several workgroups per CU; the rest use the global scratch.  A latency
+// shape (one workgroup per CU is enough: few tiles) may take up to 144 KB of
+// the CU's 160 KB.
+void place_partials(MfmaShape& s, bool fp64, int n_slots, int slot_tiles, int n_leaves, bool latency = false)
 {
     const size_t meta = (size_t)mfma_meta_bytes(n_slots, s.w, !fp64) + (size_t)mfma_count_bytes(n_leaves);
     const size_t frag = (size_t)n_slots * slot_tiles * 64 * 4 * (fp64 ? 8 : 4);
-    s.lds = s.w <= 4 && meta + frag + 4096 <= 64 * 1024 && (meta + frag) / s.w <= 20 * 1024;
+    if (latency)
+        s.lds = s.w <= 8 && meta + frag + 4096 <= 144 * 1024;
+    else
+        s.lds = s.w <= 4 && meta + frag + 4096 <= 64 * 1024 && (meta + frag) / s.w <= 20 * 1024;
     s.shm = meta + (s.lds ? frag : 0);
 }
 
@@ -798,10 +818,18 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     for (auto& kc : pp.classes) {
         kc.s32 = mfma_shape(kc.nt, false);
         kc.s64 = mfma_shape(kc.nt, true);
+        kc.r64 = mfma_shape_chain(kc.nt);
         place_partials(kc.s32, false, ctx->n_slots, kc.slot_tiles, ctx->n_leaves);
         place_partials(kc.s64, true, ctx->n_slots, kc.slot_tiles, ctx->n_leaves);
+        place_partials(kc.r64, true, ctx->n_slots, kc.slot_tiles, ctx->n_leaves, true);
         HIP_OR_FAIL(mfma_occupancy(false, kc.s32, &kc.s32.per_cu));
         HIP_OR_FAIL(mfma_occupancy(true, kc.s64, &kc.s64.per_cu));
+        HIP_OR_FAIL(mfma_occupancy(true, kc.r64, &kc.r64.per_cu));
+        if (kc.r64.per_cu == 0 && kc.r64.lds) {   // the device does not fit that much LDS: global scratch
+            kc.r64.lds = false;
+            kc.r64.shm = (size_t)mfma_meta_bytes(ctx->n_slots, kc.r64.w, false) + (size_t)mfma_count_bytes(ctx->n_leaves);
+            HIP_OR_FAIL(mfma_occupancy(true, kc.r64, &kc.r64.per_cu));
+        }
     }
     // Main-pass classes to side streams, longest first onto the least loaded
     // stream.  Estimated cost: contraction work (tiles x nt^2, plus a per-tile
@@ -876,7 +904,7 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         main_sz += align256(mb);
         if (path == CAFE_GPU_PATH_BATCHED32) {
             kc.off_r64 = r64_sz;
-            r64_sz += align256(mfma_scratch_bytes(kc.s64, true, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s64, n)));
+            r64_sz += align256(mfma_scratch_bytes(kc.r64, true, ctx->n_slots, kc.slot_tiles, pp.grid(kc.r64, n)));
         }
         if (path != CAFE_GPU_PATH_PLAIN64) {
             kc.off_rp = rp_sz;
@@ -981,7 +1009,7 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
         q.tile_counter = d_ctr + n_cls + ci;
         q.slot_tiles = kc.slot_tiles;
         q.scratch = scratch + kc.off_r64;
-        HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64, pp.class_counts[ci]), q, s));
+        HIP_OR_FAIL(launch_prune_mfma(true, kc.r64, pp.grid(kc.r64, pp.class_counts[ci]), q, s));
         return CAFE_GPU_OK;
     };
     auto stage_plain = [&](int ci, hipStream_t s) -> int {

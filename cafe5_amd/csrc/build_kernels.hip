@@ -30,6 +30,41 @@ __device__ __forceinline__ bool degenerate_key(double alpha, double coeff)
 // linear scan, done across lanes with a Kogge-Stone affine scan and carried
 // between 64-row blocks.  Stores for a fixed column are contiguous in s.
 // ---------------------------------------------------------------------------
+// Lane moves of a double within the wave (DPP on both 32-bit halves; lanes
+// without a source, or outside the row mask, read 0).
+constexpr int kDppRowShr = 0x110;       // row_shr:n, n = 1..15 (rows of 16 lanes)
+constexpr int kDppWaveShr1 = 0x138;     // wave_shr:1
+constexpr int kDppRowBcast15 = 0x142;   // lane 15 of each row into the next row
+constexpr int kDppRowBcast31 = 0x143;   // lane 31 into rows 2 and 3
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, ROWS, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)u, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double int_pow(double base, int e)
+{
+    double r = 1.0;
+    while (e) {
+        if (e & 1) r *= base;
+        base *= base;
+        e >>= 1;
+    }
+    return r;
+}
+
 template <int NB>
 __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restrict__ lam_q,
                                                           const double* __restrict__ t_q,
@@ -59,20 +94,15 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
         return;
     }
 
-    // a^d for the scan distances d = 1, 2, 4, ..., 32 and a^(l+1) per lane
-    double apow[6];
+    // a^d for the scan distances d = 1, 2, 4, 8, and per lane l the factors
+    // a^(l mod 16 + 1), a^(l mod 32 + 1) (row broadcasts) and a^(l + 1)
+    // (carry from the previous 64-row block)
+    double apow[4];
     apow[0] = alpha;
-    for (int i = 1; i < 6; ++i) apow[i] = apow[i - 1] * apow[i - 1];
-    double a_lane = 1.0;
-    {
-        int e = lane + 1;
-        double base = alpha;
-        while (e) {
-            if (e & 1) a_lane *= base;
-            base *= base;
-            e >>= 1;
-        }
-    }
+    for (int i = 1; i < 4; ++i) apow[i] = apow[i - 1] * apow[i - 1];
+    const double a16 = int_pow(alpha, (lane & 15) + 1);
+    const double a32 = int_pow(alpha, (lane & 31) + 1);
+    const double a_lane = int_pow(alpha, lane + 1);
 
     double prev[NB];   // P_s(c-1) for the lane's rows
 #pragma unroll
@@ -83,8 +113,8 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
         // K_s = (1-2a) P_{s-1}(c-1) + a P_s(c-1); row 0 is delta_0 itself
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-            double below = __shfl_up(prev[b], 1);
-            const double from_prev_block = b > 0 ? __shfl(prev[b > 0 ? b - 1 : 0], 63) : 0.0;
+            double below = dpp_move<kDppWaveShr1, 0xF>(prev[b]);   // lane l-1, 0 at lane 0
+            const double from_prev_block = b > 0 ? read_lane(prev[b > 0 ? b - 1 : 0], 63) : 0.0;
             if (lane == 0) below = from_prev_block;
             K[b] = coeff * below + alpha * prev[b];
         }
@@ -93,15 +123,18 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
         double carry = 0.0;
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
+            // y_s = K_s + a y_{s-1}: within rows of 16 lanes (shifts 1..8),
+            // then across rows (broadcast of lane 15 into rows 1 and 3, of
+            // lane 31 into rows 2 and 3)
             double y = K[b];
-#pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                const int d = 1 << i;
-                const double up = __shfl_up(y, d);
-                if (lane >= d) y += apow[i] * up;
-            }
+            y = fma(apow[0], dpp_move<kDppRowShr + 1, 0xF>(y), y);
+            y = fma(apow[1], dpp_move<kDppRowShr + 2, 0xF>(y), y);
+            y = fma(apow[2], dpp_move<kDppRowShr + 4, 0xF>(y), y);
+            y = fma(apow[3], dpp_move<kDppRowShr + 8, 0xF>(y), y);
+            y = fma(a16, dpp_move<kDppRowBcast15, 0xA>(y), y);
+            y = fma(a32, dpp_move<kDppRowBcast31, 0xC>(y), y);
             if (b > 0) y += a_lane * carry;
-            carry = __shfl(y, 63);
+            carry = read_lane(y, 63);
             prev[b] = y;
             const int s = lane + 64 * b;
             if (s < N) {

@@ -201,10 +201,13 @@ __device__ __forceinline__ void mma_step(typename Vec4<T>::type (&x)[NTW], const
 // buffer.  A one-wave kernel holds whole vectors (nts <= NTW), so its ring
 // only needs the NTW - PK k-tiles past the prefetched ones (none for NTW <= PK).
 template <typename T, int NTW, int W> struct Ring {
-    // two k-tiles in flight; a deeper ring costs registers (the 4-tile
-    // shapes would drop to one wave per SIMD) without shortening the
-    // large vectors' step time.  W = 16 leaves 128 VGPRs per wave: one.
-    static constexpr int deep = W >= 16 ? 1 : 2;
+    // two k-tiles in flight; a deeper ring costs registers (the 4-tile FP32
+    // shapes would drop to one wave per SIMD) without shortening their step
+    // time, which other waves' work covers.  The FP64 recompute shapes with
+    // one or two tiles per wave run a few workgroups whose step time is the
+    // k-loop's memory latency: four in flight.  W = 16 leaves 128 VGPRs per
+    // wave: one.
+    static constexpr int deep = W >= 16 ? 1 : (sizeof(T) == 8 && NTW <= 2 ? 4 : 2);
     static constexpr int tail = NTW > Prefetch<T>::PK ? NTW - Prefetch<T>::PK : 0;
     static constexpr int D = W == 1 ? tail : deep;
 };
@@ -651,6 +654,11 @@ void k_prune_mfma(PruneParams p)
     X(T, 3, 4, S, true) X(T, 3, 4, S, false) X(T, 4, 4, S, true) X(T, 4, 4, S, false) \
     X(T, 3, 8, S, false) X(T, 4, 8, S, false)
 #define CAFE_MFMA_WIDE(X, T, S) X(T, 3, 16, S, false) X(T, 4, 16, S, false)
+// FP64 recompute shapes: one or two output tiles per wave, so that one
+// family tile's chain of steps is as short as the workgroup allows
+#define CAFE_MFMA_CHAIN(X, T, S) \
+    X(T, 1, 4, S, true) X(T, 1, 4, S, false) X(T, 1, 8, S, true) X(T, 1, 8, S, false) X(T, 2, 8, S, true) \
+    X(T, 2, 8, S, false) X(T, 2, 16, S, false)
 
 // One instance per shape; with per_cu set, reports how many workgroups of it
 // fit on one CU instead of launching.
@@ -667,6 +675,7 @@ static hipError_t shape_dispatch(const MfmaShape& s, int grid, const PruneParams
     }
     CAFE_MFMA_SHAPES(CAFE_MFMA_CASE, T, SCALED)
     CAFE_MFMA_WIDE(CAFE_MFMA_CASE, T, SCALED)
+    if constexpr (!SCALED) { CAFE_MFMA_CHAIN(CAFE_MFMA_CASE, T, SCALED) }
 #undef CAFE_MFMA_CASE
     return hipErrorInvalidValue;
 }
