@@ -108,6 +108,18 @@ __device__ __forceinline__ doublex4 load_a(const double* M, int off, int kt, int
 
 // Operand prefetch depth: FP32 prefetches k-tiles 0 and 1 of an internal
 // step (two 16-byte loads per output tile), FP64 k-tile 0 (four 8-byte loads).
+// ds_swizzle pattern: lane l reads lane l ^ 16 (within each 32-lane half).
+constexpr int kSwizzleXor16 = 0x401F;
+
+// max(v, v of lane l ^ 32): v_permlane32_swap exchanges the upper half of its
+// first operand with the lower half of its second.
+__device__ __forceinline__ float xor32_max(float v)
+{
+    const unsigned u = __builtin_bit_cast(unsigned, v);
+    const auto sw = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    return fmaxf(__builtin_bit_cast(float, (unsigned)sw[0]), __builtin_bit_cast(float, (unsigned)sw[1]));
+}
+
 // Makes the wave wait for a loaded value without using it (an empty asm that
 // reads each element), so that no load of it is left in flight.
 template <typename V>
@@ -376,6 +388,15 @@ void k_prune_mfma(PruneParams p)
             offL0[i] = kTile * t0 + 4 * g;
             offL1[i] = kTile * t1 + 4 * g;
         }
+        // one-wave FP32 kernels start a non-root product from a 0/1 row mask
+        // (rows past the bound are 0 and stay 0), so the node's store needs no
+        // masking; the others mask when they store
+        constexpr bool kMaskFirst = SCALED && W == 1 && NTW == 2;   // registers to spare: the 2-tile class
+        V4 keep[kMaskFirst ? NTW : 1];
+#pragma unroll
+        for (int i = 0; i < (kMaskFirst ? NTW : 1); ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) keep[i][r] = kTile * (wv + W * i) + 4 * g + r < bk.L ? T(1) : T(0);
 
         // loads of step st (clamped to the last step) into pf; its descriptor into sd
         auto prefetch = [&](V4 (&pf)[PK][NTW], StepDev& sd, int st) {
@@ -430,7 +451,10 @@ void k_prune_mfma(PruneParams p)
             }
             if (sd.ctl & kStepFirst) {
 #pragma unroll
-                for (int i = 0; i < NTW; ++i) acc[i] = V4{1, 1, 1, 1};
+                for (int i = 0; i < NTW; ++i) {
+                    if constexpr (kMaskFirst) acc[i] = root ? V4{1, 1, 1, 1} : keep[i];
+                    else acc[i] = V4{1, 1, 1, 1};
+                }
                 e_sum = 0;
                 log2S = 0;
                 cert = 0.0;
@@ -493,18 +517,32 @@ void k_prune_mfma(PruneParams p)
                 for (int i = 0; i < NTW; ++i) {
                     if (!live[i]) continue;
                     V4 v = acc[i];
+                    if constexpr (!kMaskFirst)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        if (kTile * tt[i] + 4 * g + r >= bk.L) v[r] = 0;   // rows past the bound
+                        for (int r = 0; r < 4; ++r)
+                            if (kTile * tt[i] + 4 * g + r >= bk.L) v[r] = 0;   // rows past the bound
                     frag[((long long)out_slot * slot_tiles + tt[i]) * 64 + lane] = v;
                     if (SCALED)
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) wmax = wmax > v[r] ? wmax : v[r];
+                        for (int r = 0; r < 4; ++r) {
+                            if constexpr (W == 1) wmax = fmax(wmax, v[r]);   // entries are >= 0
+                            else wmax = wmax > v[r] ? wmax : v[r];
+                        }
                 }
                 if (SCALED) {
+                    // maximum over the family's four lanes f, f+16, f+32, f+48
+                    // (the one-wave kernels use lane swaps; the 4-tile
+                    // multi-wave shapes keep shuffles, which leave them the
+                    // registers for two waves per SIMD)
                     float m = (float)wmax;
-                    m = fmaxf(m, __shfl_xor(m, 16));
-                    m = fmaxf(m, __shfl_xor(m, 32));
+                    if constexpr (W == 1) {
+                        m = fmaxf(m, __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, m),
+                                                                                          kSwizzleXor16)));
+                        m = xor32_max(m);
+                    } else {
+                        m = fmaxf(m, __shfl_xor(m, 16));
+                        m = fmaxf(m, __shfl_xor(m, 32));
+                    }
                     const int q = out_slot * 16 + fl;
                     if (W == 1) {
                         // finish the stored vector: shift, exponent, flush bound
