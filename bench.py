@@ -40,6 +40,7 @@ from cafe5_amd import native, priors, workloads as wl  # noqa: E402
 METRIC = "family likelihoods/sec, batched FP32-MFMA vs plain FP64 on GPU at |ΔlnL|≤1e-6"
 PEAK_FP32_MFMA = 157.3e12     # MI355X dense FP32 matrix, FLOP/s
 PEAK_HBM = 8.0e12             # MI355X HBM3E, B/s
+KERNEL_KIND = {0: "k_prune_mfma", 1: "k_prune_lane", 2: "k_prune_plain"}
 
 
 def log(msg: str):
@@ -105,18 +106,19 @@ def time_plan(plan: native.Plan, steps: int, warmup: int, profile_steps: int):
     return (t1 - t0) * 1e3 / steps, phase / n, graph_ok, cls
 
 
-def read_traffic(nt: int):
+def read_traffic(nt: int, kernel: str):
     """HBM bytes per launch of the FP32 main-pass kernel of the nt-tile size
     class, from the committed rocprofv3 counter summary (FETCH_SIZE and
     WRITE_SIZE of that kernel, each scaled by what the counter reads for a
-    device copy of known size); None when the summary does not cover it."""
+    device copy of known size); None when the summary does not cover it or
+    was taken on another kernel (name prefix `kernel`)."""
     path = os.path.join(ROOT, "profiles", "prune_fp32_uneven34_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         t = json.load(fh)
     k = t.get("kernels", {}).get(str(nt))
-    if not k:
+    if not k or not k["kernel"].startswith(kernel):
         return None
     return {"bytes": k["fetch_kib"] * 1024 * t["fetch_scale"] + k["write_kib"] * 1024 * t["write_scale"],
             "source": f"profiles/prune_fp32_uneven34_traffic.json ({t['command']})"}
@@ -341,19 +343,30 @@ def main():
     dom_ms = float(ms_cls[dom])
     achieved = dom_flops / (dom_ms * 1e-3) if dom_ms > 0 else None
     phase_flops = sum(cnt * T * 2.0 * (B + 1) * (n1 * (B + 1) + n2 * R) for B, R, cnt in buckets)
-    traffic = read_traffic(dom_nt)
+    dl = launch_cls[dom]
+    if int(dl[0]) == 1:
+        dom_kernel = f"void k_prune_lane<{int(dl[1])}, "
+        dom_desc = (f"k_prune_lane<{int(dl[1])},*> (one family per lane, FP32 FMA; the same 157.3 TF/s dense FP32 "
+                    f"peak) main-pass launch of the {dom_nt}-tile size class (buckets B={dom_buckets}, "
+                    f"{int(tiles_cls[dom])} tiles of 64 families)")
+    else:
+        dom_kernel = f"void k_prune_mfma<float, {int(dl[1])}, {int(dl[2])}, true, {'true' if dl[3] else 'false'}>"
+        dom_desc = (f"k_prune_mfma<float,{int(dl[1])},{int(dl[2])},true,*> main-pass launch of the {dom_nt}-tile "
+                    f"size class (buckets B={dom_buckets}, {int(tiles_cls[dom])} tiles of 16 families)")
+    traffic = read_traffic(dom_nt, dom_kernel)
     build_bytes = len(lam_q) * N * N * (8 + 4 + 4)
     roofline = {"bound": "mfma", "achieved": achieved / 1e12 if achieved else None, "peak": PEAK_FP32_MFMA / 1e12,
                 "unit": "TFLOP/s", "frac": (achieved / PEAK_FP32_MFMA) if achieved else None,
                 "traffic": traffic["bytes"] if traffic else None,
-                "kernel": f"k_prune_mfma<float,*,*,true,*> main-pass launch of the {dom_nt}-tile size class "
-                          f"(buckets B={dom_buckets}, {int(tiles_cls[dom])} tiles of 16 families)",
+                "kernel": dom_desc,
                 "algorithmic_flops_per_launch": dom_flops, "avg_ms": dom_ms,
                 "flops_per_family_likelihood": "2 L (n1 L + n2 R), n1=%d n2=%d" % (n1, n2),
                 "traffic_source": traffic["source"] if traffic else None,
                 "classes": [{"tiles_per_vector": int(a_), "tiles": int(b_), "ms": float(c_),
-                             "launch": {"tiles_per_wave": int(l_[0]), "waves": int(l_[1]),
-                                        "partials_in_lds": bool(l_[2]), "workgroups": int(l_[3])}}
+                             "launch": {"kernel": KERNEL_KIND[int(l_[0])],
+                                        "tiles_per_wave_or_rows_per_lane": int(l_[1]), "waves": int(l_[2]),
+                                        "partials_in_lds": bool(l_[3]), "workgroups": int(l_[4]),
+                                        "families_per_tile": int(l_[5])}}
                             for a_, b_, c_, l_ in zip(nt_cls, tiles_cls, ms_cls, launch_cls)],
                 "prune_phase": {"ms": ph32[1], "algorithmic_flops": phase_flops,
                                 "TFLOP_per_s": phase_flops / (ph32[1] * 1e-3) / 1e12 if ph32[1] > 0 else None},

@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -24,6 +25,9 @@ hipError_t launch_fold(const double* mt64, const float* lf32, double* out64, flo
 hipError_t launch_prune_plain(int nq, int grid, const PruneParams& p, hipStream_t st);
 hipError_t launch_prune_mfma(bool fp64, const MfmaShape& s, int grid, const PruneParams& p, hipStream_t st);
 hipError_t mfma_occupancy(bool fp64, const MfmaShape& s, int* per_cu);
+int lane_rows_for(int rows);
+int lane_max_slots(int lane_rows);
+hipError_t lane_dispatch(int lr, int ns, int grid, size_t shm, const PruneParams& p, hipStream_t st, int* per_cu);
 hipError_t launch_pack(const SegDesc* segs, int n_segs, const BucketDev* buckets, const int* perm,
                        const int* flags, int F, int mask, int* fam_list, TileDesc* out_tiles,
                        const int* class_off, int* class_count, int* fam_count, int tile, hipStream_t st);
@@ -131,6 +135,8 @@ struct cafe_gpu_ctx {
     std::vector<ChildDev> kids;
     DevBuf d_ops, d_kids, d_steps;
     int n_steps = 0;
+    DevBuf d_steps_lane;     // the same steps with output-reusing slots (lane kernel)
+    int n_slots_lane = 0;
 
     // families
     int F = 0, n_leaves = 0;
@@ -188,9 +194,12 @@ struct Planner {
     std::vector<OpDev> ops;
     std::vector<ChildDev> out_kids;
     int max_slot = -1;
+    // reuse: a node's output may take one of its children's slots (a kernel
+    // whose lanes read every child before writing the output, in order)
+    bool reuse = false;
 
-    Planner(const std::vector<std::vector<int>>& k, const std::vector<int>& lc)
-        : kids(k), leaf_col(lc), need(k.size(), -1), slot_of(k.size(), -1) {}
+    Planner(const std::vector<std::vector<int>>& k, const std::vector<int>& lc, bool r = false)
+        : kids(k), leaf_col(lc), need(k.size(), -1), slot_of(k.size(), -1), reuse(r) {}
 
     std::vector<int> interior_order(int u)
     {
@@ -228,6 +237,8 @@ struct Planner {
     {
         auto in = interior_order(u);
         for (int c : in) emit(c);
+        if (reuse)
+            for (int c : in) used[slot_of[c]] = 0;
         const int out = alloc();
         OpDev op;
         op.node = u;
@@ -243,7 +254,8 @@ struct Planner {
             out_kids.push_back(ch);
         }
         ops.push_back(op);
-        for (int c : in) used[slot_of[c]] = 0;
+        if (!reuse)
+            for (int c : in) used[slot_of[c]] = 0;
         slot_of[u] = out;
     }
 };
@@ -302,7 +314,7 @@ int cafe_gpu_destroy(cafe_gpu_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf* bufs[] = {&ctx->lg, &ctx->mt64, &ctx->mt32, &ctx->lf32, &ctx->key_lam, &ctx->key_t, &ctx->d_ops,
-                      &ctx->d_kids, &ctx->d_steps, &ctx->d_counts, &ctx->d_perm, &ctx->d_err64,
+                      &ctx->d_kids, &ctx->d_steps, &ctx->d_steps_lane, &ctx->d_counts, &ctx->d_perm, &ctx->d_err64,
                       &ctx->d_err32, &ctx->mt64e, &ctx->mt32e};
     for (DevBuf* b : bufs) b->release();
     ctx->eager.release_all();
@@ -447,21 +459,35 @@ int cafe_gpu_set_tree(cafe_gpu_ctx* ctx, int n_nodes, const int* parent, const i
     HIP_OR_FAIL(hipMemcpy(ctx->d_kids.p, ctx->kids.data(), sizeof(ChildDev) * ctx->kids.size(),
                           hipMemcpyHostToDevice));
     // the same program as a flat list of child steps
-    std::vector<StepDev> steps;
-    for (size_t o = 0; o < ctx->ops.size(); ++o) {
-        const OpDev& op = ctx->ops[o];
-        for (int k = 0; k < op.n_child; ++k) {
-            const ChildDev& ch = ctx->kids[op.child_begin + k];
-            int ctl = op.out_slot;
-            if (k == 0) ctl |= kStepFirst;
-            if (k == op.n_child - 1) ctl |= kStepLast;
-            if (o + 1 == ctx->ops.size()) ctl |= kStepRoot;
-            steps.push_back(StepDev{ch.node, ch.leaf_col, ch.in_slot, ctl});
+    auto flatten = [](const std::vector<OpDev>& ops, const std::vector<ChildDev>& kids) {
+        std::vector<StepDev> steps;
+        for (size_t o = 0; o < ops.size(); ++o) {
+            const OpDev& op = ops[o];
+            for (int k = 0; k < op.n_child; ++k) {
+                const ChildDev& ch = kids[op.child_begin + k];
+                int ctl = op.out_slot;
+                if (k == 0) ctl |= kStepFirst;
+                if (k == op.n_child - 1) ctl |= kStepLast;
+                if (o + 1 == ops.size()) ctl |= kStepRoot;
+                steps.push_back(StepDev{ch.node, ch.leaf_col, ch.in_slot, ctl});
+            }
         }
-    }
+        return steps;
+    };
+    const std::vector<StepDev> steps = flatten(ctx->ops, ctx->kids);
     ctx->n_steps = (int)steps.size();
     HIP_OR_FAIL(ctx->d_steps.ensure(sizeof(StepDev) * steps.size()));
     HIP_OR_FAIL(hipMemcpy(ctx->d_steps.p, steps.data(), sizeof(StepDev) * steps.size(), hipMemcpyHostToDevice));
+    // the lane kernel's program: the same order, outputs reusing child slots
+    Planner pr(kids, lc, true);
+    pr.compute_need(root);
+    pr.used.clear();
+    pr.emit(root);
+    const std::vector<StepDev> lsteps = flatten(pr.ops, pr.out_kids);
+    ctx->n_slots_lane = pr.max_slot + 1;
+    HIP_OR_FAIL(ctx->d_steps_lane.ensure(sizeof(StepDev) * lsteps.size()));
+    HIP_OR_FAIL(hipMemcpy(ctx->d_steps_lane.p, lsteps.data(), sizeof(StepDev) * lsteps.size(),
+                          hipMemcpyHostToDevice));
     return CAFE_GPU_OK;
 }
 
@@ -563,6 +589,11 @@ struct KernelClass {
     int slot_tiles;      // 16-row tiles of its largest stored (non-root) vector
     MfmaShape s32, s64;  // launch shapes of the FP32 and FP64 MFMA kernels
     MfmaShape r64;       // FP64 recompute of the FP32 pass (a few tiles: latency)
+    int lane_rows = 0;   // > 0: the FP32 main pass runs k_prune_lane with this many rows (64-family tiles)
+    int lane_slots = 0;  // its slot instance
+    int lane_per_cu = 0; // its workgroups resident per CU
+    int r16 = 0;         // 16-family tiles its families fill (the recompute lists' capacity)
+    int r_begin = 0;     // offset of its FP64 recompute list
     int begin, end;      // its main tiles
     int seg_begin = 0, seg_end = 0;   // its (bucket, point) segments
     // byte offsets of its global scratch in the main pass, the FP64 and the
@@ -581,6 +612,7 @@ struct PrunePlan {
     std::vector<int> main_order;     // main-pass launch order (longest first)
     int plain_list_size = 0;
     int n_tiles = 0, n_segs = 0, maxLR = 0;
+    int r16_total = 0;               // capacity of the FP64 recompute lists (16-family tiles)
     int waves_per_cu = 16;           // persistent grid: resident waves per CU
     int n_cu = 256;
     long long per_wg_plain = 0;
@@ -594,6 +626,13 @@ struct PrunePlan {
     int grid(int w, int max_tiles) const
     {
         const int g = std::max(1, std::min(n_cu * waves_per_cu / std::max(1, w), max_tiles));
+        return g >= 8 ? g / 8 * 8 : g;
+    }
+    // a lane-kernel launch: as an MFMA launch with partials in LDS
+    int grid_lane(int per_cu, int max_tiles) const
+    {
+        const long long want = (long long)n_cu * std::max(1, per_cu) * 16;
+        const int g = (int)std::max(1LL, std::min(want, (long long)max_tiles));
         return g >= 8 ? g / 8 * 8 : g;
     }
     // an MFMA launch: a multiple of the workgroups of its shape that are
@@ -654,6 +693,9 @@ void place_partials(MfmaShape& s, bool fp64, int n_slots, int slot_tiles, int n_
         s.lds = s.w <= 4 && meta + frag + 4096 <= 64 * 1024 && (meta + frag) / s.w <= 20 * 1024;
     s.shm = meta + (s.lds ? frag : 0);
 }
+
+// Dynamic LDS of a lane-kernel launch: the tile's leaf counts.
+size_t lane_shm(int n_leaves) { return (size_t)n_leaves * 64 * sizeof(int); }
 
 // Global scratch of an MFMA launch (partials not in LDS), bytes.
 size_t mfma_scratch_bytes(const MfmaShape& s, bool fp64, int n_slots, int slot_tiles, int grid)
@@ -785,6 +827,19 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         if (x.first != y.first) return x.first > y.first;
         return ctx->buckets[x.second].B > ctx->buckets[y.second].B;
     });
+    // size classes the one-family-per-lane FP32 kernel takes: all vectors and
+    // root rows within one of its row counts, and the tree's slots within its
+    // instances
+    std::map<int, int> key_rows;
+    for (auto& bc : bucket_class) {
+        const BucketHost& bk = ctx->buckets[bc.second];
+        key_rows[bc.first] = std::max(key_rows[bc.first], std::max(bk.B + 1, bk.R + 1));
+    }
+    auto lane_rows_of = [&](int key) {
+        if (path != CAFE_GPU_PATH_BATCHED32) return 0;
+        const int lr = lane_rows_for(key_rows[key]);
+        return lr > 0 && ctx->n_slots_lane <= lane_max_slots(lr) ? lr : 0;
+    };
     std::vector<TileDesc> tiles;
     std::vector<SegDesc> segs;
     std::vector<int> cls_fams;
@@ -795,6 +850,7 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
             KernelClass kc{};
             kc.begin = kc.end = (int)tiles.size();
             kc.seg_begin = kc.seg_end = (int)segs.size();
+            kc.lane_rows = lane_rows_of(bc.first);
             pp.classes.push_back(kc);
             cls_fams.push_back(0);
             prev_key = bc.first;
@@ -804,12 +860,14 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         kc.nt = std::max(kc.nt, nt_of(bk));
         kc.nq = std::max(kc.nq, nq_of(bk));
         kc.slot_tiles = std::max(kc.slot_tiles, round_up(bk.B + 1, kTile) / kTile);
+        const int tw = kc.lane_rows > 0 ? 64 : kTile;   // families per tile
         for (int pi = 0; pi < n_points; ++pi) {
-            for (int f0 = 0; f0 < bk.count; f0 += kTile)
-                tiles.push_back(TileDesc{bc.second, point_ids[pi], f0, std::min(kTile, bk.count - f0)});
+            for (int f0 = 0; f0 < bk.count; f0 += tw)
+                tiles.push_back(TileDesc{bc.second, point_ids[pi], f0, std::min(tw, bk.count - f0)});
             for (int f0 = 0; f0 < bk.count; f0 += kSegChunk)
                 segs.push_back(SegDesc{bc.second, point_ids[pi], ci, f0});
         }
+        kc.r16 += n_points * ((bk.count + kTile - 1) / kTile);
         cls_fams[ci] += bk.count * n_points;
         kc.end = (int)tiles.size();
         kc.seg_end = (int)segs.size();
@@ -830,6 +888,13 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
             kc.r64.shm = (size_t)mfma_meta_bytes(ctx->n_slots, kc.r64.w, false) + (size_t)mfma_count_bytes(ctx->n_leaves);
             HIP_OR_FAIL(mfma_occupancy(true, kc.r64, &kc.r64.per_cu));
         }
+        if (kc.lane_rows > 0) {
+            kc.lane_slots = ctx->n_slots_lane;
+            const PruneParams none{};
+            HIP_OR_FAIL(lane_dispatch(kc.lane_rows, kc.lane_slots, 0, lane_shm(ctx->n_leaves), none, nullptr,
+                                      &kc.lane_per_cu));
+            if (kc.lane_per_cu == 0) kc.lane_rows = 0;   // no instance fits: matrix-core kernel
+        }
     }
     // Main-pass classes to side streams, longest first onto the least loaded
     // stream.  Estimated cost: contraction work (tiles x nt^2, plus a per-tile
@@ -839,7 +904,7 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         std::vector<int> order(n_cls);
         for (int ci = 0; ci < n_cls; ++ci) {
             const KernelClass& kc = pp.classes[ci];
-            const double tiles = kc.end - kc.begin;
+            const double tiles = kc.r16;   // in 16-family tiles
             cost[ci] = tiles * (kc.nt * kc.nt + 8.0) + 2500.0 * kc.nt * kc.nt;
             order[ci] = ci;
         }
@@ -855,17 +920,22 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     pp.n_tiles = (int)tiles.size();
     pp.class_counts.assign(n_cls + 4, 0);
     for (int i = 0; i < n_cls; ++i) pp.class_counts[i] = pp.classes[i].end - pp.classes[i].begin;
-    // [0, n_cls] offsets of the 16-family recompute lists (one slot per main
-    // tile), [n_cls + 1, 2 n_cls + 1] of the one-family PLAIN64 lists
+    // [0, n_cls] offsets of the 16-family recompute lists (room for all of a
+    // class's families in 16-family tiles), [n_cls + 1, 2 n_cls + 1] of the
+    // one-family PLAIN64 lists
     std::vector<int> class_off(2 * (n_cls + 1), 0);
-    for (int i = 0; i < n_cls; ++i) class_off[i + 1] = class_off[i] + pp.class_counts[i];
+    for (int i = 0; i < n_cls; ++i) {
+        pp.classes[i].r_begin = class_off[i];
+        class_off[i + 1] = class_off[i] + pp.classes[i].r16;
+    }
+    pp.r16_total = class_off[n_cls];
     for (int i = 0; i < n_cls; ++i) class_off[n_cls + 1 + i + 1] = class_off[n_cls + 1 + i] + cls_fams[i];
     pp.plain_list_size = class_off[2 * n_cls + 1];
     pp.plain_off.assign(class_off.begin() + n_cls + 1, class_off.end());
     pp.n_segs = (int)segs.size();
 
     HIP_OR_FAIL(b.tiles.ensure(sizeof(TileDesc) * std::max(1, pp.n_tiles)));
-    HIP_OR_FAIL(b.fb_tiles.ensure(sizeof(TileDesc) * ((size_t)std::max(1, pp.n_tiles) + std::max(1, pp.plain_list_size))));
+    HIP_OR_FAIL(b.fb_tiles.ensure(sizeof(TileDesc) * ((size_t)std::max(1, pp.r16_total) + std::max(1, pp.plain_list_size))));
     HIP_OR_FAIL(b.ntiles.ensure(sizeof(int) * (n_cls + 4)));
     HIP_OR_FAIL(b.counters.ensure(sizeof(int) * (3 * n_cls + 8)));
     HIP_OR_FAIL(b.fb_count.ensure(sizeof(int) * (2 * n_cls + 4)));
@@ -899,12 +969,12 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         if (path == CAFE_GPU_PATH_PLAIN64) mb = plain_bytes(n);
         else if (path == CAFE_GPU_PATH_BATCHED64)
             mb = mfma_scratch_bytes(kc.s64, true, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s64, n));
-        else mb = mfma_scratch_bytes(kc.s32, false, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s32, n));
+        else if (kc.lane_rows == 0) mb = mfma_scratch_bytes(kc.s32, false, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s32, n));
         kc.off_main = main_sz;
         main_sz += align256(mb);
         if (path == CAFE_GPU_PATH_BATCHED32) {
             kc.off_r64 = r64_sz;
-            r64_sz += align256(mfma_scratch_bytes(kc.r64, true, ctx->n_slots, kc.slot_tiles, pp.grid(kc.r64, n)));
+            r64_sz += align256(mfma_scratch_bytes(kc.r64, true, ctx->n_slots, kc.slot_tiles, pp.grid(kc.r64, kc.r16)));
         }
         if (path != CAFE_GPU_PATH_PLAIN64) {
             kc.off_rp = rp_sz;
@@ -936,6 +1006,7 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     p.Np = ctx->Np;
     p.leaf64 = ctx->n_dev ? ctx->mt64e.as<double>() : ctx->mt64.as<double>();
     p.leaf32 = ctx->n_dev ? ctx->mt32e.as<float>() : ctx->lf32.as<float>();
+    p.cm32 = ctx->lf32.as<float>();
     p.n_dev = ctx->n_dev;
     p.prior64 = b.prior64.as<double>();
     p.prior32 = b.prior32.as<float>();
@@ -974,7 +1045,7 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
     // Each recompute stage packs its class's flagged families into a list
     // sized on the device and prunes it.
     TileDesc* fb64 = b.fb_tiles.as<TileDesc>();
-    TileDesc* fbplain = fb64 + std::max(1, pp.n_tiles);
+    TileDesc* fbplain = fb64 + std::max(1, pp.r16_total);
     const int* coff = b.class_off.as<int>();
     const int fam_off = pp.fam_count_off();
     const SegDesc* segs = b.segs.as<SegDesc>();
@@ -992,6 +1063,11 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
             HIP_OR_FAIL(launch_prune_plain(kc.nq, pp.grid(1, n), q, s));
         } else if (pp.path == CAFE_GPU_PATH_BATCHED64) {
             HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64, n), q, s));
+        } else if (kc.lane_rows > 0) {
+            q.steps = ctx->d_steps_lane.as<StepDev>();
+            q.n_slots = ctx->n_slots_lane;
+            HIP_OR_FAIL(lane_dispatch(kc.lane_rows, kc.lane_slots, pp.grid_lane(kc.lane_per_cu, n),
+                                      lane_shm(ctx->n_leaves), q, s, nullptr));
         } else {
             HIP_OR_FAIL(launch_prune_mfma(false, kc.s32, pp.grid(kc.s32, n), q, s));
         }
@@ -1004,12 +1080,12 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
                                 pp.base.perm, b.flag.as<int>(), pp.base.F, kFlagRecompute64, b.fam_list.as<int>(),
                                 fb64, coff, fbc, fbc + fam_off, kTile, s));
         PruneParams q = pp.base;
-        q.tiles = fb64 + kc.begin;
+        q.tiles = fb64 + kc.r_begin;
         q.n_tiles_dev = fbc + ci;
         q.tile_counter = d_ctr + n_cls + ci;
         q.slot_tiles = kc.slot_tiles;
         q.scratch = scratch + kc.off_r64;
-        HIP_OR_FAIL(launch_prune_mfma(true, kc.r64, pp.grid(kc.r64, pp.class_counts[ci]), q, s));
+        HIP_OR_FAIL(launch_prune_mfma(true, kc.r64, pp.grid(kc.r64, kc.r16), q, s));
         return CAFE_GPU_OK;
     };
     auto stage_plain = [&](int ci, hipStream_t s) -> int {
@@ -1354,18 +1430,29 @@ int cafe_gpu_plan_class_times(cafe_gpu_plan* pl, int cap, int* n_classes, int* c
         if (launch) {
             const KernelClass& kc = pl->pp.classes[i];
             const int n = pl->pp.class_counts[i];
-            int* l = launch + 4 * i;
+            int* l = launch + 6 * i;
             if (pl->path == CAFE_GPU_PATH_PLAIN64) {
-                l[0] = kc.nq;
-                l[1] = 1;
-                l[2] = 0;
-                l[3] = pl->pp.grid(1, n);
+                l[0] = 2;
+                l[1] = kc.nq;
+                l[2] = 1;
+                l[3] = 0;
+                l[4] = pl->pp.grid(1, n);
+                l[5] = 1;
+            } else if (pl->path == CAFE_GPU_PATH_BATCHED32 && kc.lane_rows > 0) {
+                l[0] = 1;
+                l[1] = kc.lane_rows;
+                l[2] = 1;
+                l[3] = 0;
+                l[4] = pl->pp.grid_lane(kc.lane_per_cu, n);
+                l[5] = 64;
             } else {
                 const MfmaShape& sh = pl->path == CAFE_GPU_PATH_BATCHED64 ? kc.s64 : kc.s32;
-                l[0] = sh.ntw;
-                l[1] = sh.w;
-                l[2] = sh.lds ? 1 : 0;
-                l[3] = pl->pp.grid(sh, n);
+                l[0] = 0;
+                l[1] = sh.ntw;
+                l[2] = sh.w;
+                l[3] = sh.lds ? 1 : 0;
+                l[4] = pl->pp.grid(sh, n);
+                l[5] = kTile;
             }
         }
     }

@@ -119,6 +119,7 @@ struct PruneParams {
     const float* mt32;
     const double* leaf64;
     const float* leaf32;
+    const float* cm32;        // the FP32 matrices c-major (unfolded): lane kernel columns
     long long slot_elems;     // Np * Np
     int Np;
     int n_dev;                // error-model deviations (0: none); folded into leaf64/leaf32

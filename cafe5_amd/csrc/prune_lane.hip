@@ -1,0 +1,291 @@
+// prune_lane.hip -- batched FP32 pruning of small vectors, one family per lane
+// (replaces inference_prune / compute_node_probability, src/core.cpp:134-145,
+// src/probability.cpp:175-330, and the root reductions of base_model.cpp:77-95
+// and gamma_core.cpp:143-165, for the families of the smallest size buckets).
+//
+// A wave carries 64 families of one size bucket at one parameter point through
+// the whole tree.  Lane l owns family l: its partial vectors (sizes 0..LR-1),
+// the running product of the current node and the node's factor all live in
+// the lane's registers, so a node costs no data movement between lanes and the
+// scaled-FP32 bookkeeping (exponent, shift, flush bound) is lane-local.  An
+// interior child's factor x = M' v is LR x LR fused multiply-adds per lane
+// whose matrix operand is a wave-uniform scalar register: the wave streams
+// column k of M' (c-major FP32 pool, entries times 2^kMatExp) through scalar
+// loads while v[k] is the lane's own value.  A leaf child's factor is column
+// count of M' gathered per lane.  No row is padded beyond LR, against the
+// 16-row tiles of the matrix-core kernel (21 rows: 1.0x the work instead of
+// 2.3x), and a SIMD holds 64 families per wave.
+//
+// Partial vectors are in registers, indexed by the program's slot numbers
+// through wave-uniform branches (one copy of the code per slot).  This
+// kernel's program lets a node's output take the slot of one of its children
+// (the children are all read before the output is written, by the same lane),
+// so a binary tree of 34 leaves needs 3 slots.
+//
+// Numerics follow the single-wave matrix-core kernel exactly: stored vectors
+// are renormalised to a maximum just below 2^vec_exp, the exponent is carried
+// in an integer, and a bound on what FP32 flushing to zero could have dropped
+// decides at the root whether the family is kept or recomputed in FP64.
+#include "prune_common.h"
+
+namespace cafe {
+
+// Slot access with a wave-uniform slot number s.  The register arrays are
+// indexed by constants only: slot k-entries are blended with uniform 0/1
+// weights (stored entries are finite, so the blend is exact).  Selects would
+// be turned by the compiler into one load at a variable index, which moves
+// the arrays to scratch memory.
+template <int NS>
+struct SlotWeights {
+    float w[NS];
+    __device__ __forceinline__ explicit SlotWeights(int s)
+    {
+#pragma unroll
+        for (int c = 0; c < NS; ++c) w[c] = s == c ? 1.f : 0.f;
+    }
+};
+template <int NS, int LR>
+__device__ __forceinline__ float pick(const float (&sv)[NS][LR], const SlotWeights<NS>& sw, int k)
+{
+    float r = sv[0][k] * sw.w[0];
+#pragma unroll
+    for (int c = 1; c < NS; ++c) r = __builtin_fmaf(sv[c][k], sw.w[c], r);
+    return r;
+}
+template <int NS, typename V>
+__device__ __forceinline__ V pick(const V (&a)[NS], const SlotWeights<NS>& sw)
+{
+    V r = 0;
+#pragma unroll
+    for (int c = 0; c < NS; ++c) r += sw.w[c] != 0.f ? a[c] : V(0);
+    return r;
+}
+// (the store is a select per element as well: a branch per slot would let
+// the compiler merge the stores into one at a variable index, which moves the
+// arrays to scratch memory)
+template <int NS, int LR>
+__device__ __forceinline__ void put(float (&sv)[NS][LR], int (&se)[NS], double (&sc)[NS], int out,
+                                    const float (&v)[LR], int e, double c)
+{
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        const bool here = out == q;
+#pragma unroll
+        for (int i = 0; i < LR; ++i) sv[q][i] = here ? v[i] : sv[q][i];
+        se[q] = here ? e : se[q];
+        sc[q] = here ? c : sc[q];
+    }
+}
+
+// Three waves per SIMD (168 VGPRs) hold 21-row vectors in up to 3 slots;
+// larger shapes ask for two.
+template <int LR, int NS> struct LaneWaves { static constexpr int v = LR * (NS + 3) <= 130 ? 3 : 2; };
+
+template <int LR, int NS>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LaneWaves<LR, NS>::v))) void k_prune_lane(PruneParams p)
+{
+    const int lane = threadIdx.x;
+    const int Np = p.Np;
+    const int a_exp = p.vec_exp;
+    const int n_steps = p.n_steps;
+    const long long slot_elems = p.slot_elems;
+    // FP32 leaf factors: up to n_dev flushed terms (one without an error model)
+    const double leaf_cert = (p.n_dev > 1 ? p.n_dev : 1) * kTau32 * kTwoPowMinusMatExp;
+    const double two_m_a = ldexp(1.0, -a_exp);
+    // flush-bound terms of an interior factor relative to its nominal scale:
+    // LR matrix entries and products that may flush, and the stored vector's
+    // own entries
+    const double cint = kTau32 * (LR * kTwoPowMinusMatExp * (1.0 + two_m_a) + two_m_a);
+    extern __shared__ int s_cnt[];   // [leaf][64]: the tile's leaf counts, lane l's in column l
+
+    const int n_tiles = *as_const(p.n_tiles_dev);
+    const int groups = (gridDim.x % 8 == 0) ? 8 : 1;   // XCD-aware static split (see k_prune_mfma)
+    const int per_group = gridDim.x / groups;
+    const int grp = blockIdx.x % groups;
+    const int t_lo = (int)((long long)n_tiles * grp / groups);
+    const int t_hi = (int)((long long)n_tiles * (grp + 1) / groups);
+    for (int tile = t_lo + (int)blockIdx.x / groups; tile < t_hi; tile += per_group) {
+        const TileDesc td = ld_const(p.tiles, tile);
+        const BucketDev bk = ld_const(p.buckets, td.bucket);
+        const int nf = td.nf < 0 ? -td.nf : td.nf;
+        const bool valid = lane < nf;
+        const int fi = valid ? lane : 0;
+        const int fs = td.nf < 0 ? p.fam_list[td.fam0 + fi] : bk.fam_begin + td.fam0 + fi;
+        const int dst = p.perm[fs];   // the caller's index of this lane's family
+        for (int l = 0; l < p.n_leaves; ++l) s_cnt[l * 64 + lane] = p.counts_T[(long long)l * p.F + fs];
+        const __attribute__((address_space(4))) int* nslot = as_const(p.node_slot + (long long)td.point * p.n_nodes);
+
+        float sv[NS][LR];   // stored partial vectors (renormalised)
+        int se[NS];         // their exponents: true vector = stored 2^se
+        double sc[NS];      // their flush bounds, relative to their maximum
+        float acc[LR];      // the current node's running product
+        int e_sum = 0, log2S = 0;
+        double cert = 0.0;
+#pragma unroll
+        for (int c = 0; c < NS; ++c) {
+            se[c] = 0;
+            sc[c] = 0.0;
+#pragma unroll
+            for (int i = 0; i < LR; ++i) sv[c][i] = 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < LR; ++i) acc[i] = 1.f;
+
+        for (int st = 0; st < n_steps; ++st) {
+            const StepDev sd = ld_const(p.steps, st);
+            const bool root = (sd.ctl & kStepRoot) != 0;
+            if (sd.ctl & kStepFirst) {
+                // a non-root product starts from the row mask of the bound
+                // (rows past B are 0 and stay 0); root rows are sizes 0..R
+#pragma unroll
+                for (int i = 0; i < LR; ++i) acc[i] = (root || i < bk.L) ? 1.f : 0.f;
+                e_sum = 0;
+                log2S = 0;
+                cert = 0.0;
+            }
+            const long long base = (long long)nslot[sd.node] * slot_elems;
+            if (sd.leaf_col >= 0) {
+                // leaf: column `count` of the (error-model folded) matrix
+                const int cnt = s_cnt[sd.leaf_col * 64 + lane];
+                const float* __restrict__ col = p.leaf32 + base + (long long)cnt * Np;
+                e_sum -= kMatExp;
+                log2S += kMatExp;
+                cert += leaf_cert;
+#pragma unroll
+                for (int i = 0; i < LR; ++i) acc[i] *= col[i];
+            } else {
+                // interior child in slot s: x = M' v, column k of M' from the
+                // scalar cache, v[k] the lane's own
+                const SlotWeights<NS> sw(sd.in_slot);
+                const __attribute__((address_space(4))) float* M = as_const(p.cm32 + base);
+                float x[LR];
+#pragma unroll
+                for (int i = 0; i < LR; ++i) x[i] = 0.f;
+                // columns through two scalar-register buffers: the loads of
+                // column k + 1 are in flight while column k is used
+                float ca[LR], cb[LR];
+#pragma unroll
+                for (int i = 0; i < LR; ++i) ca[i] = M[i];
+#pragma unroll
+                for (int k = 0; k < LR; ++k) {
+                    float (&cur)[LR] = (k & 1) ? cb : ca;
+                    float (&nxt)[LR] = (k & 1) ? ca : cb;
+                    if (k + 1 < LR) {
+#pragma unroll
+                        for (int i = 0; i < LR; ++i) nxt[i] = M[(k + 1) * Np + i];
+                    }
+                    const float vk = pick(sv, sw, k);
+#pragma unroll
+                    for (int i = 0; i < LR; ++i) x[i] = __builtin_fmaf(cur[i], vk, x[i]);
+                    __builtin_amdgcn_sched_barrier(0);   // no loads further ahead than column k + 1
+                }
+                e_sum += pick(se, sw) - kMatExp;
+                cert += pick(sc, sw) + cint;
+                log2S += kMatExp + a_exp;
+#pragma unroll
+                for (int i = 0; i < LR; ++i) acc[i] *= x[i];
+            }
+            // a node's last non-root step stores it: shift to a maximum just
+            // below 2^a, exponent, flush bound re-expressed relative to the new
+            // maximum (it grows by S / m, the factor by which the product fell
+            // short of its nominal scale S = 2^log2S).  The slot update itself
+            // runs on every step (with no slot matched when nothing is
+            // stored): a branch around it makes the compiler keep copies of
+            // all slots.
+            {
+                const bool fin = (sd.ctl & kStepLast) && !root;
+                float v[LR];
+                int enew = 0;
+                double cnew = 0.0;
+                int out = NS;
+                if (fin) {
+                    float m = 0.f;
+#pragma unroll
+                    for (int i = 0; i < LR; ++i) m = fmaxf(m, acc[i]);   // entries are >= 0
+                    int e = 0;
+                    const float mant = m > 0.f ? frexpf(m, &e) : 1.f;
+                    const int sh = a_exp - e;
+                    cnew = m > 0.f ? (ldexp(cert * 1.01, log2S - e) + ldexp(kTau32, -e)) * rcp_up(mant) : 1e300;
+                    enew = e_sum + e - a_exp;
+                    out = sd.ctl & kStepSlotMask;
+#pragma unroll
+                    for (int i = 0; i < LR; ++i) v[i] = ldexpf(acc[i], sh);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < LR; ++i) v[i] = 0.f;
+                }
+                put(sv, se, sc, out, v, enew, cnew);
+            }
+            if (!((sd.ctl & kStepLast) && root)) continue;
+
+            // root: best x_j * prior_j over root sizes j + 1, j = 0..R-1
+            const __attribute__((address_space(4))) double* pr = as_const(p.prior64 + bk.prior_off);
+            RootBest rb{-__builtin_huge_val(), 0.0, -1, 0.0};
+#pragma unroll
+            for (int j = 0; j + 1 < LR; ++j) {
+                if (j >= bk.R) continue;
+                const double xv = (double)acc[j + 1];
+                if (p.root_out && valid) p.root_out[(long long)dst * p.root_stride + j] = ldexp(xv, e_sum);
+                root_take(rb, xv * pr[j], xv, j);
+            }
+            double val;
+            int flag = 0;
+            // rb.key = x_j * prior_j in units of 2^e_sum
+            if (!(rb.key > 0)) {
+                flag = kFlagRecompute64;
+                val = 0.0;
+            } else {
+                const double log_scale = (double)e_sum * kLn2;
+                if (p.model == 0)
+                    val = log(rb.x) + log(p.prior64[bk.prior_off + rb.j]) + log_scale;
+                else
+                    val = ldexp(rb.key, e_sum);
+                // flush bound (absolute, in units of 2^e_sum) against the
+                // likelihood, and FP64 range of the unscaled values
+                const double bound = (ldexp(cert, log2S) + kTau32) * 1.01;
+                if (!(bound <= kCertEta * rb.key)) flag |= kFlagRecompute64;
+                if (!(log(rb.xmax) + log_scale >= log(kFp64Floor))) flag |= kFlagRecompute64;
+                if (!(log(rb.key) + log_scale >= log(kFp64Floor))) flag |= kFlagRecompute64;
+            }
+            if (valid) {
+                p.out_val[(long long)td.point * p.F + dst] = val;
+                p.out_flag[(long long)td.point * p.F + dst] = flag;
+            }
+        }
+    }
+}
+
+// Rows per lane the kernel instances cover (vectors of sizes 0..B and root
+// sizes 0..R with max(B, R) + 1 <= rows): the buckets of bound 20 and 32.
+int lane_rows_for(int rows)
+{
+    if (rows <= 21) return 21;
+    if (rows <= 33) return 33;
+    return 0;
+}
+
+// Largest slot count the instances of a row count cover.
+int lane_max_slots(int lane_rows) { return lane_rows == 21 ? 5 : lane_rows == 33 ? 4 : 0; }
+
+// One instance per (rows, slots); with per_cu set, reports how many
+// workgroups of it fit on one CU instead of launching.
+hipError_t lane_dispatch(int lr, int ns, int grid, size_t shm, const PruneParams& p, hipStream_t st, int* per_cu)
+{
+#define CAFE_LANE_CASE(LR, NS)                                                                             \
+    if (lr == LR && ns <= NS) {                                                                            \
+        if (per_cu) return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_prune_lane<LR, NS>, 64, shm); \
+        hipLaunchKernelGGL((k_prune_lane<LR, NS>), dim3(grid), dim3(64), shm, st, p);                      \
+        return hipGetLastError();                                                                          \
+    }
+    CAFE_LANE_CASE(21, 2)
+    CAFE_LANE_CASE(21, 3)
+    CAFE_LANE_CASE(21, 4)
+    CAFE_LANE_CASE(21, 5)
+    CAFE_LANE_CASE(33, 2)
+    CAFE_LANE_CASE(33, 3)
+    CAFE_LANE_CASE(33, 4)
+#undef CAFE_LANE_CASE
+    return hipErrorInvalidValue;
+}
+
+}  // namespace cafe

@@ -24,7 +24,7 @@
 // Persistent kernels: a fixed grid walks the class's tile list (its length is
 // device-resident, so recompute lists built on the device need no host round
 // trip); every wave of a workgroup works on the same tile.
-#include "engine.h"
+#include "prune_common.h"
 
 namespace cafe {
 
@@ -56,32 +56,6 @@ __device__ __forceinline__ void lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// Read-only tables of a launch (program steps, tiles, buckets, node -> matrix
-// slots) read through the constant address space: uniform loads of them go
-// through the scalar cache and do not count against the vector-memory
-// counter that the prefetched operands are waited on.  The kernels never
-// write them; earlier kernels that do (the recompute-list packing) have
-// finished before a launch starts.
-template <typename T>
-__device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(const T* q)
-{
-    return (const __attribute__((address_space(4))) T*)q;
-}
-
-// Element i of a read-only table of 16-byte-multiple records, via as_const.
-template <typename T>
-__device__ __forceinline__ T ld_const(const T* q, long long i)
-{
-    typedef int int4v __attribute__((ext_vector_type(4)));
-    static_assert(sizeof(T) % sizeof(int4v) == 0, "records are whole 16-byte units");
-    constexpr int n = sizeof(T) / sizeof(int4v);
-    const __attribute__((address_space(4))) int4v* c = as_const((const int4v*)(q + i));
-    struct Words { int4v w[n]; } r;
-#pragma unroll
-    for (int k = 0; k < n; ++k) r.w[k] = c[k];
-    return __builtin_bit_cast(T, r);
-}
-
 // Size that A row m of a 16-row tile carries (see the fragment layout above).
 __device__ __forceinline__ int a_row(float*, int m) { return m; }
 __device__ __forceinline__ int a_row(double*, int m) { return 4 * (m & 3) + (m >> 2); }
@@ -106,8 +80,6 @@ __device__ __forceinline__ doublex4 load_a(const double* M, int off, int kt, int
     return doublex4{M[o], M[o + Np], M[o + 2 * Np], M[o + 3 * Np]};
 }
 
-// Operand prefetch depth: FP32 prefetches k-tiles 0 and 1 of an internal
-// step (two 16-byte loads per output tile), FP64 k-tile 0 (four 8-byte loads).
 // ds_swizzle pattern: lane l reads lane l ^ 16 (within each 32-lane half).
 constexpr int kSwizzleXor16 = 0x401F;
 
@@ -129,52 +101,9 @@ __device__ __forceinline__ void await_loaded(const V& v)
     for (int j = 0; j < 4; ++j) asm volatile("" ::"v"(v[j]));
 }
 
+// Operand prefetch depth: FP32 prefetches k-tiles 0 and 1 of an internal
+// step (two 16-byte loads per output tile), FP64 k-tile 0 (four 8-byte loads).
 template <typename T> struct Prefetch { static constexpr int PK = sizeof(T) == 4 ? 2 : 1; };
-
-constexpr double kLn2 = 0.69314718055994530942;
-// FP32 flush threshold (smallest normal float): the flush bound charges this
-// for every quantity FP32 could have flushed to zero.
-constexpr double kTau32 = 1.1754943508222875e-38;
-constexpr double kTwoPowMinusMatExp = 1.0 / (double)(1 << kMatExp);
-// an FP32 likelihood is accepted when the flush bound is below this fraction
-constexpr double kCertEta = 1e-9;
-// below this, FP64 values may lose bits to subnormals; such families are
-// recomputed in the reference's summation order (PLAIN64)
-constexpr double kFp64Floor = 1e-290;
-
-// 1/mant rounded up (mant in [0.5, 1)): the FP32 reciprocal is within one
-// ulp, and the factor covers that and the rounding of mant to FP32.
-__device__ __forceinline__ double rcp_up(float mant)
-{
-    return (double)__builtin_amdgcn_rcpf(mant) * (1.0 + 0x1p-20);
-}
-
-// Running root reduction of one lane: best key over its rows.
-struct RootBest {
-    double key;   // x * prior (scaled path: in units of 2^e_sum)
-    double x;     // x at the best key
-    int j;        // its index (smallest j on ties)
-    double xmax;  // max_j x_j
-};
-
-__device__ __forceinline__ void root_take(RootBest& b, double key, double x, int j)
-{
-    if (b.j < 0 || b.key < key || (b.key == key && j < b.j)) {
-        b.key = key;
-        b.x = x;
-        b.j = j;
-    }
-    b.xmax = b.xmax > x ? b.xmax : x;
-}
-
-__device__ __forceinline__ void root_merge(RootBest& b, double key, double x, int j, double xmax)
-{
-    if (j >= 0) {
-        const double xm = b.xmax > xmax ? b.xmax : xmax;
-        root_take(b, key, x, j);
-        b.xmax = xm;
-    }
-}
 
 __device__ __forceinline__ RootBest root_combine(RootBest b)
 {

@@ -278,15 +278,16 @@ class Plan:
 
     def class_times(self):
         """Per size class of the main pass, from the last profile() run:
-        (16-row tiles per vector, 16-family tiles, kernel ms, launch) arrays;
-        launch rows are (tiles per wave, waves per workgroup, partials in
-        LDS, workgroups)."""
+        (16-row tiles per vector, tiles, kernel ms, launch) arrays; launch
+        rows are (kernel: 0 matrix-core, 1 one family per lane, 2 PLAIN64;
+        tiles per wave or rows per lane; waves per workgroup; partials in
+        LDS; workgroups; families per tile)."""
         n = ctypes.c_int(0)
         self.eng._check(self.lib.cafe_gpu_plan_class_times(self.h, 0, ctypes.byref(n), None, None, None, None))
         nt = np.zeros(n.value, dtype=np.int32)
         tiles = np.zeros(n.value, dtype=np.int64)
         ms = np.zeros(n.value)
-        launch = np.zeros((n.value, 4), dtype=np.int32)
+        launch = np.zeros((n.value, 6), dtype=np.int32)
         self.eng._check(self.lib.cafe_gpu_plan_class_times(self.h, n.value, ctypes.byref(n), _ip(nt),
                                                            tiles.ctypes.data_as(_pll), _dp(ms), _ip(launch)))
         return nt, tiles, ms, launch

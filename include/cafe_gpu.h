@@ -198,10 +198,12 @@ int cafe_gpu_plan_recompute_counts(cafe_gpu_plan *plan, long long *fp64, long lo
  * cafe_gpu_plan_profile run (no reference counterpart; measurement): the
  * class's 16-row tile count per vector, its number of 16-family tiles and
  * the milliseconds of its kernel launch (HIP events on the context stream),
- * and, in launch[4 i .. 4 i + 3], that launch's shape: 16-row tiles per wave
- * (PLAIN64: rows per lane), waves per workgroup, partial vectors in LDS
- * (0/1) and workgroups.  Writes at most cap entries (any output may be
- * null); *n_classes is the number of classes. */
+ * and, in launch[6 i .. 6 i + 5], that launch's shape: kernel (0 matrix-core
+ * k_prune_mfma, 1 one-family-per-lane k_prune_lane, 2 PLAIN64 k_prune_plain),
+ * 16-row tiles per wave (lane and PLAIN64 kernels: rows per lane), waves per
+ * workgroup, partial vectors in LDS (0/1), workgroups, and families per tile.
+ * Writes at most cap entries (any output may be null); *n_classes is the
+ * number of classes. */
 int cafe_gpu_plan_class_times(cafe_gpu_plan *plan, int cap, int *n_classes, int *class_nt,
                               long long *class_tiles, double *ms, int *launch);
 int cafe_gpu_plan_destroy(cafe_gpu_plan *plan);
