@@ -131,93 +131,90 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LaneWaves<LR
 #pragma unroll
         for (int i = 0; i < LR; ++i) acc[i] = 1.f;
 
-        for (int st = 0; st < n_steps; ++st) {
-            const StepDev sd = ld_const(p.steps, st);
-            const bool root = (sd.ctl & kStepRoot) != 0;
-            if (sd.ctl & kStepFirst) {
-                // a non-root product starts from the row mask of the bound
-                // (rows past B are 0 and stay 0); root rows are sizes 0..R
+        // node by node (the program's ops, root last): the children's factors
+        // into acc, then the node is stored in its slot.  Every non-root
+        // node ends in the same store, with no branch around it: a branch
+        // makes the compiler keep copies of all slots.
+        int st = 0;
+        for (;;) {
+            bool root = false;
+            int out = 0;
+            for (;;) {
+                const StepDev sd = ld_const(p.steps, st++);
+                root = (sd.ctl & kStepRoot) != 0;
+                out = sd.ctl & kStepSlotMask;
+                if (sd.ctl & kStepFirst) {
+                    // a non-root product starts from the row mask of the bound
+                    // (rows past B are 0 and stay 0); root rows are sizes 0..R
 #pragma unroll
-                for (int i = 0; i < LR; ++i) acc[i] = (root || i < bk.L) ? 1.f : 0.f;
-                e_sum = 0;
-                log2S = 0;
-                cert = 0.0;
-            }
-            const long long base = (long long)nslot[sd.node] * slot_elems;
-            if (sd.leaf_col >= 0) {
-                // leaf: column `count` of the (error-model folded) matrix
-                const int cnt = s_cnt[sd.leaf_col * 64 + lane];
-                const float* __restrict__ col = p.leaf32 + base + (long long)cnt * Np;
-                e_sum -= kMatExp;
-                log2S += kMatExp;
-                cert += leaf_cert;
-#pragma unroll
-                for (int i = 0; i < LR; ++i) acc[i] *= col[i];
-            } else {
-                // interior child in slot s: x = M' v, column k of M' from the
-                // scalar cache, v[k] the lane's own
-                const SlotWeights<NS> sw(sd.in_slot);
-                const __attribute__((address_space(4))) float* M = as_const(p.cm32 + base);
-                float x[LR];
-#pragma unroll
-                for (int i = 0; i < LR; ++i) x[i] = 0.f;
-                // columns through two scalar-register buffers: the loads of
-                // column k + 1 are in flight while column k is used
-                float ca[LR], cb[LR];
-#pragma unroll
-                for (int i = 0; i < LR; ++i) ca[i] = M[i];
-#pragma unroll
-                for (int k = 0; k < LR; ++k) {
-                    float (&cur)[LR] = (k & 1) ? cb : ca;
-                    float (&nxt)[LR] = (k & 1) ? ca : cb;
-                    if (k + 1 < LR) {
-#pragma unroll
-                        for (int i = 0; i < LR; ++i) nxt[i] = M[(k + 1) * Np + i];
-                    }
-                    const float vk = pick(sv, sw, k);
-#pragma unroll
-                    for (int i = 0; i < LR; ++i) x[i] = __builtin_fmaf(cur[i], vk, x[i]);
-                    __builtin_amdgcn_sched_barrier(0);   // no loads further ahead than column k + 1
+                    for (int i = 0; i < LR; ++i) acc[i] = (root || i < bk.L) ? 1.f : 0.f;
+                    e_sum = 0;
+                    log2S = 0;
+                    cert = 0.0;
                 }
-                e_sum += pick(se, sw) - kMatExp;
-                cert += pick(sc, sw) + cint;
-                log2S += kMatExp + a_exp;
-#pragma unroll
-                for (int i = 0; i < LR; ++i) acc[i] *= x[i];
-            }
-            // a node's last non-root step stores it: shift to a maximum just
-            // below 2^a, exponent, flush bound re-expressed relative to the new
-            // maximum (it grows by S / m, the factor by which the product fell
-            // short of its nominal scale S = 2^log2S).  The slot update itself
-            // runs on every step (with no slot matched when nothing is
-            // stored): a branch around it makes the compiler keep copies of
-            // all slots.
-            {
-                const bool fin = (sd.ctl & kStepLast) && !root;
-                float v[LR];
-                int enew = 0;
-                double cnew = 0.0;
-                int out = NS;
-                if (fin) {
-                    float m = 0.f;
-#pragma unroll
-                    for (int i = 0; i < LR; ++i) m = fmaxf(m, acc[i]);   // entries are >= 0
-                    int e = 0;
-                    const float mant = m > 0.f ? frexpf(m, &e) : 1.f;
-                    const int sh = a_exp - e;
-                    cnew = m > 0.f ? (ldexp(cert * 1.01, log2S - e) + ldexp(kTau32, -e)) * rcp_up(mant) : 1e300;
-                    enew = e_sum + e - a_exp;
-                    out = sd.ctl & kStepSlotMask;
-#pragma unroll
-                    for (int i = 0; i < LR; ++i) v[i] = ldexpf(acc[i], sh);
+                const long long base = (long long)nslot[sd.node] * slot_elems;
+                if (sd.leaf_col >= 0) {
+                    // leaf: column `count` of the (error-model folded) matrix
+                    const int cnt = s_cnt[sd.leaf_col * 64 + lane];
+                    const float* __restrict__ col = p.leaf32 + base + (long long)cnt * Np;
+                    e_sum -= kMatExp;
+                    log2S += kMatExp;
+                    cert += leaf_cert;
+    #pragma unroll
+                    for (int i = 0; i < LR; ++i) acc[i] *= col[i];
                 } else {
-#pragma unroll
-                    for (int i = 0; i < LR; ++i) v[i] = 0.f;
+                    // interior child in slot s: x = M' v, column k of M' from the
+                    // scalar cache, v[k] the lane's own
+                    const SlotWeights<NS> sw(sd.in_slot);
+                    const __attribute__((address_space(4))) float* M = as_const(p.cm32 + base);
+                    float x[LR];
+    #pragma unroll
+                    for (int i = 0; i < LR; ++i) x[i] = 0.f;
+                    // columns through two scalar-register buffers: the loads of
+                    // column k + 1 are in flight while column k is used
+                    float ca[LR], cb[LR];
+    #pragma unroll
+                    for (int i = 0; i < LR; ++i) ca[i] = M[i];
+    #pragma unroll
+                    for (int k = 0; k < LR; ++k) {
+                        float (&cur)[LR] = (k & 1) ? cb : ca;
+                        float (&nxt)[LR] = (k & 1) ? ca : cb;
+                        if (k + 1 < LR) {
+    #pragma unroll
+                            for (int i = 0; i < LR; ++i) nxt[i] = M[(k + 1) * Np + i];
+                        }
+                        const float vk = pick(sv, sw, k);
+    #pragma unroll
+                        for (int i = 0; i < LR; ++i) x[i] = __builtin_fmaf(cur[i], vk, x[i]);
+                        __builtin_amdgcn_sched_barrier(0);   // no loads further ahead than column k + 1
+                    }
+                    e_sum += pick(se, sw) - kMatExp;
+                    cert += pick(sc, sw) + cint;
+                    log2S += kMatExp + a_exp;
+    #pragma unroll
+                    for (int i = 0; i < LR; ++i) acc[i] *= x[i];
                 }
-                put(sv, se, sc, out, v, enew, cnew);
+                if ((sd.ctl & kStepLast) || st >= n_steps) break;
             }
-            if (!((sd.ctl & kStepLast) && root)) continue;
-
+            if (root || st >= n_steps) break;
+            // store the node: shift to a maximum just below 2^a, exponent,
+            // flush bound re-expressed relative to the new maximum (it grows
+            // by S / m, the factor by which the product fell short of its
+            // nominal scale S = 2^log2S)
+            float m = 0.f;
+#pragma unroll
+            for (int i = 0; i < LR; ++i) m = fmaxf(m, acc[i]);   // entries are >= 0
+            int e = 0;
+            const float mant = m > 0.f ? frexpf(m, &e) : 1.f;
+            const int sh = a_exp - e;
+            const double cnew =
+                m > 0.f ? (ldexp(cert * 1.01, log2S - e) + ldexp(kTau32, -e)) * rcp_up(mant) : 1e300;
+            float v[LR];
+#pragma unroll
+            for (int i = 0; i < LR; ++i) v[i] = ldexpf(acc[i], sh);
+            put(sv, se, sc, out, v, e_sum + e - a_exp, cnew);
+        }
+        {
             // root: best x_j * prior_j over root sizes j + 1, j = 0..R-1
             const __attribute__((address_space(4))) double* pr = as_const(p.prior64 + bk.prior_off);
             RootBest rb{-__builtin_huge_val(), 0.0, -1, 0.0};
