@@ -160,8 +160,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LaneWaves<LR
                     e_sum -= kMatExp;
                     log2S += kMatExp;
                     cert += leaf_cert;
-    #pragma unroll
-                    for (int i = 0; i < LR; ++i) acc[i] *= col[i];
+                        // 16-byte loads: a column starts 128-byte aligned (Np is a
+                    // multiple of 32) and holds Np >= 4 ceil(LR / 4) entries
+                    // (Np >= 32 always; 33-row classes have N >= 22, Np >= 64)
+                    typedef float f4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+                    for (int q = 0; q < (LR + 3) / 4; ++q) {
+                        const f4 c4 = *(const f4*)(col + 4 * q);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            if (4 * q + r < LR) acc[4 * q + r] *= c4[r];
+                    }
                 } else {
                     // interior child in slot s: x = M' v, column k of M' from the
                     // scalar cache, v[k] the lane's own
@@ -170,23 +179,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LaneWaves<LR
                     float x[LR];
     #pragma unroll
                     for (int i = 0; i < LR; ++i) x[i] = 0.f;
-                    // columns through two scalar-register buffers: the loads of
-                    // column k + 1 are in flight while column k is used
-                    float ca[LR], cb[LR];
-    #pragma unroll
-                    for (int i = 0; i < LR; ++i) ca[i] = M[i];
-    #pragma unroll
+                    // columns through scalar-register buffers: the loads of the
+                    // next D columns are in flight while column k is used
+                    // (D = 2 for 21 rows, 1 for larger vectors: SGPR budget)
+                    constexpr int D = LR <= 24 ? 2 : 1;
+                    float cbuf[D + 1][LR];
+#pragma unroll
+                    for (int d = 0; d < D; ++d)
+#pragma unroll
+                        for (int i = 0; i < LR; ++i) cbuf[d][i] = d < LR ? M[d * Np + i] : 0.f;
+#pragma unroll
                     for (int k = 0; k < LR; ++k) {
-                        float (&cur)[LR] = (k & 1) ? cb : ca;
-                        float (&nxt)[LR] = (k & 1) ? ca : cb;
-                        if (k + 1 < LR) {
-    #pragma unroll
-                            for (int i = 0; i < LR; ++i) nxt[i] = M[(k + 1) * Np + i];
+                        float (&cur)[LR] = cbuf[k % (D + 1)];
+                        float (&nxt)[LR] = cbuf[(k + D) % (D + 1)];
+                        if (k + D < LR) {
+#pragma unroll
+                            for (int i = 0; i < LR; ++i) nxt[i] = M[(k + D) * Np + i];
                         }
                         const float vk = pick(sv, sw, k);
-    #pragma unroll
+#pragma unroll
                         for (int i = 0; i < LR; ++i) x[i] = __builtin_fmaf(cur[i], vk, x[i]);
-                        __builtin_amdgcn_sched_barrier(0);   // no loads further ahead than column k + 1
+                        __builtin_amdgcn_sched_barrier(0);   // no loads further ahead than column k + D
                     }
                     e_sum += pick(se, sw) - kMatExp;
                     cert += pick(sc, sw) + cint;
