@@ -18,6 +18,8 @@ namespace cafe {
 hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const double* d_lg, int n,
                         int first_slot, int N, int Np, long long slot_elems, double* mt64, float* mt32,
                         float* lf32, hipStream_t stream);
+hipError_t launch_transpose32(const float* lf32, float* mt32, int first_slot, int n, int Np, long long slot_elems,
+                              hipStream_t stream);
 hipError_t launch_fill_zero(void* p, long long bytes, hipStream_t stream);
 hipError_t launch_fold(const double* mt64, const float* lf32, double* out64, float* out32, const double* err64,
                        const float* err32, int n_dev, int n_rows, int N, int Np, long long slot_elems,
@@ -293,7 +295,7 @@ int cafe_gpu_create(int device, cafe_gpu_ctx** out)
             cafe_gpu_destroy(ctx);
             return fail(CAFE_GPU_EDEVICE, "hipStreamCreate failed");
         }
-    ctx->join_ev.assign(3 * (kSideStreams + 1), nullptr);
+    ctx->join_ev.assign(3 * (kSideStreams + 1) + 1, nullptr);   // + the deferred transpose
     for (auto& e : ctx->join_ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
     // log-gamma of integers from the host libm: the reference's
     // lgamma cache (probability.cpp:69-75)
@@ -600,7 +602,8 @@ struct KernelClass {
     // PLAIN64 recompute: disjoint, since one class's recompute can run
     // beside another class's main pass
     size_t off_main = 0, off_r64 = 0, off_rp = 0;
-    int stream = 0;      // side stream of its launches
+    int stream = 0;         // side stream of its launches
+    int stream_defer = 0;   // the same beside a deferred transpose (0: per-lane classes)
 };
 
 // Host-side description of one pruning sequence over a tile list.
@@ -909,11 +912,17 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
             order[ci] = ci;
         }
         std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return cost[a] > cost[b2]; });
-        std::vector<double> load(kSideStreams, 0.0);
+        std::vector<double> load(kSideStreams, 0.0), load_d(kSideStreams, 0.0);
         for (int ci : order) {
             const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
             pp.classes[ci].stream = k;
             load[k] += cost[ci];
+            // beside a deferred transpose: stream 0 for the per-lane classes,
+            // the matrix-core classes over the others
+            int kd = 0;
+            if (pp.classes[ci].lane_rows == 0) kd = (int)(std::min_element(load_d.begin() + 1, load_d.end()) - load_d.begin());
+            pp.classes[ci].stream_defer = kd;
+            load_d[kd] += cost[ci];
         }
         pp.main_order = order;
     }
@@ -1027,9 +1036,13 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
 // Enqueues the pruning sequence: the main path per size class, then
 // (device-side lists, no host round trip) the FP64 recompute of the FP32
 // families the flush bound did not certify and the PLAIN64 recompute of
-// families near the subnormal range.
+// families near the subnormal range.  transpose_keys > 0: the row-major FP32
+// pool of slots 0 .. transpose_keys-1 is still to be made from the c-major
+// one; it is made here, beside the per-lane classes (which read only the
+// c-major pool), and the matrix-core classes wait for it.
 int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream_t st, hipEvent_t ev_main,
-                  hipEvent_t ev_fb, bool concurrent, const std::vector<hipEvent_t>* cls_ev = nullptr)
+                  hipEvent_t ev_fb, bool concurrent, const std::vector<hipEvent_t>* cls_ev = nullptr,
+                  int transpose_keys = 0)
 {
     const int n_cls = (int)pp.classes.size();
     int* d_cnt = b.ntiles.as<int>();
@@ -1112,6 +1125,13 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
     // all of them before the next stage: a recompute is a latency-bound chain
     // on a few tiles, and beside a main pass it would slow that pass down.
     const int S = concurrent ? std::min((int)ctx->side.size(), n_cls) : 0;
+    auto transpose = [&]() -> int {
+        HIP_OR_FAIL(launch_transpose32(ctx->lf32.as<float>(), ctx->mt32.as<float>(), 0, transpose_keys, ctx->Np,
+                                       ctx->slot_elems, st));
+        return CAFE_GPU_OK;
+    };
+    if (transpose_keys > 0 && S <= 1)
+        if (int rc = transpose()) return rc;
     auto stage = [&](int idx, auto&& launch) -> int {
         if (S <= 1) {
             for (int ci = 0; ci < n_cls; ++ci) {
@@ -1123,11 +1143,28 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
         }
         hipEvent_t* ev = ctx->join_ev.data() + idx * (kSideStreams + 1);
         HIP_OR_FAIL(hipEventRecord(ev[0], st));
-        for (int k = 0; k < S; ++k) HIP_OR_FAIL(hipStreamWaitEvent(ctx->side[k], ev[0], 0));
-        for (int i = 0; i < n_cls; ++i) {
-            const int ci = pp.main_order[i];
-            const int k = idx == 0 ? pp.classes[ci].stream % S : i % S;
-            if (int rc = launch(ci, ctx->side[k])) return rc;
+        if (idx == 0 && transpose_keys > 0) {
+            // a fork/join tree: side stream 0 takes the per-lane classes at
+            // once, st makes the row-major pool meanwhile, and the other side
+            // streams fork after it with the matrix-core classes
+            HIP_OR_FAIL(hipStreamWaitEvent(ctx->side[0], ev[0], 0));
+            for (int ci : pp.main_order)
+                if (pp.classes[ci].lane_rows > 0)
+                    if (int rc = launch(ci, ctx->side[0])) return rc;
+            if (int rc = transpose()) return rc;
+            hipEvent_t ev_t = ctx->join_ev.back();
+            HIP_OR_FAIL(hipEventRecord(ev_t, st));
+            for (int k = 1; k < S; ++k) HIP_OR_FAIL(hipStreamWaitEvent(ctx->side[k], ev_t, 0));
+            for (int ci : pp.main_order)
+                if (pp.classes[ci].lane_rows == 0)
+                    if (int rc = launch(ci, ctx->side[1 + (pp.classes[ci].stream_defer - 1) % (S - 1)])) return rc;
+        } else {
+            for (int k = 0; k < S; ++k) HIP_OR_FAIL(hipStreamWaitEvent(ctx->side[k], ev[0], 0));
+            for (int i = 0; i < n_cls; ++i) {
+                const int ci = pp.main_order[i];
+                const int k = idx == 0 ? pp.classes[ci].stream % S : i % S;
+                if (int rc = launch(ci, ctx->side[k])) return rc;
+            }
         }
         for (int k = 0; k < S; ++k) {
             HIP_OR_FAIL(hipEventRecord(ev[1 + k], ctx->side[k]));
@@ -1207,16 +1244,22 @@ int plan_enqueue(cafe_gpu_plan* pl, bool timed)
 {
     cafe_gpu_ctx* ctx = pl->ctx;
     hipStream_t st = ctx->stream;
+    // the recurrence build leaves the row-major FP32 pool to a transpose,
+    // which the per-lane classes do not need: concurrent runs with such
+    // classes transpose beside them
+    bool lane = false;
+    for (const KernelClass& kc : pl->pp.classes) lane |= kc.lane_rows > 0;
+    const bool defer = !timed && pl->algo != 0 && lane && pl->pp.path == CAFE_GPU_PATH_BATCHED32;
     if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[0], st));
     HIP_OR_FAIL(launch_build(pl->algo, pl->d_lam.as<double>(), pl->d_t.as<double>(), ctx->lg.as<double>(), pl->n_keys,
-                             0, ctx->N, ctx->Np, ctx->slot_elems, ctx->mt64.as<double>(), ctx->mt32.as<float>(),
-                             ctx->lf32.as<float>(), st));
+                             0, ctx->N, ctx->Np, ctx->slot_elems, ctx->mt64.as<double>(),
+                             defer ? nullptr : ctx->mt32.as<float>(), ctx->lf32.as<float>(), st));
     if (int rc = enqueue_fold(ctx, 0, pl->n_keys, st)) return rc;
     if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[1], st));
     // profile runs issue the size classes one after another (standalone kernel
     // times); executions and the captured graph run them concurrently
     if (int rc = enqueue_prune(ctx, pl->bufs, pl->pp, st, timed ? pl->ev[2] : nullptr, timed ? pl->ev[3] : nullptr,
-                               !timed, timed ? &pl->cls_ev : nullptr))
+                               !timed, timed ? &pl->cls_ev : nullptr, defer ? pl->n_keys : 0))
         return rc;
     if (int rc = enqueue_finalize(ctx, pl->bufs, pl->model, pl->T, pl->K, st)) return rc;
     if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[4], st));

@@ -284,10 +284,12 @@ namespace cafe {
 
 // Row-major FP32 copy of slots first_slot .. first_slot+n-1 (Np is a
 // multiple of 32).
-static hipError_t launch_transpose32(const float* lf32, float* mt32, int first_slot, int n, int Np,
-                                     long long slot_elems, hipStream_t stream)
+// Row-major FP32 copies of slots first_slot .. first_slot+n-1 from the c-major
+// pool (nothing when mt32 is null: the caller transposes later).
+hipError_t launch_transpose32(const float* lf32, float* mt32, int first_slot, int n, int Np,
+                              long long slot_elems, hipStream_t stream)
 {
-    if (!mt32 || !lf32) return hipSuccess;
+    if (!mt32 || !lf32) return hipGetLastError();
     if (Np % 32 != 0) return hipErrorInvalidValue;
     for (int k0 = 0; k0 < n; k0 += 65535) {
         const int nk = n - k0 < 65535 ? n - k0 : 65535;
