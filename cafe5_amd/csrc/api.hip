@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <map>
 #include <numeric>
 #include <string>
@@ -18,8 +19,6 @@ namespace cafe {
 hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const double* d_lg, int n,
                         int first_slot, int N, int Np, long long slot_elems, double* mt64, float* mt32,
                         float* lf32, hipStream_t stream);
-hipError_t launch_transpose32(const float* lf32, float* mt32, int first_slot, int n, int Np, long long slot_elems,
-                              hipStream_t stream);
 hipError_t launch_fill_zero(void* p, long long bytes, hipStream_t stream);
 hipError_t launch_fold(const double* mt64, const float* lf32, double* out64, float* out32, const double* err64,
                        const float* err32, int n_dev, int n_rows, int N, int Np, long long slot_elems,
@@ -176,13 +175,14 @@ int ensure_fold_pools(cafe_gpu_ctx* ctx)
 }
 
 // Enqueues the error-model fold of slots first .. first+n-1 (nothing without
-// an error model).
-int enqueue_fold(cafe_gpu_ctx* ctx, int first, int n, hipStream_t st)
+// an error model) into the FP64 pool (fp64) and the FP32 one (fp32).
+int enqueue_fold(cafe_gpu_ctx* ctx, int first, int n, hipStream_t st, bool fp64 = true, bool fp32 = true)
 {
     if (!ctx->n_dev || n <= 0) return CAFE_GPU_OK;
-    HIP_OR_FAIL(launch_fold(ctx->mt64.as<double>(), ctx->lf32.as<float>(), ctx->mt64e.as<double>(),
-                            ctx->mt32e.as<float>(), ctx->d_err64.as<double>(), ctx->d_err32.as<float>(), ctx->n_dev,
-                            ctx->err_rows, ctx->N, ctx->Np, ctx->slot_elems, first, n, st));
+    HIP_OR_FAIL(launch_fold(ctx->mt64.as<double>(), ctx->lf32.as<float>(), fp64 ? ctx->mt64e.as<double>() : nullptr,
+                            fp32 ? ctx->mt32e.as<float>() : nullptr, ctx->d_err64.as<double>(),
+                            ctx->d_err32.as<float>(), ctx->n_dev, ctx->err_rows, ctx->N, ctx->Np, ctx->slot_elems,
+                            first, n, st));
     return CAFE_GPU_OK;
 }
 
@@ -295,7 +295,7 @@ int cafe_gpu_create(int device, cafe_gpu_ctx** out)
             cafe_gpu_destroy(ctx);
             return fail(CAFE_GPU_EDEVICE, "hipStreamCreate failed");
         }
-    ctx->join_ev.assign(3 * (kSideStreams + 1) + 1, nullptr);   // + the deferred transpose
+    ctx->join_ev.assign(3 * (kSideStreams + 1), nullptr);
     for (auto& e : ctx->join_ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
     // log-gamma of integers from the host libm: the reference's
     // lgamma cache (probability.cpp:69-75)
@@ -602,8 +602,7 @@ struct KernelClass {
     // PLAIN64 recompute: disjoint, since one class's recompute can run
     // beside another class's main pass
     size_t off_main = 0, off_r64 = 0, off_rp = 0;
-    int stream = 0;         // side stream of its launches
-    int stream_defer = 0;   // the same beside a deferred transpose (0: per-lane classes)
+    int stream = 0;      // side stream of its launches
 };
 
 // Host-side description of one pruning sequence over a tile list.
@@ -613,6 +612,7 @@ struct PrunePlan {
     std::vector<int> class_counts;   // main tiles per class (+ padding)
     std::vector<int> plain_off;      // offsets of the one-family PLAIN64 recompute lists
     std::vector<int> main_order;     // main-pass launch order (longest first)
+    int light_stream = 0;            // the side stream expected to finish its main-pass work first
     int plain_list_size = 0;
     int n_tiles = 0, n_segs = 0, maxLR = 0;
     int r16_total = 0;               // capacity of the FP64 recompute lists (16-family tiles)
@@ -912,19 +912,23 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
             order[ci] = ci;
         }
         std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return cost[a] > cost[b2]; });
-        std::vector<double> load(kSideStreams, 0.0), load_d(kSideStreams, 0.0);
+        std::vector<double> load(kSideStreams, 0.0);
         for (int ci : order) {
             const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
             pp.classes[ci].stream = k;
             load[k] += cost[ci];
-            // beside a deferred transpose: stream 0 for the per-lane classes,
-            // the matrix-core classes over the others
-            int kd = 0;
-            if (pp.classes[ci].lane_rows == 0) kd = (int)(std::min_element(load_d.begin() + 1, load_d.end()) - load_d.begin());
-            pp.classes[ci].stream_defer = kd;
-            load_d[kd] += cost[ci];
         }
         pp.main_order = order;
+        // the stream that frees up first: that of the per-lane classes (at
+        // full VALU rate they end well before their estimate), else the one
+        // with the least estimated work
+        pp.light_stream = (int)(std::min_element(load.begin(), load.begin() + std::max(1, std::min(kSideStreams, n_cls))) -
+                                load.begin());
+        for (int ci : order)
+            if (pp.classes[ci].lane_rows > 0) {
+                pp.light_stream = pp.classes[ci].stream;
+                break;
+            }
     }
     pp.n_tiles = (int)tiles.size();
     pp.class_counts.assign(n_cls + 4, 0);
@@ -1036,13 +1040,12 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
 // Enqueues the pruning sequence: the main path per size class, then
 // (device-side lists, no host round trip) the FP64 recompute of the FP32
 // families the flush bound did not certify and the PLAIN64 recompute of
-// families near the subnormal range.  transpose_keys > 0: the row-major FP32
-// pool of slots 0 .. transpose_keys-1 is still to be made from the c-major
-// one; it is made here, beside the per-lane classes (which read only the
-// c-major pool), and the matrix-core classes wait for it.
+// families near the subnormal range.  late_build (when set) enqueues the
+// FP64 pool, which only the recompute stages read: it runs on the side stream
+// expected to finish its main-pass work first, after that work.
 int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream_t st, hipEvent_t ev_main,
                   hipEvent_t ev_fb, bool concurrent, const std::vector<hipEvent_t>* cls_ev = nullptr,
-                  int transpose_keys = 0)
+                  const std::function<int(hipStream_t)>* late_build = nullptr)
 {
     const int n_cls = (int)pp.classes.size();
     int* d_cnt = b.ntiles.as<int>();
@@ -1125,13 +1128,8 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
     // all of them before the next stage: a recompute is a latency-bound chain
     // on a few tiles, and beside a main pass it would slow that pass down.
     const int S = concurrent ? std::min((int)ctx->side.size(), n_cls) : 0;
-    auto transpose = [&]() -> int {
-        HIP_OR_FAIL(launch_transpose32(ctx->lf32.as<float>(), ctx->mt32.as<float>(), 0, transpose_keys, ctx->Np,
-                                       ctx->slot_elems, st));
-        return CAFE_GPU_OK;
-    };
-    if (transpose_keys > 0 && S <= 1)
-        if (int rc = transpose()) return rc;
+    if (late_build && S <= 1)
+        if (int rc = (*late_build)(st)) return rc;
     auto stage = [&](int idx, auto&& launch) -> int {
         if (S <= 1) {
             for (int ci = 0; ci < n_cls; ++ci) {
@@ -1143,29 +1141,14 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
         }
         hipEvent_t* ev = ctx->join_ev.data() + idx * (kSideStreams + 1);
         HIP_OR_FAIL(hipEventRecord(ev[0], st));
-        if (idx == 0 && transpose_keys > 0) {
-            // a fork/join tree: side stream 0 takes the per-lane classes at
-            // once, st makes the row-major pool meanwhile, and the other side
-            // streams fork after it with the matrix-core classes
-            HIP_OR_FAIL(hipStreamWaitEvent(ctx->side[0], ev[0], 0));
-            for (int ci : pp.main_order)
-                if (pp.classes[ci].lane_rows > 0)
-                    if (int rc = launch(ci, ctx->side[0])) return rc;
-            if (int rc = transpose()) return rc;
-            hipEvent_t ev_t = ctx->join_ev.back();
-            HIP_OR_FAIL(hipEventRecord(ev_t, st));
-            for (int k = 1; k < S; ++k) HIP_OR_FAIL(hipStreamWaitEvent(ctx->side[k], ev_t, 0));
-            for (int ci : pp.main_order)
-                if (pp.classes[ci].lane_rows == 0)
-                    if (int rc = launch(ci, ctx->side[1 + (pp.classes[ci].stream_defer - 1) % (S - 1)])) return rc;
-        } else {
-            for (int k = 0; k < S; ++k) HIP_OR_FAIL(hipStreamWaitEvent(ctx->side[k], ev[0], 0));
-            for (int i = 0; i < n_cls; ++i) {
-                const int ci = pp.main_order[i];
-                const int k = idx == 0 ? pp.classes[ci].stream % S : i % S;
-                if (int rc = launch(ci, ctx->side[k])) return rc;
-            }
+        for (int k = 0; k < S; ++k) HIP_OR_FAIL(hipStreamWaitEvent(ctx->side[k], ev[0], 0));
+        for (int i = 0; i < n_cls; ++i) {
+            const int ci = pp.main_order[i];
+            const int k = idx == 0 ? pp.classes[ci].stream % S : i % S;
+            if (int rc = launch(ci, ctx->side[k])) return rc;
         }
+        if (idx == 0 && late_build)
+            if (int rc = (*late_build)(ctx->side[pp.light_stream % S])) return rc;
         for (int k = 0; k < S; ++k) {
             HIP_OR_FAIL(hipEventRecord(ev[1 + k], ctx->side[k]));
             HIP_OR_FAIL(hipStreamWaitEvent(st, ev[1 + k], 0));
@@ -1244,22 +1227,26 @@ int plan_enqueue(cafe_gpu_plan* pl, bool timed)
 {
     cafe_gpu_ctx* ctx = pl->ctx;
     hipStream_t st = ctx->stream;
-    // the recurrence build leaves the row-major FP32 pool to a transpose,
-    // which the per-lane classes do not need: concurrent runs with such
-    // classes transpose beside them
-    bool lane = false;
-    for (const KernelClass& kc : pl->pp.classes) lane |= kc.lane_rows > 0;
-    const bool defer = !timed && pl->algo != 0 && lane && pl->pp.path == CAFE_GPU_PATH_BATCHED32;
+    // BATCHED32 reads the FP64 pool only in its recompute stages: concurrent
+    // runs with the recurrence build make the FP32 pools first and the FP64
+    // one beside the main pass
+    const bool split = !timed && pl->algo != 0 && pl->pp.path == CAFE_GPU_PATH_BATCHED32;
     if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[0], st));
     HIP_OR_FAIL(launch_build(pl->algo, pl->d_lam.as<double>(), pl->d_t.as<double>(), ctx->lg.as<double>(), pl->n_keys,
-                             0, ctx->N, ctx->Np, ctx->slot_elems, ctx->mt64.as<double>(),
-                             defer ? nullptr : ctx->mt32.as<float>(), ctx->lf32.as<float>(), st));
-    if (int rc = enqueue_fold(ctx, 0, pl->n_keys, st)) return rc;
+                             0, ctx->N, ctx->Np, ctx->slot_elems, split ? nullptr : ctx->mt64.as<double>(),
+                             ctx->mt32.as<float>(), ctx->lf32.as<float>(), st));
+    if (int rc = enqueue_fold(ctx, 0, pl->n_keys, st, !split, true)) return rc;
     if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[1], st));
+    const std::function<int(hipStream_t)> build64 = [&](hipStream_t s) -> int {
+        HIP_OR_FAIL(launch_build(pl->algo, pl->d_lam.as<double>(), pl->d_t.as<double>(), ctx->lg.as<double>(),
+                                 pl->n_keys, 0, ctx->N, ctx->Np, ctx->slot_elems, ctx->mt64.as<double>(), nullptr,
+                                 nullptr, s));
+        return enqueue_fold(ctx, 0, pl->n_keys, s, true, false);
+    };
     // profile runs issue the size classes one after another (standalone kernel
     // times); executions and the captured graph run them concurrently
     if (int rc = enqueue_prune(ctx, pl->bufs, pl->pp, st, timed ? pl->ev[2] : nullptr, timed ? pl->ev[3] : nullptr,
-                               !timed, timed ? &pl->cls_ev : nullptr, defer ? pl->n_keys : 0))
+                               !timed, timed ? &pl->cls_ev : nullptr, split ? &build64 : nullptr))
         return rc;
     if (int rc = enqueue_finalize(ctx, pl->bufs, pl->model, pl->T, pl->K, st)) return rc;
     if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[4], st));

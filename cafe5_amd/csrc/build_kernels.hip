@@ -2,8 +2,8 @@
 // (replaces matrix_cache::precalculate_matrices, src/matrix_cache.cpp:113-163).
 //
 // The kernels write the pool layouts of engine.h: an FP64 copy (c-major) and,
-// when requested, an FP32 copy (row-major) of every matrix scaled by
-// 2^kMatExp.  Entries are those of
+// when requested, FP32 copies (c-major and row-major) of every matrix scaled
+// by 2^kMatExp.  Entries are those of
 // get_from_parent_fam_size_to_c (matrix_cache.cpp:78-85): row 0 is delta_0,
 // rows s >= 1 come from the birth-death process with a = lt/(1+lt) and
 // coeff = 1 - 2a; keys with coeff <= 0 or coeff == 1 (saturated, lt == 1, or
@@ -28,7 +28,8 @@ __device__ __forceinline__ bool degenerate_key(double alpha, double coeff)
 // One wavefront per matrix.  Lane l owns rows s = l + 64 b.  Columns advance
 // one at a time; along s the recurrence y_s = a y_{s-1} + K_s is a first-order
 // linear scan, done across lanes with a Kogge-Stone affine scan and carried
-// between 64-row blocks.  Stores for a fixed column are contiguous in s.
+// between 64-row blocks.  Stores for a fixed column are contiguous in s; the
+// row-major FP32 copy is written every CW columns from registers.
 // ---------------------------------------------------------------------------
 // Lane moves of a double within the wave (DPP on both 32-bit halves; lanes
 // without a source, or outside the row mask, read 0).
@@ -65,21 +66,32 @@ __device__ __forceinline__ double int_pow(double base, int e)
     return r;
 }
 
+// Columns per row-major write group: a lane keeps CW consecutive columns of
+// each of its NB rows in registers (64 floats at most) and then writes
+// CW * 4 contiguous bytes per row.
+template <int NB> struct RowGroup { static constexpr int v = NB >= 32 ? 2 : NB >= 16 ? 4 : NB >= 2 ? 64 / NB : 32; };
+
+// Any of the three pools may be null: the FP64 pool (c-major), the FP32 copy
+// (c-major, entries times 2^kMatExp) and its row-major twin.
 template <int NB>
 __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restrict__ lam_q,
                                                           const double* __restrict__ t_q,
                                                           int first_slot, int N, int Np,
                                                           long long slot_elems,
                                                           double* __restrict__ mt64,
-                                                          float* __restrict__ lf32)
+                                                          float* __restrict__ lf32,
+                                                          float* __restrict__ mt32)
 {
+    constexpr int CW = RowGroup<NB>::v;
     const int key = blockIdx.x;
     const int lane = threadIdx.x;
     const double lt = lam_q[key] * t_q[key];
     const double alpha = lt / (1 + lt);
     const double coeff = 1 - 2 * alpha;
-    double* M64 = mt64 + (long long)(first_slot + key) * slot_elems;
-    float* L32 = lf32 ? lf32 + (long long)(first_slot + key) * slot_elems : nullptr;
+    const long long base = (long long)(first_slot + key) * slot_elems;
+    double* M64 = mt64 ? mt64 + base : nullptr;
+    float* L32 = lf32 ? lf32 + base : nullptr;
+    float* R32 = mt32 ? mt32 + base : nullptr;
 
     if (degenerate_key(alpha, coeff)) {
         for (int c = 0; c < N; ++c)
@@ -87,8 +99,9 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
                 const int s = lane + 64 * b;
                 if (s < N) {
                     const double v = (s == 0 && c == 0) ? 1.0 : 0.0;
-                    M64[(long long)c * Np + s] = v;
+                    if (M64) M64[(long long)c * Np + s] = v;
                     if (L32) L32[(long long)c * Np + s] = (float)ldexp(v, kMatExp);
+                    if (R32) R32[(long long)s * Np + c] = (float)ldexp(v, kMatExp);
                 }
             }
         return;
@@ -108,60 +121,75 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
 #pragma unroll
     for (int b = 0; b < NB; ++b) prev[b] = 0.0;
 
-    for (int c = 0; c < N; ++c) {
-        double K[NB];
-        // K_s = (1-2a) P_{s-1}(c-1) + a P_s(c-1); row 0 is delta_0 itself
+    const int n_groups = (N + CW - 1) / CW;
+    for (int g = 0; g < n_groups; ++g) {
+        float row[NB][CW];   // this group's columns of the lane's rows (FP32 copy)
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            double below = dpp_move<kDppWaveShr1, 0xF>(prev[b]);   // lane l-1, 0 at lane 0
-            const double from_prev_block = b > 0 ? read_lane(prev[b > 0 ? b - 1 : 0], 63) : 0.0;
-            if (lane == 0) below = from_prev_block;
-            K[b] = coeff * below + alpha * prev[b];
-        }
-        if (lane == 0) K[0] = (c == 0) ? 1.0 : 0.0;   // P_0(c) = delta_{c0}
+        for (int cc = 0; cc < CW; ++cc) {
+            const int c = g * CW + cc;
+            if (c >= N) {   // past the last column (wave-uniform): zero padding
+#pragma unroll
+                for (int b = 0; b < NB; ++b) row[b][cc] = 0.f;
+                continue;
+            }
+            double K[NB];
+            // K_s = (1-2a) P_{s-1}(c-1) + a P_s(c-1); row 0 is delta_0 itself
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                double below = dpp_move<kDppWaveShr1, 0xF>(prev[b]);   // lane l-1, 0 at lane 0
+                const double from_prev_block = b > 0 ? read_lane(prev[b > 0 ? b - 1 : 0], 63) : 0.0;
+                if (lane == 0) below = from_prev_block;
+                K[b] = coeff * below + alpha * prev[b];
+            }
+            if (lane == 0) K[0] = (c == 0) ? 1.0 : 0.0;   // P_0(c) = delta_{c0}
 
-        double carry = 0.0;
+            double carry = 0.0;
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            // y_s = K_s + a y_{s-1}: within rows of 16 lanes (shifts 1..8),
-            // then across rows (broadcast of lane 15 into rows 1 and 3, of
-            // lane 31 into rows 2 and 3)
-            double y = K[b];
-            y = fma(apow[0], dpp_move<kDppRowShr + 1, 0xF>(y), y);
-            y = fma(apow[1], dpp_move<kDppRowShr + 2, 0xF>(y), y);
-            y = fma(apow[2], dpp_move<kDppRowShr + 4, 0xF>(y), y);
-            y = fma(apow[3], dpp_move<kDppRowShr + 8, 0xF>(y), y);
-            y = fma(a16, dpp_move<kDppRowBcast15, 0xA>(y), y);
-            y = fma(a32, dpp_move<kDppRowBcast31, 0xC>(y), y);
-            if (b > 0) y += a_lane * carry;
-            carry = read_lane(y, 63);
-            prev[b] = y;
-            const int s = lane + 64 * b;
-            if (s < N) {
+            for (int b = 0; b < NB; ++b) {
+                // y_s = K_s + a y_{s-1}: within rows of 16 lanes (shifts 1..8),
+                // then across rows (broadcast of lane 15 into rows 1 and 3, of
+                // lane 31 into rows 2 and 3)
+                double y = K[b];
+                y = fma(apow[0], dpp_move<kDppRowShr + 1, 0xF>(y), y);
+                y = fma(apow[1], dpp_move<kDppRowShr + 2, 0xF>(y), y);
+                y = fma(apow[2], dpp_move<kDppRowShr + 4, 0xF>(y), y);
+                y = fma(apow[3], dpp_move<kDppRowShr + 8, 0xF>(y), y);
+                y = fma(a16, dpp_move<kDppRowBcast15, 0xA>(y), y);
+                y = fma(a32, dpp_move<kDppRowBcast31, 0xC>(y), y);
+                if (b > 0) y += a_lane * carry;
+                carry = read_lane(y, 63);
+                prev[b] = y;
+                const int s = lane + 64 * b;
                 const double v = y < 1.0 ? y : 1.0;   // [0,1] clamp, probability.cpp:147
-                M64[(long long)c * Np + s] = v;
-                if (L32) L32[(long long)c * Np + s] = (float)ldexp(v, kMatExp);
+                const float v32 = (float)ldexp(v, kMatExp);
+                row[b][cc] = s < N ? v32 : 0.f;
+                if (s < N) {
+                    if (M64) M64[(long long)c * Np + s] = v;
+                    if (L32) L32[(long long)c * Np + s] = v32;
+                }
+            }
+        }
+        if (R32) {
+            // row s, columns g CW .. g CW + CW - 1: CW * 4 contiguous bytes
+            // (aligned: Np and g CW are multiples of CW)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int s = lane + 64 * b;
+                if (s >= N) continue;
+                float* dst = R32 + (long long)s * Np + g * CW;
+                if constexpr (CW % 4 == 0) {
+                    typedef float f4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+                    for (int q = 0; q < CW / 4; ++q)
+                        *(f4*)(dst + 4 * q) = f4{row[b][4 * q], row[b][4 * q + 1], row[b][4 * q + 2], row[b][4 * q + 3]};
+                } else {
+                    typedef float f2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+                    for (int q = 0; q < CW / 2; ++q) *(f2*)(dst + 2 * q) = f2{row[b][2 * q], row[b][2 * q + 1]};
+                }
             }
         }
     }
-}
-
-// ---------------------------------------------------------------------------
-// Row-major FP32 pool from the c-major FP32 copy: 32 x 32 tiles through LDS
-// (reads contiguous along s, writes contiguous along c), padding included.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_transpose32(const float* __restrict__ lf32, float* __restrict__ mt32,
-                                                     int first_slot, int Np, long long slot_elems)
-{
-    __shared__ float tile[32][33];
-    const long long base = (long long)(first_slot + blockIdx.z) * slot_elems;
-    const int c0 = blockIdx.y * 32, s0 = blockIdx.x * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
-#pragma unroll
-    for (int k = 0; k < 32; k += 8) tile[ty + k][tx] = lf32[base + (long long)(c0 + ty + k) * Np + s0 + tx];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 32; k += 8) mt32[base + (long long)(s0 + ty + k) * Np + c0 + tx] = tile[tx][ty + k];
 }
 
 // ---------------------------------------------------------------------------
@@ -282,23 +310,8 @@ __global__ void k_fill_zero_u32(uint32_t* __restrict__ p, long long n)
 // Host-side launchers (called from api.cpp).
 namespace cafe {
 
-// Row-major FP32 copy of slots first_slot .. first_slot+n-1 (Np is a
-// multiple of 32).
-// Row-major FP32 copies of slots first_slot .. first_slot+n-1 from the c-major
-// pool (nothing when mt32 is null: the caller transposes later).
-hipError_t launch_transpose32(const float* lf32, float* mt32, int first_slot, int n, int Np,
-                              long long slot_elems, hipStream_t stream)
-{
-    if (!mt32 || !lf32) return hipGetLastError();
-    if (Np % 32 != 0) return hipErrorInvalidValue;
-    for (int k0 = 0; k0 < n; k0 += 65535) {
-        const int nk = n - k0 < 65535 ? n - k0 : 65535;
-        hipLaunchKernelGGL(k_transpose32, dim3(Np / 32, Np / 32, nk), dim3(256), 0, stream, lf32, mt32,
-                           first_slot + k0, Np, slot_elems);
-    }
-    return hipGetLastError();
-}
-
+// Matrices of keys 0 .. n-1 into slots first_slot ..: the closed form (algo
+// 0) writes every pool; the recurrence skips the pools passed as null.
 hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const double* d_lg,
                         int n, int first_slot, int N, int Np, long long slot_elems, double* mt64,
                         float* mt32, float* lf32, hipStream_t stream)
@@ -313,12 +326,13 @@ hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const 
                                d_lg, first_slot + k0, N, Np, slot_elems, mt64, mt32, lf32);
         }
     } else {
+        if (Np % 32 != 0) return hipErrorInvalidValue;   // row groups of up to 32 columns
         const int nb = (N + 63) / 64;
 #define CAFE_BUILD_NB(NB)                                                                                 \
     if (nb <= NB) {                                                                                       \
         hipLaunchKernelGGL(k_build_recurrence<NB>, dim3(n), dim3(64), 0, stream, d_lam, d_t, first_slot, N, \
-                           Np, slot_elems, mt64, lf32);                                                   \
-        return launch_transpose32(lf32, mt32, first_slot, n, Np, slot_elems, stream);                     \
+                           Np, slot_elems, mt64, lf32, mt32);                                             \
+        return hipGetLastError();                                                                         \
     }
         CAFE_BUILD_NB(1)
         CAFE_BUILD_NB(2)
@@ -333,17 +347,20 @@ hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const 
 }
 
 // Folds the error model into slots first_slot .. first_slot+n-1 of the FP64
-// and c-major FP32 pools (every column; padding stays zero).
+// and c-major FP32 pools (every column; padding stays zero); a null output
+// pool is skipped.
 hipError_t launch_fold(const double* mt64, const float* lf32, double* out64, float* out32, const double* err64,
                        const float* err32, int n_dev, int n_rows, int N, int Np, long long slot_elems,
                        int first_slot, int n, hipStream_t stream)
 {
     for (int k0 = 0; k0 < n; k0 += 65535) {
         const int nk = n - k0 < 65535 ? n - k0 : 65535;
-        hipLaunchKernelGGL(k_fold_err64, dim3(Np, nk), dim3(256), 0, stream, mt64, out64, err64, n_dev, n_rows, N,
-                           Np, slot_elems, first_slot + k0);
-        hipLaunchKernelGGL(k_fold_err32, dim3(Np, nk), dim3(256), 0, stream, lf32, out32, err32, n_dev, n_rows, N,
-                           Np, slot_elems, first_slot + k0);
+        if (out64)
+            hipLaunchKernelGGL(k_fold_err64, dim3(Np, nk), dim3(256), 0, stream, mt64, out64, err64, n_dev, n_rows,
+                               N, Np, slot_elems, first_slot + k0);
+        if (out32)
+            hipLaunchKernelGGL(k_fold_err32, dim3(Np, nk), dim3(256), 0, stream, lf32, out32, err32, n_dev, n_rows,
+                               N, Np, slot_elems, first_slot + k0);
     }
     return hipGetLastError();
 }

@@ -17,7 +17,9 @@
 // 2.3x), and a SIMD holds 64 families per wave.
 //
 // Partial vectors are in registers, indexed by the program's slot numbers
-// through wave-uniform branches (one copy of the code per slot).  This
+// through arithmetic blends with wave-uniform 0/1 weights (per-slot copies of
+// the code under wave-uniform branches run ~6% faster alone but hold ~10%
+// more registers, which costs more beside the other classes).  This
 // kernel's program lets a node's output take the slot of one of its children
 // (the children are all read before the output is written, by the same lane),
 // so a binary tree of 34 leaves needs 3 slots.
@@ -146,8 +148,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LaneWaves<LR
                 if (sd.ctl & kStepFirst) {
                     // a non-root product starts from the row mask of the bound
                     // (rows past B are 0 and stay 0); root rows are sizes 0..R
+                    if (root || bk.L >= LR) {
 #pragma unroll
-                    for (int i = 0; i < LR; ++i) acc[i] = (root || i < bk.L) ? 1.f : 0.f;
+                        for (int i = 0; i < LR; ++i) acc[i] = 1.f;
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < LR; ++i) acc[i] = i < bk.L ? 1.f : 0.f;
+                    }
                     e_sum = 0;
                     log2S = 0;
                     cert = 0.0;
