@@ -82,6 +82,19 @@ const clade* clade::find_descendant(const std::string& name) const
     return nullptr;
 }
 
+void clade::apply_prefix_order(const std::function<void(const clade*)>& f) const
+{
+    // depth first with an explicit stack; children pushed right to left so
+    // the leftmost is visited next
+    std::vector<const clade*> stack{this};
+    while (!stack.empty()) {
+        const clade* c = stack.back();
+        stack.pop_back();
+        for (auto it = c->_descendants.rbegin(); it != c->_descendants.rend(); ++it) stack.push_back(*it);
+        f(c);
+    }
+}
+
 void clade::name_interior()
 {
     for (clade* c : _descendants) c->name_interior();
@@ -279,6 +292,13 @@ void matrix::multiply(const std::vector<double>& v, int s_min, int s_max, int c_
     }
 }
 
+int matrix::select_random_y(int x, int max) const
+{
+    const double* row = _values.data() + (size_t)x * _size;
+    std::discrete_distribution<int> draw(row, row + max);
+    return draw(randomizer_engine);
+}
+
 matrix_cache::key_t matrix_cache::make_key(double lambda, double branch_length)
 {
     return key_t(long(branch_length * 1000), long(lambda * 1000000000));
@@ -398,6 +418,12 @@ model::model(lambda* p_lambda, const clade* p_tree, const std::vector<gene_famil
 {
 }
 
+void model::set_family_bounds(std::vector<int> bound_B, std::vector<int> bound_R)
+{
+    _bound_B = std::move(bound_B);
+    _bound_R = std::move(bound_R);
+}
+
 void model::upload()
 {
     // engine node order: reverse of a breadth-first walk that enqueues
@@ -426,8 +452,12 @@ void model::upload()
     for (int f = 0; f < F; ++f)
         for (size_t l = 0; l < leaves.size(); ++l)
             counts[(size_t)f * leaves.size() + l] = _p_gene_families->at(f).get_species_size(leaves[l]);
-    check(cafe_gpu_set_families(_gpu.get(), F, (int)leaves.size(), counts.data(), nullptr, nullptr,
-                                _max_family_size, _max_root_family_size),
+    if ((!_bound_B.empty() && (int)_bound_B.size() != F) || (!_bound_R.empty() && (int)_bound_R.size() != F))
+        throw std::runtime_error("per-family bounds do not match the family count");
+    check(cafe_gpu_set_families(_gpu.get(), F, (int)leaves.size(), counts.data(),
+                                _bound_B.empty() ? nullptr : _bound_B.data(),
+                                _bound_R.empty() ? nullptr : _bound_R.data(), _max_family_size,
+                                _max_root_family_size),
           "cafe_gpu_set_families");
     if (_p_error_model) {
         const int nd = (int)_p_error_model->n_deviations();
@@ -745,6 +775,170 @@ void get_gamma(std::vector<double>& v_freq, std::vector<double>& v_rate, double 
     for (int i = 0; i < K; i++) t += v_rate[i];
     for (int i = 0; i < K; i++) v_rate[i] *= factor / t;
     for (int i = 0; i < K; i++) v_freq[i] = 1.0 / K;
+}
+
+// ---------------------------------------------------------------------------
+// p-values
+// ---------------------------------------------------------------------------
+std::mt19937 randomizer_engine;
+
+void set_weighted_random_family_size(const clade* node, clademap<int>* sizemap, const lambda* p_lambda,
+                                     int max_family_size, const matrix_cache& cache)
+{
+    if (node->is_root()) return;
+    const int parent_size = (*sizemap)[node->get_parent()];
+    int c = 0;   // a lost family stays lost
+    if (parent_size > 0) {
+        const double lam = p_lambda->get_value_for_clade(node);
+        const double t = node->get_branch_length();
+        const matrix* m = cache.get_matrix(t, lam);
+        if (matrix_cache::is_saturated(t, lam)) {
+            // the reference draws a uniform size here and then overwrites it
+            // with the row draw below; the draw still advances the engine
+            std::uniform_int_distribution<int> uniform(0, max_family_size - 1);
+            (void)uniform(randomizer_engine);
+        }
+        c = m->select_random_y(parent_size, max_family_size);
+    }
+    (*sizemap)[node] = c;
+}
+
+clademap<int> create_family(pvalue_parameters p, int root_family_size)
+{
+    clademap<int> sizes;
+    sizes[p.p_tree] = root_family_size;
+    // no error model for simulated families (docs/architecture/0006)
+    p.p_tree->apply_prefix_order([&](const clade* c) {
+        set_weighted_random_family_size(c, &sizes, p.p_lambda, p.max_family_size, p.cache);
+    });
+    return sizes;
+}
+
+namespace {
+
+// Root vectors of families whose leaf sizes come from clademaps, each with
+// its own (B, R), on the PLAIN64 path (the reference's summation order):
+// out[f] holds bound_R[f] entries, root size j+1 at index j.
+std::vector<std::vector<double>> plain_root_vectors(const pvalue_parameters& p, const std::vector<gene_family>& fams,
+                                                    std::vector<int> bound_B, std::vector<int> bound_R)
+{
+    std::vector<std::vector<double>> out(fams.size());
+    if (fams.empty()) return out;
+    matrix_cache& calc = const_cast<matrix_cache&>(p.cache);
+    const int B = *std::max_element(bound_B.begin(), bound_B.end());
+    const int R = *std::max_element(bound_R.begin(), bound_R.end());
+    if (std::max(B, R) + 1 > calc.get_matrix_size())
+        throw std::runtime_error("family bounds exceed the matrix size of the cache");
+    std::vector<int> rows(bound_R);
+    base_model m(const_cast<lambda*>(p.p_lambda), p.p_tree, &fams, B, R, nullptr, calc.gpu());
+    m.set_family_bounds(std::move(bound_B), std::move(bound_R));
+    m.upload();
+    std::vector<int> slots = m.node_slots(calc, p.p_lambda, {1.0});
+    int n_buckets = 0;
+    check(cafe_gpu_bucket_count(calc.gpu().get(), &n_buckets), "cafe_gpu_bucket_count");
+    std::vector<double> prior((size_t)std::max(1, n_buckets) * R, 1.0);   // root vectors only
+    cafe_gpu_eval_desc d{CAFE_GPU_MODEL_BASE, CAFE_GPU_PATH_PLAIN64, 1, 1, slots.data(), nullptr, prior.data(), R};
+    std::vector<double> flat(std::accumulate(rows.begin(), rows.end(), (size_t)0));
+    check(cafe_gpu_root_vectors(calc.gpu().get(), &d, 0, 0, 0, (int)fams.size(), flat.data()),
+          "cafe_gpu_root_vectors");
+    size_t o = 0;
+    for (size_t f = 0; f < fams.size(); ++f) {
+        out[f].assign(flat.begin() + o, flat.begin() + o + rows[f]);
+        o += rows[f];
+    }
+    return out;
+}
+
+// compute_family_probabilities with a root size per family
+std::vector<double> family_probabilities(const pvalue_parameters& p, const std::vector<clademap<int>>& sizes,
+                                         const std::vector<int>& root_sizes)
+{
+    const size_t F = sizes.size();
+    std::vector<gene_family> fams(F);
+    std::vector<int> bound_B(F), bound_R(root_sizes);
+    for (size_t f = 0; f < F; ++f) {
+        if (sizes[f].empty()) throw std::runtime_error("empty family size map");
+        int mx = 0;
+        for (const auto& kv : sizes[f]) mx = std::max(mx, kv.second);
+        // probability.cpp:397-399: the interior vectors' bound
+        bound_B[f] = std::min(p.max_family_size, mx + std::max(50, mx / 5));
+        for (const auto& kv : sizes[f])   // gene_family::init_from_clademap
+            if (kv.first->is_leaf()) fams[f].set_species_size(kv.first->get_taxon_name(), kv.second);
+    }
+    auto roots = plain_root_vectors(p, fams, bound_B, bound_R);
+    std::vector<double> out(F);
+    for (size_t f = 0; f < F; ++f) {
+        // the root vector is (L(r), 0, ..., 0): its largest element
+        const double v = roots[f][root_sizes[f] - 1];
+        out[f] = v < 0.0 ? 0.0 : v;
+    }
+    return out;
+}
+
+}  // namespace
+
+std::vector<double> compute_family_probabilities(pvalue_parameters p, const std::vector<clademap<int>>& sizes,
+                                                 int root_family_size)
+{
+    if (root_family_size < 1) throw std::runtime_error("root family size must be positive");
+    return family_probabilities(p, sizes, std::vector<int>(sizes.size(), root_family_size));
+}
+
+std::vector<double> get_random_probabilities(pvalue_parameters p, int number_of_simulations, int root_family_size)
+{
+    std::vector<clademap<int>> fams;
+    for (int i = 0; i < number_of_simulations; ++i) fams.push_back(create_family(p, root_family_size));
+    auto out = compute_family_probabilities(p, fams, root_family_size);
+    std::sort(out.begin(), out.end());
+    return out;
+}
+
+double pvalue(double v, const std::vector<double>& conddist)
+{
+    const auto above = std::upper_bound(conddist.begin(), conddist.end(), v);
+    const long idx = above != conddist.end() ? long(above - conddist.begin()) : long(conddist.size()) - 1;
+    return idx / (double)conddist.size();
+}
+
+double find_best_pvalue(const gene_family& fam, const std::vector<double>& root_probabilities,
+                        const std::vector<std::vector<double>>& conditional_distribution)
+{
+    std::vector<double> pvalues(root_probabilities.size());
+    const int check_to = (int)std::rint(fam.get_max_size() * 1.25);
+    for (int j = 0; j < check_to && j < (int)pvalues.size() && j < (int)conditional_distribution.size(); ++j)
+        pvalues[j] = pvalue(root_probabilities[j], conditional_distribution[j]);
+    return pvalues.empty() ? 0.0 : *std::max_element(pvalues.begin(), pvalues.end());
+}
+
+std::vector<double> compute_pvalues(pvalue_parameters p, const std::vector<gene_family>& families,
+                                    int number_of_simulations)
+{
+    const int R = p.max_root_family_size;
+    // every root size's simulated families, drawn in the reference's order
+    // (root size 1 first), then all of them in one evaluation
+    std::vector<clademap<int>> sims;
+    std::vector<int> roots;
+    for (int i = 0; i < R; ++i)
+        for (int k = 0; k < number_of_simulations; ++k) {
+            sims.push_back(create_family(p, i + 1));
+            roots.push_back(i + 1);
+        }
+    std::vector<std::vector<double>> conditional(R);
+    if (!sims.empty()) {
+        auto probs = family_probabilities(p, sims, roots);
+        for (int i = 0; i < R; ++i) {
+            conditional[i].assign(probs.begin() + (size_t)i * number_of_simulations,
+                                  probs.begin() + (size_t)(i + 1) * number_of_simulations);
+            std::sort(conditional[i].begin(), conditional[i].end());
+        }
+    }
+    // the observed families' root vectors over sizes 1..R (probability.cpp:556-569)
+    std::vector<double> result(families.size());
+    if (families.empty()) return result;
+    auto obs = plain_root_vectors(p, families, std::vector<int>(families.size(), p.max_family_size),
+                                  std::vector<int>(families.size(), R));
+    for (size_t i = 0; i < families.size(); ++i) result[i] = find_best_pvalue(families[i], obs[i], conditional);
+    return result;
 }
 
 }  // namespace cafe5_gpu

@@ -21,9 +21,11 @@
 // saturated gamma families give +inf from infer_family_likelihoods.
 #pragma once
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <ostream>
+#include <random>
 #include <set>
 #include <stdexcept>
 #include <string>
@@ -82,6 +84,8 @@ public:
     // node name -> lambda index - 1 (clade.cpp:194-204)
     std::map<std::string, int> get_lambda_index_map() const;
     const clade* find_descendant(const std::string& name) const;
+    // node first, then its descendants left to right (clade.cpp:275-291)
+    void apply_prefix_order(const std::function<void(const clade*)>& f) const;
 
     friend clade* parse_newick(const std::string& text, bool parse_to_lambdas);
 
@@ -199,6 +203,9 @@ public:
     int size() const { return _size; }
     // matrix_cache.cpp:32-58
     void multiply(const std::vector<double>& v, int s_min, int s_max, int c_min, int c_max, double* result) const;
+    // a size y drawn from row x's entries 0..max-1 with randomizer_engine
+    // (matrix_cache.cpp:60-66)
+    int select_random_y(int x, int max) const;
     double* data() { return _values.data(); }
 
 private:
@@ -264,6 +271,9 @@ public:
     // engine plumbing: upload tree, families and error model; map every
     // branch of every (lambda x multiplier) point to its matrix slot
     void upload();
+    // per-family (max_family_size, max_root_family_size) for the engine's size
+    // buckets; empty: the model's bounds for every family
+    void set_family_bounds(std::vector<int> bound_B, std::vector<int> bound_R);
     std::vector<int> node_slots(matrix_cache& cache, const lambda* lam, const std::vector<double>& multipliers);
 
 protected:
@@ -277,6 +287,7 @@ protected:
     error_model* _p_error_model;
     gpu_context& _gpu;
     std::vector<const clade*> _nodes;     // engine node order (children before parents)
+    std::vector<int> _bound_B, _bound_R;  // per-family bounds (empty: the model's)
     std::vector<family_info_stash> results;
 };
 
@@ -325,5 +336,51 @@ void get_gamma(std::vector<double>& v_freq, std::vector<double>& v_rate, double 
 std::vector<double> inference_prune(const gene_family& gf, matrix_cache& calc, const lambda* p_lambda,
                                     const error_model* p_error_model, const clade* p_tree,
                                     double lambda_multiplier, int max_root_family_size, int max_family_size);
+
+// ---------------------------------------------------------------------------
+// p-values (probability.cpp:335-581)
+// ---------------------------------------------------------------------------
+template <typename T>
+using clademap = std::map<const clade*, T>;
+
+// The engine the reference's executables define (cafexp.cpp, test.cpp);
+// create_family draws from it.
+extern std::mt19937 randomizer_engine;
+
+struct pvalue_parameters {
+    const clade* p_tree;
+    const lambda* p_lambda;
+    const int max_family_size;
+    const int max_root_family_size;
+    const matrix_cache& cache;
+};
+
+// probability.cpp:481-507: a node's size drawn from its parent's matrix row
+// (nothing for the root; leaves keep the draw, no error model)
+void set_weighted_random_family_size(const clade* node, clademap<int>* sizemap, const lambda* p_lambda,
+                                     int max_family_size, const matrix_cache& cache);
+// probability.cpp:355-375: sizes drawn down the tree from root_family_size
+clademap<int> create_family(pvalue_parameters p, int root_family_size);
+// probability.cpp:379-428: per family, the likelihood of its leaf sizes given
+// root size root_family_size, interior vectors stopping at
+// min(max_family_size, max + max(50, max / 5)) for the largest size max in
+// its map.  All families run in one engine evaluation (PLAIN64 path, the
+// reference's summation order).
+std::vector<double> compute_family_probabilities(pvalue_parameters p, const std::vector<clademap<int>>& sizes,
+                                                 int root_family_size);
+// probability.cpp:435-446: number_of_simulations drawn families, sorted
+// family probabilities
+std::vector<double> get_random_probabilities(pvalue_parameters p, int number_of_simulations, int root_family_size);
+// probability.cpp:517-527
+double pvalue(double v, const std::vector<double>& conddist);
+// probability.cpp:529-541 (the loop also stops at the end of the root vector
+// and of the distributions, where the reference would index past them)
+double find_best_pvalue(const gene_family& fam, const std::vector<double>& root_probabilities,
+                        const std::vector<std::vector<double>>& conditional_distribution);
+// probability.cpp:544-581: the conditional distributions of every root size
+// (their simulated families in one engine evaluation, drawn in the
+// reference's order), then each family's best p-value
+std::vector<double> compute_pvalues(pvalue_parameters p, const std::vector<gene_family>& families,
+                                    int number_of_simulations);
 
 }  // namespace cafe5_gpu

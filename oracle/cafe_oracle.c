@@ -220,13 +220,18 @@ static int orc_postorder(const orc_tree *tree, int *order)
     return 0;
 }
 
-int orc_prune(const orc_tree *tree, const int *counts, const double *mat_pool,
-              const int *node_mat, int N, const double *err, int n_dev,
-              int B, int R, double *out_root)
+/* core.cpp:134-145 / compute_node_probability (probability.cpp:175-234) with
+ * internal vectors of B + 1 entries and the root's child factors taken over
+ * rows s_min..s_max (root_size_range); the root vector has root_len entries
+ * and entry i is the product of the children's row s_min + i (0 past
+ * s_max, as the reference's zero-filled result buffer gives). */
+static int orc_prune_rows(const orc_tree *tree, const int *counts, const double *mat_pool,
+                          const int *node_mat, int N, const double *err, int n_dev,
+                          int B, int s_min, int s_max, int root_len, double *out_root)
 {
     int n = tree->n_nodes;
     int root = orc_find_root(tree);
-    if (root < 0 || B + 1 > N || R + 1 > N)
+    if (root < 0 || B + 1 > N || s_max + 1 > N || s_min < 0 || s_max < s_min || root_len < 1)
         return -1;
     int *order = (int *)malloc(sizeof(int) * n);
     if (orc_postorder(tree, order) != 0) {
@@ -234,9 +239,12 @@ int orc_prune(const orc_tree *tree, const int *counts, const double *mat_pool,
         return -1;
     }
     int L = B + 1;
+    int rows = s_max - s_min + 1;
     /* core.cpp:137-139: root vector has R entries, others B+1, all zero. */
-    double *probs = (double *)calloc((size_t)n * (L > R ? L : R), sizeof(double));
-    int stride = L > R ? L : R;
+    int stride = L > root_len ? L : root_len;
+    if (rows > stride)
+        stride = rows;
+    double *probs = (double *)calloc((size_t)n * stride, sizeof(double));
     double *result = (double *)malloc(sizeof(double) * stride);
     int rc = 0;
     for (int oi = 0; oi < n && rc == 0; ++oi) {
@@ -247,7 +255,7 @@ int orc_prune(const orc_tree *tree, const int *counts, const double *mat_pool,
             continue;
         }
         int is_root = (u == root);
-        int len = is_root ? R : L;
+        int len = is_root ? root_len : L;
         for (int i = 0; i < len; ++i)
             v[i] = 1;
         for (int ch = 0; ch < n; ++ch) {
@@ -255,8 +263,10 @@ int orc_prune(const orc_tree *tree, const int *counts, const double *mat_pool,
                 continue;
             const double *M = mat_pool + (size_t)node_mat[ch] * N * N;
             const double *vc = probs + (size_t)ch * stride;
-            if (is_root)   /* probability.cpp:206-219: rows 1..R, cols 0..B */
-                orc_matvec(M, N, vc, 1, R, 0, B, result);
+            for (int i = 0; i < stride; ++i)
+                result[i] = 0;
+            if (is_root)   /* probability.cpp:206-219: rows s_min..s_max, cols 0..B */
+                orc_matvec(M, N, vc, s_min, s_max, 0, B, result);
             else           /* probability.cpp:220-233: rows 0..B, cols 0..B */
                 orc_matvec(M, N, vc, 0, B, 0, B, result);
             for (int i = 0; i < len; ++i)
@@ -264,11 +274,74 @@ int orc_prune(const orc_tree *tree, const int *counts, const double *mat_pool,
         }
     }
     if (rc == 0)
-        memcpy(out_root, probs + (size_t)root * stride, sizeof(double) * R);
+        memcpy(out_root, probs + (size_t)root * stride, sizeof(double) * root_len);
     free(result);
     free(probs);
     free(order);
     return rc;
+}
+
+int orc_prune(const orc_tree *tree, const int *counts, const double *mat_pool,
+              const int *node_mat, int N, const double *err, int n_dev,
+              int B, int R, double *out_root)
+{
+    /* root sizes 1..R (core.cpp:140-142) */
+    if (R + 1 > N)
+        return -1;
+    return orc_prune_rows(tree, counts, mat_pool, node_mat, N, err, n_dev, B, 1, R, R, out_root);
+}
+
+int orc_family_probability(const orc_tree *tree, const int *counts, const double *mat_pool,
+                           const int *node_mat, int N, int B, int max_size, int r, double *out)
+{
+    /* probability.cpp:462-466 and :468-474: m = min(B, max + max(50, max / 5)) */
+    int grow = max_size / 5 > 50 ? max_size / 5 : 50;
+    int m = B < max_size + grow ? B : max_size + grow;
+    if (r < 1 || m < 0)
+        return -1;
+    double root[2];
+    /* the root vector's first entry is row r; the others are products with
+     * the zero tail of the result buffer (probability.cpp:210-217) */
+    if (orc_prune_rows(tree, counts, mat_pool, node_mat, N, NULL, 0, m, r, r, 2, root) != 0)
+        return -1;
+    *out = root[0] < root[1] ? root[1] : root[0];   /* max_element, :474 */
+    return 0;
+}
+
+/* probability.cpp:517-527 */
+double orc_pvalue(double v, const double *conddist, int n)
+{
+    int idx = n - 1;
+    /* std::upper_bound: the first element greater than v */
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = lo + (hi - lo) / 2;
+        if (v < conddist[mid])
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    if (lo != n)
+        idx = lo;
+    return idx / (double)n;
+}
+
+/* probability.cpp:529-541 */
+double orc_find_best_pvalue(const double *root_probs, int n_root, int max_size,
+                            const double *conddist, int n_dist, int n_sim)
+{
+    if (n_root < 1)
+        return 0;
+    double *pv = (double *)calloc((size_t)n_root, sizeof(double));
+    int check = (int)rint(max_size * 1.25);
+    for (int j = 0; j < check && j < n_root && j < n_dist; ++j)
+        pv[j] = orc_pvalue(root_probs[j], conddist + (size_t)j * n_sim, n_sim);
+    double best = pv[0];
+    for (int j = 1; j < n_root; ++j)   /* std::max_element: the first largest */
+        if (best < pv[j])
+            best = pv[j];
+    free(pv);
+    return best;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -20,6 +20,8 @@
  *   src/gamma.cpp:15-241              PAML discrete gamma (AS91, AS32, AS70)
  *   src/root_equilibrium_distribution.cpp:15-87   root priors (float valued)
  *   src/poisson.cpp:21-24             poisspdf
+ *   src/probability.cpp:448-477       compute_family_probabilities
+ *   src/probability.cpp:517-541       pvalue, find_best_pvalue
  */
 #ifndef CAFE_ORACLE_H
 #define CAFE_ORACLE_H
@@ -79,6 +81,28 @@ int orc_leaf_vector(int count, const double *err, int n_dev, int B, double *out)
 int orc_prune(const orc_tree *tree, const int *counts, const double *mat_pool,
               const int *node_mat, int N, const double *err, int n_dev,
               int B, int R, double *out_root);
+
+/* compute_family_probabilities (probability.cpp:448-477) for one simulated
+ * family: max_size is the largest size of the family's clademap (internal
+ * nodes included); internal vectors hold m + 1 entries with
+ * m = min(B, max_size + max(50, max_size / 5)); the root takes row r only
+ * (root size range (r, r)), so the root vector is (L(r), 0, ..., 0) and the
+ * result is its maximum element.  node_mat / N as orc_prune (N > max(B, r)).
+ * Returns -1 (and writes nothing) for malformed input. */
+int orc_family_probability(const orc_tree *tree, const int *counts, const double *mat_pool,
+                           const int *node_mat, int N, int B, int max_size, int r, double *out);
+
+/* probability.cpp:517-527: the rank of the first conditional probability
+ * greater than v (n - 1 when there is none) over n. */
+double orc_pvalue(double v, const double *conddist, int n);
+
+/* probability.cpp:529-541: the largest p-value over root sizes j + 1,
+ * j < rint(max_size * 1.25), of root_probs[j] against the sorted row j of
+ * conddist (n_sim entries per row, n_dist rows).  p-values of the sizes not
+ * checked count as 0; the loop also stops at the end of root_probs and of
+ * conddist (the reference indexes past them there). */
+double orc_find_best_pvalue(const double *root_probs, int n_root, int max_size,
+                            const double *conddist, int n_dist, int n_sim);
 
 /* base_model.cpp:77-95 for F families whose root vectors are given
  * (roots: F x R).  prior[j] = double(root_equilibrium_distribution::compute(j)).

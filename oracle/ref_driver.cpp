@@ -6,8 +6,10 @@
 // (cafe_oracle.c) to the reference at ~1 ulp; nothing in the product loads it.
 //
 // Every function here only calls reference code: matrix_cache, inference_prune,
-// base_model / gamma_model::infer_family_likelihoods, get_gamma, and the
-// root_equilibrium_distribution constructors.
+// base_model / gamma_model::infer_family_likelihoods, get_gamma, the
+// root_equilibrium_distribution constructors, and the p-value functions
+// (create_family, compute_family_probabilities, pvalue, find_best_pvalue).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -311,4 +313,79 @@ REF_API int ref_set_threads(int n)
     (void)n;
     return 1;
 #endif
+}
+
+// compute_family_probabilities (probability.cpp:448-477) for F families whose
+// clademaps hold counts[f] at the leaves (leaf_names order) and inner[f] at
+// every interior node; out[F].
+REF_API int ref_family_probabilities(const char* newick, const char* leaf_names, int F, const int* counts,
+                                     const int* inner, double lambda, int B, int R, int root_size, double* out)
+{
+    try {
+        clade* tree = parse_newick(newick);
+        auto names = split_names(leaf_names);
+        single_lambda lam(lambda);
+        matrix_cache cache(std::max(R, B) + 1);
+        cache.precalculate_matrices({lambda}, tree->get_branch_lengths());
+        pvalue_parameters p = {tree, &lam, B, R, cache};
+        std::vector<clademap<int>> sizes(F);
+        for (int f = 0; f < F; ++f)
+            tree->apply_prefix_order([&](const clade* c) {
+                if (c->is_leaf()) {
+                    const auto it = std::find(names.begin(), names.end(), c->get_taxon_name());
+                    sizes[f][c] = counts[(size_t)f * names.size() + (it - names.begin())];
+                } else {
+                    sizes[f][c] = inner[f];
+                }
+            });
+        auto v = compute_family_probabilities(p, sizes, root_size);
+        std::copy(v.begin(), v.end(), out);
+        delete tree;
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+REF_API double ref_pvalue(double v, const double* conddist, int n)
+{
+    return pvalue(v, std::vector<double>(conddist, conddist + n));
+}
+
+// find_best_pvalue (probability.cpp:529-541) for a family whose largest count
+// is max_size; conddist holds n_dist sorted rows of n_sim values.  The caller
+// keeps rint(max_size * 1.25) within both n_root and n_dist.
+REF_API double ref_find_best_pvalue(int max_size, const double* root_probs, int n_root, const double* conddist,
+                                    int n_dist, int n_sim)
+{
+    gene_family fam;
+    fam.set_species_size("A", max_size);
+    std::vector<std::vector<double>> cd(n_dist);
+    for (int j = 0; j < n_dist; ++j) cd[j].assign(conddist + (size_t)j * n_sim, conddist + (size_t)(j + 1) * n_sim);
+    return find_best_pvalue(fam, std::vector<double>(root_probs, root_probs + n_root), cd);
+}
+
+// create_family (probability.cpp:355-375) after randomizer_engine.seed(seed):
+// out[i] = the size drawn for leaf i (leaf_names order), n_draws families in a row.
+REF_API int ref_create_families(const char* newick, const char* leaf_names, double lambda, int B, int R,
+                                int N, unsigned seed, int root_size, int n_draws, int* out)
+{
+    try {
+        clade* tree = parse_newick(newick);
+        auto names = split_names(leaf_names);
+        single_lambda lam(lambda);
+        matrix_cache cache(N);
+        cache.precalculate_matrices({lambda}, tree->get_branch_lengths());
+        pvalue_parameters p = {tree, &lam, B, R, cache};
+        randomizer_engine.seed(seed);
+        for (int d = 0; d < n_draws; ++d) {
+            auto sizes = create_family(p, root_size);
+            for (size_t i = 0; i < names.size(); ++i)
+                out[(size_t)d * names.size() + i] = sizes.at(tree->find_descendant(names[i]));
+        }
+        delete tree;
+        return 0;
+    } catch (...) {
+        return -1;
+    }
 }

@@ -5,6 +5,14 @@
 //                                "PASS name" / "FAIL name ..." lines
 //   adapter_test base FILE       evaluates the base or gamma model described
 //                                in FILE and prints per-family values
+//   adapter_test condprob FILE   compute_family_probabilities for the
+//                                families of FILE at root size `root`:
+//                                "PROB id value"
+//   adapter_test pvalues FILE    compute_pvalues for the families of FILE
+//                                with the engine seeded `seed`: the draws
+//                                ("SIM r max c1 c2 ..."), each root size's
+//                                sorted distribution ("COND r v ..."), then
+//                                "PVALUE id value"
 //
 // FILE (written by tests/test_gpu_cpp_adapter.py):
 //   newick <text>
@@ -16,10 +24,14 @@
 //   error eps | none
 //   species name1 name2 ...
 //   family id c1 c2 ...          (one line per family)
+//   root r / seed s / sims n     (condprob, pvalues)
+//   inner id v                   (condprob: interior sizes of family id)
+#include <algorithm>
 #include <cmath>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <map>
 #include <memory>
 #include <sstream>
 #include <string>
@@ -177,6 +189,93 @@ static int run_kats(gpu_context& gpu)
         report("error_model_probabilities", r.size() == 3 && r[0] == 0.2 && r[1] == 0.6 && threw);
     }
     // gamma.cpp (no reference test pins the values; printed for the oracle)
+    // test.cpp:1227-1243
+    {
+        std::vector<double> cd(10);
+        double n = 0;
+        for (auto& v : cd) v = (n += 0.01);
+        report("pvalues", pvalue(0.05, cd) == 0.5 && pvalue(0.0001, cd) == 0.0 && pvalue(0.099, cd) == 0.9);
+        const double v = pvalue(.35, {.1, .2, .3, .4, .5, .6, .7, .8, .9});
+        report("pvalues_2", close_to(v, 3.0 / 9.0, 1e-12));
+    }
+    // test.cpp:2276-2300 (the Inference fixture's family: A = 1, B = 2)
+    {
+        gene_family fam;
+        fam.set_species_size("A", 1);
+        fam.set_species_size("B", 2);
+        std::vector<std::vector<double>> cd(3, std::vector<double>{.1, .2, .3, .4});
+        std::vector<double> root(3);
+        root[0] = .15;
+        const double a = find_best_pvalue(fam, root, cd);
+        root[2] = .35;
+        const double b = find_best_pvalue(fam, root, cd);
+        report("find_best_pvalue", a == 0.25 && b == 0.25);
+    }
+    // test.cpp:636-654: the reference's own draw sequence (engine seeded 10)
+    // over matrices built in the reference's term order
+    {
+        const int algo = gpu.build_algo;
+        gpu.build_algo = 0;
+        std::unique_ptr<clade> t(parse_newick("((A:1,B:1):1,(C:1,D:1):1);"));
+        single_lambda lam(0.2);
+        matrix_cache cache(15, gpu);
+        cache.precalculate_matrices({0.2}, std::set<double>{1});
+        pvalue_parameters p = {t.get(), &lam, 10, 8, cache};
+        randomizer_engine.seed(10);
+        auto sizes = create_family(p, 6);
+        auto at = [&](const char* n) { return sizes.at(t->find_descendant(n)); };
+        report("generate_family", at("A") == 5 && at("B") == 5 && at("C") == 7 && at("D") == 4,
+               std::to_string(at("A")) + " " + std::to_string(at("B")) + " " + std::to_string(at("C")) + " " +
+                   std::to_string(at("D")));
+        gpu.build_algo = algo;
+    }
+    // test.cpp:1245-1266 (the Reconstruction fixture; interior sizes 5)
+    {
+        std::unique_ptr<clade> t(parse_newick("((A:1,B:3):7,(C:11,D:17):23);"));
+        single_lambda lam(0.03);
+        matrix_cache cache(100, gpu);
+        cache.precalculate_matrices({0.03}, t->get_branch_lengths());
+        pvalue_parameters p = {t.get(), &lam, 20, 15, cache};
+        const std::map<std::string, int> leaf{{"A", 11}, {"B", 2}, {"C", 5}, {"D", 6}};
+        clademap<int> cm;
+        t->apply_prefix_order([&](const clade* c) { cm[c] = c->is_leaf() ? leaf.at(c->get_taxon_name()) : 5; });
+        auto res = compute_family_probabilities(p, {cm}, 5);
+        // doctest::Approx(1e-10) at its default epsilon
+        const bool ok = res.size() == 1 && std::fabs(res[0] - 1e-10) < 1.1920928955078125e-05 * (1 + 1e-10);
+        std::ostringstream ost;
+        ost << std::setprecision(17) << (res.empty() ? -1.0 : res[0]);
+        report("compute_family_probabilities", ok, ost.str());
+    }
+    // test.cpp:619-630
+    {
+        std::unique_ptr<clade> t(parse_newick("((A:1,B:1):1,(C:1,D:1):1);"));
+        single_lambda lam(0.05);
+        matrix_cache cache(100, gpu);
+        cache.precalculate_matrices({0.05}, std::set<double>{1});
+        pvalue_parameters p = {t.get(), &lam, 12, 8, cache};
+        auto probs = get_random_probabilities(p, 10, 3);
+        // Approx(0.001905924).scale(10000) at doctest's default epsilon
+        const double tol = 1.1920928955078125e-05 * (10000 + 0.001905924);
+        report("get_random_probabilities",
+               probs.size() == 10 && std::is_sorted(probs.begin(), probs.end()) &&
+                   std::fabs(probs[0] - 0.001905924) < tol);
+    }
+    // test.cpp:2265-2275 (the Inference fixture: engine seeded 10)
+    {
+        std::unique_ptr<clade> t(parse_newick("(A:1,B:1);"));
+        single_lambda lam(0.05);
+        std::vector<gene_family> fams(1);
+        fams[0].set_id("TestFamily1");
+        fams[0].set_species_size("A", 1);
+        fams[0].set_species_size("B", 2);
+        randomizer_engine.seed(10);
+        matrix_cache cache(100, gpu);
+        cache.precalculate_matrices(get_lambda_values(&lam), t->get_branch_lengths());
+        pvalue_parameters p = {t.get(), &lam, 10, 8, cache};
+        auto values = compute_pvalues(p, fams, 3);
+        report("estimator_compute_pvalues", values.size() == 1 && values[0] == 0.0,
+               values.empty() ? "" : std::to_string(values[0]));
+    }
     {
         std::vector<double> f(4), r(4);
         get_gamma(f, r, 0.5);
@@ -263,6 +362,107 @@ static int run_model(gpu_context& gpu, const std::string& file)
     return 0;
 }
 
+// condprob / pvalues input: tree, one lambda, bounds, families and the
+// mode's parameters
+struct pvalue_input {
+    std::string newick;
+    double lambda = 0;
+    int B = 0, R = 0, root = 1, sims = 0, algo = 1;
+    unsigned seed = 0;
+    std::vector<std::string> species;
+    std::vector<gene_family> families;
+    std::map<std::string, int> inner;
+};
+
+static pvalue_input read_pvalue_input(const std::string& file)
+{
+    std::ifstream in(file);
+    if (!in) throw std::runtime_error("cannot open " + file);
+    pvalue_input d;
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream ist(line);
+        std::string tag;
+        ist >> tag;
+        if (tag == "newick") ist >> d.newick;
+        else if (tag == "lambdas") ist >> d.lambda;
+        else if (tag == "bounds") ist >> d.B >> d.R;
+        else if (tag == "root") ist >> d.root;
+        else if (tag == "seed") ist >> d.seed;
+        else if (tag == "sims") ist >> d.sims;
+        else if (tag == "build") ist >> d.algo;
+        else if (tag == "species") { std::string s; while (ist >> s) d.species.push_back(s); }
+        else if (tag == "inner") { std::string id; int v; ist >> id >> v; d.inner[id] = v; }
+        else if (tag == "family") {
+            gene_family f;
+            std::string id;
+            ist >> id;
+            f.set_id(id);
+            for (auto& s : d.species) { int c; ist >> c; f.set_species_size(s, c); }
+            d.families.push_back(f);
+        }
+    }
+    return d;
+}
+
+static int run_condprob(gpu_context& gpu, const std::string& file)
+{
+    pvalue_input d = read_pvalue_input(file);
+    gpu.build_algo = d.algo;
+    std::unique_ptr<clade> tree(parse_newick(d.newick));
+    single_lambda lam(d.lambda);
+    matrix_cache cache(std::max(d.B, d.R) + 1, gpu);
+    cache.precalculate_matrices({d.lambda}, tree->get_branch_lengths());
+    pvalue_parameters p = {tree.get(), &lam, d.B, d.R, cache};
+    std::vector<clademap<int>> sizes;
+    for (auto& f : d.families) {
+        clademap<int> cm;
+        tree->apply_prefix_order([&](const clade* c) {
+            cm[c] = c->is_leaf() ? f.get_species_size(c->get_taxon_name()) : d.inner.at(f.id());
+        });
+        sizes.push_back(cm);
+    }
+    auto probs = compute_family_probabilities(p, sizes, d.root);
+    std::cout << std::setprecision(17);
+    for (size_t i = 0; i < probs.size(); ++i) std::cout << "PROB " << d.families[i].id() << " " << probs[i] << "\n";
+    return 0;
+}
+
+static int run_pvalues(gpu_context& gpu, const std::string& file)
+{
+    pvalue_input d = read_pvalue_input(file);
+    gpu.build_algo = d.algo;
+    std::unique_ptr<clade> tree(parse_newick(d.newick));
+    single_lambda lam(d.lambda);
+    matrix_cache cache(std::max(d.B, d.R) + 1, gpu);
+    cache.precalculate_matrices({d.lambda}, tree->get_branch_lengths());
+    pvalue_parameters p = {tree.get(), &lam, d.B, d.R, cache};
+    std::cout << std::setprecision(17);
+    // the draws compute_pvalues makes (root sizes ascending), with their
+    // probabilities through compute_family_probabilities
+    randomizer_engine.seed(d.seed);
+    for (int r = 1; r <= d.R; ++r) {
+        std::vector<clademap<int>> sims;
+        for (int k = 0; k < d.sims; ++k) sims.push_back(create_family(p, r));
+        for (auto& cm : sims) {
+            int mx = 0;
+            for (auto& kv : cm) mx = std::max(mx, kv.second);
+            std::cout << "SIM " << r << " " << mx;
+            for (auto& s : d.species) std::cout << " " << cm.at(tree->find_descendant(s));
+            std::cout << "\n";
+        }
+        auto probs = compute_family_probabilities(p, sims, r);
+        std::sort(probs.begin(), probs.end());
+        std::cout << "COND " << r;
+        for (double v : probs) std::cout << " " << v;
+        std::cout << "\n";
+    }
+    randomizer_engine.seed(d.seed);
+    auto pv = compute_pvalues(p, d.families, d.sims);
+    for (size_t i = 0; i < pv.size(); ++i) std::cout << "PVALUE " << d.families[i].id() << " " << pv[i] << "\n";
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     try {
@@ -270,7 +470,9 @@ int main(int argc, char** argv)
         const std::string mode = argc > 1 ? argv[1] : "kats";
         if (mode == "kats") return run_kats(gpu);
         if (mode == "base" && argc > 2) return run_model(gpu, argv[2]);
-        std::cerr << "usage: adapter_test kats | base FILE" << std::endl;
+        if (mode == "condprob" && argc > 2) return run_condprob(gpu, argv[2]);
+        if (mode == "pvalues" && argc > 2) return run_pvalues(gpu, argv[2]);
+        std::cerr << "usage: adapter_test kats | base FILE | condprob FILE | pvalues FILE" << std::endl;
         return 2;
     } catch (std::exception& e) {
         std::cerr << "error: " << e.what() << std::endl;

@@ -70,6 +70,9 @@ def _load_oracle() -> ctypes.CDLL:
         "orc_prior_uniform": (None, [_i, _i, _pd]),
         "orc_prior_poisson": (None, [_d, _i, _i, _pd]),
         "orc_prior_map": (None, [_i, _pi, _pi, _i, _pd]),
+        "orc_family_probability": (_i, [ctypes.POINTER(OrcTree), _pi, _pd, _pi, _i, _i, _i, _i, _pd]),
+        "orc_pvalue": (_d, [_d, _pd, _i]),
+        "orc_find_best_pvalue": (_d, [_pd, _i, _i, _pd, _i, _i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -95,6 +98,10 @@ def _load_ref() -> Optional[ctypes.CDLL]:
         "ref_gamma_model": (_d, [_cp, _cp, _i, _pi, _i, _pd, _cp, _i, _i, _i, _pd, _pd, _i, _i, _d,
                                  _i, _i, _pd, _pd, _pd, _pd]),
         "ref_get_gamma": (None, [_i, _d, _pd, _pd]),
+        "ref_family_probabilities": (_i, [_cp, _cp, _i, _pi, _pi, _d, _i, _i, _i, _pd]),
+        "ref_pvalue": (_d, [_d, _pd, _i]),
+        "ref_find_best_pvalue": (_d, [_i, _pd, _i, _pd, _i, _i]),
+        "ref_create_families": (_i, [_cp, _cp, _d, _i, _i, _i, ctypes.c_uint, _i, _i, _pi]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -213,6 +220,32 @@ class Oracle:
         ORC.orc_get_gamma(K, alpha, _dp(f), _dp(r))
         return f, r
 
+    @staticmethod
+    def family_probability(tree: Tree, counts_row, lam: float, B: int, max_size: int, r: int,
+                           N: Optional[int] = None) -> float:
+        """compute_family_probabilities for one family (single lambda)."""
+        N = N or max(B, r) + 1
+        pool, node_mat = Oracle.node_matrices(tree, [lam] * tree.n_nodes, N)
+        st, keep = Oracle.tree_struct(tree)
+        cnt = np.ascontiguousarray(counts_row, dtype=np.int32)
+        out = np.zeros(1)
+        rc = ORC.orc_family_probability(ctypes.byref(st), _ip(cnt), _dp(pool), _ip(node_mat), N, B,
+                                        int(max_size), int(r), _dp(out))
+        if rc != 0:
+            raise ValueError("orc_family_probability rejected its input")
+        return float(out[0])
+
+    @staticmethod
+    def pvalue(v: float, conddist) -> float:
+        cd = np.ascontiguousarray(conddist, dtype=np.float64)
+        return ORC.orc_pvalue(float(v), _dp(cd), len(cd))
+
+    @staticmethod
+    def find_best_pvalue(root_probs, max_size: int, conddist: np.ndarray) -> float:
+        rp = np.ascontiguousarray(root_probs, dtype=np.float64)
+        cd = np.ascontiguousarray(conddist, dtype=np.float64)
+        return ORC.orc_find_best_pvalue(_dp(rp), len(rp), int(max_size), _dp(cd), cd.shape[0], cd.shape[1])
+
 
 class Ref:
     """Numpy-level helpers over the compiled reference (libcaferef.so)."""
@@ -305,3 +338,41 @@ class Ref:
         r = np.zeros(K)
         REF.ref_get_gamma(K, alpha, _dp(f), _dp(r))
         return f, r
+
+    @staticmethod
+    def family_probabilities(tree: Tree, counts: np.ndarray, inner, lam: float, B: int, R: int,
+                             root_size: int) -> np.ndarray:
+        """compute_family_probabilities over families whose interior nodes
+        all hold inner[f] in the clademap."""
+        F = counts.shape[0]
+        cnt = np.ascontiguousarray(counts, dtype=np.int32)
+        inn = np.ascontiguousarray(inner, dtype=np.int32)
+        out = np.zeros(F)
+        rc = REF.ref_family_probabilities(tree.newick().encode(), " ".join(tree.leaf_names()).encode(), F,
+                                          _ip(cnt), _ip(inn), lam, B, R, root_size, _dp(out))
+        if rc != 0:
+            raise RuntimeError("ref_family_probabilities failed")
+        return out
+
+    @staticmethod
+    def pvalue(v: float, conddist) -> float:
+        cd = np.ascontiguousarray(conddist, dtype=np.float64)
+        return REF.ref_pvalue(float(v), _dp(cd), len(cd))
+
+    @staticmethod
+    def find_best_pvalue(root_probs, max_size: int, conddist: np.ndarray) -> float:
+        rp = np.ascontiguousarray(root_probs, dtype=np.float64)
+        cd = np.ascontiguousarray(conddist, dtype=np.float64)
+        return REF.ref_find_best_pvalue(int(max_size), _dp(rp), len(rp), _dp(cd), cd.shape[0], cd.shape[1])
+
+    @staticmethod
+    def create_families(tree: Tree, lam: float, B: int, R: int, N: int, seed: int, root_size: int,
+                        n_draws: int) -> np.ndarray:
+        """create_family n_draws times after seeding the reference's engine:
+        leaf sizes [n_draws, n_leaves] in leaf_names order."""
+        out = np.zeros((n_draws, tree.n_leaves), dtype=np.int32)
+        rc = REF.ref_create_families(tree.newick().encode(), " ".join(tree.leaf_names()).encode(), lam, B, R,
+                                     N, seed, root_size, n_draws, _ip(out))
+        if rc != 0:
+            raise RuntimeError("ref_create_families failed")
+        return out
