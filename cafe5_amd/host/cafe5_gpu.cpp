@@ -210,6 +210,11 @@ double multiple_lambda::get_value_for_clade(const clade* c) const
     return _lambdas.at(_index.at(c->get_taxon_name()));
 }
 
+int multiple_lambda::index_of_clade(const clade* c) const
+{
+    return _index.at(c->get_taxon_name());
+}
+
 bool multiple_lambda::is_valid() const
 {
     return std::none_of(_lambdas.begin(), _lambdas.end(), [](double d) { return d < 0; });
@@ -317,6 +322,7 @@ void matrix_cache::precalculate_matrices(const std::vector<double>& lambdas, con
             if (!_slots.count(k) && std::find(fresh.begin(), fresh.end(), k) == fresh.end()) fresh.push_back(k);
         }
     if (fresh.empty() && _capacity > 0) return;
+    _gpu.owner = this;
     std::vector<key_t> build;
     const int need = (int)(_slots.size() + fresh.size());
     int first = (int)_slots.size();
@@ -446,6 +452,7 @@ void model::upload()
             leaves.push_back(_nodes[i]->get_taxon_name());
         }
     }
+    _gpu.owner = this;
     check(cafe_gpu_set_tree(_gpu.get(), (int)_nodes.size(), parent.data(), leaf_col.data()), "cafe_gpu_set_tree");
     const int F = (int)_p_gene_families->size();
     std::vector<int> counts((size_t)F * leaves.size());
@@ -939,6 +946,272 @@ std::vector<double> compute_pvalues(pvalue_parameters p, const std::vector<gene_
                                   std::vector<int>(families.size(), R));
     for (size_t i = 0; i < families.size(); ++i) result[i] = find_best_pvalue(families[i], obs[i], conditional);
     return result;
+}
+
+// ---------------------------------------------------------------------------
+// device-resident base model
+// ---------------------------------------------------------------------------
+resident_base_model::~resident_base_model()
+{
+    if (_plan) (void)cafe_gpu_plan_destroy(_plan);
+}
+
+double resident_base_model::infer_family_likelihoods(const root_equilibrium_distribution& prior, const lambda*)
+{
+    if (!_p_lambda->is_valid()) return -std::log(0.0);   // base_model.cpp:56-60
+    const std::vector<double> lams = get_lambda_values(_p_lambda);
+    const std::vector<double> pr = prior_row(prior);
+    // keys: lambda i x branch length j at i * |t| + j, quantized as
+    // matrix_cache_key does (matrix_cache.h:44-63)
+    auto keys = [&](std::vector<double>& lq, std::vector<double>& tq) {
+        lq.clear();
+        tq.clear();
+        for (double l : lams)
+            for (double t : _t) {
+                lq.push_back(double(long(l * 1000000000)) / 1000000000.0);
+                tq.push_back(double(long(t * 1000)) / 1000.0);
+            }
+    };
+    std::vector<double> lq, tq;
+    if (!_plan || _gpu.owner != this || pr != _plan_prior || (int)lams.size() != _n_lambda) {
+        if (_plan) {
+            (void)cafe_gpu_plan_destroy(_plan);
+            _plan = nullptr;
+        }
+        upload();
+        const std::set<double> ts = _p_tree->get_branch_lengths();
+        _t.assign(ts.begin(), ts.end());
+        _n_lambda = (int)lams.size();
+        const int n_keys = std::max(1, _n_lambda * (int)_t.size());
+        check(cafe_gpu_matrix_reset(_gpu.get(), std::max(_max_root_family_size, _max_family_size) + 1, n_keys),
+              "cafe_gpu_matrix_reset");
+        std::vector<int> slots;
+        for (const clade* c : _nodes) {
+            if (c->is_root()) {
+                slots.push_back(0);
+                continue;
+            }
+            const int j = (int)(std::lower_bound(_t.begin(), _t.end(), c->get_branch_length()) - _t.begin());
+            slots.push_back(_p_lambda->index_of_clade(c) * (int)_t.size() + j);
+        }
+        _plan_prior = pr;
+        cafe_gpu_eval_desc d{CAFE_GPU_MODEL_BASE, _gpu.path, 1, 1, slots.data(), nullptr, _plan_prior.data(),
+                             _max_root_family_size};
+        keys(lq, tq);
+        if (lq.empty()) {   // a tree without positive branch lengths
+            lq.push_back(0.0);
+            tq.push_back(0.0);
+        }
+        check(cafe_gpu_plan_create(_gpu.get(), &d, n_keys, lq.data(), tq.data(), _gpu.build_algo, 1, &_plan),
+              "cafe_gpu_plan_create");
+    } else {
+        keys(lq, tq);
+        if (!lq.empty()) check(cafe_gpu_plan_set_keys(_plan, lq.data(), tq.data()), "cafe_gpu_plan_set_keys");
+    }
+    check(cafe_gpu_plan_execute(_plan), "cafe_gpu_plan_execute");
+    const size_t F = _p_gene_families->size();
+    std::vector<double> lnl(F);
+    double total = 0;
+    check(cafe_gpu_plan_results(_plan, lnl.data(), nullptr, nullptr, &total, nullptr), "cafe_gpu_plan_results");
+    results.resize(F);
+    for (size_t i = 0; i < F; ++i)
+        results[i] = family_info_stash(_p_gene_families->at(i).id(), 0.0, 0.0, 0.0, lnl[i], false);
+    return total;
+}
+
+// ---------------------------------------------------------------------------
+// optimizer
+// ---------------------------------------------------------------------------
+nelder_mead::nelder_mead(optimizer_scorer* s, int count)
+    : scorer(s), n(count), pool(count + 1, candidate(count)), x_mean(count), x_r(count), x_tmp(count)
+{
+    for (auto& c : pool) candidates.push_back(&c);
+}
+
+void nelder_mead::sort()
+{
+    // n + 1 <= 16 candidates: libstdc++ sorts them by insertion, so equal
+    // scores keep their order
+    std::sort(candidates.begin(), candidates.end(), [](const candidate* a, const candidate* b) {
+        return a->score < b->score;
+    });
+}
+
+bool nelder_mead::check_values() const
+{
+    double big = -std::numeric_limits<double>::max();
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            big = std::max(big, std::fabs(candidates[i + 1]->values[j] - candidates[i]->values[j]));
+    return big <= tolx;
+}
+
+bool nelder_mead::check_scores() const
+{
+    double big = -std::numeric_limits<double>::max();
+    for (int i = 1; i <= n; ++i) big = std::max(big, std::fabs(candidates[i]->score - candidates[0]->score));
+    return big <= tolf;
+}
+
+void nelder_mead::init(const double* x0)
+{
+    for (int i = 0; i <= n; ++i) {
+        // after a point that scored +inf the next step is 100x wider
+        const double step = (i > 1 && std::isinf(candidates[i - 1]->score)) ? 1 + delta * 100 : 1 + delta;
+        for (int j = 0; j < n; ++j)
+            candidates[i]->values[j] = (i - 1 == j) ? (x0[j] ? step * x0[j] : zero_delta) : x0[j];
+        candidates[i]->score = scorer->calculate_score(candidates[i]->values.data());
+    }
+    sort();
+}
+
+void nelder_mead::mean()
+{
+    for (int i = 0; i < n; ++i) {
+        double s = 0;
+        for (int j = 0; j < n; ++j) s += candidates[j]->values[i];
+        x_mean[i] = s / n;
+    }
+}
+
+double nelder_mead::reflection()
+{
+    for (int i = 0; i < n; ++i) x_r[i] = x_mean[i] + rho * (x_mean[i] - candidates[n]->values[i]);
+    return scorer->calculate_score(x_r.data());
+}
+
+double nelder_mead::expansion()
+{
+    for (int i = 0; i < n; ++i) x_tmp[i] = x_mean[i] + chi * (x_r[i] - x_mean[i]);
+    return scorer->calculate_score(x_tmp.data());
+}
+
+double nelder_mead::contract_outside()
+{
+    for (int i = 0; i < n; ++i) x_tmp[i] = x_mean[i] + psi * (x_r[i] - x_mean[i]);
+    return scorer->calculate_score(x_tmp.data());
+}
+
+double nelder_mead::contract_inside()
+{
+    for (int i = 0; i < n; ++i) x_tmp[i] = x_mean[i] + psi * (x_mean[i] - candidates[n]->values[i]);
+    return scorer->calculate_score(x_tmp.data());
+}
+
+void nelder_mead::shrink()
+{
+    for (int i = 1; i <= n; ++i) {
+        for (int j = 0; j < n; ++j)
+            candidates[i]->values[j] =
+                candidates[0]->values[j] + sigma * (candidates[i]->values[j] - candidates[0]->values[j]);
+        candidates[i]->score = scorer->calculate_score(candidates[i]->values.data());
+    }
+    sort();
+}
+
+void nelder_mead::set_last(const double* x, double f)
+{
+    std::copy(x, x + n, candidates.back()->values.begin());
+    candidates.back()->score = f;
+    sort();
+}
+
+bool nelder_mead::minimize(const double* x0)
+{
+    init(x0);
+    int i = 0;
+    for (; i < maxiters && !threshold_achieved(); ++i) {
+        mean();
+        const double fr = reflection();
+        const double worst = candidates[n]->score;
+        if (fr < candidates[0]->score) {
+            const double fe = expansion();
+            if (fe < fr) set_last(x_tmp.data(), fe);
+            else set_last(x_r.data(), fr);
+        } else if (fr >= worst) {
+            if (fr > worst) {
+                const double fc = contract_inside();
+                if (fc < worst) set_last(x_tmp.data(), fc);
+                else shrink();
+            } else {
+                const double fc = contract_outside();
+                if (fc <= fr) set_last(x_tmp.data(), fc);
+                else shrink();
+            }
+        } else {
+            set_last(x_r.data(), fr);
+        }
+    }
+    iters = i;
+    return i == maxiters;
+}
+
+std::vector<double> optimizer::get_initial_guesses()
+{
+    // NUM_OPTIMIZER_INITIALIZATION_ATTEMPTS = 1 (CMakeLists.txt:13)
+    auto initial = _p_scorer->initial_guesses();
+    double first = _p_scorer->calculate_score(initial.data());
+    for (int i = 0; std::isinf(first) && i < 1; ++i) {
+        initial = _p_scorer->initial_guesses();
+        first = _p_scorer->calculate_score(initial.data());
+    }
+    if (std::isinf(first)) throw std::runtime_error("Failed to initialize any reasonable values");
+    return initial;
+}
+
+optimizer::result optimizer::optimize(const optimizer_parameters& params)
+{
+    const auto before = std::chrono::system_clock::now();
+    result r;
+    auto initial = get_initial_guesses();
+    nelder_mead nm(_p_scorer, (int)initial.size());
+    nm.chi = params.neldermead_expansion;
+    nm.rho = params.neldermead_reflection;
+    nm.maxiters = params.neldermead_iterations;
+    nm.tolx = nm.tolf = 1e-6;
+    nm.minimize(initial.data());
+    r.score = nm.candidates[0]->score;
+    r.values = nm.candidates[0]->values;
+    r.num_iterations = nm.iters;
+    r.duration = std::chrono::duration_cast<std::chrono::seconds>(std::chrono::system_clock::now() - before);
+    return r;
+}
+
+std::ostream& operator<<(std::ostream& ost, const optimizer::result& r)
+{
+    if (r.score == -std::log(0.0)) {
+        ost << "\nFailed to find reasonable starting values to initialize search." << std::endl;
+        return ost;
+    }
+    const long s = (long)r.duration.count();
+    ost << "\nCompleted " << r.num_iterations << " iterations" << std::endl;
+    ost << "Time: " << s / 3600 << "H " << (s / 60) % 60 << "M " << s % 60 << "S" << std::endl;
+    ost << "Best match" << (r.values.size() == 1 ? " is: " : "es are: ") << std::setw(15) << std::setprecision(14);
+    for (size_t i = 0; i + 1 < r.values.size(); ++i) ost << r.values[i] << ',';
+    if (!r.values.empty()) ost << r.values.back() << std::endl;
+    ost << "Final -lnL: " << r.score << std::endl;
+    return ost;
+}
+
+std::vector<double> lambda_optimizer::initial_guesses()
+{
+    // a normal draw centred so that lambda starts near 0.002, times
+    // 1 / longest branch, redrawn while negative
+    const double distmean = 0.002 / (1.0 / _longest_branch);
+    std::normal_distribution<double> draw(distmean, 0.2);
+    std::vector<double> out(_p_lambda->count());
+    for (auto& v : out) {
+        v = 1.0 / _longest_branch * draw(randomizer_engine);
+        while (v < 0) v = 1.0 / _longest_branch * draw(randomizer_engine);
+    }
+    return out;
+}
+
+double lambda_optimizer::calculate_score(const double* values)
+{
+    _p_lambda->update(values);
+    const double score = _p_model->infer_family_likelihoods(*_p_distribution, _p_lambda);
+    return std::isnan(score) ? -std::log(0.0) : score;
 }
 
 }  // namespace cafe5_gpu

@@ -21,6 +21,7 @@
 // saturated gamma families give +inf from infer_family_likelihoods.
 #pragma once
 
+#include <chrono>
 #include <functional>
 #include <map>
 #include <memory>
@@ -32,6 +33,7 @@
 #include <vector>
 
 struct cafe_gpu_ctx;
+struct cafe_gpu_plan;
 
 namespace cafe5_gpu {
 
@@ -49,6 +51,10 @@ public:
     int path = 2;
     // matrix build: 0 closed form (reference term order), 1 recurrence
     int build_algo = 1;
+    // the object that last changed the context's tree, families or matrix
+    // pool through this interface (a resident model rebuilds its plan when
+    // it is not itself)
+    const void* owner = nullptr;
 
 private:
     cafe_gpu_ctx* _ctx = nullptr;
@@ -128,6 +134,11 @@ public:
     virtual bool is_valid() const = 0;
     virtual std::vector<double> values() const = 0;
     virtual lambda* clone() const = 0;
+    // lambda.h: the optimizer's parameters in and out
+    virtual void update(const double* values) = 0;
+    virtual int count() const = 0;
+    // which of the values a clade's branch takes
+    virtual int index_of_clade(const clade* c) const = 0;
 };
 
 class single_lambda : public lambda {
@@ -140,6 +151,9 @@ public:
     std::vector<double> values() const override { return {_lambda}; }
     lambda* clone() const override { return new single_lambda(_lambda); }
     void update(double v) { _lambda = v; }
+    void update(const double* values) override { _lambda = values[0]; }
+    int count() const override { return 1; }
+    int index_of_clade(const clade*) const override { return 0; }
 
 private:
     double _lambda;
@@ -154,6 +168,9 @@ public:
     bool is_valid() const override;                               // none negative (lambda.cpp:58-61)
     std::vector<double> values() const override { return _lambdas; }
     lambda* clone() const override { return new multiple_lambda(_index, _lambdas); }
+    void update(const double* values) override { std::copy(values, values + _lambdas.size(), _lambdas.begin()); }
+    int count() const override { return (int)_lambdas.size(); }
+    int index_of_clade(const clade* c) const override;
 
 private:
     std::map<std::string, int> _index;
@@ -328,6 +345,26 @@ private:
     double _alpha = 0.0;
 };
 
+// A base model whose evaluations stay on the device (no reference counterpart:
+// the reference rebuilds its matrix cache and re-prunes every family on each
+// call).  The first call uploads tree and families and captures one plan
+// (matrix build for every lambda x branch-length key, pruning, reduction);
+// later calls with new lambda values only set the plan's keys and replay it.
+// The plan is rebuilt when the prior, the lambda count or the context's owner
+// changed in between.
+class resident_base_model : public base_model {
+public:
+    using base_model::base_model;
+    ~resident_base_model() override;
+    double infer_family_likelihoods(const root_equilibrium_distribution& prior, const lambda* p_lambda) override;
+
+private:
+    ::cafe_gpu_plan* _plan = nullptr;
+    std::vector<double> _plan_prior;
+    std::vector<double> _t;   // the tree's branch lengths, key order
+    int _n_lambda = 0;
+};
+
 // PAML discrete gamma rates (gamma.cpp:15-52, 225-241)
 void get_gamma(std::vector<double>& v_freq, std::vector<double>& v_rate, double alpha);
 
@@ -382,5 +419,98 @@ double find_best_pvalue(const gene_family& fam, const std::vector<double>& root_
 // reference's order), then each family's best p-value
 std::vector<double> compute_pvalues(pvalue_parameters p, const std::vector<gene_family>& families,
                                     int number_of_simulations);
+
+// ---------------------------------------------------------------------------
+// optimizer (optimizer.h, optimizer.cpp, optimizer_scorer.h/.cpp)
+// ---------------------------------------------------------------------------
+class optimizer_scorer {
+public:
+    virtual ~optimizer_scorer() {}
+    virtual std::vector<double> initial_guesses() = 0;
+    virtual double calculate_score(const double* values) = 0;
+    virtual void finalize(double* results) { (void)results; }
+};
+
+// optimizer.h:26-32 (the Standard strategy, the default build's)
+struct optimizer_parameters {
+    double neldermead_expansion = 2.0;
+    double neldermead_reflection = 1.0;
+    int neldermead_iterations = 300;
+};
+
+struct candidate {
+    explicit candidate(int size) : values(size), score(0) {}
+    std::vector<double> values;
+    double score;
+};
+
+// FMinSearch and its steps (optimizer.cpp:61-322): the simplex of n + 1
+// candidates sorted by score, reflection 1, expansion 2, contraction and
+// shrink 0.5, tolerances 1e-6, initial steps of 5% (0.00025 from zero)
+struct nelder_mead {
+    nelder_mead(optimizer_scorer* scorer, int n);
+    optimizer_scorer* scorer;
+    int n;
+    int maxiters = 250, iters = 0;
+    double rho = 1, chi = 2, psi = 0.5, sigma = 0.5;
+    double tolx = 1e-6, tolf = 1e-6;
+    double delta = 0.05, zero_delta = 0.00025;
+    std::vector<candidate> pool;
+    std::vector<candidate*> candidates;   // sorted best first
+    std::vector<double> x_mean, x_r, x_tmp;
+
+    void sort();
+    bool check_values() const;   // largest coordinate step between neighbours <= tolx
+    bool check_scores() const;   // largest score gap to the best <= tolf
+    bool threshold_achieved() const { return check_values() && check_scores(); }
+    void init(const double* x0);
+    void mean();
+    double reflection();
+    double expansion();
+    double contract_outside();
+    double contract_inside();
+    void shrink();
+    void set_last(const double* x, double f);
+    // the search from x0; true when it stopped at the iteration cap
+    bool minimize(const double* x0);
+};
+
+class optimizer {
+public:
+    struct result {
+        std::vector<double> values;
+        double score = 0;
+        int num_iterations = 0;
+        std::chrono::seconds duration{0};
+    };
+    explicit optimizer(optimizer_scorer* p_scorer) : _p_scorer(p_scorer) {}
+    // optimizer.cpp:348-363: throws when the guesses score +inf
+    std::vector<double> get_initial_guesses();
+    // optimizer.cpp:546-573 with the Standard strategy (:365-388)
+    result optimize(const optimizer_parameters& params);
+
+private:
+    optimizer_scorer* _p_scorer;
+};
+std::ostream& operator<<(std::ostream& ost, const optimizer::result& r);   // optimizer.cpp:575-601
+
+// inference_optimizer_scorer + lambda_optimizer (optimizer_scorer.cpp:22-70):
+// -lnL of the model at the given lambda values, NaN read as +inf
+class lambda_optimizer : public optimizer_scorer {
+public:
+    lambda_optimizer(lambda* p_lambda, model* p_model, const root_equilibrium_distribution* p_distribution,
+                     double longest_branch)
+        : _p_lambda(p_lambda), _p_model(p_model), _p_distribution(p_distribution), _longest_branch(longest_branch) {}
+    // normal draws around 0.002 scaled to the longest branch (:40-56)
+    std::vector<double> initial_guesses() override;
+    double calculate_score(const double* values) override;
+    void finalize(double* results) override { _p_lambda->update(results); }
+
+private:
+    lambda* _p_lambda;
+    model* _p_model;
+    const root_equilibrium_distribution* _p_distribution;
+    double _longest_branch;
+};
 
 }  // namespace cafe5_gpu

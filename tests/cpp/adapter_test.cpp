@@ -8,6 +8,11 @@
 //   adapter_test condprob FILE   compute_family_probabilities for the
 //                                families of FILE at root size `root`:
 //                                "PROB id value"
+//   adapter_test optimize FILE   optimizes lambda for the base model of FILE
+//                                (Standard Nelder-Mead, engine seeded `seed`)
+//                                on a device-resident model: "START x...",
+//                                one "EVAL x... score" per evaluation, then
+//                                "RESULT iters score x..."
 //   adapter_test pvalues FILE    compute_pvalues for the families of FILE
 //                                with the engine seeded `seed`: the draws
 //                                ("SIM r max c1 c2 ..."), each root size's
@@ -28,6 +33,7 @@
 //   inner id v                   (condprob: interior sizes of family id)
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -276,6 +282,96 @@ static int run_kats(gpu_context& gpu)
         report("estimator_compute_pvalues", values.size() == 1 && values[0] == 0.0,
                values.empty() ? "" : std::to_string(values[0]));
     }
+    // the reference's Nelder-Mead step tests (test.cpp:2802-3030) on a
+    // scorer that multiplies the first two values (test.cpp:2844-2862)
+    {
+        struct product_scorer : optimizer_scorer {
+            std::vector<double> initial_guesses() override { return {5, 3}; }
+            double calculate_score(const double* v) override { return v[0] * v[1]; }
+        } ms;
+        auto fresh = [&]() { return nelder_mead(&ms, 2); };
+        {
+            auto nm = fresh();
+            candidate* c[3] = {nm.candidates[0], nm.candidates[1], nm.candidates[2]};
+            c[0]->score = 3, c[1]->score = 5, c[2]->score = 1;
+            nm.sort();
+            report("fminsearch_sort", nm.candidates[0] == c[2] && nm.candidates[1] == c[0] && nm.candidates[2] == c[1]);
+        }
+        {
+            auto nm = fresh();
+            const double v[3][2] = {{1, 3}, {2, 4}, {3, 5}};
+            for (int i = 0; i < 3; ++i) nm.candidates[i]->values = {v[i][0], v[i][1]};
+            nm.tolx = 3;
+            const bool a = nm.check_values();
+            nm.tolx = .5;
+            report("fminsearch_checkV", a && !nm.check_values());
+            nm.candidates[0]->score = 1, nm.candidates[1]->score = 3, nm.candidates[2]->score = 5;
+            nm.tolf = 5;
+            const bool b = nm.check_scores();
+            nm.tolf = 1;
+            report("fminsearch_checkF", b && !nm.check_scores());
+        }
+        {
+            auto nm = fresh();
+            auto init = ms.initial_guesses();
+            nm.init(init.data());
+            report("fminsearch_min_init", nm.candidates[0]->score == 15 && nm.candidates[1]->score == 15.75 &&
+                                              close_to(nm.candidates[2]->score, 15.75, 1e-12));
+        }
+        {
+            auto nm = fresh();
+            nm.candidates[0]->values = {300, 12};
+            nm.candidates[1]->values = {200, 44};
+            nm.mean();
+            report("fminsearch_x_mean", nm.x_mean[0] == 250 && nm.x_mean[1] == 28);
+            const double fr = nm.reflection();   // the third candidate is (0, 0)
+            report("fminsearch_x_reflection", nm.x_r[0] == 500 && nm.x_r[1] == 56 && fr == 28000);
+            const double fe = nm.expansion();
+            report("fminsearch_x_expansion", nm.x_tmp[0] == 750 && nm.x_tmp[1] == 84 && fe == 63000);
+            const double fo = nm.contract_outside();
+            report("fminsearch_x_contract_outside", nm.x_tmp[0] == 375 && nm.x_tmp[1] == 42 && fo == 15750);
+            nm.candidates[2]->values = {26, 12};
+            const double fi = nm.contract_inside();
+            report("fminsearch_x_contract_inside", nm.x_tmp[0] == 362 && nm.x_tmp[1] == 36 && fi == 13032);
+        }
+        {
+            auto nm = fresh();
+            nm.candidates[0]->values = {300, 200};
+            nm.candidates[1]->values = {42, 64};
+            nm.candidates[2]->values = {26, 12};
+            nm.shrink();
+            report("fminsearch_x_shrink", nm.candidates[0]->values == std::vector<double>{300, 200} &&
+                                              nm.candidates[1]->values == std::vector<double>{163, 106} &&
+                                              nm.candidates[2]->values == std::vector<double>{171, 132});
+        }
+        {
+            auto nm = fresh();
+            nm.candidates[0]->values = {300, 200}, nm.candidates[0]->score = 2;
+            nm.candidates[1]->values = {42, 64}, nm.candidates[1]->score = 4;
+            nm.candidates[2]->values = {26, 12}, nm.candidates[2]->score = 6;
+            const double x[2] = {99, 14};
+            nm.set_last(x, 3);
+            report("fminsearch_set_last_element", nm.candidates[0]->score == 2 && nm.candidates[1]->score == 3 &&
+                                                      nm.candidates[2]->score == 4);
+        }
+        // test.cpp:2400-2416: guesses come from the scorer; +inf scores throw
+        struct fixed_scorer : optimizer_scorer {
+            bool fail = false;
+            std::vector<double> initial_guesses() override { return {0.2}; }
+            double calculate_score(const double*) override { return fail ? INFINITY : 1000; }
+        } fs;
+        optimizer opt(&fs);
+        auto g = opt.get_initial_guesses();
+        report("optimizer_gets_initial_guesses_from_scorer", g.size() == 1 && g[0] == 0.2);
+        fs.fail = true;
+        bool threw = false;
+        try {
+            opt.get_initial_guesses();
+        } catch (std::runtime_error& e) {
+            threw = std::string(e.what()) == "Failed to initialize any reasonable values";
+        }
+        report("optimizer_disallows_bad_initializations", threw);
+    }
     {
         std::vector<double> f(4), r(4);
         get_gamma(f, r, 0.5);
@@ -428,6 +524,53 @@ static int run_condprob(gpu_context& gpu, const std::string& file)
     return 0;
 }
 
+// Logs every point the optimizer scores.
+struct logging_scorer : optimizer_scorer {
+    explicit logging_scorer(optimizer_scorer& s) : inner(s) {}
+    optimizer_scorer& inner;
+    int n = 1;
+    std::vector<double> initial_guesses() override
+    {
+        auto g = inner.initial_guesses();
+        n = (int)g.size();
+        std::cout << "START";
+        for (double v : g) std::cout << " " << v;
+        std::cout << "\n";
+        return g;
+    }
+    double calculate_score(const double* x) override
+    {
+        const double v = inner.calculate_score(x);
+        std::cout << "EVAL";
+        for (int i = 0; i < n; ++i) std::cout << " " << x[i];
+        std::cout << " " << v << "\n";
+        return v;
+    }
+};
+
+static int run_optimize(gpu_context& gpu, const std::string& file)
+{
+    pvalue_input d = read_pvalue_input(file);
+    gpu.build_algo = d.algo;
+    std::unique_ptr<clade> tree(parse_newick(d.newick));
+    single_lambda lam(d.lambda);
+    resident_base_model model(&lam, tree.get(), &d.families, d.B, d.R, nullptr, gpu);
+    root_equilibrium_distribution prior((size_t)d.R);
+    double longest = 0;
+    for (double t : tree->get_branch_lengths()) longest = std::max(longest, t);
+    lambda_optimizer scorer(&lam, &model, &prior, longest);
+    logging_scorer log(scorer);
+    std::cout << std::setprecision(17);
+    randomizer_engine.seed(d.seed);
+    optimizer opt(&log);
+    optimizer_parameters params;
+    auto r = opt.optimize(params);
+    std::cout << "RESULT " << r.num_iterations << " " << r.score;
+    for (double v : r.values) std::cout << " " << v;
+    std::cout << "\n";
+    return 0;
+}
+
 static int run_pvalues(gpu_context& gpu, const std::string& file)
 {
     pvalue_input d = read_pvalue_input(file);
@@ -472,7 +615,12 @@ int main(int argc, char** argv)
         if (mode == "base" && argc > 2) return run_model(gpu, argv[2]);
         if (mode == "condprob" && argc > 2) return run_condprob(gpu, argv[2]);
         if (mode == "pvalues" && argc > 2) return run_pvalues(gpu, argv[2]);
-        std::cerr << "usage: adapter_test kats | base FILE | condprob FILE | pvalues FILE" << std::endl;
+        if (mode == "optimize" && argc > 2) {
+            if (argc > 3) gpu.path = std::atoi(argv[3]);
+            return run_optimize(gpu, argv[2]);
+        }
+        std::cerr << "usage: adapter_test kats | base FILE | condprob FILE | pvalues FILE | optimize FILE [path]"
+                  << std::endl;
         return 2;
     } catch (std::exception& e) {
         std::cerr << "error: " << e.what() << std::endl;

@@ -203,3 +203,71 @@ def test_pvalues_through_cpp_interface(tmp_path):
     if Ref.available():
         first = Ref.create_families(tree, lam, B, R, max(B, R) + 1, seed, 1, nsim)
         assert np.array_equal(np.stack([leaf for r, _, leaf in sims[:nsim]]), first)
+
+
+# --------------------------------------------------------------------------
+# lambda search (optimizer.cpp, optimizer_scorer.cpp) on a device-resident
+# model through the C++ interface
+# --------------------------------------------------------------------------
+
+def _parse_optimize(out):
+    start, evals, result = None, [], None
+    for line in out.splitlines():
+        p = line.split()
+        if p[0] == "START":
+            start = [float(x) for x in p[1:]]
+        elif p[0] == "EVAL":
+            evals.append([float(x) for x in p[1:]])
+        elif p[0] == "RESULT":
+            result = (int(p[1]), float(p[2]), [float(x) for x in p[3:]])
+    return start, np.array(evals), result
+
+
+def _optimize_case(tmp_path, path):
+    w = wl.config_base(seed=5, F=120)
+    B = R = 60
+    counts = np.minimum(w.counts, B)
+    f = tmp_path / "opt.txt"
+    write_pvalue_input(f, w.tree, counts, 0.002, B, R, seed=77)
+    rc, out, err = run("optimize", str(f), str(path), timeout=300)
+    assert rc == 0, err
+    return w, counts, B, R, _parse_optimize(out)
+
+
+def test_lambda_search_through_cpp_interface(tmp_path):
+    """PLAIN64: the search visits the points the oracle's Nelder-Mead visits
+    with the oracle's scores, and ends at the same lambda."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import nelder_mead as nmr
+
+    w, counts, B, R, (start, evals, result) = _optimize_case(tmp_path, 0)
+    prior = Oracle.prior_uniform(R, R)
+
+    def oracle_score(x):
+        if not x[0] > 0:                                  # lambda.h: an invalid lambda scores +inf
+            return float("inf")
+        v = Oracle.evaluate(w.tree, np.full(w.tree.n_nodes, x[0]), counts, B, R, prior, threads=8)["total"]
+        return float("inf") if np.isnan(v) else v
+
+    seen = []
+    vals, score, iters = nmr.optimize(lambda x: (seen.append((x[0], oracle_score(x))), seen[-1][1])[1], start)
+    # the first evaluation checks the initial guess; the search follows
+    assert evals[0][0] == start[0]
+    got = evals[1:]
+    assert len(got) == len(seen), (len(got), len(seen))
+    want = np.array(seen)
+    assert np.array_equal(got[:, 0], want[:, 0])
+    fin = np.isfinite(want[:, 1])
+    assert np.array_equal(np.isfinite(got[:, 1]), fin)
+    assert np.all(np.abs(got[fin, 1] - want[fin, 1]) <= 1e-9 * np.abs(want[fin, 1]))
+    assert result[0] == iters and result[2][0] == vals[0]
+    assert abs(result[1] - score) <= 1e-9 * abs(score)
+
+
+def test_lambda_search_batched32_through_cpp_interface(tmp_path):
+    """BATCHED32 scores (1e-6 of the total) lead to the same optimum."""
+    _, _, _, _, (_, _, r32) = _optimize_case(tmp_path, 2)
+    _, _, _, _, (_, _, r64) = _optimize_case(tmp_path, 0)
+    assert abs(r32[1] - r64[1]) <= 1e-6 * abs(r64[1])
+    assert abs(r32[2][0] - r64[2][0]) <= 1e-3 * r64[2][0]
