@@ -20,6 +20,10 @@ hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const 
                         int first_slot, int N, int Np, long long slot_elems, double* mt64, float* mt32,
                         float* lf32, hipStream_t stream);
 hipError_t launch_fill_zero(void* p, long long bytes, hipStream_t stream);
+hipError_t launch_reconstruct(const ReconNode* nodes, int n_nodes, const int* kids, const int* parent,
+                              const double* mt64, int N, int Np, long long slot_elems, const int* counts_T, int F,
+                              const int* fam_sorted, int nf, const double* prior, int B, int Rr, double* L_all,
+                              int* C_all, double* prod_all, int Lp, int* states, int* err, hipStream_t st);
 hipError_t launch_fold(const double* mt64, const float* lf32, double* out64, float* out32, const double* err64,
                        const float* err32, int n_dev, int n_rows, int N, int Np, long long slot_elems,
                        int first_slot, int n, hipStream_t stream);
@@ -132,6 +136,9 @@ struct cafe_gpu_ctx {
 
     // tree program
     int n_nodes = 0, tree_leaves = 0, n_slots = 0, max_children = 2;
+    std::vector<int> parent_h, leaf_col_h;   // the tree as set
+    DevBuf d_recon;                          // reconstruction: program, families, prior, states, flags
+    DevBuf d_recon_scratch;                  // reconstruction: per-family L, C and product rows
     std::vector<OpDev> ops;
     std::vector<ChildDev> kids;
     DevBuf d_ops, d_kids, d_steps;
@@ -450,6 +457,8 @@ int cafe_gpu_set_tree(cafe_gpu_ctx* ctx, int n_nodes, const int* parent, const i
         return fail(CAFE_GPU_EINVAL, "tree needs %d live partial vectors (max %d)", pl.max_slot + 1, kMaxSlots);
     ctx->n_nodes = n_nodes;
     ctx->tree_leaves = n_leaves;
+    ctx->parent_h.assign(parent, parent + n_nodes);
+    ctx->leaf_col_h.assign(leaf_col, leaf_col + n_nodes);
     ctx->max_children = 1;
     for (auto& k : kids) ctx->max_children = std::max(ctx->max_children, (int)k.size());
     ctx->n_slots = pl.max_slot + 1;
@@ -1331,6 +1340,93 @@ int cafe_gpu_root_vectors(cafe_gpu_ctx* ctx, const cafe_gpu_eval_desc* d, int tr
     size_t o = 0;
     for (int f = f0; f < f0 + nf; ++f)
         for (int j = 0; j < famR[f]; ++j) out[o++] = h[(size_t)f * maxR + j];
+    return CAFE_GPU_OK;
+}
+
+int cafe_gpu_reconstruct(cafe_gpu_ctx* ctx, const int* node_slot, const double* prior, int B, int R, int f0, int nf,
+                         int* states)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!node_slot || !prior || !states) return fail(CAFE_GPU_EINVAL, "null argument");
+    if (ctx->n_nodes == 0 || ctx->F == 0) return fail(CAFE_GPU_ESTATE, "tree and families must be set first");
+    if (ctx->N == 0) return fail(CAFE_GPU_ESTATE, "matrix_reset has not been called");
+    if (B < 0 || R < 0 || B + 1 > ctx->N) return fail(CAFE_GPU_EINVAL, "bounds B=%d R=%d do not fit matrices of %d", B, R, ctx->N);
+    if (f0 < 0 || nf < 0 || f0 + nf > ctx->F) return fail(CAFE_GPU_EINVAL, "family range out of range");
+    if (nf == 0) return CAFE_GPU_OK;
+    const int n = ctx->n_nodes;
+    const int Rr = std::min(B, R) + 1;   // gene_family_reconstructor.cpp:149-151
+    // children first (depth first, children in increasing index)
+    std::vector<std::vector<int>> kids(n);
+    int root = -1;
+    for (int i = 0; i < n; ++i) {
+        if (ctx->parent_h[i] < 0) root = i;
+        else kids[ctx->parent_h[i]].push_back(i);
+    }
+    std::vector<ReconNode> prog;
+    std::vector<int> kid_list;
+    std::vector<std::pair<int, int>> stack{{root, 0}};
+    while (!stack.empty()) {
+        auto& top = stack.back();
+        const int u = top.first;
+        if (top.second < (int)kids[u].size()) {
+            stack.push_back({kids[u][top.second++], 0});
+            continue;
+        }
+        stack.pop_back();
+        ReconNode nd{u, ctx->leaf_col_h[u], (int)kid_list.size(), (int)kids[u].size(), -1};
+        for (int c : kids[u]) kid_list.push_back(c);
+        if (u != root) {
+            nd.slot = node_slot[u];
+            if (nd.slot < 0 || nd.slot >= ctx->capacity)
+                return fail(CAFE_GPU_EINVAL, "node %d: matrix slot %d outside the pool", u, nd.slot);
+        }
+        prog.push_back(nd);
+    }
+    // count-row (sorted) index of each caller family
+    std::vector<int> sorted_of(ctx->F);
+    for (int s = 0; s < ctx->F; ++s) sorted_of[ctx->perm[s]] = s;
+    const int chunk = std::min(nf, 2048);
+    const int Lp = B + 1;
+    // device layout: program | kids | parent | prior | families | states | flags
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t o_kids = up(sizeof(ReconNode) * prog.size());
+    const size_t o_par = o_kids + up(sizeof(int) * std::max<size_t>(1, kid_list.size()));
+    const size_t o_pri = o_par + up(sizeof(int) * n);
+    const size_t o_fam = o_pri + up(sizeof(double) * Rr);
+    const size_t o_st = o_fam + up(sizeof(int) * chunk);
+    const size_t o_err = o_st + up(sizeof(int) * (size_t)chunk * n);
+    HIP_OR_FAIL(ctx->d_recon.ensure(o_err + 256));
+    HIP_OR_FAIL(ctx->d_recon_scratch.ensure((size_t)chunk * Lp * ((size_t)n * (sizeof(double) + sizeof(int)) + sizeof(double))));
+    char* base = (char*)ctx->d_recon.p;
+    double* L_all = (double*)ctx->d_recon_scratch.p;
+    int* C_all = (int*)(L_all + (size_t)chunk * n * Lp);
+    double* prod_all = (double*)(C_all + (size_t)chunk * n * Lp);
+    hipStream_t st = ctx->stream;
+    HIP_OR_FAIL(hipMemcpyAsync(base, prog.data(), sizeof(ReconNode) * prog.size(), hipMemcpyHostToDevice, st));
+    if (!kid_list.empty())
+        HIP_OR_FAIL(hipMemcpyAsync(base + o_kids, kid_list.data(), sizeof(int) * kid_list.size(), hipMemcpyHostToDevice, st));
+    HIP_OR_FAIL(hipMemcpyAsync(base + o_par, ctx->parent_h.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    HIP_OR_FAIL(hipMemcpyAsync(base + o_pri, prior, sizeof(double) * Rr, hipMemcpyHostToDevice, st));
+    HIP_OR_FAIL(launch_fill_zero(base + o_err, 2 * sizeof(int), st));
+    for (int c0 = 0; c0 < nf; c0 += chunk) {
+        const int m = std::min(chunk, nf - c0);
+        std::vector<int> fams(m);
+        for (int k = 0; k < m; ++k) fams[k] = sorted_of[f0 + c0 + k];
+        HIP_OR_FAIL(hipMemcpyAsync(base + o_fam, fams.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+        HIP_OR_FAIL(launch_reconstruct((const ReconNode*)base, (int)prog.size(), (const int*)(base + o_kids),
+                                       (const int*)(base + o_par), ctx->mt64.as<double>(), ctx->N, ctx->Np,
+                                       ctx->slot_elems, ctx->d_counts.as<int>(), ctx->F, (const int*)(base + o_fam), m,
+                                       (const double*)(base + o_pri), B, Rr, L_all, C_all, prod_all, Lp,
+                                       (int*)(base + o_st), (int*)(base + o_err), st));
+        HIP_OR_FAIL(hipMemcpyAsync(states + (size_t)c0 * n, base + o_st, sizeof(int) * (size_t)m * n,
+                                   hipMemcpyDeviceToHost, st));
+        HIP_OR_FAIL(hipStreamSynchronize(st));   // fams and the state rows are reused by the next chunk
+    }
+    int flags[2] = {0, 0};
+    HIP_OR_FAIL(hipMemcpyAsync(flags, base + o_err, sizeof(flags), hipMemcpyDeviceToHost, st));
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    if (flags[0]) return fail(CAFE_GPU_EINVAL, "a leaf count is outside the matrices (N = %d)", ctx->N);
+    if (flags[1]) return fail(CAFE_GPU_EINVAL, "a reconstructed state fell outside 0..%d", B);
     return CAFE_GPU_OK;
 }
 

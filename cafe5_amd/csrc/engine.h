@@ -146,6 +146,15 @@ struct PruneParams {
     int plain_lds;            // PLAIN64: partial vectors in dynamic LDS
 };
 
+// One node of the reconstruction program (children before parents).
+struct ReconNode {
+    int node;       // engine node index
+    int leaf_col;   // >= 0: a leaf's column in the count rows
+    int kid0;       // first child in the program's kids list
+    int n_kids;
+    int slot;       // matrix slot of the branch above the node (-1: root)
+};
+
 // Launch shape of the MFMA pruning kernel: ntw 16-row output tiles per wave,
 // w waves per workgroup, partial vectors in LDS or global scratch, dynamic
 // LDS bytes.

@@ -297,6 +297,11 @@ void matrix::multiply(const std::vector<double>& v, int s_min, int s_max, int c_
     }
 }
 
+bool matrix::is_zero() const
+{
+    return std::all_of(_values.begin(), _values.end(), [](double v) { return v == 0; });
+}
+
 int matrix::select_random_y(int x, int max) const
 {
     const double* row = _values.data() + (size_t)x * _size;
@@ -946,6 +951,49 @@ std::vector<double> compute_pvalues(pvalue_parameters p, const std::vector<gene_
                                   std::vector<int>(families.size(), R));
     for (size_t i = 0; i < families.size(); ++i) result[i] = find_best_pvalue(families[i], obs[i], conditional);
     return result;
+}
+
+// ---------------------------------------------------------------------------
+// ancestral reconstruction
+// ---------------------------------------------------------------------------
+std::vector<clademap<int>> reconstruct_gene_families(const lambda* p_lambda, const clade* p_tree,
+                                                     const std::vector<gene_family>& families, matrix_cache& calc,
+                                                     const root_equilibrium_distribution& prior, int max_family_size,
+                                                     int max_root_family_size)
+{
+    std::vector<clademap<int>> out(families.size());
+    if (families.empty()) return out;
+    base_model m(const_cast<lambda*>(p_lambda), p_tree, &families, max_family_size, max_root_family_size, nullptr,
+                 calc.gpu());
+    m.upload();
+    std::vector<int> slots = m.node_slots(calc, p_lambda, {1.0});
+    // gene_family_reconstructor.cpp:84-85, for the interior branches
+    std::vector<const clade*> nodes = p_tree->reverse_level_order();   // every node once
+    for (const clade* c : nodes)
+        if (!c->is_leaf() && !c->is_root() &&
+            calc.get_matrix(c->get_branch_length(), p_lambda->get_value_for_clade(c))->is_zero())
+            throw std::runtime_error("Zero matrix found");
+    const int Rr = std::min(max_family_size, max_root_family_size) + 1;
+    std::vector<double> pr(Rr);
+    for (int j = 0; j < Rr; ++j) pr[j] = prior.compute(j);   // the root scores size j with compute(j)
+    // the engine's node order is the one upload() gave the tree
+    std::vector<const clade*> order;
+    {
+        std::vector<const clade*> bfs{p_tree};
+        for (size_t i = 0; i < bfs.size(); ++i) {
+            std::vector<const clade*> kids(bfs[i]->descendant_begin(), bfs[i]->descendant_end());
+            for (auto it = kids.rbegin(); it != kids.rend(); ++it) bfs.push_back(*it);
+        }
+        order.assign(bfs.rbegin(), bfs.rend());
+    }
+    const int n = (int)order.size();
+    std::vector<int> states((size_t)families.size() * n);
+    check(cafe_gpu_reconstruct(calc.gpu().get(), slots.data(), pr.data(), max_family_size, max_root_family_size, 0,
+                               (int)families.size(), states.data()),
+          "cafe_gpu_reconstruct");
+    for (size_t f = 0; f < families.size(); ++f)
+        for (int i = 0; i < n; ++i) out[f][order[i]] = states[f * n + i];
+    return out;
 }
 
 // ---------------------------------------------------------------------------

@@ -224,6 +224,7 @@ public:
     // (matrix_cache.cpp:60-66)
     int select_random_y(int x, int max) const;
     double* data() { return _values.data(); }
+    bool is_zero() const;   // every entry 0
 
 private:
     int _size;
@@ -419,6 +420,21 @@ double find_best_pvalue(const gene_family& fam, const std::vector<double>& root_
 // reference's order), then each family's best p-value
 std::vector<double> compute_pvalues(pvalue_parameters p, const std::vector<gene_family>& families,
                                     int number_of_simulations);
+
+// ---------------------------------------------------------------------------
+// ancestral reconstruction (gene_family_reconstructor.cpp:30-189)
+// ---------------------------------------------------------------------------
+// Pupko's joint reconstruction of every family on the engine (one wave per
+// family): per family, the reconstructed size of every interior node and the
+// root (leaves keep their counts), the result of
+// pupko_reconstructor::reconstruct_gene_family with vectors of sizes
+// 0..max_family_size and root sizes 1..min(max_family_size,
+// max_root_family_size).  The cache must hold every branch's matrix; an
+// all-zero interior matrix throws "Zero matrix found" as the reference does.
+std::vector<clademap<int>> reconstruct_gene_families(const lambda* p_lambda, const clade* p_tree,
+                                                     const std::vector<gene_family>& families, matrix_cache& calc,
+                                                     const root_equilibrium_distribution& prior, int max_family_size,
+                                                     int max_root_family_size);
 
 // ---------------------------------------------------------------------------
 // optimizer (optimizer.h, optimizer.cpp, optimizer_scorer.h/.cpp)

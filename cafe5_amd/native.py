@@ -67,6 +67,7 @@ def load_library() -> ctypes.CDLL:
         "cafe_gpu_set_error_model": (_i, [_p, _i, _i, _pd]),
         "cafe_gpu_eval": (_i, [_p, ctypes.POINTER(EvalDesc), _pd, _pd, _pd, _pd, _pll]),
         "cafe_gpu_root_vectors": (_i, [_p, ctypes.POINTER(EvalDesc), _i, _i, _i, _i, _pd]),
+        "cafe_gpu_reconstruct": (_i, [_p, _pi, _pd, _i, _i, _i, _i, _pi]),
         "cafe_gpu_last_timings": (_i, [_p, _pd]),
         "cafe_gpu_plan_create": (_i, [_p, ctypes.POINTER(EvalDesc), _i, _pd, _pd, _i, _i, ctypes.POINTER(_p)]),
         "cafe_gpu_plan_set_keys": (_i, [_p, _pd, _pd]),
@@ -89,7 +90,7 @@ def load_library() -> ctypes.CDLL:
 EXPORTED_SYMBOLS = ["cafe_gpu_create", "cafe_gpu_destroy", "cafe_gpu_last_error", "cafe_gpu_matrix_reset",
                     "cafe_gpu_matrix_build", "cafe_gpu_matrix_get", "cafe_gpu_set_tree",
                     "cafe_gpu_set_families", "cafe_gpu_bucket_count", "cafe_gpu_bucket_info",
-                    "cafe_gpu_set_error_model", "cafe_gpu_eval", "cafe_gpu_root_vectors",
+                    "cafe_gpu_set_error_model", "cafe_gpu_eval", "cafe_gpu_root_vectors", "cafe_gpu_reconstruct",
                     "cafe_gpu_last_timings", "cafe_gpu_plan_create", "cafe_gpu_plan_set_keys",
                     "cafe_gpu_plan_execute", "cafe_gpu_plan_synchronize", "cafe_gpu_plan_results",
                     "cafe_gpu_plan_profile", "cafe_gpu_plan_recompute_counts", "cafe_gpu_plan_class_times",
@@ -219,6 +220,18 @@ class Engine:
         out = np.zeros(self.F * R)
         self._check(self.lib.cafe_gpu_root_vectors(self.ctx, ctypes.byref(d), trial, cat, 0, self.F, _dp(out)))
         return out.reshape(self.F, R)
+
+    def reconstruct(self, node_slot: np.ndarray, prior: np.ndarray, B: int, R: int) -> np.ndarray:
+        """Pupko joint reconstruction of every family: states[F, n_nodes]
+        (gene_family_reconstructor.cpp:30-189); prior[j], j = 0..min(B, R)."""
+        n_nodes = self.tree.n_nodes
+        ns = np.ascontiguousarray(np.asarray(node_slot).reshape(-1)[:n_nodes], dtype=np.int32)
+        pr = np.ascontiguousarray(prior, dtype=np.float64)
+        if pr.size < min(B, R) + 1:
+            raise ValueError("prior needs min(B, R) + 1 entries")
+        out = np.zeros((self.F, n_nodes), dtype=np.int32)
+        self._check(self.lib.cafe_gpu_reconstruct(self.ctx, _ip(ns), _dp(pr), B, R, 0, self.F, _ip(out)))
+        return out
 
     def timings(self) -> np.ndarray:
         out = np.zeros(4)

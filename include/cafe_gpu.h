@@ -163,6 +163,22 @@ int cafe_gpu_eval(cafe_gpu_ctx *ctx, const cafe_gpu_eval_desc *desc, double *lnl
 int cafe_gpu_root_vectors(cafe_gpu_ctx *ctx, const cafe_gpu_eval_desc *desc, int trial,
                           int cat, int f0, int nf, double *out);
 
+/* --- ancestral reconstruction -------------------------------------------- */
+
+/* Pupko's joint reconstruction (pupko_reconstructor::reconstruct_gene_family,
+ * src/gene_family_reconstructor.cpp:30-189) of families f0 .. f0+nf-1 with
+ * vectors of sizes 0..B and root sizes 1..min(B, R):
+ *   node_slot[i]  matrix slot of the branch above node i (root ignored)
+ *   prior[j]      double(root_equilibrium_distribution::compute(j)),
+ *                 j = 0 .. min(B, R) (the root scores size j with compute(j))
+ *   states[(f - f0) * n_nodes + i]  the reconstructed size of node i (a
+ *                 leaf's own count)
+ * Uses the FP64 matrices and the reference's order of every product, so the
+ * states are the reference's.  The caller rejects all-zero matrices of
+ * interior branches as the reference does (:84-85). */
+int cafe_gpu_reconstruct(cafe_gpu_ctx *ctx, const int *node_slot, const double *prior, int B, int R,
+                         int f0, int nf, int *states);
+
 /* --- device-resident plans (no reference counterpart) ----------------------
  * A plan keeps everything an evaluation needs on the device and replays it
  * as one HIP graph: matrix build for n_keys keys into pool slots

@@ -308,6 +308,111 @@ int orc_family_probability(const orc_tree *tree, const int *counts, const double
     return 0;
 }
 
+/* gene_family_reconstructor.cpp:30-189 */
+int orc_reconstruct(const orc_tree *tree, const int *counts, const double *mat_pool, const int *node_mat,
+                    int N, int B, int R, const double *prior, int *states, double *node_L)
+{
+    int n = tree->n_nodes;
+    int root = orc_find_root(tree);
+    if (root < 0 || B < 0 || R < 0 || B + 1 > N)
+        return -1;
+    int *order = (int *)malloc(sizeof(int) * n);
+    if (orc_postorder(tree, order) != 0) {
+        free(order);
+        return -1;
+    }
+    int L1 = B + 1;
+    int Rr = (B < R ? B : R) + 1;   /* :149-151 */
+    double *L = (double *)calloc((size_t)n * L1, sizeof(double));
+    int *C = (int *)calloc((size_t)n * L1, sizeof(int));
+    int root_c = 0, rc = 0;
+    for (int oi = 0; oi < n && rc == 0; ++oi) {
+        int u = order[oi];
+        double *Lu = L + (size_t)u * L1;
+        int *Cu = C + (size_t)u * L1;
+        if (tree->leaf_col[u] >= 0) {
+            /* :30-46: the leaf's count for every parent size i */
+            int obs = counts[tree->leaf_col[u]];
+            if (obs < 0 || obs >= N) {
+                rc = -1;
+                break;
+            }
+            const double *M = mat_pool + (size_t)node_mat[u] * N * N;
+            for (int i = 0; i < L1; ++i) {
+                Lu[i] = M[(size_t)i * N + obs];
+                Cu[i] = obs;
+            }
+            continue;
+        }
+        if (u == root) {
+            /* :48-74: every i scans the same j = 1..Rr-1; C[0] keeps the
+             * last scan's first maximum */
+            for (int i = 1; i < Rr; ++i) {
+                double max_val = -1;
+                for (int j = 1; j < Rr; ++j) {
+                    double value = 1.0;
+                    for (int ch = 0; ch < n; ++ch)
+                        if (tree->parent[ch] == u)
+                            value *= L[(size_t)ch * L1 + j];
+                    double val = value * prior[j];
+                    if (val > max_val) {
+                        max_val = val;
+                        root_c = j;
+                    }
+                }
+                Lu[i] = max_val;
+            }
+            continue;
+        }
+        const double *M = mat_pool + (size_t)node_mat[u] * N * N;
+        int zero = 1;   /* matrix::is_zero */
+        for (size_t e = 0; e < (size_t)N * N && zero; ++e)
+            zero = M[e] == 0.0;
+        if (zero) {
+            rc = -2;
+            break;
+        }
+        /* :76-111: i is the parent size, j this node's */
+        for (int i = 0; i < L1; ++i) {
+            int max_j = 0;
+            double max_val = -1;
+            for (int j = 0; j < L1; ++j) {
+                double value = 1.0;
+                for (int ch = 0; ch < n; ++ch)
+                    if (tree->parent[ch] == u)
+                        value *= L[(size_t)ch * L1 + j];
+                double val = value * M[(size_t)i * N + j];
+                if (val > max_val) {
+                    max_j = j;
+                    max_val = val;
+                }
+            }
+            Lu[i] = max_val;
+            Cu[i] = max_j;
+        }
+    }
+    if (rc == 0) {
+        /* :170-189: the root's pick, then each interior child's C at its
+         * parent's state (parents before children: reverse post-order) */
+        states[root] = root_c;
+        for (int oi = n - 1; oi >= 0; --oi) {
+            int u = order[oi];
+            if (u == root)
+                continue;
+            if (tree->leaf_col[u] >= 0)
+                states[u] = counts[tree->leaf_col[u]];
+            else
+                states[u] = C[(size_t)u * L1 + states[tree->parent[u]]];
+        }
+        if (node_L)
+            memcpy(node_L, L, sizeof(double) * (size_t)n * L1);
+    }
+    free(C);
+    free(L);
+    free(order);
+    return rc;
+}
+
 /* probability.cpp:517-527 */
 double orc_pvalue(double v, const double *conddist, int n)
 {

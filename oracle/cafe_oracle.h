@@ -22,6 +22,7 @@
  *   src/poisson.cpp:21-24             poisspdf
  *   src/probability.cpp:448-477       compute_family_probabilities
  *   src/probability.cpp:517-541       pvalue, find_best_pvalue
+ *   src/gene_family_reconstructor.cpp:30-189   Pupko joint reconstruction
  */
 #ifndef CAFE_ORACLE_H
 #define CAFE_ORACLE_H
@@ -91,6 +92,16 @@ int orc_prune(const orc_tree *tree, const int *counts, const double *mat_pool,
  * Returns -1 (and writes nothing) for malformed input. */
 int orc_family_probability(const orc_tree *tree, const int *counts, const double *mat_pool,
                            const int *node_mat, int N, int B, int max_size, int r, double *out);
+
+/* pupko_reconstructor::reconstruct_gene_family (gene_family_reconstructor.cpp:
+ * 30-189) for one family: vectors of sizes 0..B, root sizes 1..min(B, R)
+ * scored with prior[j] = double(float(compute(j))), j = 0..min(B, R).
+ * Writes states[n_nodes] (leaves: their counts) and, when node_L is not NULL,
+ * every node's L vector (node_L[i * (B + 1) + s]; the root's entries 1..
+ * min(B, R)).  Returns 0, -1 for malformed input, -2 when an interior
+ * branch's matrix is all zero (:84-85). */
+int orc_reconstruct(const orc_tree *tree, const int *counts, const double *mat_pool, const int *node_mat,
+                    int N, int B, int R, const double *prior, int *states, double *node_L);
 
 /* probability.cpp:517-527: the rank of the first conditional probability
  * greater than v (n - 1 when there is none) over n. */

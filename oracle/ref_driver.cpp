@@ -7,8 +7,9 @@
 //
 // Every function here only calls reference code: matrix_cache, inference_prune,
 // base_model / gamma_model::infer_family_likelihoods, get_gamma, the
-// root_equilibrium_distribution constructors, and the p-value functions
-// (create_family, compute_family_probabilities, pvalue, find_best_pvalue).
+// root_equilibrium_distribution constructors, the p-value functions
+// (create_family, compute_family_probabilities, pvalue, find_best_pvalue) and
+// Pupko's reconstruction (initialize_at_node, reconstruct_gene_family).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -28,6 +29,7 @@ INITIALIZE_EASYLOGGINGPP
 #include "gamma.h"
 #include "gamma_core.h"
 #include "gene_family.h"
+#include "gene_family_reconstructor.h"
 #include "lambda.h"
 #include "matrix_cache.h"
 #include "probability.h"
@@ -383,6 +385,39 @@ REF_API int ref_create_families(const char* newick, const char* leaf_names, doub
             for (size_t i = 0; i < names.size(); ++i)
                 out[(size_t)d * names.size() + i] = sizes.at(tree->find_descendant(names[i]));
         }
+        delete tree;
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// pupko_reconstructor::reconstruct_gene_family (gene_family_reconstructor.cpp:
+// 30-189) of one family with the prior of ref_prior's kinds; out[i] is the
+// state of the node named node_names[i] (a leaf's own count).
+REF_API int ref_reconstruct(const char* newick, const char* leaf_names, const char* node_names, const int* counts,
+                            double lambda, int B, int R, int prior_kind, int prior_max, double poisson_lambda,
+                            int n_map, const int* map_sizes, const int* map_counts, int* out)
+{
+    try {
+        clade* tree = parse_newick(newick);
+        auto fams = make_families(1, leaf_names, counts);
+        single_lambda lam(lambda);
+        matrix_cache cache(std::max(R, B) + 1);
+        cache.precalculate_matrices({lambda}, tree->get_branch_lengths());
+        auto* prior = make_prior(prior_kind, prior_max, poisson_lambda, n_map, map_sizes, map_counts);
+        clademap<int> result;
+        clademap<std::vector<int>> Cs;
+        clademap<std::vector<double>> Ls;
+        for (auto it = tree->reverse_level_begin(); it != tree->reverse_level_end(); ++it)
+            pupko_reconstructor::initialize_at_node(*it, Cs, Ls, B, R);
+        pupko_reconstructor::reconstruct_gene_family(&lam, tree, &fams[0], &cache, prior, result, Cs, Ls);
+        auto names = split_names(node_names);
+        for (size_t i = 0; i < names.size(); ++i) {
+            const clade* c = tree->find_descendant(names[i]);
+            out[i] = c->is_leaf() ? fams[0].get_species_size(names[i]) : result[c];
+        }
+        delete prior;
         delete tree;
         return 0;
     } catch (...) {

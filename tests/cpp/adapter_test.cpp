@@ -252,6 +252,23 @@ static int run_kats(gpu_context& gpu)
         ost << std::setprecision(17) << (res.empty() ? -1.0 : res[0]);
         report("compute_family_probabilities", ok, ost.str());
     }
+    // test.cpp:1090-1118 through the batched reconstruction
+    {
+        std::unique_ptr<clade> t(parse_newick("(A:1,B:3):7"));
+        std::vector<gene_family> fams(1);
+        fams[0].set_species_size("A", 3);
+        fams[0].set_species_size("B", 6);
+        single_lambda lam(0.005);
+        matrix_cache cache(11, gpu);
+        cache.precalculate_matrices(get_lambda_values(&lam), std::set<double>{1, 3, 7});
+        std::map<int, int> rootdist;
+        const int v[9] = {1, 2, 3, 4, 5, 4, 3, 2, 1};
+        for (int i = 0; i < 9; ++i) rootdist[i] = v[i];
+        root_equilibrium_distribution dist(rootdist);
+        auto res = reconstruct_gene_families(&lam, t.get(), fams, cache, dist, 10, 8);
+        const int ab = res[0].at(t.get());
+        report("reconstruct_gene_family", ab == 4, std::to_string(ab));
+    }
     // test.cpp:619-630
     {
         std::unique_ptr<clade> t(parse_newick("((A:1,B:1):1,(C:1,D:1):1);"));

@@ -73,6 +73,7 @@ def _load_oracle() -> ctypes.CDLL:
         "orc_family_probability": (_i, [ctypes.POINTER(OrcTree), _pi, _pd, _pi, _i, _i, _i, _i, _pd]),
         "orc_pvalue": (_d, [_d, _pd, _i]),
         "orc_find_best_pvalue": (_d, [_pd, _i, _i, _pd, _i, _i]),
+        "orc_reconstruct": (_i, [ctypes.POINTER(OrcTree), _pi, _pd, _pi, _i, _i, _i, _pd, _pi, _pd]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -102,6 +103,7 @@ def _load_ref() -> Optional[ctypes.CDLL]:
         "ref_pvalue": (_d, [_d, _pd, _i]),
         "ref_find_best_pvalue": (_d, [_i, _pd, _i, _pd, _i, _i]),
         "ref_create_families": (_i, [_cp, _cp, _d, _i, _i, _i, ctypes.c_uint, _i, _i, _pi]),
+        "ref_reconstruct": (_i, [_cp, _cp, _cp, _pi, _d, _i, _i, _i, _i, _d, _i, _pi, _pi, _pi]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -234,6 +236,25 @@ class Oracle:
         if rc != 0:
             raise ValueError("orc_family_probability rejected its input")
         return float(out[0])
+
+    @staticmethod
+    def reconstruct(tree: Tree, counts_row, lam, B: int, R: int, prior, N: Optional[int] = None,
+                    with_L: bool = False):
+        """Pupko joint reconstruction of one family: states per node (and the
+        nodes' L vectors).  lam: one value or one per node."""
+        N = N or max(B, R) + 1
+        node_lambda = np.full(tree.n_nodes, lam) if np.ndim(lam) == 0 else np.asarray(lam)
+        pool, node_mat = Oracle.node_matrices(tree, node_lambda, N)
+        st, keep = Oracle.tree_struct(tree)
+        cnt = np.ascontiguousarray(counts_row, dtype=np.int32)
+        pr = np.ascontiguousarray(prior, dtype=np.float64)
+        states = np.zeros(tree.n_nodes, dtype=np.int32)
+        L = np.zeros((tree.n_nodes, B + 1))
+        rc = ORC.orc_reconstruct(ctypes.byref(st), _ip(cnt), _dp(pool), _ip(node_mat), N, B, R, _dp(pr),
+                                 _ip(states), _dp(L))
+        if rc != 0:
+            raise ValueError(f"orc_reconstruct failed ({rc})")
+        return (states, L) if with_L else states
 
     @staticmethod
     def pvalue(v: float, conddist) -> float:
@@ -375,4 +396,20 @@ class Ref:
                                      N, seed, root_size, n_draws, _ip(out))
         if rc != 0:
             raise RuntimeError("ref_create_families failed")
+        return out
+
+    @staticmethod
+    def reconstruct(tree: Tree, counts_row, lam: float, B: int, R: int, prior_kind: int = 0, prior_max: int = 0,
+                    poisson_lambda: float = 0.0, map_sizes=None, map_counts=None) -> np.ndarray:
+        """reconstruct_gene_family of one family: the state of every node, in
+        the tree's node order."""
+        cnt = np.ascontiguousarray(counts_row, dtype=np.int32)
+        ms = None if map_sizes is None else np.ascontiguousarray(map_sizes, dtype=np.int32)
+        mc = None if map_counts is None else np.ascontiguousarray(map_counts, dtype=np.int32)
+        out = np.zeros(tree.n_nodes, dtype=np.int32)
+        rc = REF.ref_reconstruct(tree.newick().encode(), " ".join(tree.leaf_names()).encode(),
+                                 " ".join(tree.names).encode(), _ip(cnt), lam, B, R, prior_kind, prior_max,
+                                 poisson_lambda, 0 if ms is None else len(ms), _ip(ms), _ip(mc), _ip(out))
+        if rc != 0:
+            raise RuntimeError("ref_reconstruct failed")
         return out
