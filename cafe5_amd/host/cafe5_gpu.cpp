@@ -474,8 +474,11 @@ void model::upload()
     if (_p_error_model) {
         const int nd = (int)_p_error_model->n_deviations();
         std::vector<double> rows((size_t)(_max_family_size + 1) * nd);
+        std::vector<double> pr;
         for (int k = 0; k <= _max_family_size; ++k) {
-            auto pr = _p_error_model->get_probs(k);
+            // rows past the model's range repeat its last one (the reference
+            // reads only observed counts' rows and leaves those undefined)
+            if (_p_error_model->covers((size_t)k) || pr.empty()) pr = _p_error_model->get_probs(k);
             std::copy(pr.begin(), pr.end(), rows.begin() + (size_t)k * nd);
         }
         check(cafe_gpu_set_error_model(_gpu.get(), nd, _max_family_size + 1, rows.data()), "cafe_gpu_set_error_model");
@@ -951,6 +954,125 @@ std::vector<double> compute_pvalues(pvalue_parameters p, const std::vector<gene_
                                   std::vector<int>(families.size(), R));
     for (size_t i = 0; i < families.size(); ++i) result[i] = find_best_pvalue(families[i], obs[i], conditional);
     return result;
+}
+
+// ---------------------------------------------------------------------------
+// input and output files
+// ---------------------------------------------------------------------------
+namespace {
+
+std::vector<std::string> split_on(const std::string& text, char delim)
+{
+    std::vector<std::string> out;
+    std::string cur;
+    std::istringstream ist(text);
+    while (std::getline(ist, cur, delim)) out.push_back(cur);
+    return out;
+}
+
+}  // namespace
+
+void read_gene_families(std::istream& input_file, const clade* p_tree, std::vector<gene_family>& gene_families)
+{
+    std::map<size_t, std::string> cafe_columns;   // CAFE layout: column -> species
+    std::map<size_t, std::string> node_columns;   // CAFExp layout: column -> leaf name
+    bool header = true;
+    size_t hash_lines = 0;
+    std::string line;
+    while (std::getline(input_file, line)) {
+        if (line.empty()) continue;
+        const std::vector<std::string> cells = split_on(line, '\t');
+        if (!node_columns.empty() && line[0] != '#') header = false;
+        if (header) {
+            if (line[0] == '#') {
+                // one tree node per "#name" line, in column order; leaves only count
+                if (!p_tree) throw std::runtime_error("No tree was provided.");
+                std::string name = line.substr(1);
+                if (!name.empty() && name.back() == '\r') name.pop_back();
+                const clade* c = p_tree->find_descendant(name);
+                if (!c) throw std::runtime_error(name + " not located in tree");
+                if (c->is_leaf()) node_columns[hash_lines] = name;
+                ++hash_lines;
+            } else {
+                header = false;
+                if (node_columns.empty())
+                    for (size_t i = 0; i < cells.size(); ++i) cafe_columns[i] = cells[i];
+            }
+            continue;
+        }
+        gene_family fam;
+        for (size_t i = 0; i < cells.size(); ++i) {
+            if (node_columns.empty()) {
+                if (i == 0) fam.set_desc(cells[i]);
+                else if (i == 1) fam.set_id(cells[i]);
+                else fam.set_species_size(cafe_columns[i], std::atoi(cells[i].c_str()));
+            } else {
+                auto it = node_columns.find(i);
+                if (it != node_columns.end()) fam.set_species_size(it->second, std::atoi(cells[i].c_str()));
+                else if (i == cells.size() - 1) fam.set_id(cells[i]);
+            }
+        }
+        gene_families.push_back(fam);
+    }
+    if (gene_families.empty()) throw std::runtime_error("No families found");
+}
+
+void read_error_model_file(std::istream& error_model_file, error_model* p_error_model)
+{
+    std::string line;
+    while (std::getline(error_model_file, line)) {
+        if (line.compare(0, 3, "max") == 0) {
+            auto parts = split_on(line, ':');
+            std::string v = parts.size() > 1 ? parts[1] : "";
+            v.erase(std::remove_if(v.begin(), v.end(), [](unsigned char ch) { return std::isspace(ch); }), v.end());
+            p_error_model->set_max_family_size((size_t)std::stoi(v));
+        } else if (line.compare(0, 3, "cnt") == 0) {
+            auto parts = split_on(line, ' ');
+            if (parts.size() % 2 != 0)
+                throw std::runtime_error("Number of different count differences in the error model (including 0) is "
+                                         "not an odd number. Exiting...");
+            std::vector<int> devs;
+            for (size_t i = 1; i < parts.size(); ++i) devs.push_back(std::stoi(parts[i]));
+            p_error_model->set_deviations(devs);
+        } else {
+            auto parts = split_on(line, ' ');
+            if (parts.empty()) continue;
+            const int size = std::stoi(parts[0]);
+            std::vector<double> probs;
+            for (size_t i = 1; i < parts.size(); ++i) probs.push_back(std::stod(parts[i]));
+            p_error_model->set_probabilities((size_t)size, probs);
+        }
+    }
+}
+
+void write_error_model_file(std::ostream& ost, const error_model& model)
+{
+    ost << "maxcnt: " << model.get_max_family_size() - 1 << "\n";
+    ost << "cntdiff:";
+    for (int d : model.deviations()) ost << " " << d;
+    ost << "\n";
+    std::vector<double> last;
+    for (size_t k = 0; k < model.get_max_family_size(); ++k) {
+        const auto probs = model.get_probs(k);
+        if (probs == last) continue;   // only the rows that change
+        last = probs;
+        ost << k;
+        for (double v : probs) ost << " " << v;
+        ost << std::endl;
+    }
+}
+
+std::map<int, int> read_rootdist(std::istream& rootdist_file)
+{
+    std::map<int, int> out;
+    std::string line;
+    while (std::getline(rootdist_file, line)) {
+        std::istringstream ist(line);
+        int size = 0, count = 0;
+        ist >> size >> count;
+        out[size] = count;
+    }
+    return out;
 }
 
 // ---------------------------------------------------------------------------

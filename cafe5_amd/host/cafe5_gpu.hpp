@@ -116,13 +116,15 @@ class gene_family {
 public:
     void set_id(std::string id) { _id = std::move(id); }
     std::string id() const { return _id; }
+    void set_desc(std::string d) { _desc = std::move(d); }
+    std::string desc() const { return _desc; }
     void set_species_size(const std::string& species, int count);
     // throws if the species is missing (gene_family.cpp:38-45)
     int get_species_size(const std::string& species) const;
     int get_max_size() const;
 
 private:
-    std::string _id;
+    std::string _id, _desc;
     std::map<std::string, int> _sizes;   // lower-cased species -> count
 };
 
@@ -190,6 +192,11 @@ public:
     // error_model.cpp:52-57
     std::vector<double> get_probs(size_t fam_size) const;
     size_t n_deviations() const { return _deviations.size(); }
+    const std::vector<int>& deviations() const { return _deviations; }
+    // rows set so far (error_model.h:55-57)
+    size_t get_max_family_size() const { return _dists.size(); }
+    // get_probs(k) is defined: a row was set for k, or k is within maxcnt
+    bool covers(size_t k) const { return !_dists.empty() && (k < _dists.size() || k <= _max_family_size); }
 
 private:
     size_t _max_family_size = 0;
@@ -420,6 +427,20 @@ double find_best_pvalue(const gene_family& fam, const std::vector<double>& root_
 // reference's order), then each family's best p-value
 std::vector<double> compute_pvalues(pvalue_parameters p, const std::vector<gene_family>& families,
                                     int number_of_simulations);
+
+// ---------------------------------------------------------------------------
+// input and output files (io.cpp:134-297, user_data.cpp:105-118)
+// ---------------------------------------------------------------------------
+// Gene family table: the CAFE layout (a "Desc<TAB>Family ID<TAB>species..."
+// header, then description, id and counts per line) or the CAFExp layout
+// ("#name" lines naming the tree's nodes in column order, then counts and a
+// final id column); blank lines are skipped; throws "No families found".
+void read_gene_families(std::istream& input_file, const clade* p_tree, std::vector<gene_family>& gene_families);
+// "maxcnt: n", "cntdiff: d...", then "size p..." rows
+void read_error_model_file(std::istream& error_model_file, error_model* p_error_model);
+void write_error_model_file(std::ostream& ost, const error_model& model);
+// "size count" lines
+std::map<int, int> read_rootdist(std::istream& rootdist_file);
 
 // ---------------------------------------------------------------------------
 // ancestral reconstruction (gene_family_reconstructor.cpp:30-189)

@@ -11,6 +11,7 @@
 // (create_family, compute_family_probabilities, pvalue, find_best_pvalue) and
 // Pupko's reconstruction (initialize_at_node, reconstruct_gene_family).
 #include <algorithm>
+#include <fstream>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -30,6 +31,7 @@ INITIALIZE_EASYLOGGINGPP
 #include "gamma_core.h"
 #include "gene_family.h"
 #include "gene_family_reconstructor.h"
+#include "io.h"
 #include "lambda.h"
 #include "matrix_cache.h"
 #include "probability.h"
@@ -422,5 +424,38 @@ REF_API int ref_reconstruct(const char* newick, const char* leaf_names, const ch
         return 0;
     } catch (...) {
         return -1;
+    }
+}
+
+// The files as the reference reads them (read_gene_families, io.cpp:134-217;
+// read_error_model_file, :228-275) evaluated by its base model at one lambda
+// with a uniform prior over 1..R: -lnL, out_lnl[F] in file order (F <= cap).
+REF_API double ref_example(const char* newick, const char* families_path, const char* error_model_path,
+                           double lambda, int B, int R, int cap, int* n_families, double* out_lnl)
+{
+    try {
+        clade* tree = parse_newick(newick);
+        std::ifstream ff(families_path);
+        std::vector<gene_family> fams;
+        read_gene_families(ff, tree, fams);
+        error_model* em = nullptr;
+        if (error_model_path && error_model_path[0]) {
+            std::ifstream ef(error_model_path);
+            em = new error_model();
+            read_error_model_file(ef, em);
+        }
+        single_lambda lam(lambda);
+        root_equilibrium_distribution prior((size_t)R);
+        base_probe model(&lam, tree, &fams, B, R, em);
+        const double v = model.infer_family_likelihoods(prior, &lam);
+        const auto& st = model.stash();
+        *n_families = (int)fams.size();
+        for (size_t f = 0; f < st.size() && (int)f < cap; ++f) out_lnl[f] = st[f].posterior_probability;
+        delete em;
+        delete tree;
+        return v;
+    } catch (...) {
+        *n_families = -1;
+        return 0;
     }
 }

@@ -8,6 +8,10 @@
 //   adapter_test condprob FILE   compute_family_probabilities for the
 //                                families of FILE at root size `root`:
 //                                "PROB id value"
+//   adapter_test io              the reference's file-reader tests (no GPU)
+//   adapter_test example TREE FAMILIES LAMBDA B R [ERRORMODEL]
+//                                reads the files as CAFE5 does and prints
+//                                the base model's "TOTAL" and "LNL id v"
 //   adapter_test optimize FILE   optimizes lambda for the base model of FILE
 //                                (Standard Nelder-Mead, engine seeded `seed`)
 //                                on a device-resident model: "START x...",
@@ -32,11 +36,13 @@
 //   root r / seed s / sims n     (condprob, pvalues)
 //   inner id v                   (condprob: interior sizes of family id)
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdlib>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -623,11 +629,110 @@ static int run_pvalues(gpu_context& gpu, const std::string& file)
     return 0;
 }
 
+static int run_io()
+{
+    // test.cpp:367-426
+    {
+        std::istringstream ist("Desc\tFamily ID\tA\tB\tC\tD\n\t (null)1\t5\t10\t2\t6\n\t (null)2\t5\t10\t2\t6\n"
+                               "\t (null)3\t5\t10\t2\t6\n\t (null)4\t5\t10\t2\t6");
+        std::vector<gene_family> f;
+        read_gene_families(ist, nullptr, f);
+        report("read_gene_families_reads_cafe_files", f.size() == 4 && f[0].get_species_size("A") == 5 &&
+                                                          f[0].get_species_size("B") == 10 &&
+                                                          f[0].get_species_size("C") == 2 &&
+                                                          f[0].get_species_size("D") == 6 && f[0].id() == " (null)1");
+    }
+    {
+        std::istringstream ist("#A\n#B\n#AB\n#CD\n#C\n#ABCD\n#D\n35\t36\t35\t35\t36\t34\t34\t1\n98\t96\t97\t98\t98\t98\t98\t1\n");
+        std::unique_ptr<clade> t(parse_newick("((A:1,B:1):1,(C:1,D:1):1);"));
+        std::vector<gene_family> f;
+        read_gene_families(ist, t.get(), f);
+        report("read_gene_families_reads_simulation_files",
+               f.size() == 2 && f[0].get_species_size("A") == 35 && f[0].get_species_size("B") == 36 &&
+                   f[0].get_species_size("C") == 36 && f[0].get_species_size("D") == 34 && f[0].id() == "1");
+    }
+    {
+        std::istringstream ist("");
+        std::unique_ptr<clade> t(parse_newick("((A:1,B:1):1,(C:1,D:1):1);"));
+        std::vector<gene_family> f;
+        bool threw = false;
+        try {
+            read_gene_families(ist, t.get(), f);
+        } catch (std::runtime_error& e) {
+            threw = std::string(e.what()) == "No families found";
+        }
+        report("read_gene_families_throws_if_no_families_found", threw);
+    }
+    {
+        std::istringstream ist("Desc\tFamily ID\tA\tB\tC\tD\n\n\n\n\t (null)1\t5\t10\t2\t6\n\n\n\n");
+        std::vector<gene_family> f;
+        read_gene_families(ist, nullptr, f);
+        report("read_gene_families_skips_blank_lines", f.size() == 1);
+    }
+    // test.cpp:1552-1608
+    {
+        std::istringstream ist("maxcnt: 10\ncntdiff: -1 0 1\n0 0.0 0.8 0.2\n1 0.2 0.6 0.2\n2 0.2 0.6 0.2\n"
+                               "3 0.2 0.6 0.2\n5 0.2 0.6 0.2\n");
+        error_model m;
+        read_error_model_file(ist, &m);
+        auto is = [&](size_t k, std::vector<double> want) { return m.get_probs(k) == want; };
+        report("read_error_model", is(0, {0.0, 0.8, 0.2}) && is(1, {0.2, 0.6, 0.2}) && is(4, {0.2, 0.6, 0.2}) &&
+                                       is(7, {0.2, 0.6, 0.2}) && m.deviations() == std::vector<int>{-1, 0, 1});
+    }
+    {
+        error_model m;
+        m.set_probabilities(0, {.0, .7, .3});
+        m.set_probabilities(1, {.2, .6, .2});
+        m.set_probabilities(2, {.1, .8, .1});
+        m.set_probabilities(3, {.2, .6, .2});
+        std::ostringstream ost;
+        write_error_model_file(ost, m);
+        report("write_error_model",
+               ost.str() == "maxcnt: 3\ncntdiff: -1 0 1\n0 0 0.7 0.3\n1 0.2 0.6 0.2\n2 0.1 0.8 0.1\n3 0.2 0.6 0.2\n");
+    }
+    {
+        std::istringstream ist("1 1\n2 5\n3 10\n");
+        auto rd = read_rootdist(ist);
+        report("read_rootdist", rd.size() == 3 && rd[1] == 1 && rd[2] == 5 && rd[3] == 10);
+    }
+    return failures ? 1 : 0;
+}
+
+static int run_example(gpu_context& gpu, char** a, int n)
+{
+    // TREE FAMILIES LAMBDA B R [ERRORMODEL]
+    std::ifstream tf(a[0]);
+    std::string newick((std::istreambuf_iterator<char>(tf)), std::istreambuf_iterator<char>());
+    while (!newick.empty() && std::isspace((unsigned char)newick.back())) newick.pop_back();
+    std::unique_ptr<clade> tree(parse_newick(newick));
+    std::ifstream ff(a[1]);
+    if (!ff) throw std::runtime_error(std::string("cannot open ") + a[1]);
+    std::vector<gene_family> families;
+    read_gene_families(ff, tree.get(), families);
+    single_lambda lam(std::stod(a[2]));
+    const int B = std::stoi(a[3]), R = std::stoi(a[4]);
+    std::unique_ptr<error_model> em;
+    if (n > 5) {
+        std::ifstream ef(a[5]);
+        em.reset(new error_model());
+        read_error_model_file(ef, em.get());
+    }
+    gpu.path = 0;
+    base_model m(&lam, tree.get(), &families, B, R, em.get(), gpu);
+    root_equilibrium_distribution prior((size_t)R);
+    const double total = m.infer_family_likelihoods(prior, &lam);
+    std::cout << std::setprecision(17) << "TOTAL " << total << "\n";
+    for (auto& r : m.get_results()) std::cout << "LNL " << r.family_id << " " << r.posterior_probability << "\n";
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     try {
-        gpu_context gpu(0);
         const std::string mode = argc > 1 ? argv[1] : "kats";
+        if (mode == "io") return run_io();   // no device needed
+        gpu_context gpu(0);
+        if (mode == "example" && argc > 6) return run_example(gpu, argv + 2, argc - 2);
         if (mode == "kats") return run_kats(gpu);
         if (mode == "base" && argc > 2) return run_model(gpu, argv[2]);
         if (mode == "condprob" && argc > 2) return run_condprob(gpu, argv[2]);

@@ -271,3 +271,28 @@ def test_lambda_search_batched32_through_cpp_interface(tmp_path):
     _, _, _, _, (_, _, r64) = _optimize_case(tmp_path, 0)
     assert abs(r32[1] - r64[1]) <= 1e-6 * abs(r64[1])
     assert abs(r32[2][0] - r64[2][0]) <= 1e-3 * r64[2][0]
+
+
+# --------------------------------------------------------------------------
+# CAFE5's example files (tests/golden/examples, from the reference's
+# examples/) read through the C++ readers and evaluated on PLAIN64
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("case", ["plain", "error_model"])
+def test_example_files_through_cpp_interface(case):
+    import json
+    ex = os.path.join(ROOT, "tests", "golden", "examples")
+    meta = json.load(open(os.path.join(ex, "expected.json")))
+    args = [os.path.join(ex, "mammals_tree.txt"), os.path.join(ex, "mammal_gene_families.txt"),
+            repr(meta["lambda"]), str(meta["B"]), str(meta["R"])]
+    if case == "error_model":
+        args.append(os.path.join(ex, "errormodel_0.1.txt"))
+    rc, out, err = run("example", *args)
+    assert rc == 0, err
+    total, lnl, _ = parse(out)
+    ids = [l.split()[1] for l in out.splitlines() if l.startswith("LNL")]
+    want = meta["cases"][case]
+    assert ids == meta["ids"]
+    wl_ = np.array([float(v) for v in want["lnl"]])
+    assert np.all(np.abs(lnl - wl_) <= 1e-9 * np.abs(wl_)), np.max(np.abs(lnl - wl_) / np.abs(wl_))
+    assert abs(total - float(want["total"])) <= 1e-9 * abs(float(want["total"]))
