@@ -686,7 +686,7 @@ MfmaShape mfma_shape_chain(int nt)
     if (nt <= 2) return {std::max(1, nt), 1, false, 0};
     if (nt <= 4) return {1, 4, false, 0};
     if (nt <= 8) return {1, 8, false, 0};
-    if (nt <= 16) return {2, 8, false, 0};
+    if (nt <= 16) return {1, 16, false, 0};
     if (nt <= 32) return {2, 16, false, 0};
     return {4, 16, false, 0};
 }
@@ -700,7 +700,7 @@ void place_partials(MfmaShape& s, bool fp64, int n_slots, int slot_tiles, int n_
     const size_t meta = (size_t)mfma_meta_bytes(n_slots, s.w, !fp64) + (size_t)mfma_count_bytes(n_leaves);
     const size_t frag = (size_t)n_slots * slot_tiles * 64 * 4 * (fp64 ? 8 : 4);
     if (latency)
-        s.lds = s.w <= 8 && meta + frag + 4096 <= 144 * 1024;
+        s.lds = (s.w <= 8 || (s.w == 16 && s.ntw == 1)) && meta + frag + 4096 <= 144 * 1024;
     else
         s.lds = s.w <= 4 && meta + frag + 4096 <= 64 * 1024 && (meta + frag) / s.w <= 20 * 1024;
     s.shm = meta + (s.lds ? frag : 0);

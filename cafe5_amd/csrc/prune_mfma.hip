@@ -625,7 +625,7 @@ void k_prune_mfma(PruneParams p)
 // family tile's chain of steps is as short as the workgroup allows
 #define CAFE_MFMA_CHAIN(X, T, S) \
     X(T, 1, 4, S, true) X(T, 1, 4, S, false) X(T, 1, 8, S, true) X(T, 1, 8, S, false) X(T, 2, 8, S, true) \
-    X(T, 2, 8, S, false) X(T, 2, 16, S, false)
+    X(T, 2, 8, S, false) X(T, 2, 16, S, false) X(T, 1, 16, S, true) X(T, 1, 16, S, false)
 
 // One instance per shape; with per_cu set, reports how many workgroups of it
 // fit on one CU instead of launching.
