@@ -245,6 +245,40 @@ std::vector<double> error_model::get_probs(size_t fam_size) const
     return _dists[fam_size];
 }
 
+std::vector<double> error_model::get_epsilons() const
+{
+    std::set<double> eps;
+    for (const auto& row : _dists) eps.insert(row.back());
+    return std::vector<double>(eps.begin(), eps.end());
+}
+
+void error_model::replace_epsilons(const std::map<double, double>* new_epsilons)
+{
+    for (size_t k = 0; k < _dists.size(); ++k) {
+        std::vector<double> row = _dists[k];
+        if (row.size() != 3) throw std::runtime_error("replace_epsilons needs rows of three deviations");
+        for (const auto& kv : *new_epsilons) {
+            if (!nearly_equal(kv.first, row.back())) continue;
+            row.back() = kv.second;
+            if (k == 0) {
+                row[1] = 1 - kv.second;
+            } else {
+                row.front() = kv.second;
+                row[1] = 1 - kv.second * 2;
+            }
+            set_probabilities(k, row);
+        }
+    }
+}
+
+void error_model::update_single_epsilon(double new_epsilon)
+{
+    const std::vector<double> eps = get_epsilons();
+    if (eps.size() != 1) throw std::runtime_error("update_single_epsilon needs a model with one epsilon");
+    const std::map<double, double> m{{eps[0], new_epsilon}};
+    replace_epsilons(&m);
+}
+
 root_equilibrium_distribution::root_equilibrium_distribution(const std::map<int, int>& rd)
 {
     if (rd.empty()) throw std::runtime_error("No root distribution specified");
@@ -471,20 +505,28 @@ void model::upload()
                                 _bound_R.empty() ? nullptr : _bound_R.data(), _max_family_size,
                                 _max_root_family_size),
           "cafe_gpu_set_families");
-    if (_p_error_model) {
-        const int nd = (int)_p_error_model->n_deviations();
-        std::vector<double> rows((size_t)(_max_family_size + 1) * nd);
-        std::vector<double> pr;
-        for (int k = 0; k <= _max_family_size; ++k) {
-            // rows past the model's range repeat its last one (the reference
-            // reads only observed counts' rows and leaves those undefined)
-            if (_p_error_model->covers((size_t)k) || pr.empty()) pr = _p_error_model->get_probs(k);
-            std::copy(pr.begin(), pr.end(), rows.begin() + (size_t)k * nd);
-        }
-        check(cafe_gpu_set_error_model(_gpu.get(), nd, _max_family_size + 1, rows.data()), "cafe_gpu_set_error_model");
-    } else {
+    const std::vector<double> rows = error_rows();
+    if (!rows.empty())
+        check(cafe_gpu_set_error_model(_gpu.get(), (int)_p_error_model->n_deviations(), _max_family_size + 1,
+                                       rows.data()),
+              "cafe_gpu_set_error_model");
+    else
         check(cafe_gpu_set_error_model(_gpu.get(), 0, 0, nullptr), "cafe_gpu_set_error_model");
+}
+
+std::vector<double> model::error_rows() const
+{
+    if (!_p_error_model) return {};
+    const int nd = (int)_p_error_model->n_deviations();
+    std::vector<double> rows((size_t)(_max_family_size + 1) * nd);
+    std::vector<double> pr;
+    for (int k = 0; k <= _max_family_size; ++k) {
+        // rows past the model's range repeat its last one (the reference
+        // reads only observed counts' rows and leaves those undefined)
+        if (_p_error_model->covers((size_t)k) || pr.empty()) pr = _p_error_model->get_probs(k);
+        std::copy(pr.begin(), pr.end(), rows.begin() + (size_t)k * nd);
     }
+    return rows;
 }
 
 std::vector<int> model::node_slots(matrix_cache& cache, const lambda* lam, const std::vector<double>& multipliers)
@@ -1143,7 +1185,12 @@ double resident_base_model::infer_family_likelihoods(const root_equilibrium_dist
             }
     };
     std::vector<double> lq, tq;
-    if (!_plan || _gpu.owner != this || pr != _plan_prior || (int)lams.size() != _n_lambda) {
+    // new error-model values of the same shape (an epsilon search) go to the
+    // device in place: the plan folds them into its pools at every execution
+    std::vector<double> err = error_rows();
+    const bool err_in_place = err.size() == _plan_err.size() && !err.empty();
+    if (!_plan || _gpu.owner != this || pr != _plan_prior || (int)lams.size() != _n_lambda ||
+        (err != _plan_err && !err_in_place)) {
         if (_plan) {
             (void)cafe_gpu_plan_destroy(_plan);
             _plan = nullptr;
@@ -1165,6 +1212,7 @@ double resident_base_model::infer_family_likelihoods(const root_equilibrium_dist
             slots.push_back(_p_lambda->index_of_clade(c) * (int)_t.size() + j);
         }
         _plan_prior = pr;
+        _plan_err = error_rows();
         cafe_gpu_eval_desc d{CAFE_GPU_MODEL_BASE, _gpu.path, 1, 1, slots.data(), nullptr, _plan_prior.data(),
                              _max_root_family_size};
         keys(lq, tq);
@@ -1177,6 +1225,12 @@ double resident_base_model::infer_family_likelihoods(const root_equilibrium_dist
     } else {
         keys(lq, tq);
         if (!lq.empty()) check(cafe_gpu_plan_set_keys(_plan, lq.data(), tq.data()), "cafe_gpu_plan_set_keys");
+        if (err != _plan_err) {
+            check(cafe_gpu_set_error_model(_gpu.get(), (int)_p_error_model->n_deviations(), _max_family_size + 1,
+                                           err.data()),
+                  "cafe_gpu_set_error_model");
+            _plan_err.swap(err);
+        }
     }
     check(cafe_gpu_plan_execute(_plan), "cafe_gpu_plan_execute");
     const size_t F = _p_gene_families->size();
@@ -1382,6 +1436,34 @@ double lambda_optimizer::calculate_score(const double* values)
     _p_lambda->update(values);
     const double score = _p_model->infer_family_likelihoods(*_p_distribution, _p_lambda);
     return std::isnan(score) ? -std::log(0.0) : score;
+}
+
+std::vector<double> lambda_epsilon_optimizer::initial_guesses()
+{
+    std::vector<double> out = _lambda_optimizer.initial_guesses();
+    _current = _p_error_model->get_epsilons();
+    out.insert(out.end(), _current.begin(), _current.end());
+    return out;
+}
+
+double lambda_epsilon_optimizer::calculate_score(const double* values)
+{
+    const double* eps = values + _p_lambda->count();
+    _p_lambda->update(values);
+    std::map<double, double> replacements;
+    for (size_t i = 0; i < _current.size(); ++i) {
+        replacements[_current[i]] = eps[i];
+        _current[i] = eps[i];
+    }
+    _p_error_model->replace_epsilons(&replacements);
+    const double score = _p_model->infer_family_likelihoods(*_p_distribution, _p_lambda);
+    return std::isnan(score) ? -std::log(0.0) : score;
+}
+
+void lambda_epsilon_optimizer::finalize(double* results)
+{
+    _lambda_optimizer.finalize(results);
+    _p_error_model->update_single_epsilon(results[_p_lambda->count()]);
 }
 
 }  // namespace cafe5_gpu

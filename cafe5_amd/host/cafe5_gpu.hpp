@@ -197,6 +197,17 @@ public:
     size_t get_max_family_size() const { return _dists.size(); }
     // get_probs(k) is defined: a row was set for k, or k is within maxcnt
     bool covers(size_t k) const { return !_dists.empty() && (k < _dists.size() || k <= _max_family_size); }
+    // the distinct last entries (the epsilons) of the rows, ascending
+    // (error_model.cpp:59-67)
+    std::vector<double> get_epsilons() const;
+    // every row whose epsilon is within 1% of a key takes the key's value:
+    // row 0 becomes {p0, 1 - e, e}, the others {e, 1 - 2e, e}, through
+    // set_probabilities; keys are tried in ascending order against the row as
+    // already replaced (error_model.cpp:79-109).  Rows must have three
+    // deviations (std::runtime_error otherwise).
+    void replace_epsilons(const std::map<double, double>* new_epsilons);
+    // a model with one epsilon takes new_epsilon (error_model.cpp:69-77)
+    void update_single_epsilon(double new_epsilon);
 
 private:
     size_t _max_family_size = 0;
@@ -296,6 +307,9 @@ public:
     // engine plumbing: upload tree, families and error model; map every
     // branch of every (lambda x multiplier) point to its matrix slot
     void upload();
+    // the error model's rows for counts 0..max_family_size as the engine takes
+    // them (empty without an error model)
+    std::vector<double> error_rows() const;
     // per-family (max_family_size, max_root_family_size) for the engine's size
     // buckets; empty: the model's bounds for every family
     void set_family_bounds(std::vector<int> bound_B, std::vector<int> bound_R);
@@ -369,6 +383,7 @@ public:
 private:
     ::cafe_gpu_plan* _plan = nullptr;
     std::vector<double> _plan_prior;
+    std::vector<double> _plan_err;   // error-model rows on the device
     std::vector<double> _t;   // the tree's branch lengths, key order
     int _n_lambda = 0;
 };
@@ -548,6 +563,35 @@ private:
     model* _p_model;
     const root_equilibrium_distribution* _p_distribution;
     double _longest_branch;
+};
+
+// lambda_epsilon_optimizer (optimizer_scorer.h:82-107, optimizer_scorer.cpp:
+// 72-108): the lambdas followed by the error model's epsilons.  A score puts
+// the lambdas in place and replaces each epsilon of the previous call by the
+// new one (error_model::replace_epsilons); finalize keeps the best lambdas and
+// the single epsilon.  root_distribution_map is unused, as in the reference.
+class lambda_epsilon_optimizer : public optimizer_scorer {
+public:
+    lambda_epsilon_optimizer(model* p_model, error_model* p_error_model,
+                             const root_equilibrium_distribution* p_distribution,
+                             const std::map<int, int>& root_distribution_map, lambda* p_lambda, double longest_branch)
+        : _lambda_optimizer(p_lambda, p_model, p_distribution, longest_branch), _p_error_model(p_error_model),
+          _p_lambda(p_lambda), _p_model(p_model), _p_distribution(p_distribution)
+    {
+        (void)root_distribution_map;
+    }
+    // the lambda guesses, then the model's current epsilons
+    std::vector<double> initial_guesses() override;
+    double calculate_score(const double* values) override;
+    void finalize(double* results) override;
+
+private:
+    lambda_optimizer _lambda_optimizer;
+    error_model* _p_error_model;
+    lambda* _p_lambda;
+    model* _p_model;
+    const root_equilibrium_distribution* _p_distribution;
+    std::vector<double> _current;   // the epsilons in the model now
 };
 
 }  // namespace cafe5_gpu

@@ -396,6 +396,29 @@ static int run_kats(gpu_context& gpu)
         report("optimizer_disallows_bad_initializations", threw);
     }
     {
+        // test.cpp:2212-2234: a score replaces the previous epsilon by the
+        // candidate's (the model's own score is not used)
+        struct mock_model : model {
+            using model::model;
+            double infer_family_likelihoods(const root_equilibrium_distribution&, const lambda*) override { return 0; }
+            void write_family_likelihoods(std::ostream&) override {}
+        };
+        const double initial_epsilon = 0.01;
+        error_model err;
+        err.set_probabilities(0, {.0, .99, initial_epsilon});
+        err.set_probabilities(1, {initial_epsilon, .98, initial_epsilon});
+        single_lambda lambda(0.05);
+        mock_model mm(&lambda, nullptr, nullptr, 10, 10, &err, gpu);
+        lambda_epsilon_optimizer opt(&mm, &err, nullptr, std::map<int, int>(), &lambda, 10);
+        opt.initial_guesses();
+        std::vector<double> values{0.05, 0.06};
+        opt.calculate_score(values.data());
+        const bool first = err.get_probs(0) == std::vector<double>({0, .94, .06});
+        values[1] = 0.04;
+        opt.calculate_score(values.data());
+        report("lambda_epsilon_optimizer", first && err.get_probs(0) == std::vector<double>({0, .96, .04}));
+    }
+    {
         std::vector<double> f(4), r(4);
         get_gamma(f, r, 0.5);
         std::cout << std::setprecision(17) << "GAMMA 4 0.5";
@@ -594,6 +617,40 @@ static int run_optimize(gpu_context& gpu, const std::string& file)
     return 0;
 }
 
+// lambda and epsilon together (lambda_epsilon_optimizer) on a device-resident
+// base model with the symmetric +-1 error model {0, 1-e, e}, {e, 1-2e, e}
+static int run_optimize_eps(gpu_context& gpu, const std::string& file, double epsilon)
+{
+    pvalue_input d = read_pvalue_input(file);
+    gpu.build_algo = d.algo;
+    std::unique_ptr<clade> tree(parse_newick(d.newick));
+    single_lambda lam(d.lambda);
+    error_model em;
+    em.set_max_family_size(d.B);
+    em.set_probabilities(0, {0, 1 - epsilon, epsilon});
+    em.set_probabilities(1, {epsilon, 1 - 2 * epsilon, epsilon});
+    resident_base_model model(&lam, tree.get(), &d.families, d.B, d.R, &em, gpu);
+    root_equilibrium_distribution prior((size_t)d.R);
+    double longest = 0;
+    for (double t : tree->get_branch_lengths()) longest = std::max(longest, t);
+    lambda_epsilon_optimizer scorer(&model, &em, &prior, std::map<int, int>(), &lam, longest);
+    logging_scorer log(scorer);
+    std::cout << std::setprecision(17);
+    randomizer_engine.seed(d.seed);
+    optimizer opt(&log);
+    optimizer_parameters params;
+    auto r = opt.optimize(params);
+    std::cout << "RESULT " << r.num_iterations << " " << r.score;
+    for (double v : r.values) std::cout << " " << v;
+    std::cout << "\n";
+    scorer.finalize(r.values.data());   // execute.cpp:101
+    std::cout << "FINAL " << lam.get_value_for_clade(tree.get());
+    for (double e : em.get_epsilons()) std::cout << " " << e;
+    for (double v : em.get_probs(1)) std::cout << " " << v;
+    std::cout << "\n";
+    return 0;
+}
+
 static int run_pvalues(gpu_context& gpu, const std::string& file)
 {
     pvalue_input d = read_pvalue_input(file);
@@ -695,6 +752,32 @@ static int run_io()
         auto rd = read_rootdist(ist);
         report("read_rootdist", rd.size() == 3 && rd[1] == 1 && rd[2] == 5 && rd[3] == 10);
     }
+    {
+        // test.cpp:1483-1495
+        error_model m;
+        m.set_probabilities(0, {.0, .7, .3});
+        m.set_probabilities(1, {.2, .6, .2});
+        m.set_probabilities(2, {.1, .8, .1});
+        m.set_probabilities(3, {.2, .6, .2});
+        report("error_model_get_epsilon", m.get_epsilons() == std::vector<double>({.1, .2, .3}));
+    }
+    {
+        // test.cpp:1527-1549
+        std::istringstream ist("maxcnt: 10\ncntdiff: -1 0 1\n0 0.0 0.8 0.2\n1 0.2 0.6 0.2\n2 0.2 0.6 0.2\n");
+        error_model m;
+        read_error_model_file(ist, &m);
+        std::map<double, double> rep{{.2, .3}};
+        m.replace_epsilons(&rep);
+        const auto p0 = m.get_probs(0), p1 = m.get_probs(1);
+        // a single epsilon: update_single_epsilon replaces it everywhere
+        error_model u;
+        u.set_probabilities(0, {0, .9, .1});
+        u.set_probabilities(1, {.1, .8, .1});
+        u.update_single_epsilon(.05);
+        report("error_model_replace_epsilons", p0[1] == .7 && p0[2] == .3 && p1[0] == .3 && p1[1] == .4 &&
+                                                   p1[2] == .3 && u.get_epsilons() == std::vector<double>({.05}) &&
+                                                   u.get_probs(1)[1] == 1 - .05 * 2);
+    }
     return failures ? 1 : 0;
 }
 
@@ -741,7 +824,12 @@ int main(int argc, char** argv)
             if (argc > 3) gpu.path = std::atoi(argv[3]);
             return run_optimize(gpu, argv[2]);
         }
-        std::cerr << "usage: adapter_test kats | base FILE | condprob FILE | pvalues FILE | optimize FILE [path]"
+        if (mode == "optimize_eps" && argc > 3) {
+            if (argc > 4) gpu.path = std::atoi(argv[4]);
+            return run_optimize_eps(gpu, argv[2], std::atof(argv[3]));
+        }
+        std::cerr << "usage: adapter_test kats | base FILE | condprob FILE | pvalues FILE | optimize FILE [path] | "
+                     "optimize_eps FILE EPSILON [path]"
                   << std::endl;
         return 2;
     } catch (std::exception& e) {

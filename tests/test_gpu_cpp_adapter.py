@@ -26,7 +26,8 @@ def test_reference_known_answers_through_cpp_interface():
     lines = out.splitlines()
     fails = [l for l in lines if l.startswith("FAIL")]
     assert rc == 0 and not fails, (fails, err)
-    assert sum(l.startswith("PASS") for l in lines) >= 14
+    assert sum(l.startswith("PASS") for l in lines) >= 15
+    assert "PASS lambda_epsilon_optimizer" in lines
     gamma = [l for l in lines if l.startswith("GAMMA 4 0.5")][0].split()[3:]
     f, r = Oracle.get_gamma(4, 0.5)
     assert np.array_equal(np.array([float(x) for x in gamma]), r)
@@ -271,6 +272,57 @@ def test_lambda_search_batched32_through_cpp_interface(tmp_path):
     _, _, _, _, (_, _, r64) = _optimize_case(tmp_path, 0)
     assert abs(r32[1] - r64[1]) <= 1e-6 * abs(r64[1])
     assert abs(r32[2][0] - r64[2][0]) <= 1e-3 * r64[2][0]
+
+
+def test_lambda_epsilon_search_through_cpp_interface(tmp_path):
+    """lambda_epsilon_optimizer (optimizer_scorer.cpp:72-108) on PLAIN64 over
+    a device-resident model: the search visits the (lambda, epsilon) points
+    the oracle's Nelder-Mead visits, each scored by the oracle with the error
+    model replace_epsilons leaves ({0, 1-e, e} at size 0, {e, 1-2e, e} above),
+    and finalize puts the best epsilon into every row."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import nelder_mead as nmr
+
+    w = wl.config_base(seed=6, F=100)
+    B = R = 50
+    counts = np.minimum(w.counts, B)
+    f = tmp_path / "opt_eps.txt"
+    write_pvalue_input(f, w.tree, counts, 0.002, B, R, seed=31)
+    rc, out, err = run("optimize_eps", str(f), "0.05", "0", timeout=300)
+    assert rc == 0, err
+    start, evals, result = _parse_optimize(out)
+    final = [float(x) for x in [l for l in out.splitlines() if l.startswith("FINAL")][0].split()[1:]]
+    prior = Oracle.prior_uniform(R, R)
+    assert len(start) == 2 and start[1] == 0.05
+
+    def oracle_score(x):
+        if not x[0] > 0:                                  # lambda.h: an invalid lambda scores +inf
+            return float("inf")
+        v = Oracle.evaluate(w.tree, np.full(w.tree.n_nodes, x[0]), counts, B, R, prior,
+                            err=wl.error_table(x[1], B), threads=8)["total"]
+        return float("inf") if np.isnan(v) else v
+
+    seen = []
+
+    def logged(x):
+        seen.append((x[0], x[1], oracle_score(x)))
+        return seen[-1][2]
+
+    vals, score, iters = nmr.optimize(logged, start)
+    assert np.array_equal(evals[0][:2], start)
+    got = evals[1:]
+    want = np.array(seen)
+    assert len(got) == len(want), (len(got), len(want))
+    assert np.array_equal(got[:, :2], want[:, :2])
+    fin = np.isfinite(want[:, 2])
+    assert np.array_equal(np.isfinite(got[:, 2]), fin)
+    assert np.all(np.abs(got[fin, 2] - want[fin, 2]) <= 1e-9 * np.abs(want[fin, 2]))
+    assert result[0] == iters and result[2] == vals
+    assert abs(result[1] - score) <= 1e-9 * abs(score)
+    # finalize: the best lambda, one epsilon, rows {e, 1-2e, e}
+    assert final[0] == vals[0] and final[1] == vals[1]
+    assert final[2:] == [vals[1], 1 - 2 * vals[1], vals[1]]
 
 
 # --------------------------------------------------------------------------
