@@ -1466,4 +1466,39 @@ void lambda_epsilon_optimizer::finalize(double* results)
     _p_error_model->update_single_epsilon(results[_p_lambda->count()]);
 }
 
+std::vector<double> gamma_optimizer::initial_guesses()
+{
+    std::gamma_distribution<double> draw(4.0, 0.25);
+    return {draw(randomizer_engine)};
+}
+
+double gamma_optimizer::calculate_score(const double* values)
+{
+    _p_gamma_model->set_alpha(*values);
+    const double score = _p_gamma_model->infer_family_likelihoods(*_p_distribution, _p_gamma_model->get_lambda());
+    return std::isnan(score) ? -std::log(0.0) : score;
+}
+
+std::vector<double> gamma_lambda_optimizer::initial_guesses()
+{
+    std::vector<double> out = _lambda_optimizer.initial_guesses();
+    const std::vector<double> alpha = _gamma_optimizer.initial_guesses();
+    out.insert(out.end(), alpha.begin(), alpha.end());
+    return out;
+}
+
+double gamma_lambda_optimizer::calculate_score(const double* values)
+{
+    _p_lambda->update(values);
+    _p_model->set_alpha(values[_p_lambda->count()]);
+    const double score = _p_model->infer_family_likelihoods(*_p_distribution, _p_lambda);
+    return std::isnan(score) ? -std::log(0.0) : score;
+}
+
+void gamma_lambda_optimizer::finalize(double* results)
+{
+    _lambda_optimizer.finalize(results);
+    _gamma_optimizer.finalize(results + _p_lambda->count());
+}
+
 }  // namespace cafe5_gpu

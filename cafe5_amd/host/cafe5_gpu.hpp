@@ -594,4 +594,43 @@ private:
     std::vector<double> _current;   // the epsilons in the model now
 };
 
+// gamma_optimizer (optimizer_scorer.h:111-130, optimizer_scorer.cpp:111-145):
+// the gamma shape alpha of a gamma model at its own lambda
+class gamma_optimizer : public optimizer_scorer {
+public:
+    gamma_optimizer(gamma_model* p_model, const root_equilibrium_distribution* prior)
+        : _p_gamma_model(p_model), _p_distribution(prior) {}
+    // one draw of Gamma(shape 4, scale 0.25) (mean 1) from randomizer_engine
+    std::vector<double> initial_guesses() override;
+    // set_alpha, then -lnL of the model, NaN read as +inf
+    double calculate_score(const double* values) override;
+    void finalize(double* result) override { _p_gamma_model->set_alpha(*result); }
+    double get_alpha() const { return _p_gamma_model->get_alpha(); }
+
+private:
+    gamma_model* _p_gamma_model;
+    const root_equilibrium_distribution* _p_distribution;
+};
+
+// gamma_lambda_optimizer (optimizer_scorer.h:132-152, optimizer_scorer.cpp:
+// 147-180): the lambdas followed by alpha; guesses in that order (the lambda
+// draws first)
+class gamma_lambda_optimizer : public optimizer_scorer {
+public:
+    gamma_lambda_optimizer(lambda* p_lambda, gamma_model* p_model, const root_equilibrium_distribution* p_distribution,
+                           double longest_branch)
+        : _lambda_optimizer(p_lambda, p_model, p_distribution, longest_branch), _gamma_optimizer(p_model, p_distribution),
+          _p_lambda(p_lambda), _p_model(p_model), _p_distribution(p_distribution) {}
+    std::vector<double> initial_guesses() override;
+    double calculate_score(const double* values) override;
+    void finalize(double* results) override;
+
+private:
+    lambda_optimizer _lambda_optimizer;
+    gamma_optimizer _gamma_optimizer;
+    lambda* _p_lambda;
+    gamma_model* _p_model;
+    const root_equilibrium_distribution* _p_distribution;
+};
+
 }  // namespace cafe5_gpu

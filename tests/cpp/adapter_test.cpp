@@ -419,6 +419,36 @@ static int run_kats(gpu_context& gpu)
         report("lambda_epsilon_optimizer", first && err.get_probs(0) == std::vector<double>({0, .96, .04}));
     }
     {
+        // test.cpp:777-792, :794-800: one guess per lambda, then alpha
+        single_lambda sl(0.05);
+        gamma_model m4(nullptr, nullptr, nullptr, 0, 5, 4, .25, nullptr, gpu);
+        gamma_lambda_optimizer glo(&sl, &m4, nullptr, 5);
+        const auto g = glo.initial_guesses();
+        report("gamma_lambda_optimizer__provides_two_guesses",
+               g.size() == 2 && g[0] > 0 && g[0] < 1 && g[1] > 0 && g[1] < 10);
+        gamma_model m0(nullptr, nullptr, nullptr, 0, 0, 0, 0, nullptr, gpu);
+        gamma_optimizer go(&m0, nullptr);
+        report("gamma_optimizer__creates_single_initial_guess", go.initial_guesses().size() == 1);
+    }
+    {
+        // test.cpp:2317-2332: a score puts lambda and alpha into the model
+        std::unique_ptr<clade> tree(parse_newick("(A:1,B:1);"));
+        single_lambda lam(0.05);
+        std::vector<gene_family> fams(1);
+        fams[0].set_id("TestFamily1");
+        fams[0].set_species_size("A", 1);
+        fams[0].set_species_size("B", 2);
+        root_equilibrium_distribution prior((size_t)10);
+        gamma_model m(&lam, tree.get(), &fams, 10, 10, std::vector<double>{0.3, 0.7}, std::vector<double>{0.5, 1.5},
+                      nullptr, gpu);
+        gamma_lambda_optimizer opt(&lam, &m, &prior, 7);
+        std::vector<double> values{0.01, 0.25};
+        const double score = opt.calculate_score(values.data());
+        report("gamma_lambda_optimizer_updates_model_alpha_and_lambda",
+               std::fabs(m.get_alpha() - 0.25) < 1e-5 && std::fabs(lam.get_single_lambda() - 0.01) < 1e-5 &&
+                   std::isfinite(score));
+    }
+    {
         std::vector<double> f(4), r(4);
         get_gamma(f, r, 0.5);
         std::cout << std::setprecision(17) << "GAMMA 4 0.5";
@@ -651,6 +681,35 @@ static int run_optimize_eps(gpu_context& gpu, const std::string& file, double ep
     return 0;
 }
 
+// lambda and alpha together (gamma_lambda_optimizer) on a K-category gamma
+// model
+static int run_optimize_gamma(gpu_context& gpu, const std::string& file, int K)
+{
+    pvalue_input d = read_pvalue_input(file);
+    gpu.build_algo = d.algo;
+    std::unique_ptr<clade> tree(parse_newick(d.newick));
+    single_lambda lam(d.lambda);
+    gamma_model model(&lam, tree.get(), &d.families, d.B, d.R, K, 1.0, nullptr, gpu);
+    root_equilibrium_distribution prior((size_t)d.R);
+    double longest = 0;
+    for (double t : tree->get_branch_lengths()) longest = std::max(longest, t);
+    gamma_lambda_optimizer scorer(&lam, &model, &prior, longest);
+    logging_scorer log(scorer);
+    std::cout << std::setprecision(17);
+    randomizer_engine.seed(d.seed);
+    optimizer opt(&log);
+    optimizer_parameters params;
+    auto r = opt.optimize(params);
+    std::cout << "RESULT " << r.num_iterations << " " << r.score;
+    for (double v : r.values) std::cout << " " << v;
+    std::cout << "\n";
+    scorer.finalize(r.values.data());
+    std::cout << "FINAL " << lam.get_single_lambda() << " " << model.get_alpha();
+    for (double v : model.get_lambda_multipliers()) std::cout << " " << v;
+    std::cout << "\n";
+    return 0;
+}
+
 static int run_pvalues(gpu_context& gpu, const std::string& file)
 {
     pvalue_input d = read_pvalue_input(file);
@@ -824,12 +883,16 @@ int main(int argc, char** argv)
             if (argc > 3) gpu.path = std::atoi(argv[3]);
             return run_optimize(gpu, argv[2]);
         }
+        if (mode == "optimize_gamma" && argc > 3) {
+            if (argc > 4) gpu.path = std::atoi(argv[4]);
+            return run_optimize_gamma(gpu, argv[2], std::atoi(argv[3]));
+        }
         if (mode == "optimize_eps" && argc > 3) {
             if (argc > 4) gpu.path = std::atoi(argv[4]);
             return run_optimize_eps(gpu, argv[2], std::atof(argv[3]));
         }
         std::cerr << "usage: adapter_test kats | base FILE | condprob FILE | pvalues FILE | optimize FILE [path] | "
-                     "optimize_eps FILE EPSILON [path]"
+                     "optimize_eps FILE EPSILON [path] | optimize_gamma FILE K [path]"
                   << std::endl;
         return 2;
     } catch (std::exception& e) {

@@ -26,8 +26,11 @@ def test_reference_known_answers_through_cpp_interface():
     lines = out.splitlines()
     fails = [l for l in lines if l.startswith("FAIL")]
     assert rc == 0 and not fails, (fails, err)
-    assert sum(l.startswith("PASS") for l in lines) >= 15
-    assert "PASS lambda_epsilon_optimizer" in lines
+    assert sum(l.startswith("PASS") for l in lines) >= 18
+    for name in ("lambda_epsilon_optimizer", "gamma_lambda_optimizer__provides_two_guesses",
+                 "gamma_optimizer__creates_single_initial_guess",
+                 "gamma_lambda_optimizer_updates_model_alpha_and_lambda"):
+        assert "PASS " + name in lines
     gamma = [l for l in lines if l.startswith("GAMMA 4 0.5")][0].split()[3:]
     f, r = Oracle.get_gamma(4, 0.5)
     assert np.array_equal(np.array([float(x) for x in gamma]), r)
@@ -323,6 +326,62 @@ def test_lambda_epsilon_search_through_cpp_interface(tmp_path):
     # finalize: the best lambda, one epsilon, rows {e, 1-2e, e}
     assert final[0] == vals[0] and final[1] == vals[1]
     assert final[2:] == [vals[1], 1 - 2 * vals[1], vals[1]]
+
+
+def test_lambda_alpha_search_through_cpp_interface(tmp_path):
+    """gamma_lambda_optimizer (optimizer_scorer.cpp:111-180) on PLAIN64 over
+    a 4-category gamma model: the search visits the (lambda, alpha) points
+    the oracle's Nelder-Mead visits, each scored by the oracle with the
+    categories get_gamma gives for alpha (gamma_core.cpp:61-67) and the
+    gamma model's validity rules (can_infer, gamma_core.cpp:123-141), and
+    finalize leaves the best lambda and alpha in the model."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import nelder_mead as nmr
+
+    K = 4
+    w = wl.config_gamma(seed=3, F=100, K=K, bound=50)
+    B = R = 50
+    f = tmp_path / "opt_gamma.txt"
+    write_pvalue_input(f, w.tree, w.counts, 0.003, B, R, seed=12)
+    rc, out, err = run("optimize_gamma", str(f), str(K), "0", timeout=300)
+    assert rc == 0, err
+    start, evals, result = _parse_optimize(out)
+    final = [float(x) for x in [l for l in out.splitlines() if l.startswith("FINAL")][0].split()[1:]]
+    prior = Oracle.prior_uniform(R, R)
+    longest = float(np.max(w.tree.length))
+
+    def oracle_score(x):
+        lam, alpha = x
+        if not lam > 0 or alpha < 0:
+            return float("inf")
+        a = lam * longest / (1 + lam * longest)           # matrix_cache::is_saturated
+        if 1 - 2 * a < 0:
+            return float("inf")
+        fq, r = Oracle.get_gamma(K, alpha)
+        v = Oracle.evaluate(w.tree, np.full(w.tree.n_nodes, lam), w.counts, B, R, prior, n_cat=K, multipliers=r,
+                            cat_probs=fq, threads=8)["total"]
+        return float("inf") if np.isnan(v) else v
+
+    seen = []
+
+    def logged(x):
+        seen.append((x[0], x[1], oracle_score(x)))
+        return seen[-1][2]
+
+    vals, score, iters = nmr.optimize(logged, start)
+    assert np.array_equal(evals[0][:2], start)
+    got = evals[1:]
+    want = np.array(seen)
+    assert len(got) == len(want), (len(got), len(want))
+    assert np.array_equal(got[:, :2], want[:, :2])
+    fin = np.isfinite(want[:, 2])
+    assert np.array_equal(np.isfinite(got[:, 2]), fin)
+    assert np.all(np.abs(got[fin, 2] - want[fin, 2]) <= 1e-9 * np.abs(want[fin, 2]))
+    assert result[0] == iters and result[2] == vals
+    assert abs(result[1] - score) <= 1e-9 * abs(score)
+    assert final[:2] == vals
+    assert np.array_equal(np.array(final[2:]), Oracle.get_gamma(K, vals[1])[1])
 
 
 # --------------------------------------------------------------------------
