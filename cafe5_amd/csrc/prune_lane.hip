@@ -145,16 +145,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LaneWaves<LR
                 const StepDev sd = ld_const(p.steps, st++);
                 root = (sd.ctl & kStepRoot) != 0;
                 out = sd.ctl & kStepSlotMask;
-                if (sd.ctl & kStepFirst) {
-                    // a non-root product starts from the row mask of the bound
-                    // (rows past B are 0 and stay 0); root rows are sizes 0..R
-                    if (root || bk.L >= LR) {
-#pragma unroll
-                        for (int i = 0; i < LR; ++i) acc[i] = 1.f;
-                    } else {
-#pragma unroll
-                        for (int i = 0; i < LR; ++i) acc[i] = i < bk.L ? 1.f : 0.f;
-                    }
+                // the node's first child sets the product (1 x factor, exactly),
+                // later children multiply into it
+                const bool first = (sd.ctl & kStepFirst) != 0;
+                if (first) {
                     e_sum = 0;
                     log2S = 0;
                     cert = 0.0;
@@ -171,12 +165,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LaneWaves<LR
                     // multiple of 32) and holds Np >= 4 ceil(LR / 4) entries
                     // (Np >= 32 always; 33-row classes have N >= 22, Np >= 64)
                     typedef float f4 __attribute__((ext_vector_type(4)));
+                    float lv[LR];
 #pragma unroll
                     for (int q = 0; q < (LR + 3) / 4; ++q) {
                         const f4 c4 = *(const f4*)(col + 4 * q);
 #pragma unroll
                         for (int r = 0; r < 4; ++r)
-                            if (4 * q + r < LR) acc[4 * q + r] *= c4[r];
+                            if (4 * q + r < LR) lv[4 * q + r] = c4[r];
+                    }
+                    if (first) {
+#pragma unroll
+                        for (int i = 0; i < LR; ++i) acc[i] = lv[i];
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < LR; ++i) acc[i] *= lv[i];
                     }
                 } else {
                     // interior child in slot s: x = M' v, column k of M' from the
@@ -184,8 +186,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LaneWaves<LR
                     const SlotWeights<NS> sw(sd.in_slot);
                     const __attribute__((address_space(4))) float* M = as_const(p.cm32 + base);
                     float x[LR];
-    #pragma unroll
-                    for (int i = 0; i < LR; ++i) x[i] = 0.f;
                     // columns through scalar-register buffers: the loads of the
                     // next D columns are in flight while column k is used
                     // (D = 2 for 21 rows, 1 for larger vectors: SGPR budget)
@@ -204,15 +204,28 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LaneWaves<LR
                             for (int i = 0; i < LR; ++i) nxt[i] = M[(k + D) * Np + i];
                         }
                         const float vk = pick(sv, sw, k);
+                        // column 0 starts the sums with a product: entries are
+                        // finite and >= 0, so this equals fma(., ., +0)
 #pragma unroll
-                        for (int i = 0; i < LR; ++i) x[i] = __builtin_fmaf(cur[i], vk, x[i]);
+                        for (int i = 0; i < LR; ++i) x[i] = k == 0 ? cur[i] * vk : __builtin_fmaf(cur[i], vk, x[i]);
                         __builtin_amdgcn_sched_barrier(0);   // no loads further ahead than column k + D
                     }
                     e_sum += pick(se, sw) - kMatExp;
                     cert += pick(sc, sw) + cint;
                     log2S += kMatExp + a_exp;
-    #pragma unroll
-                    for (int i = 0; i < LR; ++i) acc[i] *= x[i];
+                    if (first) {
+#pragma unroll
+                        for (int i = 0; i < LR; ++i) acc[i] = x[i];
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < LR; ++i) acc[i] *= x[i];
+                    }
+                }
+                // a non-root product keeps rows past the bound at 0 (they
+                // stay 0); root rows are sizes 0..R
+                if (first && !root && bk.L < LR) {
+#pragma unroll
+                    for (int i = 0; i < LR; ++i) acc[i] = i < bk.L ? acc[i] : 0.f;
                 }
                 if ((sd.ctl & kStepLast) || st >= n_steps) break;
             }
