@@ -33,25 +33,39 @@
 namespace cafe {
 
 // Slot access with a wave-uniform slot number s.  The register arrays are
-// indexed by constants only: slot k-entries are blended with uniform 0/1
-// weights (stored entries are finite, so the blend is exact).  Selects would
-// be turned by the compiler into one load at a variable index, which moves
-// the arrays to scratch memory.
+// indexed by constants only: a vector entry is selected bitwise under uniform
+// masks, the scalar bookkeeping is blended with uniform 0/1 weights (stored
+// values are finite, so the blend is exact).  Plain selects would be turned
+// by the compiler into one load at a variable index, which moves the arrays
+// to scratch memory.
 template <int NS>
 struct SlotWeights {
     float w[NS];
+    unsigned m[NS];   // all ones for slot s, else zero
     __device__ __forceinline__ explicit SlotWeights(int s)
     {
 #pragma unroll
-        for (int c = 0; c < NS; ++c) w[c] = s == c ? 1.f : 0.f;
+        for (int c = 0; c < NS; ++c) {
+            w[c] = s == c ? 1.f : 0.f;
+            m[c] = s == c ? ~0u : 0u;
+        }
     }
 };
+// (m & a) | (~m & b) with a wave-uniform mask: one bitfield insert per slot
+// past the first.  The asm operand keeps the compiler from rewriting the
+// chain as a select at a variable index.
+__device__ __forceinline__ float bit_select(unsigned m, float a, float b)
+{
+    float r;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "s"(m), "v"(a), "v"(b));
+    return r;
+}
 template <int NS, int LR>
 __device__ __forceinline__ float pick(const float (&sv)[NS][LR], const SlotWeights<NS>& sw, int k)
 {
-    float r = sv[0][k] * sw.w[0];
+    float r = sv[0][k];
 #pragma unroll
-    for (int c = 1; c < NS; ++c) r = __builtin_fmaf(sv[c][k], sw.w[c], r);
+    for (int c = 1; c < NS; ++c) r = bit_select(sw.m[c], sv[c][k], r);
     return r;
 }
 template <int NS, typename V>
