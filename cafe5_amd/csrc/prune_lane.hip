@@ -17,7 +17,7 @@
 // 2.3x), and a SIMD holds 64 families per wave.
 //
 // Partial vectors are in registers, indexed by the program's slot numbers
-// through arithmetic blends with wave-uniform 0/1 weights (per-slot copies of
+// through selects under wave-uniform masks and weights (per-slot copies of
 // the code under wave-uniform branches run ~6% faster alone but hold ~10%
 // more registers, which costs more beside the other classes).  This
 // kernel's program lets a node's output take the slot of one of its children
