@@ -51,13 +51,13 @@ struct SlotWeights {
         }
     }
 };
-// (m & a) | (~m & b) with a wave-uniform mask: one bitfield insert per slot
-// past the first.  The asm operand keeps the compiler from rewriting the
-// chain as a select at a variable index.
+// (m & a) | (~m & b) with a wave-uniform mask (all ones or zero, the same in
+// every lane): one bitfield insert per slot past the first.  The asm operand
+// keeps the compiler from rewriting the chain as a select at a variable index.
 __device__ __forceinline__ float bit_select(unsigned m, float a, float b)
 {
     float r;
-    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "s"(m), "v"(a), "v"(b));
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(b));
     return r;
 }
 template <int NS, int LR>
