@@ -151,6 +151,70 @@ def kernels_only(w: wl.Workload, steps: int):
 
 
 # ----------------------------------------------------------------------------
+# Pupko joint reconstruction (gene_family_reconstructor.cpp:30-189) of every
+# family of the workload at one lambda, matrices of the largest bound
+# ----------------------------------------------------------------------------
+RECON_LAMBDA = 0.003
+
+
+def time_reconstruct(eng: native.Engine, w: wl.Workload, reps: int = 3):
+    B = R = int(w.bounds.max())
+    eng.set_families(w.counts, B, R)
+    keys = native.KeyTable()
+    slots = keys.node_slots(w.tree, np.full((1, w.tree.n_nodes), RECON_LAMBDA))
+    lq, tq = keys.arrays()
+    eng.matrix_reset(B + 1, len(lq))
+    eng.matrix_build(lq, tq, 0, native.BUILD_RECURRENCE)
+    prior = priors.uniform(R, min(B, R) + 1)
+    states = eng.reconstruct(slots, prior, B, R)   # first call sizes the device buffers
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        states = eng.reconstruct(slots, prior, B, R)
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    F = w.counts.shape[0]
+    return {"families": F, "B": B, "R": R, "lambda": RECON_LAMBDA, "ms_per_call": ms,
+            "families_per_s": F / (ms * 1e-3),
+            "timed": "cafe_gpu_reconstruct over all families, counts and states as host arrays (PCIe included), "
+                     "matrices built beforehand"}, states
+
+
+def _cpu_reconstruct(so: str, w: wl.Workload, threads: int, gpu_states: np.ndarray, n_sample: int = 512):
+    lib = ctypes.CDLL(so)
+    lib.ref_init.restype = None
+    lib.ref_init()
+    lib.ref_set_threads.restype = ctypes.c_int
+    lib.ref_set_threads.argtypes = [ctypes.c_int]
+    used = lib.ref_set_threads(threads)
+    lib.ref_reconstruct_many.restype = ctypes.c_int
+    lib.ref_reconstruct_many.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int), ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int)]
+    B = R = int(w.bounds.max())
+    idx = _sample(w, n_sample, seed=13)
+    ip = ctypes.POINTER(ctypes.c_int)
+    t, outs = [], []
+    for n in (min(64, len(idx)), len(idx)):
+        c = np.ascontiguousarray(w.counts[idx[:n]], dtype=np.int32)
+        out = np.zeros((n, w.tree.n_nodes), dtype=np.int32)
+        t0 = time.perf_counter()
+        rc = lib.ref_reconstruct_many(w.tree.newick().encode(), " ".join(w.tree.leaf_names()).encode(),
+                                      " ".join(w.tree.names).encode(), n, c.ctypes.data_as(ip), RECON_LAMBDA, B, R,
+                                      out.ctypes.data_as(ip))
+        t.append(time.perf_counter() - t0)
+        if rc != 0:
+            raise RuntimeError("ref_reconstruct_many failed")
+        outs.append(out)
+    F = w.counts.shape[0]
+    est = _extrapolate(t[0], t[1], min(64, len(idx)), len(idx), F)
+    differ = int(np.sum(np.any(outs[1] != gpu_states[idx], axis=1)))
+    return {"families_per_s": F / est, "cores": int(used), "kind": "reference",
+            "sample": (f"reference pupko_reconstructor::reconstruct_gene_family (oracle/_ref, OpenMP over families, "
+                       f"one matrix cache) on seeded {min(64, len(idx))}- and {len(idx)}-family samples, matrix build "
+                       f"and per-family cost extrapolated to all {F} families ({sum(t):.1f} s of CPU wall time)"),
+            "families_whose_states_differ_from_gpu": differ}
+
+
+# ----------------------------------------------------------------------------
 # CPU baseline (checker libraries under oracle/, loaded only here)
 # ----------------------------------------------------------------------------
 
@@ -400,13 +464,28 @@ def main():
         "roofline": roofline,
         "cpu_baseline": None,
     }
+    for p in (p32, p64, pplain):
+        p.close()
+    # the reconstruction next to the likelihood path (an extra measurement:
+    # its failure is reported, not fatal to the headline line)
+    recon_states = None
+    try:
+        out["reconstruct"], recon_states = time_reconstruct(eng, w)
+        log(f"reconstruct: {out['reconstruct']}")
+    except Exception as e:  # noqa: BLE001
+        out["reconstruct"] = {"error": str(e)}
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads ...")
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_lambdas, args.cpu_families, threads)
         log(f"cpu baseline: {out['cpu_baseline']}")
-    for p in (p32, p64, pplain):
-        p.close()
+        ref_so = os.path.join(ROOT, "oracle", "_ref", "libcaferef.so")
+        if recon_states is not None and os.path.exists(ref_so):
+            try:
+                out["reconstruct"]["cpu_reference"] = _cpu_reconstruct(ref_so, w, threads, recon_states)
+                log(f"reconstruct cpu: {out['reconstruct']['cpu_reference']}")
+            except Exception as e:  # noqa: BLE001
+                out["reconstruct"]["cpu_reference"] = {"error": str(e)}
     eng.close()
     print(json.dumps(out))
 

@@ -427,6 +427,48 @@ REF_API int ref_reconstruct(const char* newick, const char* leaf_names, const ch
     }
 }
 
+// Pupko reconstruction of F families at one lambda (uniform prior over
+// 1..R) with one matrix cache, families over OpenMP threads as the reference's
+// base_model::reconstruct_ancestral_states does (base_model.cpp:145-160): the
+// CPU side of the reconstruction timing.  out[f * n_nodes + i] as in
+// ref_reconstruct.
+REF_API int ref_reconstruct_many(const char* newick, const char* leaf_names, const char* node_names, int F,
+                                 const int* counts, double lambda, int B, int R, int* out)
+{
+    try {
+        clade* tree = parse_newick(newick);
+        auto fams = make_families(F, leaf_names, counts);
+        single_lambda lam(lambda);
+        matrix_cache cache(std::max(R, B) + 1);
+        cache.precalculate_matrices({lambda}, tree->get_branch_lengths());
+        root_equilibrium_distribution prior((size_t)R);
+        auto names = split_names(node_names);
+        std::vector<const clade*> nodes;
+        for (auto& n : names) nodes.push_back(tree->find_descendant(n));
+        int bad = 0;
+#pragma omp parallel for reduction(+ : bad)
+        for (int f = 0; f < F; ++f) {
+            try {
+                clademap<int> result;
+                clademap<std::vector<int>> Cs;
+                clademap<std::vector<double>> Ls;
+                for (auto it = tree->reverse_level_begin(); it != tree->reverse_level_end(); ++it)
+                    pupko_reconstructor::initialize_at_node(*it, Cs, Ls, B, R);
+                pupko_reconstructor::reconstruct_gene_family(&lam, tree, &fams[f], &cache, &prior, result, Cs, Ls);
+                for (size_t i = 0; i < nodes.size(); ++i)
+                    out[(size_t)f * nodes.size() + i] =
+                        nodes[i]->is_leaf() ? fams[f].get_species_size(names[i]) : result[nodes[i]];
+            } catch (...) {
+                ++bad;
+            }
+        }
+        delete tree;
+        return bad ? -1 : 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
 // The files as the reference reads them (read_gene_families, io.cpp:134-217;
 // read_error_model_file, :228-275) evaluated by its base model at one lambda
 // with a uniform prior over 1..R: -lnL, out_lnl[F] in file order (F <= cap).
