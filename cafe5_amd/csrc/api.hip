@@ -1385,8 +1385,11 @@ int cafe_gpu_reconstruct(cafe_gpu_ctx* ctx, const int* node_slot, const double* 
     // count-row (sorted) index of each caller family
     std::vector<int> sorted_of(ctx->F);
     for (int s = 0; s < ctx->F; ++s) sorted_of[ctx->perm[s]] = s;
-    const int chunk = std::min(nf, 2048);
     const int Lp = B + 1;
+    // families per launch: enough workgroups to fill the device several times
+    // over, with the per-family L, C and product scratch kept within 2 GB
+    const size_t fam_bytes = (size_t)Lp * ((size_t)n * (sizeof(double) + sizeof(int)) + sizeof(double));
+    const int chunk = (int)std::min<size_t>((size_t)nf, std::min<size_t>(8192, std::max<size_t>(64, ((size_t)2 << 30) / fam_bytes)));
     // device layout: program | kids | parent | prior | families | states | flags
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t o_kids = up(sizeof(ReconNode) * prog.size());

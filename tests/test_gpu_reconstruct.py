@@ -59,14 +59,14 @@ def test_seeded_families(eng, algo, B, R):
 
 
 def test_many_families_in_chunks(eng):
-    # more families than one launch holds (2048): every chunk's rows land in place
-    w = wl.config_base(seed=8, F=2500)
+    # more families than one launch holds (8192): every chunk's rows land in place
+    w = wl.config_base(seed=8, F=9000)
     B = R = 24
     counts = np.minimum(w.counts, B).astype(np.int32)
     slots = _setup(eng, w.tree, counts, 0.003, B, R, native.BUILD_CLOSED_FORM)
     prior = priors.uniform(R, R + 1)
     got = eng.reconstruct(slots, prior, B, R)
     rng = np.random.default_rng(0)
-    for f in np.concatenate([rng.choice(counts.shape[0], 40, replace=False), [0, 2047, 2048, 2499]]):
+    for f in np.concatenate([rng.choice(counts.shape[0], 40, replace=False), [0, 8191, 8192, 8999]]):
         want = Oracle.reconstruct(w.tree, counts[f], 0.003, B, R, prior)
         assert np.array_equal(got[f], want), f
