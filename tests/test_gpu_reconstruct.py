@@ -58,6 +58,23 @@ def test_seeded_families(eng, algo, B, R):
         assert np.array_equal(got[f], want), (f, np.flatnonzero(got[f] != want))
 
 
+def test_large_bound_one_family_per_wave(eng):
+    # B >= 256 runs k_reconstruct (one family per workgroup, matrices read from
+    # memory); smaller bounds run k_reconstruct_shared.  Three families: the
+    # oracle builds its 301 x 301 matrices for every family (~6 s each)
+    algo = native.BUILD_RECURRENCE
+    w = wl.config_base(seed=5, F=3)
+    B = R = 300
+    counts = np.minimum(w.counts, B).astype(np.int32)
+    lam = 0.004
+    slots = _setup(eng, w.tree, counts, lam, B, R, algo)
+    prior = priors.uniform(R, R + 1)
+    got = eng.reconstruct(slots, prior, B, R)
+    for f in range(counts.shape[0]):
+        want = Oracle.reconstruct(w.tree, counts[f], lam, B, R, prior)
+        assert np.array_equal(got[f], want), (f, np.flatnonzero(got[f] != want))
+
+
 def test_many_families_in_chunks(eng):
     # more families than one launch holds (8192): every chunk's rows land in place
     w = wl.config_base(seed=8, F=9000)
