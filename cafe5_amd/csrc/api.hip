@@ -160,6 +160,10 @@ struct cafe_gpu_ctx {
     // buffers of eager evaluations (plans own their own)
     EvalBufs eager;
     long long pool_gen = 0;   // bumped by matrix_reset; plans check it
+    // bumped by set_tree, set_families and by set_error_model when it adds or
+    // removes the model or changes its shape (buffers a plan's graph points
+    // into); plans check it.  Same-shape error-model rows are updated in place.
+    long long input_gen = 0;
 };
 
 namespace {
@@ -346,21 +350,31 @@ int cafe_gpu_matrix_reset(cafe_gpu_ctx* ctx, int N, int capacity)
     if (int rc = check_ctx(ctx)) return rc;
     if (N < 2 || N * 2 + 2 > kLgammaN) return fail(CAFE_GPU_EINVAL, "matrix size %d out of range", N);
     if (capacity < 1) return fail(CAFE_GPU_EINVAL, "capacity must be positive");
-    ctx->N = N;
-    ctx->Np = round_up(N + kTile, 32);   // >= round_up(N, 16) + 16, whole 32-wide transpose tiles
-    ctx->slot_elems = (long long)ctx->Np * ctx->Np;
-    ctx->capacity = capacity;
+    const int Np = round_up(N + kTile, 32);   // >= round_up(N, 16) + 16, whole 32-wide transpose tiles
+    const long long slot_elems = (long long)Np * Np;
+    const size_t n64 = sizeof(double) * slot_elems * capacity;
+    const size_t n32 = sizeof(float) * slot_elems * capacity;
+    // every plan made over the old pool is stale from here on, and until the
+    // new pool is complete the context has none
     ctx->pool_gen++;
-    const size_t n64 = sizeof(double) * ctx->slot_elems * capacity;
-    const size_t n32 = sizeof(float) * ctx->slot_elems * capacity;
+    ctx->N = ctx->Np = ctx->capacity = 0;
+    ctx->slot_elems = 0;
+    ctx->built.clear();
     HIP_OR_FAIL(ctx->mt64.ensure(n64));
     HIP_OR_FAIL(ctx->mt32.ensure(n32));
     HIP_OR_FAIL(ctx->lf32.ensure(n32));
+    if (ctx->n_dev) {
+        HIP_OR_FAIL(ctx->mt64e.ensure(n64));
+        HIP_OR_FAIL(ctx->mt32e.ensure(n32));
+    }
     // padding rows/columns must read as zero
     HIP_OR_FAIL(launch_fill_zero(ctx->mt64.p, (long long)n64, ctx->stream));
     HIP_OR_FAIL(launch_fill_zero(ctx->mt32.p, (long long)n32, ctx->stream));
     HIP_OR_FAIL(launch_fill_zero(ctx->lf32.p, (long long)n32, ctx->stream));
-    if (int rc = ensure_fold_pools(ctx)) return rc;
+    ctx->N = N;
+    ctx->Np = Np;
+    ctx->slot_elems = slot_elems;
+    ctx->capacity = capacity;
     if (int rc = enqueue_fold(ctx, 0, capacity, ctx->stream)) return rc;
     HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
     ctx->built.assign(capacity, 0);
@@ -455,6 +469,7 @@ int cafe_gpu_set_tree(cafe_gpu_ctx* ctx, int n_nodes, const int* parent, const i
     pl.emit(root);
     if (pl.max_slot + 1 > kMaxSlots)
         return fail(CAFE_GPU_EINVAL, "tree needs %d live partial vectors (max %d)", pl.max_slot + 1, kMaxSlots);
+    ctx->input_gen++;
     ctx->n_nodes = n_nodes;
     ctx->tree_leaves = n_leaves;
     ctx->parent_h.assign(parent, parent + n_nodes);
@@ -532,6 +547,7 @@ int cafe_gpu_set_families(cafe_gpu_ctx* ctx, int F, int n_leaves, const int* cou
         b.count++;
         for (int i = 0; i < n_leaves; ++i) b.max_count = std::max(b.max_count, counts[(size_t)f * n_leaves + i]);
     }
+    ctx->input_gen++;
     std::vector<int> cT((size_t)n_leaves * F);
     for (int k = 0; k < F; ++k)
         for (int i = 0; i < n_leaves; ++i) cT[(size_t)i * F + k] = counts[(size_t)order[k] * n_leaves + i];
@@ -566,12 +582,14 @@ int cafe_gpu_set_error_model(cafe_gpu_ctx* ctx, int n_dev, int n_rows, const dou
 {
     if (int rc = check_ctx(ctx)) return rc;
     if (!rows) {
+        if (ctx->n_dev) ctx->input_gen++;
         ctx->n_dev = 0;
         ctx->err_rows = 0;
         return CAFE_GPU_OK;
     }
     if (n_dev < 1 || n_dev % 2 == 0 || n_rows < 1)
         return fail(CAFE_GPU_EINVAL, "error model needs an odd number of deviations and >= 1 row");
+    if (n_dev != ctx->n_dev || n_rows != ctx->err_rows) ctx->input_gen++;   // same shape: in place
     const size_t n = (size_t)n_dev * n_rows;
     std::vector<float> r32(rows, rows + n);
     HIP_OR_FAIL(ctx->d_err64.ensure(sizeof(double) * n));
@@ -1209,6 +1227,135 @@ double elapsed_ms(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
+// Pupko reconstruction of families f0 .. f0+nf-1 (cafe_gpu_reconstruct).  The
+// device working memory -- program, child lists, parents, prior, a launch's
+// family list, its state rows and the two error flags, then per family the
+// L, C and product rows -- comes from the context's grow-only buffers when
+// `fixed` is null, else from the fixed region [fixed, fixed + fixed_bytes),
+// in which case nothing is allocated: launches are sized to fit the region
+// and a family that does not fit on its own is rejected.
+int reconstruct_impl(cafe_gpu_ctx* ctx, const int* node_slot, const double* prior, int B, int R, int f0, int nf,
+                     int* states, char* fixed, size_t fixed_bytes)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!node_slot || !prior || !states) return fail(CAFE_GPU_EINVAL, "null argument");
+    if (ctx->n_nodes == 0 || ctx->F == 0) return fail(CAFE_GPU_ESTATE, "tree and families must be set first");
+    if (ctx->N == 0) return fail(CAFE_GPU_ESTATE, "matrix_reset has not been called");
+    if (B < 0 || R < 0 || B + 1 > ctx->N) return fail(CAFE_GPU_EINVAL, "bounds B=%d R=%d do not fit matrices of %d", B, R, ctx->N);
+    if (f0 < 0 || nf < 0 || f0 + nf > ctx->F) return fail(CAFE_GPU_EINVAL, "family range out of range");
+    if (ctx->n_leaves != ctx->tree_leaves)
+        return fail(CAFE_GPU_EINVAL, "families have %d leaves, tree has %d", ctx->n_leaves, ctx->tree_leaves);
+    if (nf == 0) return CAFE_GPU_OK;
+    const int n = ctx->n_nodes;
+    const int Rr = std::min(B, R) + 1;   // gene_family_reconstructor.cpp:149-151
+    const int Lp = B + 1;
+    // device layout: program | kids | parent | prior | families | states | flags,
+    // then the per-family L, product and C rows (the program has one record
+    // per node, the child lists one entry per non-root node)
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t fam_bytes = (size_t)Lp * ((size_t)n * (sizeof(double) + sizeof(int)) + sizeof(double));
+    const size_t o_kids = up(sizeof(ReconNode) * n);
+    const size_t o_par = o_kids + up(sizeof(int) * std::max(1, n - 1));
+    const size_t o_pri = o_par + up(sizeof(int) * n);
+    const size_t o_fam = o_pri + up(sizeof(double) * Rr);
+    // families per launch: enough workgroups to fill the device several times
+    // over, with the per-family scratch within 2 GB (or within the fixed region)
+    size_t cap = std::min<size_t>(8192, std::max<size_t>(64, ((size_t)2 << 30) / fam_bytes));
+    if (fixed) {
+        // o_fam + up(4 c) + up(4 c n) + 256 + c fam_bytes <= fixed_bytes
+        const size_t per = sizeof(int) + sizeof(int) * (size_t)n + fam_bytes;
+        const size_t head = o_fam + 3 * 256 + 256;
+        const size_t fit = fixed_bytes > head ? (fixed_bytes - head) / per : 0;
+        if (fit < 1)
+            return fail(CAFE_GPU_EINVAL,
+                        "one family of %d nodes with vectors of %d sizes needs %zu bytes of scratch; the fixed "
+                        "scratch holds %zu",
+                        n, Lp, head + per, fixed_bytes);
+        cap = std::min(cap, fit);
+    }
+    // children first (depth first, children in increasing index)
+    std::vector<std::vector<int>> kids(n);
+    int root = -1;
+    for (int i = 0; i < n; ++i) {
+        if (ctx->parent_h[i] < 0) root = i;
+        else kids[ctx->parent_h[i]].push_back(i);
+    }
+    std::vector<ReconNode> prog;
+    std::vector<int> kid_list;
+    std::vector<std::pair<int, int>> stack{{root, 0}};
+    while (!stack.empty()) {
+        auto& top = stack.back();
+        const int u = top.first;
+        if (top.second < (int)kids[u].size()) {
+            stack.push_back({kids[u][top.second++], 0});
+            continue;
+        }
+        stack.pop_back();
+        ReconNode nd{u, ctx->leaf_col_h[u], (int)kid_list.size(), (int)kids[u].size(), -1};
+        for (int c : kids[u]) kid_list.push_back(c);
+        if (u != root) {
+            nd.slot = node_slot[u];
+            if (nd.slot < 0 || nd.slot >= ctx->capacity)
+                return fail(CAFE_GPU_EINVAL, "node %d: matrix slot %d outside the pool", u, nd.slot);
+            if (!ctx->built[nd.slot])   // matrix_cache::get_matrix throws on a miss (matrix_cache.cpp:88-104)
+                return fail(CAFE_GPU_ENOTFOUND, "node %d uses matrix slot %d, which is not built", u, nd.slot);
+        }
+        prog.push_back(nd);
+    }
+    // count-row (sorted) index of each caller family
+    std::vector<int> sorted_of(ctx->F);
+    for (int s = 0; s < ctx->F; ++s) sorted_of[ctx->perm[s]] = s;
+    if (prog.size() != (size_t)n || kid_list.size() + 1 != (size_t)n)
+        return fail(CAFE_GPU_EINVAL, "reconstruction program does not cover the tree (internal)");
+    const int chunk = (int)std::min<size_t>((size_t)nf, cap);
+    const size_t o_st = o_fam + up(sizeof(int) * chunk);
+    const size_t o_err = o_st + up(sizeof(int) * (size_t)chunk * n);
+    const size_t o_scr = o_err + 256;
+    char* base;
+    char* scr;
+    if (fixed) {
+        base = fixed;
+        scr = fixed + o_scr;
+        if (o_scr + (size_t)chunk * fam_bytes > fixed_bytes)
+            return fail(CAFE_GPU_EINVAL, "fixed scratch layout overflows (internal)");
+    } else {
+        HIP_OR_FAIL(ctx->d_recon.ensure(o_scr));
+        HIP_OR_FAIL(ctx->d_recon_scratch.ensure((size_t)chunk * fam_bytes));
+        base = (char*)ctx->d_recon.p;
+        scr = (char*)ctx->d_recon_scratch.p;
+    }
+    double* L_all = (double*)scr;   // doubles first: every row stays 8-byte aligned
+    double* prod_all = L_all + (size_t)chunk * n * Lp;
+    int* C_all = (int*)(prod_all + (size_t)chunk * Lp);
+    hipStream_t st = ctx->stream;
+    HIP_OR_FAIL(hipMemcpyAsync(base, prog.data(), sizeof(ReconNode) * prog.size(), hipMemcpyHostToDevice, st));
+    if (!kid_list.empty())
+        HIP_OR_FAIL(hipMemcpyAsync(base + o_kids, kid_list.data(), sizeof(int) * kid_list.size(), hipMemcpyHostToDevice, st));
+    HIP_OR_FAIL(hipMemcpyAsync(base + o_par, ctx->parent_h.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    HIP_OR_FAIL(hipMemcpyAsync(base + o_pri, prior, sizeof(double) * Rr, hipMemcpyHostToDevice, st));
+    HIP_OR_FAIL(launch_fill_zero(base + o_err, 2 * sizeof(int), st));
+    std::vector<int> fams(chunk);
+    for (int c0 = 0; c0 < nf; c0 += chunk) {
+        const int m = std::min(chunk, nf - c0);
+        for (int k = 0; k < m; ++k) fams[k] = sorted_of[f0 + c0 + k];
+        HIP_OR_FAIL(hipMemcpyAsync(base + o_fam, fams.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+        HIP_OR_FAIL(launch_reconstruct((const ReconNode*)base, (int)prog.size(), (const int*)(base + o_kids),
+                                       (const int*)(base + o_par), ctx->mt64.as<double>(), ctx->N, ctx->Np,
+                                       ctx->slot_elems, ctx->d_counts.as<int>(), ctx->F, (const int*)(base + o_fam), m,
+                                       (const double*)(base + o_pri), B, Rr, L_all, C_all, prod_all, Lp,
+                                       (int*)(base + o_st), (int*)(base + o_err), st));
+        HIP_OR_FAIL(hipMemcpyAsync(states + (size_t)c0 * n, base + o_st, sizeof(int) * (size_t)m * n,
+                                   hipMemcpyDeviceToHost, st));
+        HIP_OR_FAIL(hipStreamSynchronize(st));   // fams and the state rows are reused by the next chunk
+    }
+    int flags[2] = {0, 0};
+    HIP_OR_FAIL(hipMemcpyAsync(flags, base + o_err, sizeof(flags), hipMemcpyDeviceToHost, st));
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    if (flags[0]) return fail(CAFE_GPU_EINVAL, "a leaf count is outside the matrices (N = %d)", ctx->N);
+    if (flags[1]) return fail(CAFE_GPU_EINVAL, "a reconstructed state fell outside 0..%d", B);
+    return CAFE_GPU_OK;
+}
+
 }  // namespace
 
 // One device-resident evaluation (see cafe_gpu_plan_* in cafe_gpu.h).
@@ -1218,7 +1365,7 @@ struct cafe_gpu_plan {
     PrunePlan pp;
     int model = 0, path = 0, T = 0, K = 0, n_keys = 0, algo = 0;
     DevBuf d_lam, d_t;
-    long long pool_gen = 0;
+    long long pool_gen = 0, input_gen = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool graph_ok = false;
@@ -1287,6 +1434,23 @@ bool plan_capture(cafe_gpu_plan* pl, bool timed)
     return true;
 }
 
+// A plan's graph and launch parameters point into the context's pool and
+// input buffers and were sized for its tree, families and error model.
+int plan_current(cafe_gpu_plan* pl)
+{
+    if (pl->pool_gen != pl->ctx->pool_gen)
+        return fail(CAFE_GPU_ESTATE, "matrix pool was reset after the plan was made");
+    if (pl->input_gen != pl->ctx->input_gen)
+        return fail(CAFE_GPU_ESTATE, "tree, families or error-model shape changed after the plan was made");
+    return CAFE_GPU_OK;
+}
+
+// Slots 0 .. n_keys-1 hold the plan's matrices once an execution is enqueued.
+void plan_mark_built(cafe_gpu_plan* pl)
+{
+    for (int i = 0; i < pl->n_keys; ++i) pl->ctx->built[i] = 1;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1346,91 +1510,7 @@ int cafe_gpu_root_vectors(cafe_gpu_ctx* ctx, const cafe_gpu_eval_desc* d, int tr
 int cafe_gpu_reconstruct(cafe_gpu_ctx* ctx, const int* node_slot, const double* prior, int B, int R, int f0, int nf,
                          int* states)
 {
-    if (int rc = check_ctx(ctx)) return rc;
-    if (!node_slot || !prior || !states) return fail(CAFE_GPU_EINVAL, "null argument");
-    if (ctx->n_nodes == 0 || ctx->F == 0) return fail(CAFE_GPU_ESTATE, "tree and families must be set first");
-    if (ctx->N == 0) return fail(CAFE_GPU_ESTATE, "matrix_reset has not been called");
-    if (B < 0 || R < 0 || B + 1 > ctx->N) return fail(CAFE_GPU_EINVAL, "bounds B=%d R=%d do not fit matrices of %d", B, R, ctx->N);
-    if (f0 < 0 || nf < 0 || f0 + nf > ctx->F) return fail(CAFE_GPU_EINVAL, "family range out of range");
-    if (nf == 0) return CAFE_GPU_OK;
-    const int n = ctx->n_nodes;
-    const int Rr = std::min(B, R) + 1;   // gene_family_reconstructor.cpp:149-151
-    // children first (depth first, children in increasing index)
-    std::vector<std::vector<int>> kids(n);
-    int root = -1;
-    for (int i = 0; i < n; ++i) {
-        if (ctx->parent_h[i] < 0) root = i;
-        else kids[ctx->parent_h[i]].push_back(i);
-    }
-    std::vector<ReconNode> prog;
-    std::vector<int> kid_list;
-    std::vector<std::pair<int, int>> stack{{root, 0}};
-    while (!stack.empty()) {
-        auto& top = stack.back();
-        const int u = top.first;
-        if (top.second < (int)kids[u].size()) {
-            stack.push_back({kids[u][top.second++], 0});
-            continue;
-        }
-        stack.pop_back();
-        ReconNode nd{u, ctx->leaf_col_h[u], (int)kid_list.size(), (int)kids[u].size(), -1};
-        for (int c : kids[u]) kid_list.push_back(c);
-        if (u != root) {
-            nd.slot = node_slot[u];
-            if (nd.slot < 0 || nd.slot >= ctx->capacity)
-                return fail(CAFE_GPU_EINVAL, "node %d: matrix slot %d outside the pool", u, nd.slot);
-        }
-        prog.push_back(nd);
-    }
-    // count-row (sorted) index of each caller family
-    std::vector<int> sorted_of(ctx->F);
-    for (int s = 0; s < ctx->F; ++s) sorted_of[ctx->perm[s]] = s;
-    const int Lp = B + 1;
-    // families per launch: enough workgroups to fill the device several times
-    // over, with the per-family L, C and product scratch kept within 2 GB
-    const size_t fam_bytes = (size_t)Lp * ((size_t)n * (sizeof(double) + sizeof(int)) + sizeof(double));
-    const int chunk = (int)std::min<size_t>((size_t)nf, std::min<size_t>(8192, std::max<size_t>(64, ((size_t)2 << 30) / fam_bytes)));
-    // device layout: program | kids | parent | prior | families | states | flags
-    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-    const size_t o_kids = up(sizeof(ReconNode) * prog.size());
-    const size_t o_par = o_kids + up(sizeof(int) * std::max<size_t>(1, kid_list.size()));
-    const size_t o_pri = o_par + up(sizeof(int) * n);
-    const size_t o_fam = o_pri + up(sizeof(double) * Rr);
-    const size_t o_st = o_fam + up(sizeof(int) * chunk);
-    const size_t o_err = o_st + up(sizeof(int) * (size_t)chunk * n);
-    HIP_OR_FAIL(ctx->d_recon.ensure(o_err + 256));
-    HIP_OR_FAIL(ctx->d_recon_scratch.ensure((size_t)chunk * Lp * ((size_t)n * (sizeof(double) + sizeof(int)) + sizeof(double))));
-    char* base = (char*)ctx->d_recon.p;
-    double* L_all = (double*)ctx->d_recon_scratch.p;
-    int* C_all = (int*)(L_all + (size_t)chunk * n * Lp);
-    double* prod_all = (double*)(C_all + (size_t)chunk * n * Lp);
-    hipStream_t st = ctx->stream;
-    HIP_OR_FAIL(hipMemcpyAsync(base, prog.data(), sizeof(ReconNode) * prog.size(), hipMemcpyHostToDevice, st));
-    if (!kid_list.empty())
-        HIP_OR_FAIL(hipMemcpyAsync(base + o_kids, kid_list.data(), sizeof(int) * kid_list.size(), hipMemcpyHostToDevice, st));
-    HIP_OR_FAIL(hipMemcpyAsync(base + o_par, ctx->parent_h.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-    HIP_OR_FAIL(hipMemcpyAsync(base + o_pri, prior, sizeof(double) * Rr, hipMemcpyHostToDevice, st));
-    HIP_OR_FAIL(launch_fill_zero(base + o_err, 2 * sizeof(int), st));
-    for (int c0 = 0; c0 < nf; c0 += chunk) {
-        const int m = std::min(chunk, nf - c0);
-        std::vector<int> fams(m);
-        for (int k = 0; k < m; ++k) fams[k] = sorted_of[f0 + c0 + k];
-        HIP_OR_FAIL(hipMemcpyAsync(base + o_fam, fams.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
-        HIP_OR_FAIL(launch_reconstruct((const ReconNode*)base, (int)prog.size(), (const int*)(base + o_kids),
-                                       (const int*)(base + o_par), ctx->mt64.as<double>(), ctx->N, ctx->Np,
-                                       ctx->slot_elems, ctx->d_counts.as<int>(), ctx->F, (const int*)(base + o_fam), m,
-                                       (const double*)(base + o_pri), B, Rr, L_all, C_all, prod_all, Lp,
-                                       (int*)(base + o_st), (int*)(base + o_err), st));
-        HIP_OR_FAIL(hipMemcpyAsync(states + (size_t)c0 * n, base + o_st, sizeof(int) * (size_t)m * n,
-                                   hipMemcpyDeviceToHost, st));
-        HIP_OR_FAIL(hipStreamSynchronize(st));   // fams and the state rows are reused by the next chunk
-    }
-    int flags[2] = {0, 0};
-    HIP_OR_FAIL(hipMemcpyAsync(flags, base + o_err, sizeof(flags), hipMemcpyDeviceToHost, st));
-    HIP_OR_FAIL(hipStreamSynchronize(st));
-    if (flags[0]) return fail(CAFE_GPU_EINVAL, "a leaf count is outside the matrices (N = %d)", ctx->N);
-    if (flags[1]) return fail(CAFE_GPU_EINVAL, "a reconstructed state fell outside 0..%d", B);
-    return CAFE_GPU_OK;
+    return reconstruct_impl(ctx, node_slot, prior, B, R, f0, nf, states, nullptr, 0);
 }
 
 int cafe_gpu_last_timings(cafe_gpu_ctx* ctx, double* ms4)
@@ -1453,10 +1533,14 @@ int cafe_gpu_plan_create(cafe_gpu_ctx* ctx, const cafe_gpu_eval_desc* d, int n_k
     if (!lambda_q || !t_q) return fail(CAFE_GPU_EINVAL, "null key arrays");
     auto* pl = new cafe_gpu_plan();
     pl->ctx = ctx;
-    // the plan builds its keys into slots 0 .. n_keys-1 at every execution
+    // the plan builds its keys into slots 0 .. n_keys-1 at every execution:
+    // they count as built while its inputs are validated, and are marked
+    // built in the context only once an execution has been enqueued
+    const std::vector<char> built_before = ctx->built;
     for (int i = 0; i < n_keys; ++i) ctx->built[i] = 1;
     int P = 0;
     int rc = upload_eval_inputs(ctx, pl->bufs, d, P);
+    ctx->built = built_before;
     std::vector<int> pts(P);
     std::iota(pts.begin(), pts.end(), 0);
     if (rc == CAFE_GPU_OK) rc = plan_prune(ctx, pl->bufs, d->path, d->model, P, pts.data(), nullptr, 0, pl->pp);
@@ -1475,6 +1559,7 @@ int cafe_gpu_plan_create(cafe_gpu_ctx* ctx, const cafe_gpu_eval_desc* d, int n_k
     pl->n_keys = n_keys;
     pl->algo = algo;
     pl->pool_gen = ctx->pool_gen;
+    pl->input_gen = ctx->input_gen;
     for (auto& e : pl->ev) (void)hipEventCreate(&e);
     pl->cls_ev.assign(pl->pp.classes.size() + 1, nullptr);
     for (auto& e : pl->cls_ev) (void)hipEventCreate(&e);
@@ -1504,13 +1589,12 @@ int cafe_gpu_plan_set_keys(cafe_gpu_plan* pl, const double* lambda_q, const doub
 int cafe_gpu_plan_execute(cafe_gpu_plan* pl)
 {
     if (!pl) return fail(CAFE_GPU_EINVAL, "null plan");
+    if (int rc = plan_current(pl)) return rc;
     cafe_gpu_ctx* ctx = pl->ctx;
-    if (pl->pool_gen != ctx->pool_gen) return fail(CAFE_GPU_ESTATE, "matrix pool was reset after the plan was made");
-    if (pl->graph_ok) {
-        HIP_OR_FAIL(hipGraphLaunch(pl->exec, ctx->stream));
-        return CAFE_GPU_OK;
-    }
-    return plan_enqueue(pl, false);
+    if (pl->graph_ok) HIP_OR_FAIL(hipGraphLaunch(pl->exec, ctx->stream));
+    else if (int rc = plan_enqueue(pl, false)) return rc;
+    plan_mark_built(pl);
+    return CAFE_GPU_OK;
 }
 
 int cafe_gpu_plan_synchronize(cafe_gpu_plan* pl)
@@ -1524,16 +1608,18 @@ int cafe_gpu_plan_results(cafe_gpu_plan* pl, double* lnl, double* cat, double* p
                           long long* n_fallback)
 {
     if (!pl) return fail(CAFE_GPU_EINVAL, "null plan");
+    if (int rc = plan_current(pl)) return rc;
     return copy_results(pl->ctx, pl->bufs, pl->pp, pl->model, pl->T, pl->K, lnl, cat, post, total, n_fallback);
 }
 
 int cafe_gpu_plan_profile(cafe_gpu_plan* pl, double* ms4, int* graph_ok)
 {
     if (!pl || !ms4) return fail(CAFE_GPU_EINVAL, "null argument");
+    if (int rc = plan_current(pl)) return rc;
     cafe_gpu_ctx* ctx = pl->ctx;
-    if (pl->pool_gen != ctx->pool_gen) return fail(CAFE_GPU_ESTATE, "matrix pool was reset after the plan was made");
     HIP_OR_FAIL(hipSetDevice(ctx->device));
     if (int rc = plan_enqueue(pl, true)) return rc;
+    plan_mark_built(pl);
     HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
     if (graph_ok) *graph_ok = pl->graph_ok ? 1 : 0;
     ms4[0] = elapsed_ms(pl->ev[0], pl->ev[1]);

@@ -221,16 +221,19 @@ class Engine:
         self._check(self.lib.cafe_gpu_root_vectors(self.ctx, ctypes.byref(d), trial, cat, 0, self.F, _dp(out)))
         return out.reshape(self.F, R)
 
-    def reconstruct(self, node_slot: np.ndarray, prior: np.ndarray, B: int, R: int) -> np.ndarray:
-        """Pupko joint reconstruction of every family: states[F, n_nodes]
-        (gene_family_reconstructor.cpp:30-189); prior[j], j = 0..min(B, R)."""
+    def reconstruct(self, node_slot: np.ndarray, prior: np.ndarray, B: int, R: int, f0: int = 0,
+                    nf: Optional[int] = None) -> np.ndarray:
+        """Pupko joint reconstruction of families f0 .. f0+nf-1 (all by
+        default): states[nf, n_nodes] (gene_family_reconstructor.cpp:30-189);
+        prior[j], j = 0..min(B, R)."""
         n_nodes = self.tree.n_nodes
+        nf = self.F - f0 if nf is None else nf
         ns = np.ascontiguousarray(np.asarray(node_slot).reshape(-1)[:n_nodes], dtype=np.int32)
         pr = np.ascontiguousarray(prior, dtype=np.float64)
         if pr.size < min(B, R) + 1:
             raise ValueError("prior needs min(B, R) + 1 entries")
-        out = np.zeros((self.F, n_nodes), dtype=np.int32)
-        self._check(self.lib.cafe_gpu_reconstruct(self.ctx, _ip(ns), _dp(pr), B, R, 0, self.F, _ip(out)))
+        out = np.zeros((max(nf, 0), n_nodes), dtype=np.int32)
+        self._check(self.lib.cafe_gpu_reconstruct(self.ctx, _ip(ns), _dp(pr), B, R, f0, nf, _ip(out)))
         return out
 
     def timings(self) -> np.ndarray:

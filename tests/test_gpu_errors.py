@@ -89,3 +89,69 @@ def test_evaluation_errors_then_a_valid_evaluation(eng):
         want = Oracle.evaluate(tree, np.full(tree.n_nodes, 0.02), counts, B, R, prior[0])["lnl"]
         tol = 1e-9 if path == native.PATH_PLAIN64 else 1e-6
         assert np.all(np.abs(got - want) <= tol * np.abs(want)), (path, got, want)
+
+
+def _small_setup(eng):
+    tree = wl.parse_newick(TREE)
+    eng.set_tree(tree)
+    counts = np.array([[1, 2, 3, 0], [4, 4, 5, 6], [0, 0, 1, 0]], dtype=np.int32)
+    B = R = 10
+    eng.set_families(counts, B, R)
+    keys = native.KeyTable()
+    slots = keys.node_slots(tree, np.full((1, tree.n_nodes), 0.02))
+    lq, tq = keys.arrays()
+    prior = Oracle.prior_uniform(R, R)[None, :]
+    return tree, counts, B, R, slots, lq, tq, prior
+
+
+def test_reconstruct_rejects_mismatched_leaves_and_unbuilt_slots(eng):
+    tree, counts, B, R, slots, lq, tq, prior = _small_setup(eng)
+    eng.matrix_reset(B + 1, len(lq))
+    rprior = Oracle.prior_uniform(R, R + 1)
+    # gene_family_reconstructor.cpp:84-85 / matrix_cache.cpp:88-104: no matrix, no reconstruction
+    _raises(lambda: eng.reconstruct(slots[0], rprior, B, R), "not built")
+    eng.matrix_build(lq, tq, 0, native.BUILD_RECURRENCE)
+    # a family table with fewer leaf columns than the tree has leaves
+    eng.set_families(counts[:, :3], B, R)
+    _raises(lambda: eng.reconstruct(slots[0], rprior, B, R), "families have 3 leaves, tree has 4")
+    eng.set_families(counts, B, R)
+    got = eng.reconstruct(slots[0], rprior, B, R)
+    for f in range(counts.shape[0]):
+        want = Oracle.reconstruct(tree, counts[f], np.full(tree.n_nodes, 0.02), B, R, rprior)
+        assert np.array_equal(got[f], want), (f, got[f], want)
+
+
+def test_plan_is_stale_after_inputs_change(eng):
+    tree, counts, B, R, slots, lq, tq, prior = _small_setup(eng)
+    eng.set_error_model(wl.error_table(0.05, B))
+    eng.matrix_reset(B + 1, len(lq))
+    p = eng.plan(slots, prior, lq, tq, path=native.PATH_BATCHED32)
+    p.execute()
+    first = p.results()["total"][0]
+    # same-shape error rows are updated in place: the plan stays valid
+    eng.set_error_model(wl.error_table(0.06, B))
+    p.execute()
+    assert p.results()["total"][0] != first
+    # a new family table, tree or error-model shape makes the plan stale
+    for change in (lambda: eng.set_families(counts, B, R), lambda: eng.set_tree(tree),
+                   lambda: eng.set_error_model(None)):
+        change()
+        _raises(p.execute, "changed after the plan was made")
+        _raises(p.results, "changed after the plan was made")
+    p.close()
+    eng.set_error_model(None)
+
+
+def test_failed_plan_leaves_slots_unbuilt(eng):
+    tree, counts, B, R, slots, lq, tq, prior = _small_setup(eng)
+    eng.matrix_reset(B + 1, len(lq))
+    with pytest.raises(CafeGpuError):
+        eng.plan(slots, prior[:, :5], lq, tq, path=native.PATH_PLAIN64)   # prior shorter than R
+    _raises(lambda: eng.matrix_get(0), "not built")
+    p = eng.plan(slots, prior, lq, tq, path=native.PATH_PLAIN64)
+    _raises(lambda: eng.matrix_get(0), "not built")   # built by the first execution
+    p.execute()
+    p.synchronize()
+    want = Oracle.matrix(lq[0], tq[0], B + 1)
+    assert np.allclose(eng.matrix_get(0), want, rtol=1e-12, atol=1e-300)
+    p.close()

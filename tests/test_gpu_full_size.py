@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 from cafe5_amd import native, priors, workloads as wl
+from bars import reduced_ok
 from oracle_lib import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -69,7 +70,7 @@ def test_batched32_matches_plain64(full):
     assert np.array_equal(np.isfinite(a), np.isfinite(b))
     fin = np.isfinite(b)
     err = np.abs(a[fin] - b[fin])
-    assert np.all(err <= 1e-4 + 1e-6 * np.abs(b[fin])), err.max()
+    assert np.all(reduced_ok(a[fin], b[fin])), err.max()
     ta, tb = out[native.PATH_BATCHED32]["total"], out[native.PATH_PLAIN64]["total"]
     assert np.all(np.abs(ta - tb) <= 1e-6 * np.abs(tb)), np.max(np.abs(ta - tb) / np.abs(tb))
     # the families the flush bound could not certify were recomputed in FP64

@@ -7,9 +7,9 @@ Tolerances:
     ulps (~1e-13 at N = 201); the recurrence is a different, cancellation-free
     summation of the same positive terms;
   * PLAIN64 / BATCHED64 per-family lnL: <= 1e-9 relative (the north-star bar);
-  * BATCHED32: per-family |dlnL| <= 1e-4 + 1e-6 |lnL| and the total within
-    1e-6 relative, with families the FP32 flush bound cannot certify
-    recomputed in FP64.
+  * BATCHED32: per-family |dlnL| <= 5e-6 + 1e-7 |lnL| and <= 1e-5 relative
+    (tests/bars.py), the total within 1e-6 relative, with families the FP32
+    flush bound cannot certify recomputed in FP64.
 """
 import math
 
@@ -18,6 +18,7 @@ import pytest
 
 from cafe5_amd import native, workloads as wl
 from cafe5_amd.model import BaseModel, GammaModel
+from bars import reduced_ok
 from oracle_lib import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -47,8 +48,8 @@ def assert_lnl(got, want, path, what=""):
     assert np.array_equal(np.isfinite(got), np.isfinite(want)), (what, np.flatnonzero(np.isfinite(got) != np.isfinite(want))[:5])
     fin = np.isfinite(want)
     if path == native.PATH_BATCHED32:
-        err = np.abs(got[fin] - want[fin])
-        assert np.all(err <= 1e-4 + 1e-6 * np.abs(want[fin])), (what, err.max())
+        ok = reduced_ok(got[fin], want[fin])
+        assert np.all(ok), (what, np.abs(got[fin] - want[fin]).max())
     else:
         r = _rel(got[fin], want[fin])
         assert np.all(r <= 1e-9), (what, r.max())
