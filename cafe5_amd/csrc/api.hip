@@ -419,7 +419,7 @@ int cafe_gpu_matrix_get(cafe_gpu_ctx* ctx, int slot, double* out)
     if (int rc = check_ctx(ctx)) return rc;
     if (!out) return fail(CAFE_GPU_EINVAL, "null output");
     if (slot < 0 || slot >= ctx->capacity || !ctx->built[slot])
-        return fail(CAFE_GPU_ENOTFOUND, "matrix slot %d has not been built", slot);
+        return fail(CAFE_GPU_ENOTFOUND, "matrix slot %d is not built", slot);
     const int N = ctx->N, Np = ctx->Np;
     std::vector<double> cm((size_t)N * Np);
     HIP_OR_FAIL(hipMemcpy(cm.data(), ctx->mt64.as<double>() + (long long)slot * ctx->slot_elems,
