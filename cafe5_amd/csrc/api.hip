@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -46,6 +47,16 @@ using namespace cafe;
 // side streams per context: the size classes of one pruning pass (as many as
 // the runtime's default hardware queues per process)
 constexpr int kSideStreams = 4;
+
+// Fixed device scratch of the entry points that allocate nothing
+// (cafe_gpu_reconstruct_noalloc): a reconstruction's program, family list,
+// state rows, error flags and per-family L, C and product rows.  Every device
+// the library runs on holds its own copy for the life of the process.  In host
+// code `scratch` names a host-side stand-in, never the device memory: its
+// device address comes from hipGetSymbolAddress, once per device (see
+// fixed_scratch_of), and only that address is handed to kernels and copies.
+constexpr size_t kFixedScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[kFixedScratchBytes];
 
 namespace {
 
@@ -139,6 +150,7 @@ struct cafe_gpu_ctx {
     std::vector<int> parent_h, leaf_col_h;   // the tree as set
     DevBuf d_recon;                          // reconstruction: program, families, prior, states, flags
     DevBuf d_recon_scratch;                  // reconstruction: per-family L, C and product rows
+    char* fixed_scratch = nullptr;           // this device's copy of `scratch` (device address)
     std::vector<OpDev> ops;
     std::vector<ChildDev> kids;
     DevBuf d_ops, d_kids, d_steps;
@@ -172,6 +184,26 @@ int check_ctx(cafe_gpu_ctx* ctx)
 {
     if (!ctx) return fail(CAFE_GPU_EINVAL, "null context");
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(CAFE_GPU_EDEVICE, "hipSetDevice failed");
+    return CAFE_GPU_OK;
+}
+
+// Device address of this device's `scratch` array, resolved once per device
+// with that device current (cafe_gpu_create makes it current first).
+int fixed_scratch_of(int device, char** out)
+{
+    static std::mutex mu;
+    static std::map<int, char*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(device);
+    if (it == addr.end()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(CAFE_GPU_EDEVICE, "hipGetSymbolAddress(scratch) failed on device %d: %s", device,
+                        hipGetErrorString(e));
+        it = addr.emplace(device, (char*)p).first;
+    }
+    *out = it->second;
     return CAFE_GPU_OK;
 }
 
@@ -306,6 +338,11 @@ int cafe_gpu_create(int device, cafe_gpu_ctx** out)
             cafe_gpu_destroy(ctx);
             return fail(CAFE_GPU_EDEVICE, "hipStreamCreate failed");
         }
+    if (int rc = fixed_scratch_of(device, &ctx->fixed_scratch)) {
+        const std::string msg = cafe_gpu_last_error();
+        cafe_gpu_destroy(ctx);
+        return fail(rc, "%s", msg.c_str());
+    }
     ctx->join_ev.assign(3 * (kSideStreams + 1), nullptr);
     for (auto& e : ctx->join_ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
     // log-gamma of integers from the host libm: the reference's
@@ -1511,6 +1548,14 @@ int cafe_gpu_reconstruct(cafe_gpu_ctx* ctx, const int* node_slot, const double* 
                          int* states)
 {
     return reconstruct_impl(ctx, node_slot, prior, B, R, f0, nf, states, nullptr, 0);
+}
+
+int cafe_gpu_reconstruct_noalloc(cafe_gpu_ctx* ctx, const int* node_slot, const double* prior, int B, int R,
+                                 int f0, int nf, int* states)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!ctx->fixed_scratch) return fail(CAFE_GPU_ESTATE, "fixed scratch not resolved for device %d", ctx->device);
+    return reconstruct_impl(ctx, node_slot, prior, B, R, f0, nf, states, ctx->fixed_scratch, kFixedScratchBytes);
 }
 
 int cafe_gpu_last_timings(cafe_gpu_ctx* ctx, double* ms4)

@@ -68,6 +68,7 @@ def load_library() -> ctypes.CDLL:
         "cafe_gpu_eval": (_i, [_p, ctypes.POINTER(EvalDesc), _pd, _pd, _pd, _pd, _pll]),
         "cafe_gpu_root_vectors": (_i, [_p, ctypes.POINTER(EvalDesc), _i, _i, _i, _i, _pd]),
         "cafe_gpu_reconstruct": (_i, [_p, _pi, _pd, _i, _i, _i, _i, _pi]),
+        "cafe_gpu_reconstruct_noalloc": (_i, [_p, _pi, _pd, _i, _i, _i, _i, _pi]),
         "cafe_gpu_last_timings": (_i, [_p, _pd]),
         "cafe_gpu_plan_create": (_i, [_p, ctypes.POINTER(EvalDesc), _i, _pd, _pd, _i, _i, ctypes.POINTER(_p)]),
         "cafe_gpu_plan_set_keys": (_i, [_p, _pd, _pd]),
@@ -91,7 +92,7 @@ EXPORTED_SYMBOLS = ["cafe_gpu_create", "cafe_gpu_destroy", "cafe_gpu_last_error"
                     "cafe_gpu_matrix_build", "cafe_gpu_matrix_get", "cafe_gpu_set_tree",
                     "cafe_gpu_set_families", "cafe_gpu_bucket_count", "cafe_gpu_bucket_info",
                     "cafe_gpu_set_error_model", "cafe_gpu_eval", "cafe_gpu_root_vectors", "cafe_gpu_reconstruct",
-                    "cafe_gpu_last_timings", "cafe_gpu_plan_create", "cafe_gpu_plan_set_keys",
+                    "cafe_gpu_reconstruct_noalloc", "cafe_gpu_last_timings", "cafe_gpu_plan_create", "cafe_gpu_plan_set_keys",
                     "cafe_gpu_plan_execute", "cafe_gpu_plan_synchronize", "cafe_gpu_plan_results",
                     "cafe_gpu_plan_profile", "cafe_gpu_plan_recompute_counts", "cafe_gpu_plan_class_times",
                     "cafe_gpu_plan_destroy"]
@@ -222,10 +223,11 @@ class Engine:
         return out.reshape(self.F, R)
 
     def reconstruct(self, node_slot: np.ndarray, prior: np.ndarray, B: int, R: int, f0: int = 0,
-                    nf: Optional[int] = None) -> np.ndarray:
+                    nf: Optional[int] = None, noalloc: bool = False) -> np.ndarray:
         """Pupko joint reconstruction of families f0 .. f0+nf-1 (all by
         default): states[nf, n_nodes] (gene_family_reconstructor.cpp:30-189);
-        prior[j], j = 0..min(B, R)."""
+        prior[j], j = 0..min(B, R).  noalloc: through
+        cafe_gpu_reconstruct_noalloc (fixed device scratch, no allocation)."""
         n_nodes = self.tree.n_nodes
         nf = self.F - f0 if nf is None else nf
         ns = np.ascontiguousarray(np.asarray(node_slot).reshape(-1)[:n_nodes], dtype=np.int32)
@@ -233,7 +235,8 @@ class Engine:
         if pr.size < min(B, R) + 1:
             raise ValueError("prior needs min(B, R) + 1 entries")
         out = np.zeros((max(nf, 0), n_nodes), dtype=np.int32)
-        self._check(self.lib.cafe_gpu_reconstruct(self.ctx, _ip(ns), _dp(pr), B, R, f0, nf, _ip(out)))
+        fn = self.lib.cafe_gpu_reconstruct_noalloc if noalloc else self.lib.cafe_gpu_reconstruct
+        self._check(fn(self.ctx, _ip(ns), _dp(pr), B, R, f0, nf, _ip(out)))
         return out
 
     def timings(self) -> np.ndarray:

@@ -179,6 +179,21 @@ int cafe_gpu_root_vectors(cafe_gpu_ctx *ctx, const cafe_gpu_eval_desc *desc, int
 int cafe_gpu_reconstruct(cafe_gpu_ctx *ctx, const int *node_slot, const double *prior, int B, int R,
                          int f0, int nf, int *states);
 
+/* The same reconstruction with no device allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as code
+ * under stream capture or a serving loop that forbids hipMalloc.  Arguments,
+ * results and errors are those of cafe_gpu_reconstruct; the working memory is
+ * a fixed 32 MiB device array compiled into the library (one per device, held
+ * until the process ends) instead of the context's grow-only buffers, so
+ * families run in launches sized to fit it.  A family whose rows alone need
+ * more (nodes x (B + 1) x 12 bytes, about 32 MiB) is rejected with
+ * CAFE_GPU_EINVAL before anything runs.  Every call on one device shares that
+ * array: calls that overlap on one device, from any context or thread, race.
+ * The inputs it reads (tree, families, matrices) are the context's, set up
+ * beforehand by the calls that allocate. */
+int cafe_gpu_reconstruct_noalloc(cafe_gpu_ctx *ctx, const int *node_slot, const double *prior, int B,
+                                 int R, int f0, int nf, int *states);
+
 /* --- device-resident plans (no reference counterpart) ----------------------
  * A plan keeps everything an evaluation needs on the device and replays it
  * as one HIP graph: matrix build for n_keys keys into pool slots
