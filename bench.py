@@ -320,6 +320,167 @@ def _cpu_port(w, n_lambda, n_sample, threads):
 
 # ----------------------------------------------------------------------------
 
+# ----------------------------------------------------------------------------
+# Error model + local lambdas (60 taxa, 3 lambda classes, eps 0.05, B = 150,
+# 100,000 families): split-FP16 MFMA against FP32, FP64 MFMA and PLAIN64
+# ----------------------------------------------------------------------------
+PEAK_FP16_MFMA = 2.5e15     # MI355X dense FP16 matrix, FLOP/s
+KERNEL_KIND[3] = "k_prune_f16"
+
+
+def _path_runs(eng, slots, prior, lam_q, t_q, paths, steps, warmup, fp64_steps, model=0, cat_probs=None):
+    out = {}
+    for path in paths:
+        p = eng.plan(slots, prior, lam_q, t_q, path=path, model=model, cat_probs=cat_probs, use_graph=True)
+        reduced = path in native.REDUCED_PATHS
+        n = steps if reduced else fp64_steps
+        ms, ph, graph_ok, cls = time_plan(p, n, warmup if reduced else 1, n if reduced else 1)
+        r = p.results()
+        rc = p.recompute_counts()
+        out[path] = {"ms": ms, "phases": ph, "graph": graph_ok, "classes": cls, "res": r, "recomputed": rc}
+        log(f"{native.PATH_NAMES[path]}: {ms:.3f} ms/step phases={np.round(ph, 3).tolist()} recomputed={rc}")
+        p.close()
+    return out
+
+
+def _accuracy(runs, test_path, ref_path):
+    a = runs[test_path]["res"]["lnl"]
+    b = runs[ref_path]["res"]["lnl"]
+    fin = np.isfinite(b)
+    d = np.abs(a[fin] - b[fin])
+    tr = np.abs(runs[test_path]["res"]["total"] - runs[ref_path]["res"]["total"]) / np.abs(runs[ref_path]["res"]["total"])
+    return {"reference_for_comparison": f"{native.PATH_NAMES[ref_path]} GPU path (matches the FP64 oracle to 1e-9 "
+                                        f"in tests)",
+            "max_abs_dlnL_per_family": float(d.max()) if d.size else 0.0,
+            "max_rel_dlnL_per_family": float((d / np.abs(b[fin])).max()) if d.size else 0.0,
+            "max_rel_dlnL_total": float(np.max(tr)),
+            "nonfinite_families_match": bool(np.array_equal(np.isfinite(a), fin)),
+            "families_recomputed_fp64": int(runs[test_path]["recomputed"][0]),
+            "families_recomputed_plain64": int(runs[test_path]["recomputed"][1])}
+
+
+def _class_roofline(run, flops, peak, unit_desc):
+    nt_cls, tiles_cls, ms_cls, launch_cls = run["classes"]
+    dom = int(np.argmax(ms_cls))
+    ms = float(ms_cls[dom])
+    dl = launch_cls[dom]
+    achieved = flops / (ms * 1e-3)
+    return {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": None,
+            "kernel": f"{KERNEL_KIND[int(dl[0])]}<{int(dl[1])},{int(dl[2])},{'lds' if dl[3] else 'global'}> main-pass "
+                      f"launch of the {int(nt_cls[dom])}-tile size class ({int(tiles_cls[dom])} tiles of "
+                      f"{int(dl[5])} families, {int(dl[4])} workgroups)",
+            "algorithmic_flops_per_launch": flops, "avg_ms": ms, "flops_per_family_likelihood": unit_desc}
+
+
+def _ref_cpu_base(w, B, R, err, node_lambda_classes, n_sample, threads):
+    """The compiled reference's base_model::infer_family_likelihoods (error
+    model, lambda tree) on a seeded subsample: one warm-up, median of 3, on
+    `threads` OpenMP threads and on one thread (a smaller sample)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Ref  # checker only
+    if not Ref.available():
+        return None
+    Ref.lib.ref_set_threads.restype = ctypes.c_int
+    Ref.lib.ref_set_threads.argtypes = [ctypes.c_int]
+    idx = _sample(w, n_sample, seed=17)
+    lams = w.lambdas
+    res = {}
+    for nt, n in ((threads, len(idx)), (1, max(1, len(idx) // 10))):
+        used = Ref.lib.ref_set_threads(nt)
+        c = np.ascontiguousarray(w.counts[idx[:n]])
+        times = []
+        for rep in range(4):
+            t0 = time.perf_counter()
+            Ref.base_model(w.tree, c, lams, B, R, err=err, node_class=node_lambda_classes)
+            times.append(time.perf_counter() - t0)
+        res[nt] = (used, n, float(np.median(times[1:])))
+    used, n, t = res[threads]
+    _, n1, t1 = res[1]
+    return {"value": n / t, "unit": "family likelihoods/s", "cores": int(used), "kind": "reference",
+            "one_thread_value": n1 / t1, "cpu_model": _cpu_model(),
+            "sample": (f"reference base_model::infer_family_likelihoods (oracle/_ref, -O3, OpenMP) with the error "
+                       f"model and the lambda tree on a seeded {n}-family subsample ({n1} on one thread): one "
+                       f"warm-up, median of 3; each call builds its matrices and runs the constructor's duplicate "
+                       f"scan (base_model.cpp:27-51)")}
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _host_threads() -> int:
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    return env or aff
+
+
+def main_config4(args):
+    F = 100000 if args.families == 20000 else args.families
+    B = R = 150
+    w = wl.config_error_multi(seed=4, F=F, bound=B)
+    err = wl.error_table(0.05, B)
+    log(f"workload: {w.name} F={F} taxa={w.tree.n_leaves} B={B} classes={w.lambdas.tolist()} eps=0.05")
+    eng = native.Engine(0)
+    eng.set_tree(w.tree)
+    eng.set_families(w.counts, B, R)
+    eng.set_error_model(err)
+    keys = native.KeyTable()
+    slots = keys.node_slots(w.tree, w.lambdas[w.node_class][None, :])
+    lam_q, t_q = keys.arrays()
+    eng.matrix_reset(B + 1, len(lam_q))
+    prior = priors.uniform(R, R)[None, :]
+    paths = (native.PATH_BATCHED16, native.PATH_BATCHED32, native.PATH_BATCHED64, native.PATH_PLAIN64)
+    runs = _path_runs(eng, slots, prior, lam_q, t_q, paths, args.steps, args.warmup, args.fp64_steps)
+    n1, n2 = tree_flop_factors(w.tree)
+    L = B + 1
+    flops = F * 2.0 * L * (n1 * L + n2 * R)
+    r16 = runs[native.PATH_BATCHED16]
+    out = {
+        "metric": "family likelihoods/sec, batched split-FP16 MFMA (error model + lambda classes) vs plain FP64 "
+                  "on GPU at |ΔlnL_total|≤1e-6",
+        "value": F / (r16["ms"] * 1e-3), "unit": "family likelihoods/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r16["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f16x2", "data": "synthetic",
+        "config": {"workload": f"errmodel-60taxa: seeded Yule tree (60 taxa, height 100), 3 branch-lambda classes "
+                               f"(0.001, 0.004, 0.01), {F} birth-death families, size bound 150, leaf error model "
+                               f"eps=0.05, {len(lam_q)} matrices of {B + 1}x{B + 1} per step",
+                   "path": "batched16 (split-FP16 MFMA + per-entry range bound + FP64 recompute)",
+                   "graph": bool(r16["graph"]), "families": F, "trials": 1},
+        "paths": {native.PATH_NAMES[p]: {"value": F / (runs[p]["ms"] * 1e-3), "ms_per_step": runs[p]["ms"],
+                                         "phases_ms": np.round(runs[p]["phases"], 4).tolist()} for p in paths},
+        "speedup_vs_plain64": runs[native.PATH_PLAIN64]["ms"] / r16["ms"],
+        "speedup_vs_batched32": runs[native.PATH_BATCHED32]["ms"] / r16["ms"],
+        "accuracy": _accuracy(runs, native.PATH_BATCHED16, native.PATH_BATCHED64),
+        "accuracy_batched32": _accuracy(runs, native.PATH_BATCHED32, native.PATH_BATCHED64),
+        "roofline": _class_roofline(r16, flops, PEAK_FP16_MFMA,
+                                    "2 L (n1 L + n2 R), n1=%d n2=%d, L=%d (algorithmic; the split operands issue "
+                                    "3 value MFMAs and 1 bound MFMA per product on 16-padded sizes)" % (n1, n2, L)),
+        "roofline_batched32": _class_roofline(runs[native.PATH_BATCHED32], flops, PEAK_FP32_MFMA,
+                                              "2 L (n1 L + n2 R)"),
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = _ref_cpu_base(w, B, R, err, w.node_class, min(2000, F), _host_threads())
+        log(f"cpu baseline: {out['cpu_baseline']}")
+    eng.close()
+    print(json.dumps(out))
+
+
+def main_config5(args):
+    raise SystemExit("config 5 is not wired yet")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -334,12 +495,19 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="run the FP32 plan eagerly --steps times plus a 256 MiB copy (for counter passes)")
+    ap.add_argument("--config", type=int, default=2, choices=(2, 4, 5),
+                    help="2: uneven-size lambda grid (the headline); 4: error model + lambda classes, split "
+                         "FP16; 5: optimizer-scale gamma run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
               "family sharding with one all-reduce of per-trial totals and is not built", file=sys.stderr)
         sys.exit(2)
 
+    if args.config == 4:
+        return main_config4(args)
+    if args.config == 5:
+        return main_config5(args)
     w = wl.config_uneven(seed=2, F=args.families, T=args.trials)
     if args.kernels_only:
         kernels_only(w, args.steps)

@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcafe_gpu.so")
-SOURCES = ["api.hip", "build_kernels.hip", "prune_kernels.hip", "prune_mfma.hip", "prune_lane.hip", "reconstruct.hip"]
+SOURCES = ["api.hip", "build_kernels.hip", "prune_kernels.hip", "prune_mfma.hip", "prune_lane.hip", "prune_f16.hip",
+           "reconstruct.hip"]
 ARCH = os.environ.get("CAFE_GPU_ARCH", "gfx950")
 
 

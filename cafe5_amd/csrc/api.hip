@@ -31,6 +31,9 @@ hipError_t launch_fold(const double* mt64, const float* lf32, double* out64, flo
 hipError_t launch_prune_plain(int nq, int grid, const PruneParams& p, hipStream_t st);
 hipError_t launch_prune_mfma(bool fp64, const MfmaShape& s, int grid, const PruneParams& p, hipStream_t st);
 hipError_t mfma_occupancy(bool fp64, const MfmaShape& s, int* per_cu);
+hipError_t f16_dispatch(const MfmaShape& s, int grid, const PruneParams& p, hipStream_t st, int* per_cu);
+hipError_t launch_split_f16(const float* mt32, _Float16* hi, _Float16* lo, int Np, long long slot_elems, int n_slots,
+                            hipStream_t st);
 int lane_rows_for(int rows);
 int lane_max_slots(int lane_rows);
 hipError_t lane_dispatch(int lr, int ns, int grid, size_t shm, const PruneParams& p, hipStream_t st, int* per_cu);
@@ -143,6 +146,7 @@ struct cafe_gpu_ctx {
     int N = 0, Np = 0, capacity = 0;
     long long slot_elems = 0;
     DevBuf mt64, mt32, lf32, key_lam, key_t;
+    DevBuf mt16h, mt16l;   // split-FP16 halves of mt32 (BATCHED16; made by its evaluations)
     std::vector<char> built;
 
     // tree program
@@ -365,7 +369,7 @@ int cafe_gpu_destroy(cafe_gpu_ctx* ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf* bufs[] = {&ctx->lg, &ctx->mt64, &ctx->mt32, &ctx->lf32, &ctx->key_lam, &ctx->key_t, &ctx->d_ops,
                       &ctx->d_kids, &ctx->d_steps, &ctx->d_steps_lane, &ctx->d_counts, &ctx->d_perm, &ctx->d_err64,
-                      &ctx->d_err32, &ctx->mt64e, &ctx->mt32e};
+                      &ctx->d_err32, &ctx->mt64e, &ctx->mt32e, &ctx->mt16h, &ctx->mt16l};
     for (DevBuf* b : bufs) b->release();
     ctx->eager.release_all();
     for (auto& e : ctx->ev)
@@ -654,6 +658,7 @@ struct KernelClass {
     int nq;              // PLAIN64 rows per lane (row s = lane + 64 q)
     int slot_tiles;      // 16-row tiles of its largest stored (non-root) vector
     MfmaShape s32, s64;  // launch shapes of the FP32 and FP64 MFMA kernels
+    MfmaShape s16;       // launch shape of the split-FP16 kernel (BATCHED16)
     MfmaShape r64;       // FP64 recompute of the FP32 pass (a few tiles: latency)
     int lane_rows = 0;   // > 0: the FP32 main pass runs k_prune_lane with this many rows (64-family tiles)
     int lane_slots = 0;  // its slot instance
@@ -678,6 +683,7 @@ struct PrunePlan {
     std::vector<int> main_order;     // main-pass launch order (longest first)
     int light_stream = 0;            // the side stream expected to finish its main-pass work first
     int plain_list_size = 0;
+    int n_split = 0;                 // BATCHED16: pool slots converted to FP16 halves before pruning
     int n_tiles = 0, n_segs = 0, maxLR = 0;
     int r16_total = 0;               // capacity of the FP64 recompute lists (16-family tiles)
     int waves_per_cu = 16;           // persistent grid: resident waves per CU
@@ -761,6 +767,29 @@ void place_partials(MfmaShape& s, bool fp64, int n_slots, int slot_tiles, int n_
     s.shm = meta + (s.lds ? frag : 0);
 }
 
+// The split-FP16 kernel takes vectors of up to 32 tiles (eight waves of four
+// tiles without spilling registers); BATCHED16 runs larger ones on the FP32
+// matrix-core kernel.
+constexpr int kF16MaxTiles = 32;
+
+// The split-FP16 kernel's partials: per 32-row block and lane three 16-byte
+// B operands (hi, lo, bound).
+size_t f16_frag_bytes(int n_slots, int slot_tiles) { return (size_t)3 * n_slots * ((slot_tiles + 1) / 2) * 64 * 16; }
+
+void place_partials16(MfmaShape& s, int n_slots, int slot_tiles, int n_leaves)
+{
+    const size_t meta = (size_t)f16_meta_bytes(n_slots, s.w) + (size_t)mfma_count_bytes(n_leaves);
+    const size_t frag = f16_frag_bytes(n_slots, slot_tiles);
+    // two workgroups per CU (its registers allow two waves per SIMD)
+    s.lds = s.w <= 4 && meta + frag + 4096 <= 76 * 1024;
+    s.shm = meta + (s.lds ? frag : 0);
+}
+
+size_t f16_scratch_bytes(const MfmaShape& s, int n_slots, int slot_tiles, int grid)
+{
+    return s.lds ? 0 : (size_t)grid * f16_frag_bytes(n_slots, slot_tiles);
+}
+
 // Dynamic LDS of a lane-kernel launch: the tile's leaf counts.
 size_t lane_shm(int n_leaves) { return (size_t)n_leaves * 64 * sizeof(int); }
 
@@ -783,7 +812,7 @@ int upload_eval_inputs(cafe_gpu_ctx* ctx, EvalBufs& b, const cafe_gpu_eval_desc*
         return fail(CAFE_GPU_EINVAL, "families have %d leaves, tree has %d", ctx->n_leaves, ctx->tree_leaves);
     if (d->model != CAFE_GPU_MODEL_BASE && d->model != CAFE_GPU_MODEL_GAMMA)
         return fail(CAFE_GPU_EINVAL, "unknown model %d", d->model);
-    if (d->path < CAFE_GPU_PATH_PLAIN64 || d->path > CAFE_GPU_PATH_BATCHED32)
+    if (d->path < CAFE_GPU_PATH_PLAIN64 || d->path > CAFE_GPU_PATH_BATCHED16)
         return fail(CAFE_GPU_EINVAL, "unknown path %d", d->path);
     const int K = d->n_cat;
     if (d->n_trials < 1 || K < 1 || (d->model == CAFE_GPU_MODEL_BASE && K != 1))
@@ -903,7 +932,7 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         key_rows[bc.first] = std::max(key_rows[bc.first], std::max(bk.B + 1, bk.R + 1));
     }
     auto lane_rows_of = [&](int key) {
-        if (path != CAFE_GPU_PATH_BATCHED32) return 0;
+        if (path != CAFE_GPU_PATH_BATCHED32 && path != CAFE_GPU_PATH_BATCHED16) return 0;
         const int lr = lane_rows_for(key_rows[key]);
         return lr > 0 && ctx->n_slots_lane <= lane_max_slots(lr) ? lr : 0;
     };
@@ -950,6 +979,19 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         HIP_OR_FAIL(mfma_occupancy(false, kc.s32, &kc.s32.per_cu));
         HIP_OR_FAIL(mfma_occupancy(true, kc.s64, &kc.s64.per_cu));
         HIP_OR_FAIL(mfma_occupancy(true, kc.r64, &kc.r64.per_cu));
+        if (path == CAFE_GPU_PATH_BATCHED16 && kc.nt <= kF16MaxTiles) {
+            // the program whose outputs reuse a child's slot (the store's
+            // maxima exchange orders all reads before the writes)
+            kc.s16 = mfma_shape(kc.nt, false);
+            place_partials16(kc.s16, ctx->n_slots_lane, kc.slot_tiles, ctx->n_leaves);
+            const PruneParams none{};
+            HIP_OR_FAIL(f16_dispatch(kc.s16, 0, none, nullptr, &kc.s16.per_cu));
+            if (kc.s16.per_cu == 0 && kc.s16.lds) {   // LDS does not fit: global scratch
+                kc.s16.lds = false;
+                kc.s16.shm = (size_t)f16_meta_bytes(ctx->n_slots_lane, kc.s16.w) + (size_t)mfma_count_bytes(ctx->n_leaves);
+                HIP_OR_FAIL(f16_dispatch(kc.s16, 0, none, nullptr, &kc.s16.per_cu));
+            }
+        }
         if (kc.r64.per_cu == 0 && kc.r64.lds) {   // the device does not fit that much LDS: global scratch
             kc.r64.lds = false;
             kc.r64.shm = (size_t)mfma_meta_bytes(ctx->n_slots, kc.r64.w, false) + (size_t)mfma_count_bytes(ctx->n_leaves);
@@ -1046,10 +1088,12 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         if (path == CAFE_GPU_PATH_PLAIN64) mb = plain_bytes(n);
         else if (path == CAFE_GPU_PATH_BATCHED64)
             mb = mfma_scratch_bytes(kc.s64, true, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s64, n));
-        else if (kc.lane_rows == 0) mb = mfma_scratch_bytes(kc.s32, false, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s32, n));
+        else if (kc.lane_rows > 0) mb = 0;
+        else if (path == CAFE_GPU_PATH_BATCHED16 && kc.nt <= kF16MaxTiles) mb = f16_scratch_bytes(kc.s16, ctx->n_slots_lane, kc.slot_tiles, pp.grid(kc.s16, n));
+        else mb = mfma_scratch_bytes(kc.s32, false, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s32, n));
         kc.off_main = main_sz;
         main_sz += align256(mb);
-        if (path == CAFE_GPU_PATH_BATCHED32) {
+        if (path == CAFE_GPU_PATH_BATCHED32 || path == CAFE_GPU_PATH_BATCHED16) {
             kc.off_r64 = r64_sz;
             r64_sz += align256(mfma_scratch_bytes(kc.r64, true, ctx->n_slots, kc.slot_tiles, pp.grid(kc.r64, kc.r16)));
         }
@@ -1094,6 +1138,17 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     p.root_stride = root_stride;
     p.fam_list = b.fam_list.as<int>();
     p.vec_exp = 120 / std::max(1, ctx->max_children) - kMatExp;
+    // split FP16: a factor is below 2^(kMatExp16 + a) and a node's product of
+    // max_children factors stays inside FP32; vectors stay inside FP16
+    p.vec_exp16 = std::min(15, 126 / std::max(1, ctx->max_children) - kMatExp16);
+    if (path == CAFE_GPU_PATH_BATCHED16) {
+        const size_t n16 = sizeof(_Float16) * (size_t)ctx->slot_elems * ctx->capacity;
+        HIP_OR_FAIL(ctx->mt16h.ensure(n16));
+        HIP_OR_FAIL(ctx->mt16l.ensure(n16));
+        pp.n_split = ctx->capacity;
+    }
+    p.mt16h = ctx->mt16h.as<_Float16>();
+    p.mt16l = ctx->mt16l.as<_Float16>();
     p.scratch = b.scratch.p;
     p.ldv = ldv;
     p.n_slots = ctx->n_slots;
@@ -1118,6 +1173,10 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
     char* scratch = (char*)pp.base.scratch;
     HIP_OR_FAIL(launch_fill_zero(b.counters.p, sizeof(int) * (3 * n_cls + 8), st));
     HIP_OR_FAIL(launch_fill_zero(b.fb_count.p, sizeof(int) * (2 * n_cls + 4), st));
+    // BATCHED16 reads the FP16 halves of the FP32 matrices
+    if (pp.path == CAFE_GPU_PATH_BATCHED16)
+        HIP_OR_FAIL(launch_split_f16(ctx->mt32.as<float>(), ctx->mt16h.as<_Float16>(), ctx->mt16l.as<_Float16>(),
+                                     ctx->Np, ctx->slot_elems, pp.n_split, st));
 
     // Three stages per size class: the main pass, then (BATCHED32) the FP64
     // recompute of the families the FP32 bound did not certify, then
@@ -1143,18 +1202,22 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
             HIP_OR_FAIL(launch_prune_plain(kc.nq, pp.grid(1, n), q, s));
         } else if (pp.path == CAFE_GPU_PATH_BATCHED64) {
             HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64, n), q, s));
-        } else if (kc.lane_rows > 0) {
+        } else if (kc.lane_rows > 0) {   // BATCHED32 / BATCHED16: small vectors, one family per lane
             q.steps = ctx->d_steps_lane.as<StepDev>();
             q.n_slots = ctx->n_slots_lane;
             HIP_OR_FAIL(lane_dispatch(kc.lane_rows, kc.lane_slots, pp.grid_lane(kc.lane_per_cu, n),
                                       lane_shm(ctx->n_leaves), q, s, nullptr));
+        } else if (pp.path == CAFE_GPU_PATH_BATCHED16 && kc.nt <= kF16MaxTiles) {
+            q.steps = ctx->d_steps_lane.as<StepDev>();
+            q.n_slots = ctx->n_slots_lane;
+            HIP_OR_FAIL(f16_dispatch(kc.s16, pp.grid(kc.s16, n), q, s, nullptr));
         } else {
             HIP_OR_FAIL(launch_prune_mfma(false, kc.s32, pp.grid(kc.s32, n), q, s));
         }
         return CAFE_GPU_OK;
     };
     auto stage_r64 = [&](int ci, hipStream_t s) -> int {
-        if (pp.path != CAFE_GPU_PATH_BATCHED32) return CAFE_GPU_OK;
+        if (pp.path != CAFE_GPU_PATH_BATCHED32 && pp.path != CAFE_GPU_PATH_BATCHED16) return CAFE_GPU_OK;
         const KernelClass& kc = pp.classes[ci];
         HIP_OR_FAIL(launch_pack(segs + kc.seg_begin, kc.seg_end - kc.seg_begin, b.buckets.as<BucketDev>(),
                                 pp.base.perm, b.flag.as<int>(), pp.base.F, kFlagRecompute64, b.fam_list.as<int>(),
@@ -1221,7 +1284,7 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
     };
     if (int rc = stage(0, stage_main)) return rc;
     if (ev_main) HIP_OR_FAIL(hipEventRecord(ev_main, st));
-    if (pp.path == CAFE_GPU_PATH_BATCHED32)
+    if (pp.path == CAFE_GPU_PATH_BATCHED32 || pp.path == CAFE_GPU_PATH_BATCHED16)
         if (int rc = stage(1, stage_r64)) return rc;
     if (pp.path != CAFE_GPU_PATH_PLAIN64)
         if (int rc = stage(2, stage_plain)) return rc;
@@ -1423,7 +1486,8 @@ int plan_enqueue(cafe_gpu_plan* pl, bool timed)
     // BATCHED32 reads the FP64 pool only in its recompute stages: concurrent
     // runs with the recurrence build make the FP32 pools first and the FP64
     // one beside the main pass
-    const bool split = !timed && pl->algo != 0 && pl->pp.path == CAFE_GPU_PATH_BATCHED32;
+    const bool split = !timed && pl->algo != 0 &&
+                       (pl->pp.path == CAFE_GPU_PATH_BATCHED32 || pl->pp.path == CAFE_GPU_PATH_BATCHED16);
     if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[0], st));
     HIP_OR_FAIL(launch_build(pl->algo, pl->d_lam.as<double>(), pl->d_t.as<double>(), ctx->lg.as<double>(), pl->n_keys,
                              0, ctx->N, ctx->Np, ctx->slot_elems, split ? nullptr : ctx->mt64.as<double>(),
@@ -1698,7 +1762,8 @@ int cafe_gpu_plan_class_times(cafe_gpu_plan* pl, int cap, int* n_classes, int* c
                 l[3] = 0;
                 l[4] = pl->pp.grid(1, n);
                 l[5] = 1;
-            } else if (pl->path == CAFE_GPU_PATH_BATCHED32 && kc.lane_rows > 0) {
+            } else if ((pl->path == CAFE_GPU_PATH_BATCHED32 || pl->path == CAFE_GPU_PATH_BATCHED16) &&
+                       kc.lane_rows > 0) {
                 l[0] = 1;
                 l[1] = kc.lane_rows;
                 l[2] = 1;
@@ -1706,8 +1771,9 @@ int cafe_gpu_plan_class_times(cafe_gpu_plan* pl, int cap, int* n_classes, int* c
                 l[4] = pl->pp.grid_lane(kc.lane_per_cu, n);
                 l[5] = 64;
             } else {
-                const MfmaShape& sh = pl->path == CAFE_GPU_PATH_BATCHED64 ? kc.s64 : kc.s32;
-                l[0] = 0;
+                const bool f16 = pl->path == CAFE_GPU_PATH_BATCHED16 && kc.nt <= kF16MaxTiles;
+                const MfmaShape& sh = pl->path == CAFE_GPU_PATH_BATCHED64 ? kc.s64 : f16 ? kc.s16 : kc.s32;
+                l[0] = f16 ? 3 : 0;
                 l[1] = sh.ntw;
                 l[2] = sh.w;
                 l[3] = sh.lds ? 1 : 0;

@@ -24,6 +24,10 @@ constexpr int kLgammaN = 4096;        // log-gamma table (integers < 4096)
 // when k (kMatExp + vec_exp) <= 126 (the host picks vec_exp from the tree's
 // largest child count) while entries far below each vector's max survive.
 constexpr int kMatExp = 27;
+// The split-FP16 pools (hi and lo halves, row-major, see prune_f16.hip) hold
+// every matrix times 2^kMatExp16: entries <= 1 stay below FP16's largest
+// finite value.
+constexpr int kMatExp16 = 15;
 
 // One unit of work of the pruning kernels: families fam0 .. fam0+nf-1 of a
 // size bucket (bucket-sorted order) at one parameter point.  nf < 0 marks a
@@ -136,6 +140,11 @@ struct PruneParams {
     // packed recompute tiles index families through this list
     const int* fam_list;
     int vec_exp;              // FP32 partial vectors are kept below 2^vec_exp
+    // split-FP16 path: matrix halves (row-major, 32-column blocks in lane
+    // order) and the renormalisation target of its partial vectors
+    const _Float16* mt16h;
+    const _Float16* mt16l;
+    int vec_exp16;
     // partial-vector scratch: per workgroup n_slots x 16 x ldv values
     void* scratch;
     long long scratch_per_wg;  // elements
@@ -174,6 +183,14 @@ __host__ __device__ inline int mfma_meta_bytes(int n_slots, int w, bool scaled)
 {
     if (!scaled) return 0;
     const int b = n_slots * 16 * 8 + n_slots * w * 16 * 4 + 2 * n_slots * 16 * 4 + n_slots * 4;
+    return (b + 31) & ~31;
+}
+
+// Dynamic-LDS bookkeeping of the split-FP16 kernel: the node's per-wave
+// maxima (float) and per slot the stored vector's exponent (int) per family.
+__host__ __device__ inline int f16_meta_bytes(int n_slots, int w)
+{
+    const int b = w * 16 * 4 + n_slots * 16 * 4;
     return (b + 31) & ~31;
 }
 

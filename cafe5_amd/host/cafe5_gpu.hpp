@@ -47,7 +47,7 @@ public:
     gpu_context(const gpu_context&) = delete;
     gpu_context& operator=(const gpu_context&) = delete;
     cafe_gpu_ctx* get() const { return _ctx; }
-    // pruning path: 0 plain FP64, 1 batched FP64 MFMA, 2 batched FP32 MFMA
+    // pruning path: 0 plain FP64, 1 batched FP64 MFMA, 2 batched FP32 MFMA, 3 batched split-FP16 MFMA
     int path = 2;
     // matrix build: 0 closed form (reference term order), 1 recurrence
     int build_algo = 1;

@@ -21,9 +21,14 @@ BUILD_RECURRENCE = 1
 PATH_PLAIN64 = 0
 PATH_BATCHED64 = 1
 PATH_BATCHED32 = 2
+PATH_BATCHED16 = 3
 MODEL_BASE = 0
 MODEL_GAMMA = 1
-PATH_NAMES = {PATH_PLAIN64: "plain64", PATH_BATCHED64: "batched64", PATH_BATCHED32: "batched32"}
+# the paths whose contractions run below FP64 (per-family accuracy bar, FP64
+# recompute of uncertified families)
+REDUCED_PATHS = (PATH_BATCHED32, PATH_BATCHED16)
+PATH_NAMES = {PATH_PLAIN64: "plain64", PATH_BATCHED64: "batched64", PATH_BATCHED32: "batched32",
+              PATH_BATCHED16: "batched16"}
 
 _i = ctypes.c_int
 _d = ctypes.c_double

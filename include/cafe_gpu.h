@@ -65,7 +65,16 @@ enum {
      * on what FP32 flushing could have lost, and families whose bound is not
      * below 1e-9 of their likelihood (or whose FP64 value would be
      * subnormal) are recomputed on the BATCHED64 path */
-    CAFE_GPU_PATH_BATCHED32 = 2
+    CAFE_GPU_PATH_BATCHED32 = 2,
+    /* families as columns, split-FP16 MFMA contractions (each operand as two
+     * FP16 halves, three products with FP32 accumulation, about FP32
+     * accuracy for in-range values) with the same per-family rescaling and
+     * exponent carry; every partial vector carries a per-entry bound on what
+     * FP16's narrow range can lose, propagated by a fourth MFMA, and families
+     * whose bound is not below 1e-9 of their likelihood are recomputed on the
+     * BATCHED64 path.  Vectors of at most 33 rows run the FP32 per-lane
+     * kernel of BATCHED32 (no matrix-core work there). */
+    CAFE_GPU_PATH_BATCHED16 = 3
 };
 
 enum {

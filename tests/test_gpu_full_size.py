@@ -4,8 +4,9 @@ keys) through the captured plan the benchmark replays, checked by properties
 that hold at any size:
 
   * BATCHED64 against PLAIN64 on every (point, family): 1e-9 relative;
-  * BATCHED32 against PLAIN64 on every (point, family): |dlnL| <= 1e-4 +
-    1e-6 |lnL|, and every point's total within 1e-6 relative;
+  * BATCHED32 and BATCHED16 against PLAIN64 on every (point, family): the
+    per-family bar of the reduced paths (tests/bars.py), and every point's
+    total within 1e-6 relative;
   * PLAIN64 against the oracle on a seeded sample of (point, family) pairs
     drawn from every bucket: 1e-9 relative;
   * replays of one plan give bit-identical per-family results (the graph
@@ -38,7 +39,7 @@ def full():
     for i, (_, r, _) in enumerate(buckets):
         prior[i, :r] = priors.uniform(r, r)
     out = {}
-    for path in (native.PATH_PLAIN64, native.PATH_BATCHED64, native.PATH_BATCHED32):
+    for path in (native.PATH_PLAIN64, native.PATH_BATCHED64, native.PATH_BATCHED32, native.PATH_BATCHED16):
         plan = eng.plan(slots, prior, lam_q, t_q, path=path)
         plan.execute()
         plan.synchronize()
@@ -64,17 +65,20 @@ def test_batched64_matches_plain64(full):
     assert np.max(np.abs(a[fin] - b[fin]) / np.abs(b[fin])) <= 1e-9
 
 
-def test_batched32_matches_plain64(full):
+@pytest.mark.parametrize("path", [native.PATH_BATCHED32, native.PATH_BATCHED16])
+def test_reduced_paths_match_plain64(full, path):
     _, _, _, out = full
-    a, b = out[native.PATH_BATCHED32]["lnl"], out[native.PATH_PLAIN64]["lnl"]
+    a, b = out[path]["lnl"], out[native.PATH_PLAIN64]["lnl"]
     assert np.array_equal(np.isfinite(a), np.isfinite(b))
     fin = np.isfinite(b)
     err = np.abs(a[fin] - b[fin])
+    ta, tb = out[path]["total"], out[native.PATH_PLAIN64]["total"]
+    print(f"{native.PATH_NAMES[path]}: max|dlnL|={err.max():.3e} max total rel="
+          f"{np.max(np.abs(ta - tb) / np.abs(tb)):.3e} recomputed={out[path]['n_fallback']}", flush=True)
     assert np.all(reduced_ok(a[fin], b[fin])), err.max()
-    ta, tb = out[native.PATH_BATCHED32]["total"], out[native.PATH_PLAIN64]["total"]
     assert np.all(np.abs(ta - tb) <= 1e-6 * np.abs(tb)), np.max(np.abs(ta - tb) / np.abs(tb))
-    # the families the flush bound could not certify were recomputed in FP64
-    assert out[native.PATH_BATCHED32]["n_fallback"] > 0
+    # the families the bounds could not certify were recomputed in FP64
+    assert out[path]["n_fallback"] > 0
 
 
 def test_plain64_matches_oracle_on_a_sample(full):

@@ -23,7 +23,7 @@ from oracle_lib import Oracle
 
 pytestmark = pytest.mark.gpu
 
-PATHS = [native.PATH_PLAIN64, native.PATH_BATCHED64, native.PATH_BATCHED32]
+PATHS = [native.PATH_PLAIN64, native.PATH_BATCHED64, native.PATH_BATCHED32, native.PATH_BATCHED16]
 
 
 @pytest.fixture(scope="module")
@@ -47,7 +47,7 @@ def assert_lnl(got, want, path, what=""):
     want = np.asarray(want)
     assert np.array_equal(np.isfinite(got), np.isfinite(want)), (what, np.flatnonzero(np.isfinite(got) != np.isfinite(want))[:5])
     fin = np.isfinite(want)
-    if path == native.PATH_BATCHED32:
+    if path in native.REDUCED_PATHS:
         ok = reduced_ok(got[fin], want[fin])
         assert np.all(ok), (what, np.abs(got[fin] - want[fin]).max())
     else:
@@ -114,7 +114,7 @@ def test_root_vector_known_values(eng, path):
     model, slots = _setup_prune(eng, tree, counts, 0.045, 20, 20, native.BUILD_CLOSED_FORM)
     got = eng.root_vectors(slots, np.full((1, 20), 0.05), path)[0]
     want = Oracle.prune(tree, counts[0], np.full(tree.n_nodes, 0.045), 20, 20)
-    tol = 1e-5 if path == native.PATH_BATCHED32 else 1e-12
+    tol = 1e-5 if path in native.REDUCED_PATHS else 1e-12
     assert np.all(_rel(got, want) <= tol), _rel(got, want).max()
 
 
@@ -127,8 +127,10 @@ def test_root_vectors_seeded_tree(eng, path):
     got = eng.root_vectors(slots, np.full((1, R), 1.0 / R), path)
     for f in range(counts.shape[0]):
         want = Oracle.prune(w.tree, counts[f], np.full(w.tree.n_nodes, 0.002), B, R)
-        big = want > want.max() * 1e-30
-        tol = 2e-5 if path == native.PATH_BATCHED32 else 1e-11
+        # split FP16 keeps about FP32 accuracy for entries within ~2^-50 of the
+        # vector's maximum (what the likelihood uses); FP32 down to 1e-30
+        big = want > want.max() * (1e-8 if path == native.PATH_BATCHED16 else 1e-30)
+        tol = 2e-5 if path in native.REDUCED_PATHS else 1e-11
         assert np.all(_rel(got[f][big], want[big]) <= tol), (f, _rel(got[f][big], want[big]).max())
 
 
@@ -181,7 +183,7 @@ def test_base_model_seeded(eng, path):
     total = m.infer_family_likelihoods(prior, 0.002)
     ref = Oracle.evaluate(w.tree, np.full(w.tree.n_nodes, 0.002), w.counts, B, R, prior, threads=8)
     assert_lnl(m.results, ref["lnl"], path)
-    assert abs(total - ref["total"]) <= 1e-6 * abs(ref["total"]) if path == native.PATH_BATCHED32 else \
+    assert abs(total - ref["total"]) <= 1e-6 * abs(ref["total"]) if path in native.REDUCED_PATHS else \
         abs(total - ref["total"]) <= 1e-9 * abs(ref["total"])
 
 
@@ -199,7 +201,7 @@ def test_uneven_buckets_lambda_grid(eng, path):
         want_total = -np.sum([Oracle.evaluate(w.tree, np.full(w.tree.n_nodes, lam),
                                               w.counts[w.bounds == B], B, B, Oracle.prior_uniform(B, B),
                                               threads=8)["lnl"].sum() for (B, _, _) in m.buckets])
-        tol = 1e-6 if path == native.PATH_BATCHED32 else 1e-9
+        tol = 1e-6 if path in native.REDUCED_PATHS else 1e-9
         assert abs(totals[ti] - want_total) <= tol * abs(want_total), (lam, totals[ti], want_total)
 
 
@@ -214,9 +216,9 @@ def test_gamma_model_seeded(eng, path):
     ref = Oracle.evaluate(w.tree, np.full(w.tree.n_nodes, 0.003), w.counts, B, R, prior, n_cat=4,
                           multipliers=r, cat_probs=f, threads=8)
     assert_lnl(m.results["lnl"], ref["lnl"], path)
-    tol = 1e-5 if path == native.PATH_BATCHED32 else 1e-9
+    tol = 1e-5 if path in native.REDUCED_PATHS else 1e-9
     assert np.all(np.abs(m.results["post"] - ref["post"]) <= tol)
-    assert abs(total - ref["total"]) <= (1e-6 if path == native.PATH_BATCHED32 else 1e-9) * abs(ref["total"])
+    assert abs(total - ref["total"]) <= (1e-6 if path in native.REDUCED_PATHS else 1e-9) * abs(ref["total"])
 
 
 @pytest.mark.parametrize("path", PATHS)
@@ -281,7 +283,7 @@ def test_large_bound_gamma_500(eng, path):
     m = GammaModel(eng, tree, counts, 500, 500, f, r, path=path, alpha=1.0)
     total = m.infer_family_likelihoods(prior, 0.002)
     assert_lnl(m.results["lnl"], ref["lnl"], path)
-    tol = 1e-6 if path == native.PATH_BATCHED32 else 1e-9
+    tol = 1e-6 if path in native.REDUCED_PATHS else 1e-9
     assert abs(total - ref["total"]) <= tol * abs(ref["total"])
 
 
@@ -292,5 +294,5 @@ def test_large_bound_base_600(eng, path):
     m = BaseModel(eng, tree, counts, 600, 600, path=path)
     total = m.infer_family_likelihoods(prior, 0.002)
     assert_lnl(m.results, ref["lnl"], path)
-    tol = 1e-6 if path == native.PATH_BATCHED32 else 1e-9
+    tol = 1e-6 if path in native.REDUCED_PATHS else 1e-9
     assert abs(total - ref["total"]) <= tol * abs(ref["total"])
