@@ -27,6 +27,7 @@ import ctypes
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -40,6 +41,8 @@ from cafe5_amd import native, priors, workloads as wl  # noqa: E402
 METRIC = "family likelihoods/sec, batched FP32-MFMA vs plain FP64 on GPU at |ΔlnL|≤1e-6"
 PEAK_FP32_MFMA = 157.3e12     # MI355X dense FP32 matrix, FLOP/s
 PEAK_HBM = 8.0e12             # MI355X HBM3E, B/s
+PEAK_FP32_VALU_UNPACKED = 78.6e12   # FP32 vector FMAs without packed FP32 (half the 157.3 figure)
+PEAK_FP64 = 78.6e12           # MI355X FP64 (vector and matrix), FLOP/s
 KERNEL_KIND = {0: "k_prune_mfma", 1: "k_prune_lane", 2: "k_prune_plain"}
 
 
@@ -241,12 +244,20 @@ def _extrapolate(times_tiny, times_sample, n_tiny, n_sample, n_full):
 
 
 def _cpu_reference(so, w, n_lambda, n_sample, threads):
+    """The compiled reference (base_model::infer_family_likelihoods with its
+    OpenMP loops) on the host: per size bucket a session on 8 families and one
+    on the bucket's share of a seeded n_sample-family subsample, each
+    evaluated at n_lambda grid lambdas with one warm-up and the median of 3;
+    the per-evaluation matrix build and the per-family cost are separated by
+    the two session sizes and extrapolated to every family.  The sessions'
+    construction (the duplicate-family scan of base_model.cpp:27-51) is timed
+    apart.  Run on `threads` threads and on one thread (an eighth of the
+    sample, two lambdas)."""
     lib = ctypes.CDLL(so)
     lib.ref_init.restype = None
     lib.ref_init()
     lib.ref_set_threads.restype = ctypes.c_int
     lib.ref_set_threads.argtypes = [ctypes.c_int]
-    used = lib.ref_set_threads(threads)
     lib.ref_base_session_create.restype = ctypes.c_void_p
     lib.ref_base_session_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
@@ -256,39 +267,54 @@ def _cpu_reference(so, w, n_lambda, n_sample, threads):
     lib.ref_base_session_free.argtypes = [ctypes.c_void_p]
     newick = w.tree.newick().encode()
     names = " ".join(w.tree.leaf_names()).encode()
-    idx = _sample(w, n_sample)
-    lams = w.lambdas[np.linspace(0, len(w.lambdas) - 1, n_lambda).round().astype(int)]
-    est_total = 0.0
-    wall = 0.0
-    for B in sorted(set(w.bounds.tolist())):
-        full = int((w.bounds == B).sum())
-        sel = idx[w.bounds[idx] == B]
-        if len(sel) == 0:
-            sel = np.flatnonzero(w.bounds == B)[:1]
-        n_tiny = min(8, len(sel))
-        sessions = []
-        for part in (sel[:n_tiny], sel):
-            c = np.ascontiguousarray(w.counts[part], dtype=np.int32)
-            h = lib.ref_base_session_create(newick, names, len(part), c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                                            B, B, 0, B, 0.0)
-            sessions.append((h, c, len(part)))
-        for lam in lams:
-            t = []
-            for h, _, _ in sessions:
-                t0 = time.perf_counter()
-                lib.ref_base_session_eval(h, float(lam), None)
-                t.append(time.perf_counter() - t0)
-            wall += sum(t)
-            est_total += _extrapolate(t[0], t[1], sessions[0][2], sessions[1][2], full)
-        for h, _, _ in sessions:
-            lib.ref_base_session_free(h)
     F = w.counts.shape[0]
-    return {"value": F * len(lams) / est_total, "unit": "family likelihoods/s", "cores": int(used),
-            "kind": "reference",
-            "sample": (f"reference base_model::infer_family_likelihoods (oracle/_ref, OpenMP) at {len(lams)} of the "
-                       f"64 grid lambdas on a seeded {len(idx)}-family subsample plus 8-family runs per bucket; "
-                       f"matrix build and per-family cost extrapolated to all {F} families "
-                       f"({wall:.1f} s of CPU wall time)")}
+
+    def run(nt, n_fam, n_lam):
+        used = lib.ref_set_threads(nt)
+        idx = _sample(w, n_fam)
+        lams = w.lambdas[np.linspace(0, len(w.lambdas) - 1, n_lam).round().astype(int)]
+        est_total = wall = scan = 0.0
+        for B in sorted(set(w.bounds.tolist())):
+            members = np.flatnonzero(w.bounds == B)
+            full = len(members)
+            sel = idx[w.bounds[idx] == B]
+            if len(sel) < 24:   # enough families to separate the build from the per-family cost
+                sel = members[:min(24, full)]
+            n_tiny = min(8, max(1, len(sel) // 3))
+            sessions = []
+            for part in (sel[:n_tiny], sel):
+                c = np.ascontiguousarray(w.counts[part], dtype=np.int32)
+                t0 = time.perf_counter()
+                h = lib.ref_base_session_create(newick, names, len(part),
+                                                c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), B, B, 0, B, 0.0)
+                scan += time.perf_counter() - t0
+                sessions.append((h, len(part)))
+            for lam in lams:
+                med = []
+                for h, _ in sessions:
+                    ts = []
+                    for _ in range(4):
+                        t0 = time.perf_counter()
+                        lib.ref_base_session_eval(h, float(lam), None)
+                        ts.append(time.perf_counter() - t0)
+                    wall += sum(ts)
+                    med.append(float(np.median(ts[1:])))
+                est_total += _extrapolate(med[0], med[1], sessions[0][1], sessions[1][1], full)
+            for h, _ in sessions:
+                lib.ref_base_session_free(h)
+        return F * len(lams) / est_total, int(used), wall, scan, len(idx), len(lams)
+
+    value, used, wall, scan, n_s, n_l = run(threads, n_sample, n_lambda)
+    v1, _, wall1, _, n_s1, n_l1 = run(1, max(64, n_sample // 8), 2)
+    return {"value": value, "unit": "family likelihoods/s", "cores": used, "kind": "reference",
+            "one_thread_value": v1, "cpu_model": _cpu_model(),
+            "duplicate_scan_s": scan,
+            "sample": (f"reference base_model::infer_family_likelihoods (oracle/_ref, -O3 -march=x86-64-v3, OpenMP "
+                       f"over families) at {n_l} of the 64 grid lambdas on a seeded {n_s}-family subsample plus "
+                       f"8-family sessions per bucket, one warm-up and the median of 3 per evaluation; matrix "
+                       f"build and per-family cost extrapolated to all {F} families ({wall:.1f} s of CPU wall "
+                       f"time); one thread: {n_l1} lambdas on {n_s1} families ({wall1:.1f} s); the sessions' "
+                       f"duplicate scan (base_model.cpp:27-51) is reported apart")}
 
 
 def _cpu_port(w, n_lambda, n_sample, threads):
@@ -375,8 +401,12 @@ def _class_roofline(run, flops, peak, unit_desc):
 
 def _ref_cpu_base(w, B, R, err, node_lambda_classes, n_sample, threads):
     """The compiled reference's base_model::infer_family_likelihoods (error
-    model, lambda tree) on a seeded subsample: one warm-up, median of 3, on
-    `threads` OpenMP threads and on one thread (a smaller sample)."""
+    model, lambda tree) on seeded subsamples of 32 and n_sample families, one
+    warm-up and the median of 3 each: the per-evaluation matrix build and the
+    per-family cost, separated by the two sizes and extrapolated to every
+    family, on `threads` OpenMP threads and on one thread (a quarter of the
+    sample).  Each call also runs the constructor's duplicate scan
+    (base_model.cpp:27-51), O(n^2) in the subsample and included."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Ref  # checker only
     if not Ref.available():
@@ -384,28 +414,39 @@ def _ref_cpu_base(w, B, R, err, node_lambda_classes, n_sample, threads):
     Ref.lib.ref_set_threads.restype = ctypes.c_int
     Ref.lib.ref_set_threads.argtypes = [ctypes.c_int]
     idx = _sample(w, n_sample, seed=17)
-    lams = w.lambdas
+    F = w.counts.shape[0]
     res = {}
-    for nt, n in ((threads, len(idx)), (1, max(1, len(idx) // 10))):
+    for nt, n in ((threads, len(idx)), (1, max(64, len(idx) // 4))):
         used = Ref.lib.ref_set_threads(nt)
-        c = np.ascontiguousarray(w.counts[idx[:n]])
-        times = []
-        for rep in range(4):
-            t0 = time.perf_counter()
-            Ref.base_model(w.tree, c, lams, B, R, err=err, node_class=node_lambda_classes)
-            times.append(time.perf_counter() - t0)
-        res[nt] = (used, n, float(np.median(times[1:])))
-    used, n, t = res[threads]
-    _, n1, t1 = res[1]
-    return {"value": n / t, "unit": "family likelihoods/s", "cores": int(used), "kind": "reference",
-            "one_thread_value": n1 / t1, "cpu_model": _cpu_model(),
-            "sample": (f"reference base_model::infer_family_likelihoods (oracle/_ref, -O3, OpenMP) with the error "
-                       f"model and the lambda tree on a seeded {n}-family subsample ({n1} on one thread): one "
-                       f"warm-up, median of 3; each call builds its matrices and runs the constructor's duplicate "
-                       f"scan (base_model.cpp:27-51)")}
+        med = []
+        for m in (32, n):
+            c = np.ascontiguousarray(w.counts[idx[:m]])
+            times = []
+            for rep in range(4):
+                t0 = time.perf_counter()
+                Ref.base_model(w.tree, c, w.lambdas, B, R, err=err, node_class=node_lambda_classes)
+                times.append(time.perf_counter() - t0)
+            med.append(float(np.median(times[1:])))
+        res[nt] = (used, n, _extrapolate(med[0], med[1], 32, n, F), med)
+    used, n, est, med = res[threads]
+    _, n1, est1, med1 = res[1]
+    return {"value": F / est, "unit": "family likelihoods/s", "cores": int(used), "kind": "reference",
+            "one_thread_value": F / est1, "cpu_model": _cpu_model(),
+            "sample": (f"reference base_model::infer_family_likelihoods (oracle/_ref, -O3 -march=x86-64-v3, OpenMP) "
+                       f"with the error model and the lambda tree on seeded 32- and {n}-family subsamples "
+                       f"({med[0]:.2f} s and {med[1]:.2f} s; one thread: 32 and {n1} families, {med1[0]:.2f} s and "
+                       f"{med1[1]:.2f} s), one warm-up and the median of 3; matrix build and per-family cost "
+                       f"extrapolated to all {F} families")}
 
 
 def _cpu_model() -> str:
+    try:
+        res = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10)
+        for line in res.stdout.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         with open("/proc/cpuinfo") as fh:
             for line in fh:
@@ -477,8 +518,150 @@ def main_config4(args):
     print(json.dumps(out))
 
 
+def _ref_cpu_session(w, B, R, lam, n_small, n_sample, threads, seed=19):
+    """The compiled reference's base_model on the workload's tree at bounds
+    (B, R) and one lambda: a session (tree, families, model; the
+    constructor's duplicate scan, timed apart) evaluated on n_small and
+    n_sample families, one warm-up and the median of 3 each, on `threads`
+    OpenMP threads.  Returns (build seconds, seconds per family, scan
+    seconds, threads used): an evaluation rebuilds every matrix and prunes
+    every family (base_model.cpp:53-100)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Ref  # checker only
+    if not Ref.available():
+        return None
+    lib = Ref.lib
+    lib.ref_set_threads.restype = ctypes.c_int
+    lib.ref_set_threads.argtypes = [ctypes.c_int]
+    lib.ref_base_session_create.restype = ctypes.c_void_p
+    lib.ref_base_session_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    lib.ref_base_session_eval.restype = ctypes.c_double
+    lib.ref_base_session_eval.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]
+    lib.ref_base_session_free.argtypes = [ctypes.c_void_p]
+    used = lib.ref_set_threads(threads)
+    idx = _sample(w, n_sample, seed=seed)
+    newick = w.tree.newick().encode()
+    names = " ".join(w.tree.leaf_names()).encode()
+    times, scan = {}, 0.0
+    for n in (n_small, n_sample):
+        c = np.ascontiguousarray(w.counts[idx[:n]], dtype=np.int32)
+        t0 = time.perf_counter()
+        h = lib.ref_base_session_create(newick, names, n, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), B, R,
+                                        0, R, 0.0)
+        if n == n_sample:
+            scan = time.perf_counter() - t0
+        ts = []
+        for _ in range(4):
+            t0 = time.perf_counter()
+            lib.ref_base_session_eval(h, float(lam), None)
+            ts.append(time.perf_counter() - t0)
+        lib.ref_base_session_free(h)
+        times[n] = float(np.median(ts[1:]))
+    per_fam = max(0.0, (times[n_sample] - times[n_small]) / (n_sample - n_small))
+    build = max(0.0, times[n_small] - n_small * per_fam)
+    return build, per_fam, scan, int(used)
+
+
 def main_config5(args):
-    raise SystemExit("config 5 is not wired yet")
+    """Optimizer-scale gamma run: 100 taxa, 200,000 families, B = 500, K = 8;
+    the first --evals of the 200 seeded (lambda, alpha) trial points scored
+    one after another through one captured plan per path (the scoring loop
+    of gamma_lambda_optimizer: keys set, graph replayed, total read)."""
+    from cafe5_amd.model import ResidentGammaModel
+    F = 200000 if args.families == 20000 else args.families
+    K, B, T = 8, 500, 200
+    w = wl.config_optimizer(seed=5, F=F, K=K, bound=B, T=T)
+    n_eval = max(1, min(args.evals, T))
+    log(f"workload: {w.name} F={F} taxa={w.tree.n_leaves} B={B} K={K}; scoring {n_eval} of {T} trial points")
+    eng = native.Engine(0)
+    prior = priors.uniform(B, B)
+    n1, n2 = tree_flop_factors(w.tree)
+    L = B + 1
+    flops_fam = K * 2.0 * L * (n1 * L + n2 * B)
+    runs = {}
+    for path in (native.PATH_BATCHED16, native.PATH_BATCHED32, native.PATH_BATCHED64):
+        m = ResidentGammaModel(eng, w.tree, w.counts, B, B, K, prior, path=path)
+        m.infer_family_likelihoods(float(w.lambdas[0]), float(w.alphas[0]), records=False)   # warm-up
+        totals, recomputed = [], []
+        t0 = time.perf_counter()
+        for t in range(n_eval):
+            totals.append(m.infer_family_likelihoods(float(w.lambdas[t]), float(w.alphas[t]), records=False))
+            recomputed.append(int(m.last["n_fallback"]))
+        ms = (time.perf_counter() - t0) * 1e3 / n_eval
+        # one profiled execution at trial 0: phases and per-class kernel times
+        m.plan.set_keys(*m._keys(float(w.lambdas[0]), get_gamma_rates(K, float(w.alphas[0]))))
+        ph, graph_ok = m.plan.profile()
+        cls = m.plan.class_times()
+        m.plan.execute()
+        lnl0 = m.plan.results()["lnl"][0]
+        runs[path] = {"ms": ms, "totals": np.array(totals), "recomputed": recomputed, "phases": ph,
+                      "graph": graph_ok, "classes": cls, "lnl0": lnl0}
+        log(f"{native.PATH_NAMES[path]}: {ms:.1f} ms per evaluation, phases={np.round(ph, 2).tolist()} "
+            f"recomputed={recomputed[:4]}...")
+        m.close()
+    head = native.PATH_BATCHED32
+    r = runs[head]
+    b64 = runs[native.PATH_BATCHED64]
+    fin = np.isfinite(b64["totals"])
+    acc = {}
+    for path in (native.PATH_BATCHED32, native.PATH_BATCHED16):
+        tr = np.abs(runs[path]["totals"][fin] - b64["totals"][fin]) / np.abs(b64["totals"][fin])
+        a, b = runs[path]["lnl0"], b64["lnl0"]
+        ok = np.isfinite(b)
+        acc[native.PATH_NAMES[path]] = {
+            "reference_for_comparison": "batched64 GPU path (matches the FP64 oracle to 1e-9 in tests)",
+            "max_rel_dlnL_total_over_evaluations": float(tr.max()) if tr.size else None,
+            "max_abs_dlnL_per_family_trial0": float(np.max(np.abs(a[ok] - b[ok]))),
+            "nonfinite_totals_match": bool(np.array_equal(np.isfinite(runs[path]["totals"]), fin)),
+            "families_recomputed_fp64_mean": float(np.mean(runs[path]["recomputed"]))}
+    out = {
+        "metric": "family likelihoods/sec, batched FP32-MFMA gamma model (K=8) over an optimizer's trial points "
+                  "vs FP64 on GPU at |ΔlnL_total|≤1e-6",
+        "value": F / (r["ms"] * 1e-3), "unit": "family likelihoods/s", "n_gpus": 1, "steps": n_eval,
+        "warmup": 1, "ms_per_step": r["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"optimizer-100taxa: seeded Yule tree (100 taxa, height 100), {F} birth-death "
+                               f"families, size bound 500, K=8 gamma categories, {n_eval} of 200 seeded (lambda, "
+                               f"alpha) trial points scored one after another through one captured plan "
+                               f"(1584 matrices of 501x501 rebuilt per evaluation)",
+                   "path": "batched32", "graph": bool(r["graph"]), "families": F, "evaluations": n_eval},
+        "ms_per_200_evaluations": r["ms"] * 200,
+        "paths": {native.PATH_NAMES[p]: {"value": F / (runs[p]["ms"] * 1e-3), "ms_per_evaluation": runs[p]["ms"],
+                                         "ms_per_200_evaluations": runs[p]["ms"] * 200,
+                                         "phases_ms": np.round(runs[p]["phases"], 3).tolist()} for p in runs},
+        "finite_evaluations": int(fin.sum()),
+        "accuracy": acc,
+        "roofline": _class_roofline(r, F * flops_fam, PEAK_FP32_MFMA,
+                                    "K 2 L (n1 L + n2 R), K=%d n1=%d n2=%d L=%d" % (K, n1, n2, L)),
+        "roofline_batched16": _class_roofline(runs[native.PATH_BATCHED16], F * flops_fam, PEAK_FP16_MFMA,
+                                              "K 2 L (n1 L + n2 R) against the dense FP16 peak"),
+        "roofline_batched64": _class_roofline(b64, F * flops_fam, 78.6e12, "K 2 L (n1 L + n2 R) against FP64"),
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        threads = _host_threads()
+        est = _ref_cpu_session(w, B, B, float(w.lambdas[0]), 4, 36, threads)
+        if est:
+            build, per_fam, scan, used = est
+            eval_s = K * build + K * per_fam * F
+            out["cpu_baseline"] = {
+                "value": F / eval_s, "unit": "family likelihoods/s", "cores": used, "kind": "reference",
+                "cpu_model": _cpu_model(), "seconds_per_evaluation": eval_s,
+                "sample": (f"reference base_model::infer_family_likelihoods (oracle/_ref, -O3, OpenMP) on the same "
+                           f"tree at B=R=500 and one lambda (198 matrices of 501x501), 4- and 36-family sessions, "
+                           f"one warm-up and the median of 3: matrix build {build:.2f} s and {per_fam * 1e3:.1f} ms "
+                           f"per family, scaled by K=8 (8x the keys, 8 prunes per family) to all {F} families; the "
+                           f"constructor's duplicate scan ({scan:.3f} s for 36 families) is not included")}
+            log(f"cpu baseline: {out['cpu_baseline']}")
+    eng.close()
+    print(json.dumps(out))
+
+
+def get_gamma_rates(K, alpha):
+    from cafe5_amd.model import get_gamma
+    return get_gamma(K, alpha)[1]
 
 
 def main():
@@ -489,12 +672,13 @@ def main():
     ap.add_argument("--families", type=int, default=20000)
     ap.add_argument("--trials", type=int, default=64)
     ap.add_argument("--fp64-steps", type=int, default=3, help="timed steps of the FP64 comparison paths")
-    ap.add_argument("--cpu-lambdas", type=int, default=8)
+    ap.add_argument("--cpu-lambdas", type=int, default=4)
     ap.add_argument("--cpu-families", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="run the FP32 plan eagerly --steps times plus a 256 MiB copy (for counter passes)")
+    ap.add_argument("--evals", type=int, default=10, help="config 5: trial points scored per path")
     ap.add_argument("--config", type=int, default=2, choices=(2, 4, 5),
                     help="2: uneven-size lambda grid (the headline); 4: error model + lambda classes, split "
                          "FP16; 5: optimizer-scale gamma run")
@@ -532,6 +716,11 @@ def main():
     ms64, ph64, _, _ = time_plan(p64, args.fp64_steps, 1, 1)
     r64 = p64.results()
     log(f"batched64: {ms64:.3f} ms/step phases={ph64.round(3).tolist()}")
+    p16 = eng.plan(slots, prior, lam_q, t_q, path=native.PATH_BATCHED16, use_graph=use_graph)
+    ms16, ph16, _, _ = time_plan(p16, args.fp64_steps, 1, 1)
+    r16 = p16.results()
+    rc16 = p16.recompute_counts()
+    log(f"batched16: {ms16:.3f} ms/step phases={ph16.round(3).tolist()} recomputed={rc16}")
     pplain = eng.plan(slots, prior, lam_q, t_q, path=native.PATH_PLAIN64, use_graph=use_graph)
     msplain, phplain, _, _ = time_plan(pplain, args.fp64_steps, 1, 1)
     rplain = pplain.results()
@@ -576,19 +765,22 @@ def main():
     achieved = dom_flops / (dom_ms * 1e-3) if dom_ms > 0 else None
     phase_flops = sum(cnt * T * 2.0 * (B + 1) * (n1 * (B + 1) + n2 * R) for B, R, cnt in buckets)
     dl = launch_cls[dom]
-    if int(dl[0]) == 1:
+    lane_dom = int(dl[0]) == 1
+    if lane_dom:
         dom_kernel = f"void k_prune_lane<{int(dl[1])}, "
-        dom_desc = (f"k_prune_lane<{int(dl[1])},*> (one family per lane, FP32 FMA; the same 157.3 TF/s dense FP32 "
-                    f"peak) main-pass launch of the {dom_nt}-tile size class (buckets B={dom_buckets}, "
-                    f"{int(tiles_cls[dom])} tiles of 64 families)")
+        dom_desc = (f"k_prune_lane<{int(dl[1])},*> (one family per lane, FP32 vector FMAs, no MFMA) main-pass "
+                    f"launch of the {dom_nt}-tile size class (buckets B={dom_buckets}, {int(tiles_cls[dom])} tiles "
+                    f"of 64 families)")
     else:
         dom_kernel = f"void k_prune_mfma<float, {int(dl[1])}, {int(dl[2])}, true, {'true' if dl[3] else 'false'}>"
         dom_desc = (f"k_prune_mfma<float,{int(dl[1])},{int(dl[2])},true,*> main-pass launch of the {dom_nt}-tile "
                     f"size class (buckets B={dom_buckets}, {int(tiles_cls[dom])} tiles of 16 families)")
     traffic = read_traffic(dom_nt, dom_kernel)
     build_bytes = len(lam_q) * N * N * (8 + 4 + 4)
-    roofline = {"bound": "mfma", "achieved": achieved / 1e12 if achieved else None, "peak": PEAK_FP32_MFMA / 1e12,
-                "unit": "TFLOP/s", "frac": (achieved / PEAK_FP32_MFMA) if achieved else None,
+    roofline = {"bound": "valu" if lane_dom else "mfma", "achieved": achieved / 1e12 if achieved else None,
+                "peak": PEAK_FP32_MFMA / 1e12, "unit": "TFLOP/s",
+                "frac": (achieved / PEAK_FP32_MFMA) if achieved else None,
+                "frac_of_unpacked_fp32_valu": (achieved / PEAK_FP32_VALU_UNPACKED) if (achieved and lane_dom) else None,
                 "traffic": traffic["bytes"] if traffic else None,
                 "kernel": dom_desc,
                 "algorithmic_flops_per_launch": dom_flops, "avg_ms": dom_ms,
@@ -606,6 +798,19 @@ def main():
                                  "GB_per_s": build_bytes / (ph32[0] * 1e-3) / 1e9 if ph32[0] > 0 else None,
                                  "hbm_frac": build_bytes / (ph32[0] * 1e-3) / PEAK_HBM if ph32[0] > 0 else None}}
 
+    # PLAIN64 (the drop-in): its flops (the same contractions, plus the dense
+    # leaf matrix-vector products the reference makes) against the FP64 peak,
+    # and the matrix bytes its per-family products read against HBM
+    n_leaf = w.tree.n_leaves
+    plain_flops = sum(cnt * T * 2.0 * (B + 1) * (n1 * (B + 1) + n2 * R + n_leaf * (B + 1)) for B, R, cnt in buckets)
+    plain_bytes = sum(cnt * T * 8.0 * (B + 1) * (n1 * (B + 1) + n2 * R + n_leaf * (B + 1)) for B, R, cnt in buckets)
+    plain_s = phplain[1] * 1e-3
+    plain_roofline = {"bound": "fp64", "achieved_TFLOP_per_s": plain_flops / plain_s / 1e12,
+                      "frac_of_fp64_peak": plain_flops / plain_s / PEAK_FP64,
+                      "matrix_bytes_GB_per_s": plain_bytes / plain_s / 1e9,
+                      "frac_of_hbm": plain_bytes / plain_s / PEAK_HBM,
+                      "note": "prune phase of one profiled PLAIN64 execution; flops and matrix bytes count every "
+                              "leaf branch as the reference's dense matrix-vector product"}
     value = F * T / (ms32 * 1e-3)
     out = {
         "metric": METRIC,
@@ -625,14 +830,20 @@ def main():
                                f"{T}-point lambda grid 1e-4..1e-2, {len(lam_q)} matrices of {N}x{N} per step",
                    "path": "batched32 (FP32 MFMA + certified FP64 recompute)",
                    "graph": bool(graph_ok), "families": F, "trials": T},
-        "plain64": {"value": F * T / (msplain * 1e-3), "ms_per_step": msplain},
+        "plain64": {"value": F * T / (msplain * 1e-3), "ms_per_step": msplain,
+                    "roofline": plain_roofline},
+        "batched16": {"value": F * T / (ms16 * 1e-3), "ms_per_step": ms16,
+                      "families_recomputed_fp64": int(rc16[0]),
+                      "max_abs_dlnL_per_family": float(np.max(np.abs(r16["lnl"][fin] - b[fin]))),
+                      "max_rel_dlnL_total": float(np.max(np.abs(r16["total"] - rplain["total"]) /
+                                                         np.abs(rplain["total"])))},
         "batched64": {"value": F * T / (ms64 * 1e-3), "ms_per_step": ms64},
         "speedup_vs_plain64": msplain / ms32,
         "accuracy": accuracy,
         "roofline": roofline,
         "cpu_baseline": None,
     }
-    for p in (p32, p64, pplain):
+    for p in (p32, p64, p16, pplain):
         p.close()
     # the reconstruction next to the likelihood path (an extra measurement:
     # its failure is reported, not fatal to the headline line)
@@ -643,7 +854,7 @@ def main():
     except Exception as e:  # noqa: BLE001
         out["reconstruct"] = {"error": str(e)}
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        threads = _host_threads()
         log(f"cpu baseline on {threads} threads ...")
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_lambdas, args.cpu_families, threads)
         log(f"cpu baseline: {out['cpu_baseline']}")

@@ -32,7 +32,8 @@
 //     off by at most 2^-36; the contraction's input bound is
 //     ê = e + 2^-36 [0 < v < 2^-14];
 //   - representation of M: at most 2^-36 per entry below 2^-14, so at most
-//     2^-36 Lp 2^a over a row (v < 2^a);
+//     2^-36 sum_x v over a row (sum_x v <= (1 + 2^-10) sum_x vhi + Lp 2^-25;
+//     sum_x vhi is one more all-ones MFMA per k-block);
 //   - through M: sum_x M ê <= (1 + 2^-9) Mhi ê + 2^-14 sum_x ê, since
 //     M <= Mhi (1 + 2^-10) where Mhi is normal and M < 2^-14 elsewhere; the
 //     first term is a fourth MFMA whose B operand is ê x 2^30 rounded up, the
@@ -189,8 +190,6 @@ __global__ __launch_bounds__(64 * W) void k_prune_f16(PruneParams p)
         const int nts = bk.Lp / kTile;           // 16-row tiles of every child vector
         const int nkb = (nts + 1) >> 1;          // 32-row k-blocks
         const int rows_root = bk.Rp / kTile;
-        // the matrix's representation floor over a row (v below 2^a)
-        const float floorA = kFloor16 * (float)bk.Lp * ldexpf(1.f, a_exp);
 
         int offA0[NTW], offA1[NTW], offL0[NTW], offL1[NTW];
 #pragma unroll
@@ -257,7 +256,8 @@ __global__ __launch_bounds__(64 * W) void k_prune_f16(PruneParams p)
                 const _Float16* __restrict__ Ml = pooll + mo;
                 const long long b0 = (long long)sd.in_slot * slot_blocks * 64 + lane;
                 floatx4 xh[NTW], xl[NTW], xb[NTW];
-                floatx4 xs = floatx4{0, 0, 0, 0};
+                floatx4 xs = floatx4{0, 0, 0, 0};   // sum_x ê x 2^30, every row
+                floatx4 xv = floatx4{0, 0, 0, 0};   // sum_x vhi, every row
 #pragma unroll
                 for (int i = 0; i < NTW; ++i) {
                     xh[i] = floatx4{0, 0, 0, 0};
@@ -278,7 +278,8 @@ __global__ __launch_bounds__(64 * W) void k_prune_f16(PruneParams p)
                         xl[i] = mfma32(a_l, bh, xl[i]);
                         xb[i] = mfma32(a_h, be, xb[i]);
                     }
-                    xs = mfma32(ones, be, xs);   // sum over the block of ê, every row
+                    xs = mfma32(ones, be, xs);   // sums over the block of ê and of vhi
+                    xv = mfma32(ones, bh, xv);
                 };
                 block(pf[0], pf[1], fh[b0], flo[b0], fe[b0]);
                 if (nkb > 1) {
@@ -329,6 +330,8 @@ __global__ __launch_bounds__(64 * W) void k_prune_f16(PruneParams p)
                     }
                 }
                 const float s_e = xs[0] * 0x1p-14f;
+                // the matrix's representation floor over a row
+                const float floorA = kFloor16 * (xv[0] * (1.f + 0x1p-9f) + (float)bk.Lp * 0x1p-25f);
 #pragma unroll
                 for (int i = 0; i < NTW; ++i) {
                     x[i] = xh[i] + xl[i] * kLoInv;

@@ -1502,3 +1502,21 @@ void gamma_lambda_optimizer::finalize(double* results)
 }
 
 }  // namespace cafe5_gpu
+
+// get_gamma for callers outside C++ (the Python mirror's gamma model): K
+// category frequencies and rate multipliers at shape alpha; 0 on success.
+extern "C" int cafe5_get_gamma(int K, double alpha, double* freq, double* rate)
+{
+    if (K < 1 || !freq || !rate) return 1;
+    try {
+        std::vector<double> f(K), r(K);
+        cafe5_gpu::get_gamma(f, r, alpha);
+        for (int i = 0; i < K; ++i) {
+            freq[i] = f[i];
+            rate[i] = r[i];
+        }
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}

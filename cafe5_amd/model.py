@@ -11,7 +11,9 @@ evaluated in one call: the engine batches all of them in one launch.
 """
 from __future__ import annotations
 
+import ctypes
 import math
+import os
 from typing import Optional, Sequence
 
 import numpy as np
@@ -137,3 +139,108 @@ class GammaModel(_ModelBase):
         self.results = {"cat": res["cat"][0], "post": res["post"][0], "lnl": res["lnl"][0],
                         "fam": np.exp(res["lnl"][0])}
         return float(res["total"][0])
+
+
+# --------------------------------------------------------------------------
+# discrete gamma categories through the C++ mirror (gamma.cpp:15-52)
+# --------------------------------------------------------------------------
+_HOST = None
+
+
+def _host_library():
+    global _HOST
+    if _HOST is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcafe5_host.so")
+        if not os.path.exists(path):
+            raise native.CafeGpuError(f"{path} is not built (python -m cafe5_amd.build)")
+        lib = ctypes.CDLL(path)
+        lib.cafe5_get_gamma.restype = ctypes.c_int
+        lib.cafe5_get_gamma.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double)]
+        _HOST = lib
+    return _HOST
+
+
+def get_gamma(K: int, alpha: float):
+    """(frequencies, rate multipliers) of K median-discretised gamma
+    categories of shape alpha, the PAML routines CAFE5 uses (gamma.cpp)."""
+    freq = np.zeros(K)
+    rate = np.zeros(K)
+    dp = ctypes.POINTER(ctypes.c_double)
+    if _host_library().cafe5_get_gamma(K, float(alpha), freq.ctypes.data_as(dp), rate.ctypes.data_as(dp)) != 0:
+        raise ValueError(f"get_gamma failed for K={K} alpha={alpha}")
+    return freq, rate
+
+
+class ResidentGammaModel(_ModelBase):
+    """gamma_model (gamma_core.cpp:168-237) for an optimizer's sequence of
+    (lambda, alpha) trial points (gamma_lambda_optimizer,
+    optimizer_scorer.cpp:147-180), with everything kept on the device.
+
+    The first call captures one plan for K categories: key k * n_t + i is
+    (lambda m_k, t_i) over the tree's distinct branch lengths t_i, and every
+    branch's matrix slot is fixed.  A trial point only recomputes the K
+    multipliers (get_gamma at alpha), quantizes the K n_t keys as
+    matrix_cache_key does, sets them and replays the plan (matrix build,
+    pruning, recompute, reductions)."""
+
+    def __init__(self, engine, tree, counts, max_family_size, max_root_family_size, K: int, prior,
+                 error_model=None, path=native.PATH_BATCHED32, build_algo=native.BUILD_RECURRENCE,
+                 use_graph: bool = True):
+        super().__init__(engine, tree, counts, max_family_size, max_root_family_size, error_model, path,
+                         build_algo)
+        self.K = K
+        lengths = [float(tree.length[i]) for i in range(tree.n_nodes) if i != tree.root]
+        self.t_keys = sorted({native.key_branch_length(t) for t in lengths})
+        t_index = {t: i for i, t in enumerate(self.t_keys)}
+        n_t = len(self.t_keys)
+        slots = np.zeros((K, tree.n_nodes), dtype=np.int32)
+        for k in range(K):
+            for u in range(tree.n_nodes):
+                if u != tree.root:
+                    slots[k, u] = k * n_t + t_index[native.key_branch_length(float(tree.length[u]))]
+        self.node_slots = slots
+        self.longest = max(lengths)
+        self.cat_probs = np.full(K, 1.0 / K)
+        freq, rate = get_gamma(K, 1.0)
+        lam_q, t_q = self._keys(0.001, rate)
+        if engine.N != self.N or getattr(engine, "_capacity", 0) < len(lam_q):
+            engine.matrix_reset(self.N, len(lam_q))
+            engine._capacity = len(lam_q)
+        self.plan = engine.plan(slots, self._prior_table(prior), lam_q, t_q, path=path, model=native.MODEL_GAMMA,
+                                cat_probs=self.cat_probs, algo=build_algo, use_graph=use_graph)
+        self.results = None
+
+    def _keys(self, lam: float, rate: np.ndarray):
+        n_t = len(self.t_keys)
+        lam_q = np.empty(self.K * n_t)
+        t_q = np.empty(self.K * n_t)
+        for k in range(self.K):
+            lam_q[k * n_t:(k + 1) * n_t] = native.key_lambda(lam * float(rate[k]))
+            t_q[k * n_t:(k + 1) * n_t] = self.t_keys
+        return lam_q, t_q
+
+    def can_infer(self, lam: float, alpha: float) -> bool:
+        """gamma_core.cpp:123-141: a valid lambda, alpha >= 0, and the
+        un-multiplied lambda does not saturate the longest branch."""
+        return lam > 0 and alpha >= 0 and not is_saturated(self.longest, lam)
+
+    def infer_family_likelihoods(self, lam: float, alpha: float, records: bool = True) -> float:
+        """-lnL at (lambda, alpha); +inf where the reference returns it.
+        Per-family records in self.results (cat, post, lnl) unless
+        records=False (an optimizer's score needs only the total)."""
+        if not self.can_infer(lam, alpha):
+            self.results = None
+            return math.inf
+        freq, rate = get_gamma(self.K, alpha)
+        self.multipliers = rate
+        self.plan.set_keys(*self._keys(lam, rate))
+        self.plan.execute()
+        res = self.plan.results(records)
+        self.last = res
+        self.results = {"cat": res["cat"][0], "post": res["post"][0], "lnl": res["lnl"][0]} if records else None
+        return float(res["total"][0])
+
+    def close(self):
+        self.plan.close()
+

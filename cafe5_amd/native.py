@@ -281,11 +281,13 @@ class Plan:
     def synchronize(self):
         self.eng._check(self.lib.cafe_gpu_plan_synchronize(self.h))
 
-    def results(self) -> Dict[str, np.ndarray]:
+    def results(self, records: bool = True) -> Dict[str, np.ndarray]:
+        """The last execution's outputs; records=False copies only the
+        totals and the recompute count (what an optimizer's score needs)."""
         F = self.eng.F
-        lnl = np.zeros((self.T, F))
-        cat = np.zeros((self.T, F, self.K)) if self.model == MODEL_GAMMA else None
-        post = np.zeros((self.T, F, self.K)) if self.model == MODEL_GAMMA else None
+        lnl = np.zeros((self.T, F)) if records else None
+        cat = np.zeros((self.T, F, self.K)) if self.model == MODEL_GAMMA and records else None
+        post = np.zeros((self.T, F, self.K)) if self.model == MODEL_GAMMA and records else None
         total = np.zeros(self.T)
         nfb = ctypes.c_longlong(0)
         self.eng._check(self.lib.cafe_gpu_plan_results(self.h, _dp(lnl), _dp(cat), _dp(post), _dp(total),

@@ -318,6 +318,7 @@ class Workload:
     n_cat: int = 0
     alpha: float = 0.0
     epsilon: float = 0.0
+    alphas: Optional[np.ndarray] = None       # gamma shape of each trial point (optimizer runs)
 
 
 def config_base(seed: int = 1, F: int = 2000) -> Workload:
@@ -389,14 +390,16 @@ def config_error_multi(seed: int = 4, F: int = 100000, bound: int = 150) -> Work
 
 
 def config_optimizer(seed: int = 5, F: int = 200000, K: int = 8, bound: int = 500, T: int = 200) -> Workload:
-    """100 taxa, F families, bound 500, K = 8, T seeded (lambda, alpha) points."""
+    """100 taxa, F families, bound 500, K = 8, T seeded (lambda, alpha) trial
+    points: lambda log-uniform in [1e-3, 5e-3], alpha log-uniform in [0.2, 2]."""
     tree = yule_tree(100, seed)
     rng = np.random.Generator(np.random.PCG64(seed + 1000))
     roots = heavy_tail_roots(rng, F, body_max=10, tail_max=300)
     counts = simulate_families(tree, roots, np.full(tree.n_nodes, 0.002), seed + 2000, cap=bound - 1)
     lam = np.exp(rng.uniform(np.log(1e-3), np.log(5e-3), T))
+    alphas = np.exp(rng.uniform(np.log(0.2), np.log(2.0), T))
     return Workload("optimizer-100taxa", tree, counts, np.full(F, bound, np.int32), lam, n_cat=K,
-                    alpha=1.0)
+                    alpha=1.0, alphas=alphas)
 
 
 def error_table(epsilon: float, B: int) -> np.ndarray:
