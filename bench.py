@@ -50,6 +50,15 @@ def log(msg: str):
     print(msg, file=sys.stderr, flush=True)
 
 
+def load_tuning(eng: native.Engine, path: str):
+    """Load the tile-shape cache into the engine; what was loaded, for the
+    output line."""
+    if not path or path == "none" or not os.path.exists(path):
+        return {"file": None, "entries": 0, "note": "built-in shapes"}
+    n = eng.tuning_load(path)
+    return {"file": os.path.relpath(path, ROOT), "entries": n}
+
+
 def tree_flop_factors(tree: wl.Tree):
     """(n1, n2): interior non-root nodes whose parent is interior / the root.
     A family likelihood at bound B (L = B+1 entries, R root sizes) costs
@@ -473,6 +482,7 @@ def main_config4(args):
     err = wl.error_table(0.05, B)
     log(f"workload: {w.name} F={F} taxa={w.tree.n_leaves} B={B} classes={w.lambdas.tolist()} eps=0.05")
     eng = native.Engine(0)
+    tuning = load_tuning(eng, args.tuning)
     eng.set_tree(w.tree)
     eng.set_families(w.counts, B, R)
     eng.set_error_model(err)
@@ -509,6 +519,7 @@ def main_config4(args):
                                     "3 value MFMAs and 1 bound MFMA per product on 16-padded sizes)" % (n1, n2, L)),
         "roofline_batched32": _class_roofline(runs[native.PATH_BATCHED32], flops, PEAK_FP32_MFMA,
                                               "2 L (n1 L + n2 R)"),
+        "tuning": tuning,
         "cpu_baseline": None,
     }
     if not args.no_cpu:
@@ -576,6 +587,7 @@ def main_config5(args):
     n_eval = max(1, min(args.evals, T))
     log(f"workload: {w.name} F={F} taxa={w.tree.n_leaves} B={B} K={K}; scoring {n_eval} of {T} trial points")
     eng = native.Engine(0)
+    tuning = load_tuning(eng, args.tuning)
     prior = priors.uniform(B, B)
     n1, n2 = tree_flop_factors(w.tree)
     L = B + 1
@@ -638,6 +650,7 @@ def main_config5(args):
         "roofline_batched16": _class_roofline(runs[native.PATH_BATCHED16], F * flops_fam, PEAK_FP16_MFMA,
                                               "K 2 L (n1 L + n2 R) against the dense FP16 peak"),
         "roofline_batched64": _class_roofline(b64, F * flops_fam, 78.6e12, "K 2 L (n1 L + n2 R) against FP64"),
+        "tuning": tuning,
         "cpu_baseline": None,
     }
     if not args.no_cpu:
@@ -679,6 +692,8 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="run the FP32 plan eagerly --steps times plus a 256 MiB copy (for counter passes)")
     ap.add_argument("--evals", type=int, default=10, help="config 5: trial points scored per path")
+    ap.add_argument("--tuning", default=os.path.join(ROOT, "cafe5_amd", "tune_cache.txt"),
+                    help="tile-shape cache (cafe5_amd/tune.py) loaded into the engine; 'none' for built-in shapes")
     ap.add_argument("--config", type=int, default=2, choices=(2, 4, 5),
                     help="2: uneven-size lambda grid (the headline); 4: error model + lambda classes, split "
                          "FP16; 5: optimizer-scale gamma run")
@@ -699,8 +714,9 @@ def main():
     F, T = w.counts.shape[0], len(w.lambdas)
     log(f"workload: {w.name} F={F} T={T} taxa={w.tree.n_leaves}")
     eng = native.Engine(0)
+    tuning = load_tuning(eng, args.tuning)
     buckets, slots, lam_q, t_q, prior, N = setup(eng, w)
-    log(f"buckets (B,R,count): {buckets}; keys={len(lam_q)} N={N}")
+    log(f"buckets (B,R,count): {buckets}; keys={len(lam_q)} N={N}; tuning={tuning}")
     use_graph = not args.no_graph
 
     # ---- headline: BATCHED32 ----
@@ -841,6 +857,7 @@ def main():
         "speedup_vs_plain64": msplain / ms32,
         "accuracy": accuracy,
         "roofline": roofline,
+        "tuning": tuning,
         "cpu_baseline": None,
     }
     for p in (p32, p64, p16, pplain):

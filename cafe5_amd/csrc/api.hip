@@ -176,6 +176,11 @@ struct cafe_gpu_ctx {
     // buffers of eager evaluations (plans own their own)
     EvalBufs eager;
     long long pool_gen = 0;   // bumped by matrix_reset; plans check it
+    // measured launch shapes: (path, tiles per vector, leaves) -> shape
+    struct TuneEntry {
+        int path, nt, leaves, kernel, ntw, w, lds, grid_mult;
+    };
+    std::vector<TuneEntry> tuning;
     // bumped by set_tree, set_families and by set_error_model when it adds or
     // removes the model or changes its shape (buffers a plan's graph points
     // into); plans check it.  Same-shape error-model rows are updated in place.
@@ -663,6 +668,8 @@ struct KernelClass {
     int lane_rows = 0;   // > 0: the FP32 main pass runs k_prune_lane with this many rows (64-family tiles)
     int lane_slots = 0;  // its slot instance
     int lane_per_cu = 0; // its workgroups resident per CU
+    int lane_grid_mult = 0;   // workgroups per resident slot (0: default)
+    bool tuned = false;  // its shape came from the context's tuning table
     int r16 = 0;         // 16-family tiles its families fill (the recompute lists' capacity)
     int r_begin = 0;     // offset of its FP64 recompute list
     int begin, end;      // its main tiles
@@ -702,9 +709,9 @@ struct PrunePlan {
         return g >= 8 ? g / 8 * 8 : g;
     }
     // a lane-kernel launch: as an MFMA launch with partials in LDS
-    int grid_lane(int per_cu, int max_tiles) const
+    int grid_lane(int per_cu, int max_tiles, int mult = 0) const
     {
-        const long long want = (long long)n_cu * std::max(1, per_cu) * 16;
+        const long long want = (long long)n_cu * std::max(1, per_cu) * (mult > 0 ? mult : 16);
         const int g = (int)std::max(1LL, std::min(want, (long long)max_tiles));
         return g >= 8 ? g / 8 * 8 : g;
     }
@@ -717,7 +724,7 @@ struct PrunePlan {
     int grid(const MfmaShape& s, int max_tiles) const
     {
         const int per_cu = s.per_cu > 0 ? s.per_cu : std::max(1, waves_per_cu / std::max(1, s.w));
-        const long long want = (long long)n_cu * per_cu * (s.lds ? 16 : 2);
+        const long long want = (long long)n_cu * per_cu * (s.grid_mult > 0 ? s.grid_mult : (s.lds ? 16 : 2));
         const int g = (int)std::max(1LL, std::min(want, (long long)max_tiles));
         return g >= 8 ? g / 8 * 8 : g;
     }
@@ -798,6 +805,67 @@ size_t mfma_scratch_bytes(const MfmaShape& s, bool fp64, int n_slots, int slot_t
 {
     if (s.lds) return 0;
     return (size_t)grid * n_slots * slot_tiles * 64 * 4 * (fp64 ? 8 : 4);
+}
+
+// The tuning entry for a class, if the context holds one.
+const cafe_gpu_ctx::TuneEntry* find_tuning(const cafe_gpu_ctx* ctx, int path, int nt)
+{
+    for (const auto& e : ctx->tuning)
+        if (e.path == path && e.nt == nt && e.leaves == ctx->n_leaves) return &e;
+    return nullptr;
+}
+
+// The matrix-core shape a tuning entry names for a class of nt tiles whose
+// stored vectors have slot_tiles tiles, when that kernel instance exists,
+// covers the class (tiles per wave x waves >= nt) and fits a CU (occupancy
+// query; LDS partials only where they fit).
+bool tuned_mfma_shape(cafe_gpu_ctx* ctx, int path, const cafe_gpu_ctx::TuneEntry& e, int nt, int slot_tiles,
+                      MfmaShape& s)
+{
+    if (e.kernel != 0 || path == CAFE_GPU_PATH_PLAIN64 || e.ntw < 1 || e.w < 1 || e.ntw * e.w < nt) return false;
+    const bool f16 = path == CAFE_GPU_PATH_BATCHED16;
+    if (f16 && nt > kF16MaxTiles) return false;
+    const bool fp64 = path == CAFE_GPU_PATH_BATCHED64;
+    s = MfmaShape{e.ntw, e.w, e.lds != 0, 0, 0};
+    if (f16) place_partials16(s, ctx->n_slots_lane, slot_tiles, ctx->n_leaves);
+    else place_partials(s, fp64, ctx->n_slots, slot_tiles, ctx->n_leaves);
+    if (e.lds && !s.lds) return false;   // LDS partials that do not fit
+    if (!e.lds && s.lds) {               // global scratch asked for
+        s.lds = false;
+        s.shm = f16 ? (size_t)f16_meta_bytes(ctx->n_slots_lane, s.w) + (size_t)mfma_count_bytes(ctx->n_leaves)
+                    : (size_t)mfma_meta_bytes(ctx->n_slots, s.w, !fp64) + (size_t)mfma_count_bytes(ctx->n_leaves);
+    }
+    const PruneParams none{};
+    const hipError_t q = f16 ? f16_dispatch(s, 0, none, nullptr, &s.per_cu) : mfma_occupancy(fp64, s, &s.per_cu);
+    if (q != hipSuccess || s.per_cu < 1) {
+        (void)hipGetLastError();
+        return false;   // no such instance, or it does not fit a CU
+    }
+    s.grid_mult = std::max(0, e.grid_mult);
+    return true;
+}
+
+// Replaces a class's built-in launch shape by its tuning entry where the entry
+// is usable (the kernel itself -- per lane or matrix cores -- was chosen when
+// the class's tiles were made); otherwise the class keeps its shape.
+int apply_tuning(cafe_gpu_ctx* ctx, int path, KernelClass& kc)
+{
+    const cafe_gpu_ctx::TuneEntry* e = find_tuning(ctx, path, kc.nt);
+    if (!e || path == CAFE_GPU_PATH_PLAIN64) return CAFE_GPU_OK;
+    if (kc.lane_rows > 0) {
+        if (e->kernel == 1) {
+            kc.lane_grid_mult = std::max(0, e->grid_mult);
+            kc.tuned = true;
+        }
+        return CAFE_GPU_OK;
+    }
+    MfmaShape s;
+    if (!tuned_mfma_shape(ctx, path, *e, kc.nt, kc.slot_tiles, s)) return CAFE_GPU_OK;
+    if (path == CAFE_GPU_PATH_BATCHED16) kc.s16 = s;
+    else if (path == CAFE_GPU_PATH_BATCHED64) kc.s64 = s;
+    else kc.s32 = s;
+    kc.tuned = true;
+    return CAFE_GPU_OK;
 }
 
 // Validates a descriptor and uploads node slots, priors, bucket table and
@@ -946,7 +1014,20 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
             KernelClass kc{};
             kc.begin = kc.end = (int)tiles.size();
             kc.seg_begin = kc.seg_end = (int)segs.size();
+            // the kernel is chosen here: its tiles carry 64 families (per
+            // lane) or 16 (matrix cores)
             kc.lane_rows = lane_rows_of(bc.first);
+            if (const cafe_gpu_ctx::TuneEntry* te = find_tuning(ctx, path, bc.first)) {
+                MfmaShape probe;
+                if (kc.lane_rows > 0 && tuned_mfma_shape(ctx, path, *te, bc.first, bc.first, probe)) kc.lane_rows = 0;
+            }
+            if (kc.lane_rows > 0) {
+                kc.lane_slots = ctx->n_slots_lane;
+                const PruneParams none{};
+                HIP_OR_FAIL(lane_dispatch(kc.lane_rows, kc.lane_slots, 0, lane_shm(ctx->n_leaves), none, nullptr,
+                                          &kc.lane_per_cu));
+                if (kc.lane_per_cu == 0) kc.lane_rows = 0;   // no instance fits: matrix-core kernel
+            }
             pp.classes.push_back(kc);
             cls_fams.push_back(0);
             prev_key = bc.first;
@@ -997,14 +1078,11 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
             kc.r64.shm = (size_t)mfma_meta_bytes(ctx->n_slots, kc.r64.w, false) + (size_t)mfma_count_bytes(ctx->n_leaves);
             HIP_OR_FAIL(mfma_occupancy(true, kc.r64, &kc.r64.per_cu));
         }
-        if (kc.lane_rows > 0) {
-            kc.lane_slots = ctx->n_slots_lane;
-            const PruneParams none{};
-            HIP_OR_FAIL(lane_dispatch(kc.lane_rows, kc.lane_slots, 0, lane_shm(ctx->n_leaves), none, nullptr,
-                                      &kc.lane_per_cu));
-            if (kc.lane_per_cu == 0) kc.lane_rows = 0;   // no instance fits: matrix-core kernel
-        }
     }
+    // Measured shapes from the context's tuning table replace the built-in
+    // ones where they exist and fit the class.
+    for (auto& kc : pp.classes)
+        if (int rc = apply_tuning(ctx, path, kc)) return rc;
     // Main-pass classes to side streams, longest first onto the least loaded
     // stream.  Estimated cost: contraction work (tiles x nt^2, plus a per-tile
     // overhead) and a floor for the serial chain of one large tile.
@@ -1205,7 +1283,7 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
         } else if (kc.lane_rows > 0) {   // BATCHED32 / BATCHED16: small vectors, one family per lane
             q.steps = ctx->d_steps_lane.as<StepDev>();
             q.n_slots = ctx->n_slots_lane;
-            HIP_OR_FAIL(lane_dispatch(kc.lane_rows, kc.lane_slots, pp.grid_lane(kc.lane_per_cu, n),
+            HIP_OR_FAIL(lane_dispatch(kc.lane_rows, kc.lane_slots, pp.grid_lane(kc.lane_per_cu, n, kc.lane_grid_mult),
                                       lane_shm(ctx->n_leaves), q, s, nullptr));
         } else if (pp.path == CAFE_GPU_PATH_BATCHED16 && kc.nt <= kF16MaxTiles) {
             q.steps = ctx->d_steps_lane.as<StepDev>();
@@ -1622,6 +1700,59 @@ int cafe_gpu_reconstruct_noalloc(cafe_gpu_ctx* ctx, const int* node_slot, const 
     return reconstruct_impl(ctx, node_slot, prior, B, R, f0, nf, states, ctx->fixed_scratch, kFixedScratchBytes);
 }
 
+int cafe_gpu_tuning_set(cafe_gpu_ctx* ctx, int path, int tiles, int leaves, int kernel, int ntw, int w, int lds,
+                        int grid_mult)
+{
+    if (!ctx) return fail(CAFE_GPU_EINVAL, "null context");
+    if (path < CAFE_GPU_PATH_PLAIN64 || path > CAFE_GPU_PATH_BATCHED16 || tiles < 1 || leaves < 1 || kernel < 0 ||
+        kernel > 1 || ntw < 0 || w < 0 || grid_mult < 0)
+        return fail(CAFE_GPU_EINVAL, "invalid tuning entry");
+    for (auto& e : ctx->tuning)
+        if (e.path == path && e.nt == tiles && e.leaves == leaves) {
+            e = cafe_gpu_ctx::TuneEntry{path, tiles, leaves, kernel, ntw, w, lds ? 1 : 0, grid_mult};
+            return CAFE_GPU_OK;
+        }
+    ctx->tuning.push_back(cafe_gpu_ctx::TuneEntry{path, tiles, leaves, kernel, ntw, w, lds ? 1 : 0, grid_mult});
+    return CAFE_GPU_OK;
+}
+
+int cafe_gpu_tuning_clear(cafe_gpu_ctx* ctx)
+{
+    if (!ctx) return fail(CAFE_GPU_EINVAL, "null context");
+    ctx->tuning.clear();
+    return CAFE_GPU_OK;
+}
+
+int cafe_gpu_tuning_load(cafe_gpu_ctx* ctx, const char* path, int* n_loaded)
+{
+    if (!ctx || !path) return fail(CAFE_GPU_EINVAL, "null argument");
+    FILE* fh = fopen(path, "r");
+    if (!fh) return fail(CAFE_GPU_EINVAL, "cannot open tuning file %s", path);
+    std::vector<cafe_gpu_ctx::TuneEntry> entries;
+    char line[512];
+    int lineno = 0;
+    while (fgets(line, sizeof(line), fh)) {
+        ++lineno;
+        char* hash = strchr(line, '#');
+        if (hash) *hash = 0;
+        cafe_gpu_ctx::TuneEntry e{};
+        const int n = sscanf(line, "%d %d %d %d %d %d %d %d", &e.path, &e.nt, &e.leaves, &e.kernel, &e.ntw, &e.w,
+                             &e.lds, &e.grid_mult);
+        if (n <= 0) continue;   // blank or comment
+        if (n != 8 || e.path < CAFE_GPU_PATH_PLAIN64 || e.path > CAFE_GPU_PATH_BATCHED16 || e.nt < 1 ||
+            e.leaves < 1 || e.kernel < 0 || e.kernel > 1 || e.ntw < 0 || e.w < 0 || e.grid_mult < 0) {
+            fclose(fh);
+            return fail(CAFE_GPU_EINVAL, "%s:%d: malformed tuning entry", path, lineno);
+        }
+        e.lds = e.lds ? 1 : 0;
+        entries.push_back(e);
+    }
+    fclose(fh);
+    ctx->tuning = entries;
+    if (n_loaded) *n_loaded = (int)entries.size();
+    return CAFE_GPU_OK;
+}
+
 int cafe_gpu_last_timings(cafe_gpu_ctx* ctx, double* ms4)
 {
     if (!ctx || !ms4) return fail(CAFE_GPU_EINVAL, "null argument");
@@ -1768,7 +1899,7 @@ int cafe_gpu_plan_class_times(cafe_gpu_plan* pl, int cap, int* n_classes, int* c
                 l[1] = kc.lane_rows;
                 l[2] = 1;
                 l[3] = 0;
-                l[4] = pl->pp.grid_lane(kc.lane_per_cu, n);
+                l[4] = pl->pp.grid_lane(kc.lane_per_cu, n, kc.lane_grid_mult);
                 l[5] = 64;
             } else {
                 const bool f16 = pl->path == CAFE_GPU_PATH_BATCHED16 && kc.nt <= kF16MaxTiles;

@@ -172,7 +172,8 @@ struct MfmaShape {
     int w;
     bool lds;
     size_t shm;
-    int per_cu;   // workgroups resident on one CU (registers and LDS)
+    int per_cu;          // workgroups resident on one CU (registers and LDS)
+    int grid_mult = 0;   // workgroups per resident slot (0: the planner's default)
 };
 
 // Dynamic-LDS bookkeeping of the scaled FP32 kernel for n_slots partial

@@ -75,6 +75,9 @@ def load_library() -> ctypes.CDLL:
         "cafe_gpu_reconstruct": (_i, [_p, _pi, _pd, _i, _i, _i, _i, _pi]),
         "cafe_gpu_reconstruct_noalloc": (_i, [_p, _pi, _pd, _i, _i, _i, _i, _pi]),
         "cafe_gpu_last_timings": (_i, [_p, _pd]),
+        "cafe_gpu_tuning_load": (_i, [_p, ctypes.c_char_p, _pi]),
+        "cafe_gpu_tuning_set": (_i, [_p, _i, _i, _i, _i, _i, _i, _i, _i]),
+        "cafe_gpu_tuning_clear": (_i, [_p]),
         "cafe_gpu_plan_create": (_i, [_p, ctypes.POINTER(EvalDesc), _i, _pd, _pd, _i, _i, ctypes.POINTER(_p)]),
         "cafe_gpu_plan_set_keys": (_i, [_p, _pd, _pd]),
         "cafe_gpu_plan_execute": (_i, [_p]),
@@ -100,7 +103,8 @@ EXPORTED_SYMBOLS = ["cafe_gpu_create", "cafe_gpu_destroy", "cafe_gpu_last_error"
                     "cafe_gpu_reconstruct_noalloc", "cafe_gpu_last_timings", "cafe_gpu_plan_create", "cafe_gpu_plan_set_keys",
                     "cafe_gpu_plan_execute", "cafe_gpu_plan_synchronize", "cafe_gpu_plan_results",
                     "cafe_gpu_plan_profile", "cafe_gpu_plan_recompute_counts", "cafe_gpu_plan_class_times",
-                    "cafe_gpu_plan_destroy"]
+                    "cafe_gpu_plan_destroy", "cafe_gpu_tuning_load", "cafe_gpu_tuning_set",
+                    "cafe_gpu_tuning_clear"]
 
 
 def _ip(a: Optional[np.ndarray]):
@@ -243,6 +247,21 @@ class Engine:
         fn = self.lib.cafe_gpu_reconstruct_noalloc if noalloc else self.lib.cafe_gpu_reconstruct
         self._check(fn(self.ctx, _ip(ns), _dp(pr), B, R, f0, nf, _ip(out)))
         return out
+
+    # ---- launch shapes (see cafe_gpu_tuning_* in include/cafe_gpu.h) ----
+    def tuning_load(self, path: str) -> int:
+        """Replace the context's shape table with a cache file's entries."""
+        n = ctypes.c_int(0)
+        self._check(self.lib.cafe_gpu_tuning_load(self.ctx, path.encode(), ctypes.byref(n)))
+        return n.value
+
+    def tuning_set(self, path: int, tiles: int, leaves: int, kernel: int, ntw: int = 0, w: int = 0,
+                   lds: bool = False, grid_mult: int = 0):
+        self._check(self.lib.cafe_gpu_tuning_set(self.ctx, path, tiles, leaves, kernel, ntw, w, 1 if lds else 0,
+                                                 grid_mult))
+
+    def tuning_clear(self):
+        self._check(self.lib.cafe_gpu_tuning_clear(self.ctx))
 
     def timings(self) -> np.ndarray:
         out = np.zeros(4)

@@ -248,6 +248,28 @@ int cafe_gpu_plan_class_times(cafe_gpu_plan *plan, int cap, int *n_classes, int 
                               long long *class_tiles, double *ms, int *launch);
 int cafe_gpu_plan_destroy(cafe_gpu_plan *plan);
 
+/* --- tile shapes (no reference counterpart) --------------------------------
+ * Plans and evaluations pick a launch shape for every size class (the
+ * buckets whose vectors have the same number of 16-row tiles): the
+ * one-family-per-lane kernel or the matrix-core kernel, and for the latter
+ * its output tiles per wave, waves per workgroup, partials in LDS or global
+ * scratch, and the grid (workgroups per resident slot).  The context holds a
+ * table of measured choices keyed by (path, tiles per vector, leaves of the
+ * tree) that overrides the built-in rules; cafe5_amd/tune.py writes it as a
+ * cache file on the GPU, one line per entry:
+ *     path tiles leaves kernel ntw w lds grid_mult [measured ms]
+ * (kernel 0 matrix core, 1 per lane; '#' starts a comment).  An entry whose
+ * shape does not exist or does not fit the class is ignored, and the class
+ * keeps its built-in shape. */
+/* Replaces the table with the entries of a cache file; *n_loaded (may be
+ * NULL) receives their number. */
+int cafe_gpu_tuning_load(cafe_gpu_ctx *ctx, const char *path, int *n_loaded);
+/* Adds or replaces one entry. */
+int cafe_gpu_tuning_set(cafe_gpu_ctx *ctx, int path, int tiles, int leaves, int kernel, int ntw, int w,
+                        int lds, int grid_mult);
+/* Empties the table (built-in shapes only). */
+int cafe_gpu_tuning_clear(cafe_gpu_ctx *ctx);
+
 /* --- timing (measurement support; no reference counterpart) --------------- */
 
 /* Milliseconds of the last eval's phases, measured with HIP events on the

@@ -1,0 +1,86 @@
+"""The tile-shape table (cafe_gpu_tuning_*): a cache file's choice replaces a
+size class's built-in launch shape -- seen in the launch the plan records --
+and the results stay those of the built-in shapes (the per-family bar of the
+reduced paths); entries naming shapes that do not exist are ignored, and a
+malformed file is rejected."""
+import os
+
+import numpy as np
+import pytest
+
+from cafe5_amd import native, priors, workloads as wl
+from cafe5_amd.native import CafeGpuError
+from bars import reduced_ok
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup():
+    w = wl.config_uneven(seed=2, F=3000, T=4)
+    eng = native.Engine(0)
+    eng.set_tree(w.tree)
+    eng.set_families(w.counts, bound_B=w.bounds, bound_R=w.bounds)
+    buckets = eng.buckets()
+    keys = native.KeyTable()
+    slots = keys.node_slots(w.tree, np.repeat(w.lambdas[:, None], w.tree.n_nodes, axis=1))
+    lq, tq = keys.arrays()
+    eng.matrix_reset(int(w.bounds.max()) + 1, len(lq))
+    stride = max(r for _, r, _ in buckets)
+    prior = np.zeros((len(buckets), stride))
+    for i, (_, r, _) in enumerate(buckets):
+        prior[i, :r] = priors.uniform(r, r)
+    yield w, eng, slots, prior, lq, tq
+    eng.close()
+
+
+def _run(eng, slots, prior, lq, tq, path):
+    p = eng.plan(slots, prior, lq, tq, path=path)
+    p.execute()
+    p.synchronize()
+    res = p.results()
+    p.profile()
+    nts, _, _, launch = p.class_times()
+    p.close()
+    return res, {int(n): tuple(int(x) for x in l) for n, l in zip(nts, launch)}
+
+
+@pytest.mark.parametrize("path", [native.PATH_BATCHED32, native.PATH_BATCHED16])
+def test_cached_shapes_are_applied(setup, tmp_path, path):
+    w, eng, slots, prior, lq, tq = setup
+    eng.tuning_clear()
+    base, shapes = _run(eng, slots, prior, lq, tq, path)
+    assert shapes[2][0] == 1                      # built in: the 2-tile class runs one family per lane
+    big = max(shapes)
+    assert shapes[big][0] in (0, 3)               # the largest class on the matrix cores
+    leaves = w.tree.n_leaves
+    cache = tmp_path / "tune.txt"
+    cache.write_text(
+        "# test cache\n"
+        f"{path} 2 {leaves} 0 2 1 1 4   0.5\n"            # the 2-tile class on the matrix cores, LDS, 4x grid
+        f"{path} {big} {leaves} 0 4 4 0 1\n"             # the largest class with global partials, 1x grid
+        f"{path} 3 {leaves} 0 7 7 1 4\n")                # no such instance: ignored
+    assert eng.tuning_load(str(cache)) == 3
+    tuned, tshapes = _run(eng, slots, prior, lq, tq, path)
+    print(f"{native.PATH_NAMES[path]}: built-in {shapes}\n   tuned {tshapes}", flush=True)
+    assert tshapes[2][:4] in ((0, 2, 1, 1), (3, 2, 1, 1))       # matrix cores, 2 tiles x 1 wave, LDS
+    assert tshapes[2][5] == 16                                   # 16-family tiles
+    assert tshapes[big][1:4] == (4, 4, 0)                        # 4 tiles x 4 waves, global partials
+    assert tshapes[big][4] <= shapes[big][4]                     # a 1x grid
+    assert tshapes[3] == shapes[3]                               # the unknown shape left the class alone
+    fin = np.isfinite(base["lnl"])
+    assert np.all(reduced_ok(tuned["lnl"][fin], base["lnl"][fin]))
+    assert np.allclose(tuned["total"], base["total"], rtol=1e-6, atol=0)
+    eng.tuning_clear()
+
+
+def test_malformed_cache_is_rejected(setup, tmp_path):
+    _, eng, *_ = setup
+    bad = tmp_path / "bad.txt"
+    bad.write_text("2 13 34 0 4\n")
+    with pytest.raises(CafeGpuError) as ei:
+        eng.tuning_load(str(bad))
+    assert "malformed tuning entry" in str(ei.value)
+    with pytest.raises(CafeGpuError):
+        eng.tuning_load(str(tmp_path / "missing.txt"))
+    eng.tuning_clear()

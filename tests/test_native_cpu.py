@@ -98,3 +98,30 @@ def test_reference_file_readers_through_cpp_interface():
     lines = res.stdout.splitlines()
     assert res.returncode == 0 and not [l for l in lines if l.startswith("FAIL")], res.stdout + res.stderr
     assert sum(l.startswith("PASS") for l in lines) == 9
+
+
+def test_tuner_candidates_and_cache_format(tmp_path):
+    from cafe5_amd import tune
+    c13 = tune.candidates(native.PATH_BATCHED32, 13)
+    assert all(k == 0 and ntw * w >= 13 for k, ntw, w, _, _ in c13)
+    assert {(ntw, w) for _, ntw, w, _, _ in c13} == {(4, 4)}
+    c2 = tune.candidates(native.PATH_BATCHED32, 2)
+    assert any(k == 1 for k, *_ in c2) and any(k == 0 for k, *_ in c2)
+    assert not any(k == 1 for k, *_ in tune.candidates(native.PATH_BATCHED64, 2))   # FP64: matrix cores only
+    assert all(w <= 8 for _, _, w, _, _ in tune.candidates(native.PATH_BATCHED16, 32))
+    out = tmp_path / "cache.txt"
+    tune.write_cache(str(out), [(2, 13, 34, 0, 4, 4, 1, 16, 0.83)], "test")
+    lines = [l for l in out.read_text().splitlines() if l and not l.startswith("#")]
+    assert lines == ["2 13 34 0 4 4 1 16  0.8300"]
+
+
+def test_committed_tuning_cache_parses():
+    path = os.path.join(ROOT, "cafe5_amd", "tune_cache.txt")
+    if not os.path.exists(path):
+        pytest.skip("no tile-shape cache committed")
+    for line in open(path):
+        f = line.split("#", 1)[0].split()
+        if f:
+            assert len(f) in (8, 9), line
+            path_, tiles, leaves, kernel, ntw, w, lds, gm = (int(x) for x in f[:8])
+            assert 0 <= path_ <= 3 and tiles >= 1 and leaves >= 1 and kernel in (0, 1) and lds in (0, 1)
