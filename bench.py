@@ -136,16 +136,19 @@ def read_traffic(nt: int, kernel: str):
             "source": f"profiles/prune_fp32_uneven34_traffic.json ({t['command']})"}
 
 
-def kernels_only(w: wl.Workload, steps: int):
-    """The FP32 plan launched eagerly `steps` times, then two device copies
-    of 256 MiB (the known-size reference for counter scaling): the program
-    that rocprofv3 counter passes run."""
+def kernels_only(w: wl.Workload, steps: int, paths=(native.PATH_BATCHED32,)):
+    """Each path's plan executed `steps` times with its size classes launched
+    one after another (plan.profile), then two device copies of 256 MiB (the
+    known-size reference for counter scaling): the program that rocprofv3
+    counter passes run."""
     eng = native.Engine(0)
     buckets, slots, lam_q, t_q, prior, N = setup(eng, w)
-    p = eng.plan(slots, prior, lam_q, t_q, path=native.PATH_BATCHED32, use_graph=False)
-    for _ in range(steps):
-        p.execute()
-        p.synchronize()
+    for path in paths:
+        p = eng.plan(slots, prior, lam_q, t_q, path=path, use_graph=False)
+        for _ in range(steps):
+            p.profile()   # the size classes one after another: each launch alone on the device
+        p.close()
+        log(f"kernels-only: {steps} profiled {native.PATH_NAMES[path]} executions")
     hip = ctypes.CDLL("libamdhip64.so")
     nbytes = ctypes.c_size_t(256 << 20)
     a, b = ctypes.c_void_p(), ctypes.c_void_p()
@@ -157,9 +160,8 @@ def kernels_only(w: wl.Workload, steps: int):
     hip.hipDeviceSynchronize()
     hip.hipFree(a)
     hip.hipFree(b)
-    p.close()
     eng.close()
-    log(f"kernels-only: {steps} eager FP32 executions, 2 x 256 MiB device copies")
+    log("kernels-only: 2 x 256 MiB device copies")
 
 
 # ----------------------------------------------------------------------------
@@ -690,7 +692,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
-                    help="run the FP32 plan eagerly --steps times plus a 256 MiB copy (for counter passes)")
+                    help="run each --kernel-paths plan eagerly --steps times plus a 256 MiB copy (counter passes)")
+    ap.add_argument("--kernel-paths", default="2", help="paths for --kernels-only, e.g. 2,3,1")
     ap.add_argument("--evals", type=int, default=10, help="config 5: trial points scored per path")
     ap.add_argument("--tuning", default=os.path.join(ROOT, "cafe5_amd", "tune_cache.txt"),
                     help="tile-shape cache (cafe5_amd/tune.py) loaded into the engine; 'none' for built-in shapes")
@@ -709,7 +712,7 @@ def main():
         return main_config5(args)
     w = wl.config_uneven(seed=2, F=args.families, T=args.trials)
     if args.kernels_only:
-        kernels_only(w, args.steps)
+        kernels_only(w, args.steps, [int(x) for x in args.kernel_paths.split(",") if x])
         return
     F, T = w.counts.shape[0], len(w.lambdas)
     log(f"workload: {w.name} F={F} T={T} taxa={w.tree.n_leaves}")
