@@ -531,14 +531,14 @@ def main_config4(args):
     print(json.dumps(out))
 
 
-def _ref_cpu_session(w, B, R, lam, n_small, n_sample, threads, seed=19):
-    """The compiled reference's base_model on the workload's tree at bounds
-    (B, R) and one lambda: a session (tree, families, model; the
-    constructor's duplicate scan, timed apart) evaluated on n_small and
-    n_sample families, one warm-up and the median of 3 each, on `threads`
-    OpenMP threads.  Returns (build seconds, seconds per family, scan
-    seconds, threads used): an evaluation rebuilds every matrix and prunes
-    every family (base_model.cpp:53-100)."""
+def _ref_cpu_prune(w, B, R, lam, n_sample, n_one, threads, seed=19):
+    """The compiled reference's two per-evaluation costs on the workload's
+    tree at bounds (B, R) and one lambda, timed apart in a pruning session:
+    its matrix_cache build (precalculate_matrices: one key per branch length)
+    and inference_prune over a seeded n_sample-family subsample in an OpenMP
+    loop over families (one warm-up and the median of 3) on `threads`
+    threads, and over n_one families on one thread.  Returns (build seconds,
+    seconds per family, seconds per family on one thread, threads used)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Ref  # checker only
     if not Ref.available():
@@ -546,35 +546,32 @@ def _ref_cpu_session(w, B, R, lam, n_small, n_sample, threads, seed=19):
     lib = Ref.lib
     lib.ref_set_threads.restype = ctypes.c_int
     lib.ref_set_threads.argtypes = [ctypes.c_int]
-    lib.ref_base_session_create.restype = ctypes.c_void_p
-    lib.ref_base_session_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
-                                            ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
-                                            ctypes.c_int, ctypes.c_int, ctypes.c_double]
-    lib.ref_base_session_eval.restype = ctypes.c_double
-    lib.ref_base_session_eval.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]
-    lib.ref_base_session_free.argtypes = [ctypes.c_void_p]
-    used = lib.ref_set_threads(threads)
+    lib.ref_prune_session_create.restype = ctypes.c_void_p
+    lib.ref_prune_session_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_int), ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    lib.ref_prune_session_run.restype = ctypes.c_double
+    lib.ref_prune_session_run.argtypes = [ctypes.c_void_p]
+    lib.ref_prune_session_free.argtypes = [ctypes.c_void_p]
     idx = _sample(w, n_sample, seed=seed)
     newick = w.tree.newick().encode()
     names = " ".join(w.tree.leaf_names()).encode()
-    times, scan = {}, 0.0
-    for n in (n_small, n_sample):
+    out = []
+    for nt, n in ((threads, n_sample), (1, n_one)):
+        used = lib.ref_set_threads(nt)
         c = np.ascontiguousarray(w.counts[idx[:n]], dtype=np.int32)
-        t0 = time.perf_counter()
-        h = lib.ref_base_session_create(newick, names, n, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), B, R,
-                                        0, R, 0.0)
-        if n == n_sample:
-            scan = time.perf_counter() - t0
-        ts = []
-        for _ in range(4):
-            t0 = time.perf_counter()
-            lib.ref_base_session_eval(h, float(lam), None)
-            ts.append(time.perf_counter() - t0)
-        lib.ref_base_session_free(h)
-        times[n] = float(np.median(ts[1:]))
-    per_fam = max(0.0, (times[n_sample] - times[n_small]) / (n_sample - n_small))
-    build = max(0.0, times[n_small] - n_small * per_fam)
-    return build, per_fam, scan, int(used)
+        build = ctypes.c_double(0.0)
+        h = lib.ref_prune_session_create(newick, names, n, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                         float(lam), 1.0, B, R, ctypes.byref(build))
+        if not h:
+            return None
+        ts = [lib.ref_prune_session_run(h) for _ in range(4)]
+        lib.ref_prune_session_free(h)
+        out.append((build.value, float(np.median(ts[1:])) / n, int(used)))
+        if nt == 1:
+            break
+    (build, per_fam, used), (_, per_fam1, _) = out
+    return build, per_fam, per_fam1, used
 
 
 def main_config5(args):
@@ -657,18 +654,21 @@ def main_config5(args):
     }
     if not args.no_cpu:
         threads = _host_threads()
-        est = _ref_cpu_session(w, B, B, float(w.lambdas[0]), 4, 36, threads)
+        est = _ref_cpu_prune(w, B, B, float(w.lambdas[0]), 64, 4, threads)
         if est:
-            build, per_fam, scan, used = est
+            build, per_fam, per_fam1, used = est
             eval_s = K * build + K * per_fam * F
+            eval1 = K * build * used + K * per_fam1 * F   # the build taken to scale with its threads
             out["cpu_baseline"] = {
                 "value": F / eval_s, "unit": "family likelihoods/s", "cores": used, "kind": "reference",
-                "cpu_model": _cpu_model(), "seconds_per_evaluation": eval_s,
-                "sample": (f"reference base_model::infer_family_likelihoods (oracle/_ref, -O3, OpenMP) on the same "
-                           f"tree at B=R=500 and one lambda (198 matrices of 501x501), 4- and 36-family sessions, "
-                           f"one warm-up and the median of 3: matrix build {build:.2f} s and {per_fam * 1e3:.1f} ms "
-                           f"per family, scaled by K=8 (8x the keys, 8 prunes per family) to all {F} families; the "
-                           f"constructor's duplicate scan ({scan:.3f} s for 36 families) is not included")}
+                "one_thread_value": F / eval1, "cpu_model": _cpu_model(), "seconds_per_evaluation": eval_s,
+                "sample": (f"reference matrix_cache::precalculate_matrices and inference_prune (oracle/_ref, -O3 "
+                           f"-march=x86-64-v3, OpenMP over families as in gamma_model) on the same tree at B=R=500 "
+                           f"and one lambda: the build of its 198 matrices of 501x501 ({build:.2f} s on {used} "
+                           f"threads) and the pruning of a seeded 64-family subsample (one warm-up, median of 3: "
+                           f"{per_fam * 1e3:.2f} ms per family; one thread on 4 families: {per_fam1 * 1e3:.1f} ms), "
+                           f"scaled by K=8 (8x the keys, 8 prunes per family; gamma_model has no duplicate scan) to "
+                           f"all {F} families; the one-thread figure scales the build by {used}")}
             log(f"cpu baseline: {out['cpu_baseline']}")
     eng.close()
     print(json.dumps(out))
@@ -697,15 +697,18 @@ def main():
     ap.add_argument("--evals", type=int, default=10, help="config 5: trial points scored per path")
     ap.add_argument("--tuning", default=os.path.join(ROOT, "cafe5_amd", "tune_cache.txt"),
                     help="tile-shape cache (cafe5_amd/tune.py) loaded into the engine; 'none' for built-in shapes")
-    ap.add_argument("--config", type=int, default=2, choices=(2, 4, 5),
-                    help="2: uneven-size lambda grid (the headline); 4: error model + lambda classes, split "
-                         "FP16; 5: optimizer-scale gamma run")
+    ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3, 4, 5),
+                    help="1: plain base model (2,000 families); 2: uneven-size lambda grid (the headline); 3: gamma "
+                         "rate variation (50,000 families, K=4); 4: error model + lambda classes, split FP16; "
+                         "5: optimizer-scale gamma run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
               "family sharding with one all-reduce of per-trial totals and is not built", file=sys.stderr)
         sys.exit(2)
 
+    if args.config in (1, 3):
+        return main_config13(args)
     if args.config == 4:
         return main_config4(args)
     if args.config == 5:

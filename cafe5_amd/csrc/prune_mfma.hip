@@ -621,8 +621,9 @@ void k_prune_mfma(PruneParams p)
     X(T, 3, 4, S, true) X(T, 3, 4, S, false) X(T, 4, 4, S, true) X(T, 4, 4, S, false) \
     X(T, 3, 8, S, false) X(T, 4, 8, S, false)
 #define CAFE_MFMA_WIDE(X, T, S) X(T, 3, 16, S, false) X(T, 4, 16, S, false)
-// FP64 recompute shapes: one or two output tiles per wave, so that one
-// family tile's chain of steps is as short as the workgroup allows
+// Latency shapes: one or two output tiles per wave, so that one family
+// tile's chain of steps is as short as the workgroup allows (the FP64
+// recompute's few tiles; FP32 classes with fewer tiles than resident slots)
 #define CAFE_MFMA_CHAIN(X, T, S) \
     X(T, 1, 4, S, true) X(T, 1, 4, S, false) X(T, 1, 8, S, true) X(T, 1, 8, S, false) X(T, 2, 8, S, true) \
     X(T, 2, 8, S, false) X(T, 2, 16, S, false) X(T, 1, 16, S, true) X(T, 1, 16, S, false)
@@ -642,7 +643,7 @@ static hipError_t shape_dispatch(const MfmaShape& s, int grid, const PruneParams
     }
     CAFE_MFMA_SHAPES(CAFE_MFMA_CASE, T, SCALED)
     CAFE_MFMA_WIDE(CAFE_MFMA_CASE, T, SCALED)
-    if constexpr (!SCALED) { CAFE_MFMA_CHAIN(CAFE_MFMA_CASE, T, SCALED) }
+    CAFE_MFMA_CHAIN(CAFE_MFMA_CASE, T, SCALED)
 #undef CAFE_MFMA_CASE
     return hipErrorInvalidValue;
 }

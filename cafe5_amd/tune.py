@@ -31,8 +31,9 @@ DEFAULT_CACHE = os.path.join(HERE, "tune_cache.txt")
 
 # instantiated matrix-core shapes (tiles per wave, waves); the FP16 kernel
 # stops at eight waves
-MFMA_SHAPES = [(1, 1), (2, 1), (3, 1), (4, 1), (3, 2), (4, 2), (3, 4), (4, 4), (3, 8), (4, 8), (3, 16), (4, 16)]
-F16_SHAPES = [s for s in MFMA_SHAPES if s[1] <= 8]
+MFMA_SHAPES = [(1, 1), (2, 1), (3, 1), (4, 1), (3, 2), (4, 2), (3, 4), (4, 4), (3, 8), (4, 8), (3, 16), (4, 16),
+               (1, 4), (1, 8), (2, 8), (1, 16), (2, 16)]   # the last five: latency shapes
+F16_SHAPES = [s for s in MFMA_SHAPES[:12] if s[1] <= 8]
 LANE_TILES = 3          # the per-lane kernel takes vectors of up to 33 rows (3 tiles)
 
 
@@ -47,6 +48,8 @@ def candidates(path: int, nt: int) -> List[Tuple[int, int, int, int, int]]:
         if ntw * w < nt or (ntw - 1) * w >= nt and ntw > 1:   # covers the class without an idle tile per wave
             continue
         for lds in (1, 0):
+            if lds and (ntw, w) == (2, 16):
+                continue   # instantiated with global partials only
             for gm in ((4, 8, 16) if lds else (1, 2, 4)):
                 out.append((0, ntw, w, lds, gm))
     return out

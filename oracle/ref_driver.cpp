@@ -11,6 +11,7 @@
 // (create_family, compute_family_probabilities, pvalue, find_best_pvalue) and
 // Pupko's reconstruction (initialize_at_node, reconstruct_gene_family).
 #include <algorithm>
+#include <chrono>
 #include <fstream>
 #include <cmath>
 #include <cstring>
@@ -349,6 +350,67 @@ REF_API int ref_family_probabilities(const char* newick, const char* leaf_names,
     } catch (...) {
         return -1;
     }
+}
+
+// A pruning session for timing the reference's two per-evaluation costs
+// apart: the matrix_cache of one lambda x multiplier over the tree's branch
+// lengths is built (precalculate_matrices, timed), then each run prunes every
+// family with inference_prune in an OpenMP loop over families (the loop of
+// base_model / gamma_model).  Outputs: build seconds; run returns seconds.
+struct prune_session {
+    clade* tree = nullptr;
+    std::vector<gene_family> fams;
+    single_lambda* lam = nullptr;
+    matrix_cache* cache = nullptr;
+    double multiplier = 1.0;
+    int B = 0, R = 0;
+};
+
+REF_API void* ref_prune_session_create(const char* newick, const char* leaf_names, int F, const int* counts,
+                                       double lambda, double multiplier, int B, int R, double* out_build_s)
+{
+    try {
+        auto* s = new prune_session();
+        s->tree = parse_newick(newick);
+        s->fams = make_families(F, leaf_names, counts);
+        s->lam = new single_lambda(lambda);
+        s->cache = new matrix_cache(std::max(R, B) + 1);
+        s->multiplier = multiplier;
+        s->B = B;
+        s->R = R;
+        const auto t0 = std::chrono::steady_clock::now();
+        s->cache->precalculate_matrices({lambda * multiplier}, s->tree->get_branch_lengths());
+        const auto t1 = std::chrono::steady_clock::now();
+        if (out_build_s) *out_build_s = std::chrono::duration<double>(t1 - t0).count();
+        return s;
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+REF_API double ref_prune_session_run(void* h)
+{
+    auto* s = (prune_session*)h;
+    const int F = (int)s->fams.size();
+    std::vector<double> first(F, 0.0);
+    const auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel for schedule(dynamic)
+    for (int f = 0; f < F; ++f) {
+        auto v = inference_prune(s->fams[f], *s->cache, s->lam, nullptr, s->tree, s->multiplier, s->R, s->B);
+        first[f] = v.empty() ? 0.0 : v[0];
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+REF_API void ref_prune_session_free(void* h)
+{
+    auto* s = (prune_session*)h;
+    if (!s) return;
+    delete s->cache;
+    delete s->lam;
+    delete s->tree;
+    delete s;
 }
 
 REF_API double ref_pvalue(double v, const double* conddist, int n)

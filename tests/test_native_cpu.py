@@ -104,7 +104,7 @@ def test_tuner_candidates_and_cache_format(tmp_path):
     from cafe5_amd import tune
     c13 = tune.candidates(native.PATH_BATCHED32, 13)
     assert all(k == 0 and ntw * w >= 13 for k, ntw, w, _, _ in c13)
-    assert {(ntw, w) for _, ntw, w, _, _ in c13} == {(4, 4)}
+    assert {(ntw, w) for _, ntw, w, _, _ in c13} == {(4, 4), (2, 8), (1, 16)}   # throughput and latency shapes
     c2 = tune.candidates(native.PATH_BATCHED32, 2)
     assert any(k == 1 for k, *_ in c2) and any(k == 0 for k, *_ in c2)
     assert not any(k == 1 for k, *_ in tune.candidates(native.PATH_BATCHED64, 2))   # FP64: matrix cores only
