@@ -410,14 +410,16 @@ def _class_roofline(run, flops, peak, unit_desc):
             "algorithmic_flops_per_launch": flops, "avg_ms": ms, "flops_per_family_likelihood": unit_desc}
 
 
-def _ref_cpu_base(w, B, R, err, node_lambda_classes, n_sample, threads):
+def _ref_cpu_base(w, B, R, err, node_lambda_classes, n_sample, threads, gamma=None):
     """The compiled reference's base_model::infer_family_likelihoods (error
-    model, lambda tree) on seeded subsamples of 32 and n_sample families, one
-    warm-up and the median of 3 each: the per-evaluation matrix build and the
-    per-family cost, separated by the two sizes and extrapolated to every
-    family, on `threads` OpenMP threads and on one thread (a quarter of the
-    sample).  Each call also runs the constructor's duplicate scan
-    (base_model.cpp:27-51), O(n^2) in the subsample and included."""
+    model, lambda tree) -- or, with gamma = (category probabilities,
+    multipliers), gamma_model's -- on seeded subsamples of 32 and n_sample
+    families, one warm-up and the median of 3 each: the per-evaluation matrix
+    build and the per-family cost, separated by the two sizes and
+    extrapolated to every family, on `threads` OpenMP threads and on one
+    thread (a quarter of the sample).  Each base-model call also runs the
+    constructor's duplicate scan (base_model.cpp:27-51), O(n^2) in the
+    subsample and included."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Ref  # checker only
     if not Ref.available():
@@ -435,7 +437,11 @@ def _ref_cpu_base(w, B, R, err, node_lambda_classes, n_sample, threads):
             times = []
             for rep in range(4):
                 t0 = time.perf_counter()
-                Ref.base_model(w.tree, c, w.lambdas, B, R, err=err, node_class=node_lambda_classes)
+                if gamma is None:
+                    Ref.base_model(w.tree, c, w.lambdas, B, R, err=err, node_class=node_lambda_classes)
+                else:
+                    Ref.gamma_model(w.tree, c, w.lambdas, B, R, gamma[0], gamma[1], err=err,
+                                    node_class=node_lambda_classes)
                 times.append(time.perf_counter() - t0)
             med.append(float(np.median(times[1:])))
         res[nt] = (used, n, _extrapolate(med[0], med[1], 32, n, F), med)
@@ -443,8 +449,10 @@ def _ref_cpu_base(w, B, R, err, node_lambda_classes, n_sample, threads):
     _, n1, est1, med1 = res[1]
     return {"value": F / est, "unit": "family likelihoods/s", "cores": int(used), "kind": "reference",
             "one_thread_value": F / est1, "cpu_model": _cpu_model(),
-            "sample": (f"reference base_model::infer_family_likelihoods (oracle/_ref, -O3 -march=x86-64-v3, OpenMP) "
-                       f"with the error model and the lambda tree on seeded 32- and {n}-family subsamples "
+            "sample": (f"reference {'gamma_model' if gamma is not None else 'base_model'}::infer_family_likelihoods "
+                       f"(oracle/_ref, -O3 -march=x86-64-v3, OpenMP) "
+                       f"{'with the error model and the lambda tree ' if err is not None else ''}"
+                       f"on seeded 32- and {n}-family subsamples "
                        f"({med[0]:.2f} s and {med[1]:.2f} s; one thread: 32 and {n1} families, {med1[0]:.2f} s and "
                        f"{med1[1]:.2f} s), one warm-up and the median of 3; matrix build and per-family cost "
                        f"extrapolated to all {F} families")}
@@ -670,6 +678,82 @@ def main_config5(args):
                            f"scaled by K=8 (8x the keys, 8 prunes per family; gamma_model has no duplicate scan) to "
                            f"all {F} families; the one-thread figure scales the build by {used}")}
             log(f"cpu baseline: {out['cpu_baseline']}")
+    eng.close()
+    print(json.dumps(out))
+
+
+def main_config13(args):
+    """One parameter point through a captured plan per path.  Config 1: the
+    plain base model (seeded Yule tree of 12 taxa, 2,000 families, lambda
+    0.002, B = R = 100; base_model.cpp:53-100).  Config 3: gamma rate
+    variation (34 taxa, 50,000 families, K = 4 categories at alpha = 0.5,
+    lambda 0.003, B = R = 100; gamma_core.cpp:97-237), with the category
+    posteriors compared across paths."""
+    from cafe5_amd.model import get_gamma
+    gamma = args.config == 3
+    if gamma:
+        F = 50000 if args.families == 20000 else args.families
+        K, alpha = 4, 0.5
+        w = wl.config_gamma(seed=3, F=F, K=K, alpha=alpha, lam=0.003)
+        freq, rate = get_gamma(K, alpha)
+    else:
+        F = 2000 if args.families == 20000 else args.families
+        K, alpha = 1, None
+        w = wl.config_base(seed=1, F=F)
+        freq, rate = np.ones(1), np.ones(1)
+    B = R = int(w.bounds[0])
+    lam = float(w.lambdas[0])
+    log(f"workload: {w.name} F={F} taxa={w.tree.n_leaves} B={B} lambda={lam} K={K}")
+    eng = native.Engine(0)
+    tuning = load_tuning(eng, args.tuning)
+    eng.set_tree(w.tree)
+    eng.set_families(w.counts, B, R)
+    keys = native.KeyTable()
+    slots = keys.node_slots(w.tree, np.stack([np.full(w.tree.n_nodes, lam * r) for r in rate]))
+    lam_q, t_q = keys.arrays()
+    eng.matrix_reset(B + 1, len(lam_q))
+    prior = priors.uniform(R, R)[None, :]
+    paths = (native.PATH_BATCHED32, native.PATH_BATCHED16, native.PATH_BATCHED64, native.PATH_PLAIN64)
+    runs = _path_runs(eng, slots, prior, lam_q, t_q, paths, args.steps, args.warmup, args.fp64_steps,
+                      model=native.MODEL_GAMMA if gamma else native.MODEL_BASE, cat_probs=freq if gamma else None)
+    n1, n2 = tree_flop_factors(w.tree)
+    L = B + 1
+    flops = F * K * 2.0 * L * (n1 * L + n2 * R)
+    head = native.PATH_BATCHED32
+    r = runs[head]
+    name = (f"gamma-34taxa: seeded Yule tree (34 taxa, height 100), {F} birth-death families with gamma-distributed "
+            f"rates, size bound 100, K=4 categories at alpha=0.5, lambda 0.003, {len(lam_q)} matrices of 101x101 "
+            f"per step" if gamma else
+            f"base-12taxa: seeded Yule tree (12 taxa, height 100), {F} birth-death families (root 1 + Poisson(4)), "
+            f"size bound 100, lambda 0.002, {len(lam_q)} matrices of 101x101 per step")
+    out = {
+        "metric": ("family likelihoods/sec, batched FP32-MFMA gamma model (K=4, posteriors) vs FP64 on GPU at "
+                   "|ΔlnL_total|≤1e-6" if gamma else
+                   "family likelihoods/sec, batched FP32-MFMA base model vs FP64 on GPU at |ΔlnL_total|≤1e-6"),
+        "value": F / (r["ms"] * 1e-3), "unit": "family likelihoods/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": name, "path": "batched32 (FP32 MFMA + certified FP64 recompute)",
+                   "graph": bool(r["graph"]), "families": F, "trials": 1},
+        "paths": {native.PATH_NAMES[p]: {"value": F / (runs[p]["ms"] * 1e-3), "ms_per_step": runs[p]["ms"],
+                                         "phases_ms": np.round(runs[p]["phases"], 4).tolist()} for p in paths},
+        "speedup_vs_plain64": runs[native.PATH_PLAIN64]["ms"] / r["ms"],
+        "accuracy": _accuracy(runs, head, native.PATH_BATCHED64),
+        "accuracy_batched16": _accuracy(runs, native.PATH_BATCHED16, native.PATH_BATCHED64),
+        "roofline": _class_roofline(r, flops, PEAK_FP32_MFMA,
+                                    "K 2 L (n1 L + n2 R), K=%d n1=%d n2=%d L=%d" % (K, n1, n2, L)),
+        "tuning": tuning,
+        "cpu_baseline": None,
+    }
+    if gamma:
+        p64 = runs[native.PATH_BATCHED64]["res"]["post"]
+        out["posteriors"] = {native.PATH_NAMES[p]: {"max_abs_dpost_vs_batched64":
+                                                    float(np.nanmax(np.abs(runs[p]["res"]["post"] - p64)))}
+                             for p in (native.PATH_BATCHED32, native.PATH_BATCHED16, native.PATH_PLAIN64)}
+    if not args.no_cpu:
+        out["cpu_baseline"] = _ref_cpu_base(w, B, R, None, None, min(2000, F), _host_threads(),
+                                            gamma=(freq, rate) if gamma else None)
+        log(f"cpu baseline: {out['cpu_baseline']}")
     eng.close()
     print(json.dumps(out))
 

@@ -669,6 +669,7 @@ struct KernelClass {
     int lane_slots = 0;  // its slot instance
     int lane_per_cu = 0; // its workgroups resident per CU
     int lane_grid_mult = 0;   // workgroups per resident slot (0: default)
+    int fb32 = 1;        // FP32 matrix-core main pass: blocks of 16 families per tile (2: 32-family tiles)
     bool tuned = false;  // its shape came from the context's tuning table
     int r16 = 0;         // 16-family tiles its families fill (the recompute lists' capacity)
     int r_begin = 0;     // offset of its FP64 recompute list
@@ -745,6 +746,18 @@ MfmaShape mfma_shape(int nt, bool fp64)
     return s;
 }
 
+// FP32 launch shape with 32-family tiles (two blocks of 16 per MFMA operand)
+// for vectors of up to 16 tiles (kMaxTilesFb2): two output tiles per wave
+// over 4 or 8 waves (the products and factors of both blocks are held per
+// output tile, so more tiles per wave would spill).
+constexpr int kMaxTilesFb2 = 16;
+MfmaShape mfma_shape_fb2(int nt)
+{
+    MfmaShape s{2, nt <= 8 ? 4 : 8, false, 0};
+    s.fb = 2;
+    return s;
+}
+
 // FP64 recompute shape: its lists hold a few tiles, so the time is one tile's
 // chain of steps; one or two output tiles per wave (four above 32 tiles) over
 // up to 8 waves (16 above 16 tiles) keep each step's k-loop short, with room
@@ -759,16 +772,25 @@ MfmaShape mfma_shape_chain(int nt)
     return {4, 16, false, 0};
 }
 
+// Dynamic LDS of an MFMA launch without its partials: the bookkeeping and the
+// tile's leaf counts.
+size_t mfma_meta_only(const MfmaShape& s, bool fp64, int n_slots, int n_leaves)
+{
+    return (size_t)mfma_meta_bytes(n_slots, s.w, !fp64, s.fb) + (size_t)mfma_count_bytes(n_leaves, s.fb);
+}
+
 // Partial vectors go to LDS when the workgroup's share stays small enough for
 // several workgroups per CU; the rest use the global scratch.  A latency
 // shape (one workgroup per CU is enough: few tiles) may take up to 144 KB of
 // the CU's 160 KB.
 void place_partials(MfmaShape& s, bool fp64, int n_slots, int slot_tiles, int n_leaves, bool latency = false)
 {
-    const size_t meta = (size_t)mfma_meta_bytes(n_slots, s.w, !fp64) + (size_t)mfma_count_bytes(n_leaves);
-    const size_t frag = (size_t)n_slots * slot_tiles * 64 * 4 * (fp64 ? 8 : 4);
+    const size_t meta = mfma_meta_only(s, fp64, n_slots, n_leaves);
+    const size_t frag = (size_t)n_slots * slot_tiles * s.fb * 64 * 4 * (fp64 ? 8 : 4);
     if (latency)
         s.lds = (s.w <= 8 || (s.w == 16 && s.ntw == 1)) && meta + frag + 4096 <= 144 * 1024;
+    else if (s.fb > 1)   // two workgroups per CU, or one of 8 waves
+        s.lds = s.w <= 8 && meta + frag + 4096 <= (s.w >= 8 ? 144 : 76) * 1024;
     else
         s.lds = s.w <= 4 && meta + frag + 4096 <= 64 * 1024 && (meta + frag) / s.w <= 20 * 1024;
     s.shm = meta + (s.lds ? frag : 0);
@@ -804,7 +826,7 @@ size_t lane_shm(int n_leaves) { return (size_t)n_leaves * 64 * sizeof(int); }
 size_t mfma_scratch_bytes(const MfmaShape& s, bool fp64, int n_slots, int slot_tiles, int grid)
 {
     if (s.lds) return 0;
-    return (size_t)grid * n_slots * slot_tiles * 64 * 4 * (fp64 ? 8 : 4);
+    return (size_t)grid * n_slots * slot_tiles * s.fb * 64 * 4 * (fp64 ? 8 : 4);
 }
 
 // The tuning entry for a class, if the context holds one.
@@ -822,18 +844,24 @@ const cafe_gpu_ctx::TuneEntry* find_tuning(const cafe_gpu_ctx* ctx, int path, in
 bool tuned_mfma_shape(cafe_gpu_ctx* ctx, int path, const cafe_gpu_ctx::TuneEntry& e, int nt, int slot_tiles,
                       MfmaShape& s)
 {
-    if (e.kernel != 0 || path == CAFE_GPU_PATH_PLAIN64 || e.ntw < 1 || e.w < 1 || e.ntw * e.w < nt) return false;
-    const bool f16 = path == CAFE_GPU_PATH_BATCHED16;
+    if ((e.kernel != 0 && e.kernel != 2) || path == CAFE_GPU_PATH_PLAIN64 || e.ntw < 1 || e.w < 1 || e.ntw * e.w < nt)
+        return false;
+    const bool f16 = path == CAFE_GPU_PATH_BATCHED16 && e.kernel == 0;
     if (f16 && nt > kF16MaxTiles) return false;
     const bool fp64 = path == CAFE_GPU_PATH_BATCHED64;
+    // 32-family tiles: the FP32 kernel's main pass (BATCHED32, or BATCHED16
+    // classes past the split-FP16 kernel's sizes)
+    if (e.kernel == 2 && (fp64 || nt > kMaxTilesFb2 || (path == CAFE_GPU_PATH_BATCHED16 && nt <= kF16MaxTiles)))
+        return false;
     s = MfmaShape{e.ntw, e.w, e.lds != 0, 0, 0};
+    s.fb = e.kernel == 2 ? 2 : 1;
     if (f16) place_partials16(s, ctx->n_slots_lane, slot_tiles, ctx->n_leaves);
     else place_partials(s, fp64, ctx->n_slots, slot_tiles, ctx->n_leaves);
     if (e.lds && !s.lds) return false;   // LDS partials that do not fit
     if (!e.lds && s.lds) {               // global scratch asked for
         s.lds = false;
         s.shm = f16 ? (size_t)f16_meta_bytes(ctx->n_slots_lane, s.w) + (size_t)mfma_count_bytes(ctx->n_leaves)
-                    : (size_t)mfma_meta_bytes(ctx->n_slots, s.w, !fp64) + (size_t)mfma_count_bytes(ctx->n_leaves);
+                    : mfma_meta_only(s, fp64, ctx->n_slots, ctx->n_leaves);
     }
     const PruneParams none{};
     const hipError_t q = f16 ? f16_dispatch(s, 0, none, nullptr, &s.per_cu) : mfma_occupancy(fp64, s, &s.per_cu);
@@ -861,7 +889,8 @@ int apply_tuning(cafe_gpu_ctx* ctx, int path, KernelClass& kc)
     }
     MfmaShape s;
     if (!tuned_mfma_shape(ctx, path, *e, kc.nt, kc.slot_tiles, s)) return CAFE_GPU_OK;
-    if (path == CAFE_GPU_PATH_BATCHED16) kc.s16 = s;
+    if ((e->kernel == 2) != (kc.fb32 == 2)) return CAFE_GPU_OK;   // the tiles were cut for the other width
+    if (path == CAFE_GPU_PATH_BATCHED16 && e->kernel == 0) kc.s16 = s;
     else if (path == CAFE_GPU_PATH_BATCHED64) kc.s64 = s;
     else kc.s32 = s;
     kc.tuned = true;
@@ -1019,7 +1048,10 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
             kc.lane_rows = lane_rows_of(bc.first);
             if (const cafe_gpu_ctx::TuneEntry* te = find_tuning(ctx, path, bc.first)) {
                 MfmaShape probe;
-                if (kc.lane_rows > 0 && tuned_mfma_shape(ctx, path, *te, bc.first, bc.first, probe)) kc.lane_rows = 0;
+                if (tuned_mfma_shape(ctx, path, *te, bc.first, bc.first, probe)) {
+                    kc.lane_rows = 0;
+                    kc.fb32 = probe.fb;
+                }
             }
             if (kc.lane_rows > 0) {
                 kc.lane_slots = ctx->n_slots_lane;
@@ -1037,7 +1069,7 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         kc.nt = std::max(kc.nt, nt_of(bk));
         kc.nq = std::max(kc.nq, nq_of(bk));
         kc.slot_tiles = std::max(kc.slot_tiles, round_up(bk.B + 1, kTile) / kTile);
-        const int tw = kc.lane_rows > 0 ? 64 : kTile;   // families per tile
+        const int tw = kc.lane_rows > 0 ? 64 : kTile * kc.fb32;   // families per tile
         for (int pi = 0; pi < n_points; ++pi) {
             for (int f0 = 0; f0 < bk.count; f0 += tw)
                 tiles.push_back(TileDesc{bc.second, point_ids[pi], f0, std::min(tw, bk.count - f0)});
@@ -1051,7 +1083,7 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     }
     const int n_cls = (int)pp.classes.size();
     for (auto& kc : pp.classes) {
-        kc.s32 = mfma_shape(kc.nt, false);
+        kc.s32 = kc.fb32 == 2 ? mfma_shape_fb2(kc.nt) : mfma_shape(kc.nt, false);
         kc.s64 = mfma_shape(kc.nt, true);
         kc.r64 = mfma_shape_chain(kc.nt);
         place_partials(kc.s32, false, ctx->n_slots, kc.slot_tiles, ctx->n_leaves);
@@ -1075,7 +1107,7 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         }
         if (kc.r64.per_cu == 0 && kc.r64.lds) {   // the device does not fit that much LDS: global scratch
             kc.r64.lds = false;
-            kc.r64.shm = (size_t)mfma_meta_bytes(ctx->n_slots, kc.r64.w, false) + (size_t)mfma_count_bytes(ctx->n_leaves);
+            kc.r64.shm = mfma_meta_only(kc.r64, true, ctx->n_slots, ctx->n_leaves);
             HIP_OR_FAIL(mfma_occupancy(true, kc.r64, &kc.r64.per_cu));
         }
     }
@@ -1705,7 +1737,7 @@ int cafe_gpu_tuning_set(cafe_gpu_ctx* ctx, int path, int tiles, int leaves, int 
 {
     if (!ctx) return fail(CAFE_GPU_EINVAL, "null context");
     if (path < CAFE_GPU_PATH_PLAIN64 || path > CAFE_GPU_PATH_BATCHED16 || tiles < 1 || leaves < 1 || kernel < 0 ||
-        kernel > 1 || ntw < 0 || w < 0 || grid_mult < 0)
+        kernel > 2 || ntw < 0 || w < 0 || grid_mult < 0)
         return fail(CAFE_GPU_EINVAL, "invalid tuning entry");
     for (auto& e : ctx->tuning)
         if (e.path == path && e.nt == tiles && e.leaves == leaves) {
@@ -1740,7 +1772,7 @@ int cafe_gpu_tuning_load(cafe_gpu_ctx* ctx, const char* path, int* n_loaded)
                              &e.lds, &e.grid_mult);
         if (n <= 0) continue;   // blank or comment
         if (n != 8 || e.path < CAFE_GPU_PATH_PLAIN64 || e.path > CAFE_GPU_PATH_BATCHED16 || e.nt < 1 ||
-            e.leaves < 1 || e.kernel < 0 || e.kernel > 1 || e.ntw < 0 || e.w < 0 || e.grid_mult < 0) {
+            e.leaves < 1 || e.kernel < 0 || e.kernel > 2 || e.ntw < 0 || e.w < 0 || e.grid_mult < 0) {
             fclose(fh);
             return fail(CAFE_GPU_EINVAL, "%s:%d: malformed tuning entry", path, lineno);
         }
@@ -1909,7 +1941,7 @@ int cafe_gpu_plan_class_times(cafe_gpu_plan* pl, int cap, int* n_classes, int* c
                 l[2] = sh.w;
                 l[3] = sh.lds ? 1 : 0;
                 l[4] = pl->pp.grid(sh, n);
-                l[5] = kTile;
+                l[5] = kTile * sh.fb;
             }
         }
     }

@@ -174,16 +174,18 @@ struct MfmaShape {
     size_t shm;
     int per_cu;          // workgroups resident on one CU (registers and LDS)
     int grid_mult = 0;   // workgroups per resident slot (0: the planner's default)
+    int fb = 1;          // blocks of 16 families per tile (2: FP32 only)
 };
 
 // Dynamic-LDS bookkeeping of the scaled FP32 kernel for n_slots partial
-// vectors and w waves: per slot a flush bound (double), an exponent and a
-// renormalisation shift (int) per family, per-wave maxima (float) per family,
-// and one nominal-scale exponent.
-__host__ __device__ inline int mfma_meta_bytes(int n_slots, int w, bool scaled)
+// vectors, w waves and fb blocks of 16 families: per slot a flush bound
+// (double), an exponent and a renormalisation shift (int) per family, per-wave
+// maxima (float) per family, and one nominal-scale exponent.
+__host__ __device__ inline int mfma_meta_bytes(int n_slots, int w, bool scaled, int fb = 1)
 {
     if (!scaled) return 0;
-    const int b = n_slots * 16 * 8 + n_slots * w * 16 * 4 + 2 * n_slots * 16 * 4 + n_slots * 4;
+    const int fw = 16 * fb;
+    const int b = n_slots * fw * 8 + n_slots * w * fw * 4 + 2 * n_slots * fw * 4 + n_slots * 4;
     return (b + 31) & ~31;
 }
 
@@ -195,10 +197,10 @@ __host__ __device__ inline int f16_meta_bytes(int n_slots, int w)
     return (b + 31) & ~31;
 }
 
-// Dynamic-LDS copy of a tile's leaf counts: n_leaves x 16 ints.
-__host__ __device__ inline int mfma_count_bytes(int n_leaves)
+// Dynamic-LDS copy of a tile's leaf counts: n_leaves x 16 fb ints.
+__host__ __device__ inline int mfma_count_bytes(int n_leaves, int fb = 1)
 {
-    return (n_leaves * 16 * 4 + 31) & ~31;
+    return (n_leaves * 16 * fb * 4 + 31) & ~31;
 }
 
 }  // namespace cafe

@@ -117,18 +117,23 @@ __device__ __forceinline__ RootBest root_combine(RootBest b)
     return b;
 }
 
-// One k-tile of MFMAs: x[i] += A_i(kt) B(kt) for the wave's NTW output tiles.
-// SCALED: the stored child fragment is renormalised on the fly by 2^bsh.
-template <typename T, int NTW, bool SCALED>
-__device__ __forceinline__ void mma_step(typename Vec4<T>::type (&x)[NTW], const typename Vec4<T>::type (&a)[NTW],
-                                         const typename Vec4<T>::type& b, int bsh)
+// One k-tile of MFMAs: x[f][i] += A_i(kt) B_f(kt) for the wave's NTW output
+// tiles and the tile's FB blocks of 16 families (each A operand feeds FB
+// MFMAs).  SCALED: the stored child fragment of block f is renormalised on the
+// fly by 2^bsh[f].
+template <typename T, int NTW, bool SCALED, int FB>
+__device__ __forceinline__ void mma_step(typename Vec4<T>::type (&x)[FB][NTW], const typename Vec4<T>::type (&a)[NTW],
+                                         const typename Vec4<T>::type (&b)[FB], const int (&bsh)[FB])
 {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        T bj = b[j];
-        if constexpr (SCALED) bj = ldexpf(bj, bsh);
 #pragma unroll
-        for (int i = 0; i < NTW; ++i) x[i] = mfma16(a[i][j], bj, x[i]);
+        for (int f = 0; f < FB; ++f) {
+            T bj = b[f][j];
+            if constexpr (SCALED) bj = ldexpf(bj, bsh[f]);
+#pragma unroll
+            for (int i = 0; i < NTW; ++i) x[f][i] = mfma16(a[i][j], bj, x[f][i]);
+        }
     }
 }
 
@@ -153,20 +158,26 @@ template <typename T, int NTW, int W> struct Ring {
     static constexpr int D = W == 1 ? tail : deep;
 };
 
-template <typename T, int NTW, int D, bool SCALED>
-__device__ __forceinline__ void child_gemm(typename Vec4<T>::type (&x)[NTW], const T* __restrict__ M, int Np,
+// Child fragments of k-tile kt, family block f: stored at (kt FB + f) 64 + lane
+// from the slot's base (both blocks of a k-tile side by side).
+template <typename T, int NTW, int D, bool SCALED, int FB>
+__device__ __forceinline__ void child_gemm(typename Vec4<T>::type (&x)[FB][NTW], const T* __restrict__ M, int Np,
                                            const int (&ao)[NTW],
                                            const typename Vec4<T>::type (&pf)[Prefetch<T>::PK][NTW],
-                                           const typename Vec4<T>::type* bsrc, int nts, int bsh)
+                                           const typename Vec4<T>::type* bsrc, int nts, const int (&bsh)[FB])
 {
     typedef typename Vec4<T>::type V4;
     constexpr int PK = Prefetch<T>::PK;
     {
-        const V4 b0 = bsrc[0];
-        mma_step<T, NTW, SCALED>(x, pf[0], b0, bsh);
+        V4 b0[FB];
+#pragma unroll
+        for (int f = 0; f < FB; ++f) b0[f] = bsrc[f * 64];
+        mma_step<T, NTW, SCALED, FB>(x, pf[0], b0, bsh);
         if (PK > 1 && nts > 1) {
-            const V4 b1 = bsrc[64];
-            mma_step<T, NTW, SCALED>(x, pf[PK - 1], b1, bsh);
+            V4 b1[FB];
+#pragma unroll
+            for (int f = 0; f < FB; ++f) b1[f] = bsrc[(FB + f) * 64];
+            mma_step<T, NTW, SCALED, FB>(x, pf[PK - 1], b1, bsh);
         } else if (PK > 1) {
             // unused second k-tile: still wait for it (see the leaf step)
 #pragma unroll
@@ -175,23 +186,25 @@ __device__ __forceinline__ void child_gemm(typename Vec4<T>::type (&x)[NTW], con
     }
     if constexpr (D == 0) return;
     if (nts <= PK) return;
-    V4 a[D > 0 ? D : 1][NTW], bb[D > 0 ? D : 1];
+    V4 a[D > 0 ? D : 1][NTW], bb[D > 0 ? D : 1][FB];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const int k = PK + d < nts ? PK + d : nts - 1;
 #pragma unroll
         for (int i = 0; i < NTW; ++i) a[d][i] = load_a(M, ao[i], k, Np);
-        bb[d] = PK + d < nts ? bsrc[k * 64] : V4{0, 0, 0, 0};
+#pragma unroll
+        for (int f = 0; f < FB; ++f) bb[d][f] = PK + d < nts ? bsrc[(k * FB + f) * 64] : V4{0, 0, 0, 0};
     }
     for (int kt = PK; kt < nts; kt += D) {
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            mma_step<T, NTW, SCALED>(x, a[d], bb[d], bsh);
+            mma_step<T, NTW, SCALED, FB>(x, a[d], bb[d], bsh);
             const int kn = kt + D + d;   // the k-tile this buffer holds next
             const int k = kn < nts ? kn : nts - 1;
 #pragma unroll
             for (int i = 0; i < NTW; ++i) a[d][i] = load_a(M, ao[i], k, Np);
-            bb[d] = kn < nts ? bsrc[k * 64] : V4{0, 0, 0, 0};
+#pragma unroll
+            for (int f = 0; f < FB; ++f) bb[d][f] = kn < nts ? bsrc[(k * FB + f) * 64] : V4{0, 0, 0, 0};
         }
     }
 }
@@ -225,13 +238,20 @@ template <int NTW, int W, int BYTES> struct WavesHint {
     static constexpr int v = (W == 1 && NTW * BYTES <= 8) ? 4 : 1;
 };
 
-template <typename T, int NTW, int W, bool SCALED, bool LDSP>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WavesHint<NTW, W, (int)sizeof(T)>::v)))
+// FB = 2: a tile carries 32 families, two blocks of 16 (block f holds
+// families 16 f .. 16 f + 15 in lanes fl of every lane group): each A operand
+// (matrix fragment) feeds both blocks' MFMAs, so a tile reads its matrices
+// once for twice the families, and a k-tile's loads in flight cover twice the
+// matrix-core work.  The per-family bookkeeping is kept per block.
+template <typename T, int NTW, int W, bool SCALED, bool LDSP, int FB>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WavesHint<NTW, W, (int)sizeof(T) * FB>::v)))
 void k_prune_mfma(PruneParams p)
 {
     typedef typename Vec4<T>::type V4;
     constexpr int PK = Prefetch<T>::PK;
     constexpr bool kF64 = sizeof(T) == 8;
+    constexpr int FW = 16 * FB;   // families per tile
+    static_assert(FB == 1 || PK >= FB, "a leaf step loads one column per family block into the prefetch buffers");
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int fl = lane & 15;
@@ -250,23 +270,23 @@ void k_prune_mfma(PruneParams p)
     const double two_m_a = ldexp(1.0, -a_exp);
 
     // dynamic LDS: scaled-path bookkeeping per slot, the tile's leaf counts,
-    // then LDS partials.  Per slot and family:
+    // then LDS partials.  Per slot and family (FW of them):
     //   m_cert  W == 1: the stored vector's flush bound; W > 1: before renormalising
     //   m_esum  W == 1: its exponent E (true = stored 2^(E + bsh)); W > 1: e_sum of the raw product
     //   m_bsh   W == 1: its renormalisation shift a - e
     //   m_wmax  W > 1: per-wave maxima;  m_l2s  W > 1: log2 of the nominal scale (per slot)
     extern __shared__ __align__(32) unsigned char s_dyn[];
     double* m_cert = (double*)s_dyn;
-    float* m_wmax = (float*)(m_cert + n_slots * 16);
-    int* m_esum = (int*)(m_wmax + n_slots * W * 16);
-    int* m_bsh = m_esum + n_slots * 16;
-    int* m_l2s = m_bsh + n_slots * 16;
-    const int meta = mfma_meta_bytes(n_slots, W, SCALED);
-    int* s_cnt = (int*)(s_dyn + meta);                // [leaf][16]
-    V4* frag = LDSP ? (V4*)(s_dyn + meta + mfma_count_bytes(p.n_leaves))
-                    : (V4*)p.scratch + (long long)blockIdx.x * n_slots * slot_tiles * 64;
-    __shared__ double s_rkey[W][16], s_rx[W][16], s_rxmax[W][16];
-    __shared__ int s_rj[W][16];
+    float* m_wmax = (float*)(m_cert + n_slots * FW);
+    int* m_esum = (int*)(m_wmax + n_slots * W * FW);
+    int* m_bsh = m_esum + n_slots * FW;
+    int* m_l2s = m_bsh + n_slots * FW;
+    const int meta = mfma_meta_bytes(n_slots, W, SCALED, FB);
+    int* s_cnt = (int*)(s_dyn + meta);                // [leaf][FW]
+    V4* frag = LDSP ? (V4*)(s_dyn + meta + mfma_count_bytes(p.n_leaves, FB))
+                    : (V4*)p.scratch + (long long)blockIdx.x * n_slots * slot_tiles * FB * 64;
+    __shared__ double s_rkey[W][FW], s_rx[W][FW], s_rxmax[W][FW];
+    __shared__ int s_rj[W][FW];
 
     // Static, XCD-aware split of the class's tiles: workgroup b runs on XCD
     // b % 8 (dispatch order), each XCD takes one contiguous range of the list
@@ -283,18 +303,23 @@ void k_prune_mfma(PruneParams p)
         const TileDesc td = ld_const(p.tiles, tile);
         const BucketDev bk = ld_const(p.buckets, td.bucket);
         const int nf = td.nf < 0 ? -td.nf : td.nf;
-        // the tile's leaf counts -> LDS (lane f of each 16 holds family f)
-        for (int i = threadIdx.x; i < p.n_leaves * 16; i += 64 * W) {
-            const int f = i & 15;
+        // the tile's leaf counts -> LDS (lane fl of block f holds family 16 f + fl)
+        for (int i = threadIdx.x; i < p.n_leaves * FW; i += 64 * W) {
+            const int f = i & (FW - 1);
             const int ff = f < nf ? f : 0;
             const int fam = td.nf < 0 ? p.fam_list[td.fam0 + ff] : bk.fam_begin + td.fam0 + ff;
-            s_cnt[i] = p.counts_T[(long long)(i >> 4) * p.F + fam];
+            s_cnt[i] = p.counts_T[(long long)(i / FW) * p.F + fam];
         }
         __syncthreads();   // counts visible
-        const bool valid = fl < nf;
-        const int fi = valid ? fl : 0;
-        const int fs = td.nf < 0 ? p.fam_list[td.fam0 + fi] : bk.fam_begin + td.fam0 + fi;
-        const int dst = p.perm[fs];   // the caller's index of this lane's family
+        bool valid[FB];
+        int dst[FB];
+#pragma unroll
+        for (int f = 0; f < FB; ++f) {
+            valid[f] = 16 * f + fl < nf;
+            const int fi = valid[f] ? 16 * f + fl : 0;
+            const int fs = td.nf < 0 ? p.fam_list[td.fam0 + fi] : bk.fam_begin + td.fam0 + fi;
+            dst[f] = p.perm[fs];   // the caller's index of this lane's family of block f
+        }
         const __attribute__((address_space(4))) int* nslot = as_const(p.node_slot + (long long)td.point * p.n_nodes);
         const int nts = bk.Lp / kTile;        // k-tiles of every child vector
         const int rows_root = bk.Rp / kTile;  // root rows are sizes 0..R
@@ -320,14 +345,16 @@ void k_prune_mfma(PruneParams p)
         // one-wave FP32 kernels start a non-root product from a 0/1 row mask
         // (rows past the bound are 0 and stay 0), so the node's store needs no
         // masking; the others mask when they store
-        constexpr bool kMaskFirst = SCALED && W == 1 && NTW == 2;   // registers to spare: the 2-tile class
+        constexpr bool kMaskFirst = SCALED && W == 1 && NTW == 2 && FB == 1;   // registers to spare: the 2-tile class
         V4 keep[kMaskFirst ? NTW : 1];
 #pragma unroll
         for (int i = 0; i < (kMaskFirst ? NTW : 1); ++i)
 #pragma unroll
             for (int r = 0; r < 4; ++r) keep[i][r] = kTile * (wv + W * i) + 4 * g + r < bk.L ? T(1) : T(0);
 
-        // loads of step st (clamped to the last step) into pf; its descriptor into sd
+        // loads of step st (clamped to the last step) into pf; its descriptor
+        // into sd.  A leaf step loads its column for family block k into
+        // pf[k] (FB = 2), or the same column twice (FB = 1).
         auto prefetch = [&](V4 (&pf)[PK][NTW], StepDev& sd, int st) {
             // wave-uniform index: the descriptor comes through the scalar
             // cache, so waiting for it does not wait for vector loads
@@ -336,16 +363,16 @@ void k_prune_mfma(PruneParams p)
             const bool root = (sd.ctl & kStepRoot) != 0;
             const bool leaf = sd.leaf_col >= 0;
             const T* __restrict__ M = (leaf ? lpool : pool) + (long long)nslot[sd.node] * p.slot_elems;
-            const int cn = s_cnt[(leaf ? sd.leaf_col : 0) * 16 + fl] * Np;
 #pragma unroll
-            for (int k = 0; k < PK; ++k)
+            for (int k = 0; k < PK; ++k) {
+                const int cn = s_cnt[(leaf ? sd.leaf_col : 0) * FW + (FB > 1 ? 16 * k : 0) + fl] * Np;
 #pragma unroll
                 for (int i = 0; i < NTW; ++i) {
                     const int oa = root ? offA1[i] : offA0[i];
                     const int ol = cn + (root ? offL1[i] : offL0[i]);
                     const int kk = k < nts ? k : nts - 1;
                     if constexpr (!kF64) {
-                        const int off = leaf ? ol + 4 * k * zero : oa + kTile * kk;
+                        const int off = leaf ? ol + (FB > 1 ? 0 : 4 * k * zero) : oa + kTile * kk;
                         pf[k][i] = *(const floatx4*)(M + off);
                     } else {
                         const int base = leaf ? ol : oa + kTile * kk * Np;
@@ -353,19 +380,26 @@ void k_prune_mfma(PruneParams p)
                         pf[k][i] = V4{M[base], M[base + step], M[base + 2 * step], M[base + 3 * step]};
                     }
                 }
+            }
         };
 
         // per-op state (SCALED bookkeeping: a renormalised child v' holds
         // v_c = v' 2^E_c with max v' < 2^a; its factor X' = M' v'
         // (M' = 2^kMatExp M) holds X_c = X' 2^(E_c - kMatExp) and is at most
         // 2^(kMatExp + a), leaf factors 2^kMatExp.
-        //   e_sum  sum_c (E_c - kMatExp): true product = raw 2^e_sum
+        //   e_sum  sum_c (E_c - kMatExp): true product = raw 2^e_sum (per family)
         //   log2S  log2 of the product's nominal scale prod_c 2^(kMatExp + a_c)
         //   cert   sum_c of the factors' relative flush errors (w.r.t. their
-        //          nominal scale): inherited bound + what flushing can drop)
-        V4 acc[NTW];
-        int e_sum = 0, log2S = 0;
-        double cert = 0.0;
+        //          nominal scale): inherited bound + what flushing can drop
+        //          (per family)
+        V4 acc[FB][NTW];
+        int e_sum[FB], log2S = 0;
+        double cert[FB];
+#pragma unroll
+        for (int f = 0; f < FB; ++f) {
+            e_sum[f] = 0;
+            cert[f] = 0.0;
+        }
 
         auto consume = [&](const V4 (&pf)[PK][NTW], const StepDev& sd) {
             const bool root = (sd.ctl & kStepRoot) != 0;
@@ -380,120 +414,145 @@ void k_prune_mfma(PruneParams p)
             }
             if (sd.ctl & kStepFirst) {
 #pragma unroll
-                for (int i = 0; i < NTW; ++i) {
-                    if constexpr (kMaskFirst) acc[i] = root ? V4{1, 1, 1, 1} : keep[i];
-                    else acc[i] = V4{1, 1, 1, 1};
+                for (int f = 0; f < FB; ++f) {
+#pragma unroll
+                    for (int i = 0; i < NTW; ++i) {
+                        if constexpr (kMaskFirst) acc[f][i] = root ? V4{1, 1, 1, 1} : keep[i];
+                        else acc[f][i] = V4{1, 1, 1, 1};
+                    }
+                    e_sum[f] = 0;
+                    cert[f] = 0.0;
                 }
-                e_sum = 0;
                 log2S = 0;
-                cert = 0.0;
             }
-            V4 x[NTW];
+            V4 x[FB][NTW];
             if (sd.leaf_col >= 0) {
                 if (SCALED) {
-                    e_sum -= kMatExp;
+#pragma unroll
+                    for (int f = 0; f < FB; ++f) {
+                        e_sum[f] -= kMatExp;
+                        cert[f] += leaf_cert;
+                    }
                     log2S += kMatExp;
-                    cert += leaf_cert;
                 }
 #pragma unroll
-                for (int i = 0; i < NTW; ++i) x[i] = pf[0][i];
+                for (int f = 0; f < FB; ++f)
+#pragma unroll
+                    for (int i = 0; i < NTW; ++i) x[f][i] = pf[f][i];
                 // The leaf step ignores the other prefetched k-tiles; taking
                 // them here makes both paths leave no load of this buffer in
                 // flight, so its next prefetch needs no wait on the other one.
 #pragma unroll
-                for (int k = 1; k < PK; ++k)
+                for (int k = FB; k < PK; ++k)
 #pragma unroll
                     for (int i = 0; i < NTW; ++i) await_loaded(pf[k][i]);
             } else {
-                int bsh = 0;
+                int bsh[FB];
+#pragma unroll
+                for (int f = 0; f < FB; ++f) bsh[f] = 0;
                 if (SCALED) {
-                    const int q = sd.in_slot * 16 + fl;
                     log2S += kMatExp + a_exp;
-                    if (W == 1) {
-                        bsh = m_bsh[q];
-                        e_sum += m_esum[q] - kMatExp;
-                        cert += m_cert[q] + cint;
-                    } else {
-                        float m = 0.f;
-                        for (int w2 = 0; w2 < W; ++w2) m = fmaxf(m, m_wmax[(sd.in_slot * W + w2) * 16 + fl]);
-                        int e = 0;
-                        const float mant = m > 0.f ? frexpf(m, &e) : 1.f;
-                        bsh = a_exp - e;
-                        e_sum += m_esum[q] + e - a_exp - kMatExp;
-                        cert += (m > 0.f ? (ldexp(m_cert[q] * 1.01, m_l2s[sd.in_slot] - e) + ldexp(kTau32, -e)) *
-                                               rcp_up(mant)
-                                         : 1e300) +
-                                cint;
+#pragma unroll
+                    for (int f = 0; f < FB; ++f) {
+                        const int q = sd.in_slot * FW + 16 * f + fl;
+                        if (W == 1) {
+                            bsh[f] = m_bsh[q];
+                            e_sum[f] += m_esum[q] - kMatExp;
+                            cert[f] += m_cert[q] + cint;
+                        } else {
+                            float m = 0.f;
+                            for (int w2 = 0; w2 < W; ++w2)
+                                m = fmaxf(m, m_wmax[(sd.in_slot * W + w2) * FW + 16 * f + fl]);
+                            int e = 0;
+                            const float mant = m > 0.f ? frexpf(m, &e) : 1.f;
+                            bsh[f] = a_exp - e;
+                            e_sum[f] += m_esum[q] + e - a_exp - kMatExp;
+                            cert[f] += (m > 0.f ? (ldexp(m_cert[q] * 1.01, m_l2s[sd.in_slot] - e) + ldexp(kTau32, -e)) *
+                                                      rcp_up(mant)
+                                                : 1e300) +
+                                       cint;
+                        }
                     }
                 }
                 const T* __restrict__ M = pool + (long long)nslot[sd.node] * p.slot_elems;
                 int ao[NTW];
 #pragma unroll
                 for (int i = 0; i < NTW; ++i) {
-                    x[i] = V4{0, 0, 0, 0};
+#pragma unroll
+                    for (int f = 0; f < FB; ++f) x[f][i] = V4{0, 0, 0, 0};
                     ao[i] = root ? offA1[i] : offA0[i];
                 }
-                const V4* bsrc = frag + (long long)sd.in_slot * slot_tiles * 64 + lane;
-                child_gemm<T, NTW, Ring<T, NTW, W>::D, SCALED>(x, M, Np, ao, pf, bsrc, nts, bsh);
+                const V4* bsrc = frag + (long long)sd.in_slot * slot_tiles * FB * 64 + lane;
+                child_gemm<T, NTW, Ring<T, NTW, W>::D, SCALED, FB>(x, M, Np, ao, pf, bsrc, nts, bsh);
             }
 #pragma unroll
-            for (int i = 0; i < NTW; ++i) acc[i] *= x[i];
+            for (int f = 0; f < FB; ++f)
+#pragma unroll
+                for (int i = 0; i < NTW; ++i) acc[f][i] *= x[f][i];
             if (!(sd.ctl & kStepLast)) return;
 
             if (!root) {
-                T wmax = 0;
+                T wmax[FB];
 #pragma unroll
-                for (int i = 0; i < NTW; ++i) {
-                    if (!live[i]) continue;
-                    V4 v = acc[i];
-                    if constexpr (!kMaskFirst)
+                for (int f = 0; f < FB; ++f) {
+                    wmax[f] = 0;
 #pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            if (kTile * tt[i] + 4 * g + r >= bk.L) v[r] = 0;   // rows past the bound
-                    frag[((long long)out_slot * slot_tiles + tt[i]) * 64 + lane] = v;
-                    if (SCALED)
+                    for (int i = 0; i < NTW; ++i) {
+                        if (!live[i]) continue;
+                        V4 v = acc[f][i];
+                        if constexpr (!kMaskFirst)
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            if constexpr (W == 1) wmax = fmax(wmax, v[r]);   // entries are >= 0
-                            else wmax = wmax > v[r] ? wmax : v[r];
-                        }
+                            for (int r = 0; r < 4; ++r)
+                                if (kTile * tt[i] + 4 * g + r >= bk.L) v[r] = 0;   // rows past the bound
+                        frag[(((long long)out_slot * slot_tiles + tt[i]) * FB + f) * 64 + lane] = v;
+                        if (SCALED)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                if constexpr (W == 1) wmax[f] = fmax(wmax[f], v[r]);   // entries are >= 0
+                                else wmax[f] = wmax[f] > v[r] ? wmax[f] : v[r];
+                            }
+                    }
                 }
                 if (SCALED) {
-                    // maximum over the family's four lanes f, f+16, f+32, f+48
-                    // (the one-wave kernels use lane swaps; the 4-tile
-                    // multi-wave shapes keep shuffles, which leave them the
-                    // registers for two waves per SIMD)
-                    float m = (float)wmax;
-                    if constexpr (W == 1) {
-                        m = fmaxf(m, __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, m),
-                                                                                          kSwizzleXor16)));
-                        m = xor32_max(m);
-                    } else {
-                        m = fmaxf(m, __shfl_xor(m, 16));
-                        m = fmaxf(m, __shfl_xor(m, 32));
-                    }
-                    const int q = out_slot * 16 + fl;
-                    if (W == 1) {
-                        // finish the stored vector: shift, exponent, flush bound
-                        // re-expressed relative to the new scale (it grows by
-                        // S / m, the factor by which the product fell short of
-                        // its nominal scale)
-                        int e = 0;
-                        const float mant = m > 0.f ? frexpf(m, &e) : 1.f;
-                        if (g == 0) {
-                            m_bsh[q] = a_exp - e;
-                            m_esum[q] = e_sum + e - a_exp;
-                            m_cert[q] = m > 0.f ? (ldexp(cert * 1.01, log2S - e) + ldexp(kTau32, -e)) * rcp_up(mant)
-                                                : 1e300;
+#pragma unroll
+                    for (int f = 0; f < FB; ++f) {
+                        // maximum over the family's four lanes fl, fl+16, fl+32,
+                        // fl+48 (the one-wave kernels use lane swaps; the 4-tile
+                        // multi-wave shapes keep shuffles, which leave them the
+                        // registers for two waves per SIMD)
+                        float m = (float)wmax[f];
+                        if constexpr (W == 1) {
+                            m = fmaxf(m, __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(
+                                                                       __builtin_bit_cast(int, m), kSwizzleXor16)));
+                            m = xor32_max(m);
+                        } else {
+                            m = fmaxf(m, __shfl_xor(m, 16));
+                            m = fmaxf(m, __shfl_xor(m, 32));
                         }
-                    } else {
-                        if (g == 0) m_wmax[(out_slot * W + wv) * 16 + fl] = m;
-                        if (wv == 0 && g == 0) {
-                            m_esum[q] = e_sum;
-                            m_cert[q] = cert;
+                        const int q = out_slot * FW + 16 * f + fl;
+                        if (W == 1) {
+                            // finish the stored vector: shift, exponent, flush
+                            // bound re-expressed relative to the new scale (it
+                            // grows by S / m, the factor by which the product
+                            // fell short of its nominal scale)
+                            int e = 0;
+                            const float mant = m > 0.f ? frexpf(m, &e) : 1.f;
+                            if (g == 0) {
+                                m_bsh[q] = a_exp - e;
+                                m_esum[q] = e_sum[f] + e - a_exp;
+                                m_cert[q] = m > 0.f ? (ldexp(cert[f] * 1.01, log2S - e) + ldexp(kTau32, -e)) *
+                                                          rcp_up(mant)
+                                                    : 1e300;
+                            }
+                        } else {
+                            if (g == 0) m_wmax[(out_slot * W + wv) * FW + 16 * f + fl] = m;
+                            if (wv == 0 && g == 0) {
+                                m_esum[q] = e_sum[f];
+                                m_cert[q] = cert[f];
+                            }
                         }
-                        if (wv == 0 && lane == 0) m_l2s[out_slot] = log2S;
                     }
+                    if (W > 1 && wv == 0 && lane == 0) m_l2s[out_slot] = log2S;
                 }
                 // the node's vector is complete before anyone reads it (one
                 // wave: LDS keeps its own order, only the compiler must not
@@ -522,64 +581,76 @@ void k_prune_mfma(PruneParams p)
             for (int i = 0; i < NTW; ++i)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) asm volatile("" ::"v"(pv[i][r]));
-            RootBest rb{-__builtin_huge_val(), 0.0, -1, 0.0};
+            RootBest rb[FB];
 #pragma unroll
-            for (int i = 0; i < NTW; ++i) {
-                if (!live[i]) continue;
+            for (int f = 0; f < FB; ++f) {
+                rb[f] = RootBest{-__builtin_huge_val(), 0.0, -1, 0.0};
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int j = kTile * tt[i] + 4 * g + r - 1;
-                    if (j < 0 || j >= bk.R) continue;
-                    const double xv = (double)acc[i][r];
-                    if (p.root_out && valid)
-                        p.root_out[(long long)dst * p.root_stride + j] = SCALED ? ldexp(xv, e_sum) : xv;
-                    root_take(rb, xv * pv[i][r], xv, j);
+                for (int i = 0; i < NTW; ++i) {
+                    if (!live[i]) continue;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int j = kTile * tt[i] + 4 * g + r - 1;
+                        if (j < 0 || j >= bk.R) continue;
+                        const double xv = (double)acc[f][i][r];
+                        if (p.root_out && valid[f])
+                            p.root_out[(long long)dst[f] * p.root_stride + j] = SCALED ? ldexp(xv, e_sum[f]) : xv;
+                        root_take(rb[f], xv * pv[i][r], xv, j);
+                    }
                 }
+                rb[f] = root_combine(rb[f]);
             }
-            rb = root_combine(rb);
             if (W > 1) {
-                if (g == 0) {
-                    s_rkey[wv][fl] = rb.key;
-                    s_rx[wv][fl] = rb.x;
-                    s_rxmax[wv][fl] = rb.xmax;
-                    s_rj[wv][fl] = rb.j;
-                }
+#pragma unroll
+                for (int f = 0; f < FB; ++f)
+                    if (g == 0) {
+                        s_rkey[wv][16 * f + fl] = rb[f].key;
+                        s_rx[wv][16 * f + fl] = rb[f].x;
+                        s_rxmax[wv][16 * f + fl] = rb[f].xmax;
+                        s_rj[wv][16 * f + fl] = rb[f].j;
+                    }
                 lds_barrier();
                 if (wv == 0)
-                    for (int w2 = 1; w2 < W; ++w2)
-                        root_merge(rb, s_rkey[w2][fl], s_rx[w2][fl], s_rj[w2][fl], s_rxmax[w2][fl]);
+#pragma unroll
+                    for (int f = 0; f < FB; ++f)
+                        for (int w2 = 1; w2 < W; ++w2)
+                            root_merge(rb[f], s_rkey[w2][16 * f + fl], s_rx[w2][16 * f + fl], s_rj[w2][16 * f + fl],
+                                       s_rxmax[w2][16 * f + fl]);
             }
             if (wv == 0) {
-                double val;
-                int flag = 0;
-                if (SCALED) {
-                    // rb.key = x_j * prior_j in units of 2^e_sum
-                    if (!(rb.key > 0)) {
-                        flag = kFlagRecompute64;
-                        val = 0.0;
+#pragma unroll
+                for (int f = 0; f < FB; ++f) {
+                    double val;
+                    int flag = 0;
+                    if (SCALED) {
+                        // rb.key = x_j * prior_j in units of 2^e_sum
+                        if (!(rb[f].key > 0)) {
+                            flag = kFlagRecompute64;
+                            val = 0.0;
+                        } else {
+                            const double log_scale = (double)e_sum[f] * kLn2;
+                            if (p.model == 0)
+                                val = log(rb[f].x) + log(p.prior64[bk.prior_off + rb[f].j]) + log_scale;
+                            else
+                                val = ldexp(rb[f].key, e_sum[f]);
+                            // flush bound (absolute, in units of 2^e_sum) against
+                            // the likelihood, and FP64 range of the unscaled values
+                            const double bound = (ldexp(cert[f], log2S) + kTau32) * 1.01;
+                            if (!(bound <= kCertEta * rb[f].key)) flag |= kFlagRecompute64;
+                            if (!(log(rb[f].xmax) + log_scale >= log(kFp64Floor))) flag |= kFlagRecompute64;
+                            if (!(log(rb[f].key) + log_scale >= log(kFp64Floor))) flag |= kFlagRecompute64;
+                        }
                     } else {
-                        const double log_scale = (double)e_sum * kLn2;
-                        if (p.model == 0)
-                            val = log(rb.x) + log(p.prior64[bk.prior_off + rb.j]) + log_scale;
-                        else
-                            val = ldexp(rb.key, e_sum);
-                        // flush bound (absolute, in units of 2^e_sum) against
-                        // the likelihood, and FP64 range of the unscaled values
-                        const double bound = (ldexp(cert, log2S) + kTau32) * 1.01;
-                        if (!(bound <= kCertEta * rb.key)) flag |= kFlagRecompute64;
-                        if (!(log(rb.xmax) + log_scale >= log(kFp64Floor))) flag |= kFlagRecompute64;
-                        if (!(log(rb.key) + log_scale >= log(kFp64Floor))) flag |= kFlagRecompute64;
+                        // base: log(x_j prior_j) at the best j (base_model.cpp:86
+                        // takes log x_j + log prior_j); gamma: x_j prior_j
+                        val = p.model == 0 ? log(rb[f].key) : rb[f].key;
+                        if (rb[f].xmax == 0.0) flag |= kFlagZeroSum;
+                        if (rb[f].xmax < kFp64Floor || rb[f].key < kFp64Floor) flag |= kFlagRecomputePlain;
                     }
-                } else {
-                    // base: log(x_j prior_j) at the best j (base_model.cpp:86
-                    // takes log x_j + log prior_j); gamma: x_j prior_j
-                    val = p.model == 0 ? log(rb.key) : rb.key;
-                    if (rb.xmax == 0.0) flag |= kFlagZeroSum;
-                    if (rb.xmax < kFp64Floor || rb.key < kFp64Floor) flag |= kFlagRecomputePlain;
-                }
-                if (valid && g == 0) {
-                    p.out_val[(long long)td.point * p.F + dst] = val;
-                    p.out_flag[(long long)td.point * p.F + dst] = flag;
+                    if (valid[f] && g == 0) {
+                        p.out_val[(long long)td.point * p.F + dst[f]] = val;
+                        p.out_flag[(long long)td.point * p.F + dst[f]] = flag;
+                    }
                 }
             }
         };
@@ -628,23 +699,38 @@ void k_prune_mfma(PruneParams p)
     X(T, 1, 4, S, true) X(T, 1, 4, S, false) X(T, 1, 8, S, true) X(T, 1, 8, S, false) X(T, 2, 8, S, true) \
     X(T, 2, 8, S, false) X(T, 2, 16, S, false) X(T, 1, 16, S, true) X(T, 1, 16, S, false)
 
+// 32-family tiles (FP32 only: a leaf step's two columns use both prefetch
+// buffers) for vectors of up to 16 tiles, whose k-loops stream their
+// matrices.  Two output tiles per wave and block hold ~190 VGPRs; four would
+// spill.
+#define CAFE_MFMA_FB2(X, T, S) \
+    X(T, 2, 4, S, true) X(T, 2, 4, S, false) X(T, 1, 8, S, true) X(T, 1, 8, S, false) X(T, 2, 8, S, true) \
+    X(T, 2, 8, S, false) X(T, 1, 16, S, false)
+
 // One instance per shape; with per_cu set, reports how many workgroups of it
 // fit on one CU instead of launching.
 template <typename T, bool SCALED>
 static hipError_t shape_dispatch(const MfmaShape& s, int grid, const PruneParams& p, hipStream_t st, int* per_cu)
 {
-#define CAFE_MFMA_CASE(T_, NTW, W, S_, LDSP)                                                                 \
-    if (s.ntw == NTW && s.w == W && s.lds == LDSP) {                                                        \
+#define CAFE_MFMA_CASE_FB(T_, NTW, W, S_, LDSP, FB)                                                          \
+    if (s.ntw == NTW && s.w == W && s.lds == LDSP && s.fb == FB) {                                          \
         if (per_cu)                                                                                         \
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_prune_mfma<T, NTW, W, SCALED, LDSP>, \
-                                                                64 * W, s.shm);                             \
-        hipLaunchKernelGGL((k_prune_mfma<T, NTW, W, SCALED, LDSP>), dim3(grid), dim3(64 * W), s.shm, st, p);  \
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(                                            \
+                per_cu, k_prune_mfma<T, NTW, W, SCALED, LDSP, FB>, 64 * W, s.shm);                          \
+        hipLaunchKernelGGL((k_prune_mfma<T, NTW, W, SCALED, LDSP, FB>), dim3(grid), dim3(64 * W), s.shm, st, p); \
         return hipGetLastError();                                                                           \
     }
+#define CAFE_MFMA_CASE(T_, NTW, W, S_, LDSP) CAFE_MFMA_CASE_FB(T_, NTW, W, S_, LDSP, 1)
+#define CAFE_MFMA_CASE2(T_, NTW, W, S_, LDSP) CAFE_MFMA_CASE_FB(T_, NTW, W, S_, LDSP, 2)
     CAFE_MFMA_SHAPES(CAFE_MFMA_CASE, T, SCALED)
     CAFE_MFMA_WIDE(CAFE_MFMA_CASE, T, SCALED)
     CAFE_MFMA_CHAIN(CAFE_MFMA_CASE, T, SCALED)
+    if constexpr (sizeof(T) == 4) {
+        CAFE_MFMA_FB2(CAFE_MFMA_CASE2, T, SCALED)
+    }
+#undef CAFE_MFMA_CASE2
 #undef CAFE_MFMA_CASE
+#undef CAFE_MFMA_CASE_FB
     return hipErrorInvalidValue;
 }
 

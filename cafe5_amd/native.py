@@ -324,9 +324,10 @@ class Plan:
     def class_times(self):
         """Per size class of the main pass, from the last profile() run:
         (16-row tiles per vector, tiles, kernel ms, launch) arrays; launch
-        rows are (kernel: 0 matrix-core, 1 one family per lane, 2 PLAIN64;
-        tiles per wave or rows per lane; waves per workgroup; partials in
-        LDS; workgroups; families per tile)."""
+        rows are (kernel: 0 matrix-core, 1 one family per lane, 2 PLAIN64,
+        3 split-FP16; tiles per wave or rows per lane; waves per workgroup;
+        partials in LDS; workgroups; families per tile: 16 or 32 for the
+        matrix-core kernel, 64 per lane)."""
         n = ctypes.c_int(0)
         self.eng._check(self.lib.cafe_gpu_plan_class_times(self.h, 0, ctypes.byref(n), None, None, None, None))
         nt = np.zeros(n.value, dtype=np.int32)

@@ -9,8 +9,9 @@ A size class is the set of buckets whose vectors have the same number of
 16-row tiles.  For each class and path the candidates are the per-lane
 kernel (where the class can take it) and every instantiated matrix-core
 shape (output tiles per wave x waves >= the class's tiles; partials in LDS
-where they fit, else global scratch) at grid multipliers 4, 8 and 16 (LDS)
-or 1, 2 and 4 (global); each is timed as that class's launch in profiled
+where they fit, else global scratch; for BATCHED32 classes of up to 16 tiles
+also 16- or 32-family tiles) at grid multipliers 4, 8 and 16 (LDS) or 1, 2
+and 4 (global); each is timed as that class's launch in profiled
 executions of the plan (median of 3), with the classes already tuned held at
 their choices.  Cache entries are keyed by (path, tiles, leaves of the tree).
 """
@@ -34,6 +35,10 @@ DEFAULT_CACHE = os.path.join(HERE, "tune_cache.txt")
 MFMA_SHAPES = [(1, 1), (2, 1), (3, 1), (4, 1), (3, 2), (4, 2), (3, 4), (4, 4), (3, 8), (4, 8), (3, 16), (4, 16),
                (1, 4), (1, 8), (2, 8), (1, 16), (2, 16)]   # the last five: latency shapes
 F16_SHAPES = [s for s in MFMA_SHAPES[:12] if s[1] <= 8]
+# shapes of the FP32 kernel with 32-family tiles (two blocks of 16 per
+# matrix operand), vectors of up to 16 tiles; (1, 16) has global partials only
+FB2_SHAPES = [(2, 4), (1, 8), (2, 8), (1, 16)]
+FB2_MAX_TILES = 16
 LANE_TILES = 3          # the per-lane kernel takes vectors of up to 33 rows (3 tiles)
 
 
@@ -52,6 +57,13 @@ def candidates(path: int, nt: int) -> List[Tuple[int, int, int, int, int]]:
                 continue   # instantiated with global partials only
             for gm in ((4, 8, 16) if lds else (1, 2, 4)):
                 out.append((0, ntw, w, lds, gm))
+    if path == native.PATH_BATCHED32 and nt <= FB2_MAX_TILES:
+        for ntw, w in FB2_SHAPES:
+            if ntw * w < nt or (ntw - 1) * w >= nt and ntw > 1:
+                continue
+            for lds in ((1, 0) if w < 16 else (0,)):
+                for gm in ((4, 8, 16) if lds else (1, 2, 4)):
+                    out.append((2, ntw, w, lds, gm))
     return out
 
 
@@ -65,17 +77,41 @@ def _class_ms(plan, nt: int, reps: int = 3) -> float:
     return float(np.median(ms))
 
 
-def tune_plan(eng: native.Engine, make_plan, path: int, leaves: int, log=print) -> List[Tuple]:
+def _step_ms(plan, reps: int = 20, rounds: int = 3) -> float:
+    """Median over rounds of the ms per execution of reps graph replays (the
+    size classes run concurrently, as in the timed steps)."""
+    for _ in range(2):
+        plan.execute()
+        plan.synchronize()
+    ms = []
+    for _ in range(rounds):
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            plan.execute()
+            plan.synchronize()
+        ms.append((time.perf_counter() - t0) * 1e3 / reps)
+    return float(np.median(ms))
+
+
+def tune_plan(eng: native.Engine, make_plan, path: int, leaves: int, log=print, objective: str = "step",
+              min_gain: float = 0.005) -> List[Tuple]:
     """Measure every class of the plan make_plan(path) builds; returns the
-    chosen entries (path, tiles, leaves, kernel, ntw, w, lds, grid_mult, ms)."""
+    chosen entries (path, tiles, leaves, kernel, ntw, w, lds, grid_mult, ms).
+    objective "class": the class's own launch time in profiled executions
+    (classes one after another); "step": the whole execution's time with
+    the classes running concurrently, as the timed steps run them (a shape
+    that is fastest alone can hold LDS or matrix cores another class needs).
+    A candidate replaces the current choice only when it is faster by more
+    than min_gain (relative), so timing noise does not pick shapes."""
     base = make_plan(path)
     base.profile()
     nts, tiles, cms, launch = base.class_times()
+    cur_ms = _step_ms(base) if objective == "step" else None
     base.close()
     chosen = []
     for nt, default_ms, l in sorted(zip(nts, cms, launch), key=lambda x: -x[1]):
         nt = int(nt)
-        best = (float(default_ms), None)
+        best = (float(default_ms) if objective == "class" else cur_ms, None)
         tried = 0
         for cand in candidates(path, nt):
             kernel, ntw, w, lds, gm = cand
@@ -84,20 +120,24 @@ def tune_plan(eng: native.Engine, make_plan, path: int, leaves: int, log=print) 
                 p = make_plan(path)
             except native.CafeGpuError:
                 continue
-            ms = _class_ms(p, nt)
+            ms = _class_ms(p, nt) if objective == "class" else _step_ms(p)
             p.close()
             tried += 1
-            if ms < best[0]:
+            if ms < best[0] * (1.0 - min_gain):
                 best = (ms, cand)
         if best[1] is None:   # the built-in shape was fastest: pin it as measured
             k = int(l[0])
-            best_c = (1, 0, 0, 0, 0) if k == 1 else (0, int(l[1]), int(l[2]), int(l[3]), 0)
+            kern = 2 if k == 0 and int(l[5]) == 32 else 0
+            best_c = (1, 0, 0, 0, 0) if k == 1 else (kern, int(l[1]), int(l[2]), int(l[3]), 0)
         else:
             best_c = best[1]
         eng.tuning_set(path, nt, leaves, *best_c[:3], bool(best_c[3]), best_c[4])
         chosen.append((path, nt, leaves, *best_c, best[0]))
-        log(f"path {native.PATH_NAMES[path]} class {nt} tiles: built-in {float(default_ms):.3f} ms, "
-            f"best {best[0]:.3f} ms with {best_c} ({tried} candidates)")
+        what = "class" if objective == "class" else "step"
+        log(f"path {native.PATH_NAMES[path]} class {nt} tiles: {what} {float(default_ms if objective == 'class' else cur_ms):.3f} "
+            f"ms before, {best[0]:.3f} ms with {best_c} ({tried} candidates)")
+        if objective == "step":
+            cur_ms = best[0]
     return chosen
 
 
@@ -133,6 +173,8 @@ def main():
     ap.add_argument("--paths", default="2,3,1")
     ap.add_argument("--out", default=DEFAULT_CACHE)
     ap.add_argument("--append", action="store_true", help="keep the entries already in --out")
+    ap.add_argument("--objective", default="step", choices=("step", "class"),
+                    help="step: whole replayed execution (classes concurrent); class: the class's launch alone")
     args = ap.parse_args()
     w, node_lams, bounds, err, K = _workload(args.workload)
     eng = native.Engine(0)
@@ -152,7 +194,8 @@ def main():
     cat = np.full(K, 1.0 / K) if K > 1 else None
 
     def make_plan(path):
-        return eng.plan(slots, prior, lam_q, t_q, path=path, model=model, cat_probs=cat, use_graph=False)
+        return eng.plan(slots, prior, lam_q, t_q, path=path, model=model, cat_probs=cat,
+                        use_graph=args.objective == "step")
 
     entries = []
     if args.append and os.path.exists(args.out):
@@ -165,10 +208,12 @@ def main():
                     entries.append(tuple(int(x) for x in line[:8]) + (float(line[8]) if len(line) > 8 else 0.0,))
     t0 = time.time()
     for path in [int(x) for x in args.paths.split(",") if x]:
-        got = tune_plan(eng, make_plan, path, w.tree.n_leaves, log=lambda m: print(m, flush=True))
+        got = tune_plan(eng, make_plan, path, w.tree.n_leaves, log=lambda m: print(m, flush=True),
+                        objective=args.objective)
         keyset = {(e[0], e[1], e[2]) for e in got}
         entries = [e for e in entries if (e[0], e[1], e[2]) not in keyset] + got
-    write_cache(args.out, entries, f"measured on {args.workload} ({w.name}) in {time.time() - t0:.0f} s")
+    write_cache(args.out, entries, f"measured on {args.workload} ({w.name}), objective {args.objective}, in "
+                                   f"{time.time() - t0:.0f} s")
     print(f"wrote {len(entries)} entries to {args.out}", flush=True)
     eng.close()
 

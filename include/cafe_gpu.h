@@ -236,12 +236,14 @@ int cafe_gpu_plan_profile(cafe_gpu_plan *plan, double *ms4, int *graph_ok);
 int cafe_gpu_plan_recompute_counts(cafe_gpu_plan *plan, long long *fp64, long long *plain);
 /* Per size class of the main pruning pass, from the last
  * cafe_gpu_plan_profile run (no reference counterpart; measurement): the
- * class's 16-row tile count per vector, its number of 16-family tiles and
- * the milliseconds of its kernel launch (HIP events on the context stream),
+ * class's 16-row tile count per vector, its number of tiles (of the
+ * families per tile given in launch) and the milliseconds of its kernel
+ * launch (HIP events on the context stream),
  * and, in launch[6 i .. 6 i + 5], that launch's shape: kernel (0 matrix-core
  * k_prune_mfma, 1 one-family-per-lane k_prune_lane, 2 PLAIN64 k_prune_plain),
  * 16-row tiles per wave (lane and PLAIN64 kernels: rows per lane), waves per
- * workgroup, partial vectors in LDS (0/1), workgroups, and families per tile.
+ * workgroup, partial vectors in LDS (0/1), workgroups, and families per tile
+ * (matrix-core kernel: 16, or 32 with two 16-family blocks per operand).
  * Writes at most cap entries (any output may be null); *n_classes is the
  * number of classes. */
 int cafe_gpu_plan_class_times(cafe_gpu_plan *plan, int cap, int *n_classes, int *class_nt,
@@ -252,13 +254,16 @@ int cafe_gpu_plan_destroy(cafe_gpu_plan *plan);
  * Plans and evaluations pick a launch shape for every size class (the
  * buckets whose vectors have the same number of 16-row tiles): the
  * one-family-per-lane kernel or the matrix-core kernel, and for the latter
- * its output tiles per wave, waves per workgroup, partials in LDS or global
- * scratch, and the grid (workgroups per resident slot).  The context holds a
+ * its families per tile (16, or 32 for the FP32 kernel on vectors of up to 16
+ * tiles: each matrix operand then feeds two 16-family blocks), output tiles
+ * per wave, waves per workgroup, partials in LDS or global scratch, and the
+ * grid (workgroups per resident slot).  The context holds a
  * table of measured choices keyed by (path, tiles per vector, leaves of the
  * tree) that overrides the built-in rules; cafe5_amd/tune.py writes it as a
  * cache file on the GPU, one line per entry:
  *     path tiles leaves kernel ntw w lds grid_mult [measured ms]
- * (kernel 0 matrix core, 1 per lane; '#' starts a comment).  An entry whose
+ * (kernel 0 matrix core, 1 per lane, 2 matrix core with 32-family tiles;
+ * '#' starts a comment).  An entry whose
  * shape does not exist or does not fit the class is ignored, and the class
  * keeps its built-in shape. */
 /* Replaces the table with the entries of a cache file; *n_loaded (may be

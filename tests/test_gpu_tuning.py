@@ -84,3 +84,37 @@ def test_malformed_cache_is_rejected(setup, tmp_path):
     with pytest.raises(CafeGpuError):
         eng.tuning_load(str(tmp_path / "missing.txt"))
     eng.tuning_clear()
+
+
+# 32-family tiles (kernel 2): every instantiated shape on every class it
+# covers.  Each family's column of an MFMA depends only on the matrix
+# operand and that family's own column, in the same k order as with
+# 16-family tiles, so the per-family results (after the same certification
+# and FP64 recompute) are bit-identical to the built-in shapes', including
+# the ragged last tile of every (bucket, point).
+@pytest.mark.parametrize("shape", [(2, 4, 1), (2, 4, 0), (1, 8, 1), (1, 8, 0), (2, 8, 1), (2, 8, 0), (1, 16, 0)])
+def test_32_family_tiles_match_16_family_tiles(setup, shape):
+    w, eng, slots, prior, lq, tq = setup
+    path = native.PATH_BATCHED32
+    eng.tuning_clear()
+    base, shapes = _run(eng, slots, prior, lq, tq, path)
+    ntw, wv, lds = shape
+    leaves = w.tree.n_leaves
+    tried = [nt for nt, l in shapes.items() if l[0] == 0 and nt <= 16 and ntw * wv >= nt]
+    assert tried
+    for nt in tried:
+        eng.tuning_set(path, nt, leaves, 2, ntw, wv, bool(lds), 4 if lds else 2)
+    tuned, tshapes = _run(eng, slots, prior, lq, tq, path)
+    eng.tuning_clear()
+    wide = [nt for nt in tried if tshapes[nt][5] == 32]
+    print(f"shape {shape}: 32-family tiles on classes {wide} of {tried}; {tshapes}", flush=True)
+    for nt in wide:
+        assert tshapes[nt][:4] == (0, ntw, wv, lds)
+    if not lds:
+        assert wide == tried                 # global partials always fit
+    assert wide
+    assert np.array_equal(np.isfinite(tuned["lnl"]), np.isfinite(base["lnl"]))
+    fin = np.isfinite(base["lnl"])
+    assert np.array_equal(tuned["lnl"][fin], base["lnl"][fin]), np.abs(tuned["lnl"][fin] - base["lnl"][fin]).max()
+    assert np.array_equal(tuned["total"], base["total"])
+    assert tuned["n_fallback"] == base["n_fallback"]

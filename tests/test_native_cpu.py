@@ -103,8 +103,12 @@ def test_reference_file_readers_through_cpp_interface():
 def test_tuner_candidates_and_cache_format(tmp_path):
     from cafe5_amd import tune
     c13 = tune.candidates(native.PATH_BATCHED32, 13)
-    assert all(k == 0 and ntw * w >= 13 for k, ntw, w, _, _ in c13)
-    assert {(ntw, w) for _, ntw, w, _, _ in c13} == {(4, 4), (2, 8), (1, 16)}   # throughput and latency shapes
+    assert all(k in (0, 2) and ntw * w >= 13 for k, ntw, w, _, _ in c13)
+    # 16-family tiles: throughput and latency shapes; 32-family tiles
+    assert {(ntw, w) for k, ntw, w, _, _ in c13 if k == 0} == {(4, 4), (2, 8), (1, 16)}
+    assert {(ntw, w) for k, ntw, w, _, _ in c13 if k == 2} == {(2, 8), (1, 16)}
+    assert not any(k == 2 for k, *_ in tune.candidates(native.PATH_BATCHED32, 17))   # up to 16 tiles
+    assert not any(k == 2 for k, *_ in tune.candidates(native.PATH_BATCHED64, 13))   # FP32 only
     c2 = tune.candidates(native.PATH_BATCHED32, 2)
     assert any(k == 1 for k, *_ in c2) and any(k == 0 for k, *_ in c2)
     assert not any(k == 1 for k, *_ in tune.candidates(native.PATH_BATCHED64, 2))   # FP64: matrix cores only
@@ -124,4 +128,4 @@ def test_committed_tuning_cache_parses():
         if f:
             assert len(f) in (8, 9), line
             path_, tiles, leaves, kernel, ntw, w, lds, gm = (int(x) for x in f[:8])
-            assert 0 <= path_ <= 3 and tiles >= 1 and leaves >= 1 and kernel in (0, 1) and lds in (0, 1)
+            assert 0 <= path_ <= 3 and tiles >= 1 and leaves >= 1 and kernel in (0, 1, 2) and lds in (0, 1)
