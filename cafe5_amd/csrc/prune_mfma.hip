@@ -146,14 +146,15 @@ __device__ __forceinline__ void mma_step(typename Vec4<T>::type (&x)[FB][NTW], c
 // fragment zeroed, which adds exactly 0, so each wait covers only the oldest
 // buffer.  A one-wave kernel holds whole vectors (nts <= NTW), so its ring
 // only needs the NTW - PK k-tiles past the prefetched ones (none for NTW <= PK).
-template <typename T, int NTW, int W> struct Ring {
+template <typename T, int NTW, int W, int FB> struct Ring {
     // two k-tiles in flight; a deeper ring costs registers (the 4-tile FP32
     // shapes would drop to one wave per SIMD) without shortening their step
     // time, which other waves' work covers.  The FP64 recompute shapes with
     // one or two tiles per wave run a few workgroups whose step time is the
     // k-loop's memory latency: four in flight.  W = 16 leaves 128 VGPRs per
-    // wave: one.
-    static constexpr int deep = W >= 16 ? 1 : (sizeof(T) == 8 && NTW <= 2 ? 4 : 2);
+    // wave: two with one output tile of one family block per wave (the FP64
+    // recompute of the largest vectors is such a chain), else one.
+    static constexpr int deep = W >= 16 ? (NTW == 1 && FB == 1 ? 2 : 1) : (sizeof(T) == 8 && NTW <= 2 ? 4 : 2);
     static constexpr int tail = NTW > Prefetch<T>::PK ? NTW - Prefetch<T>::PK : 0;
     static constexpr int D = W == 1 ? tail : deep;
 };
@@ -483,7 +484,7 @@ void k_prune_mfma(PruneParams p)
                     ao[i] = root ? offA1[i] : offA0[i];
                 }
                 const V4* bsrc = frag + (long long)sd.in_slot * slot_tiles * FB * 64 + lane;
-                child_gemm<T, NTW, Ring<T, NTW, W>::D, SCALED, FB>(x, M, Np, ao, pf, bsrc, nts, bsh);
+                child_gemm<T, NTW, Ring<T, NTW, W, FB>::D, SCALED, FB>(x, M, Np, ao, pf, bsrc, nts, bsh);
             }
 #pragma unroll
             for (int f = 0; f < FB; ++f)
@@ -700,12 +701,13 @@ void k_prune_mfma(PruneParams p)
     X(T, 2, 8, S, false) X(T, 2, 16, S, false) X(T, 1, 16, S, true) X(T, 1, 16, S, false)
 
 // 32-family tiles (FP32 only: a leaf step's two columns use both prefetch
-// buffers) for vectors of up to 16 tiles, whose k-loops stream their
-// matrices.  Two output tiles per wave and block hold ~190 VGPRs; four would
-// spill.
+// buffers) for vectors of up to 32 tiles, whose k-loops stream their
+// matrices.  Two output tiles per wave and block hold ~190 VGPRs, four would
+// spill; sixteen waves of two (the 17-32-tile vectors) are capped at 128
+// VGPRs and keep ~30 values in scratch.
 #define CAFE_MFMA_FB2(X, T, S) \
     X(T, 2, 4, S, true) X(T, 2, 4, S, false) X(T, 1, 8, S, true) X(T, 1, 8, S, false) X(T, 2, 8, S, true) \
-    X(T, 2, 8, S, false) X(T, 1, 16, S, false)
+    X(T, 2, 8, S, false) X(T, 1, 16, S, false) X(T, 2, 16, S, false)
 
 // One instance per shape; with per_cu set, reports how many workgroups of it
 // fit on one CU instead of launching.

@@ -2,14 +2,14 @@
 of a workload on the GPU and writes the fastest to a cache file that
 cafe_gpu_tuning_load reads (format in include/cafe_gpu.h).
 
-    python -m cafe5_amd.tune [--workload uneven|errmodel|optimizer] [--paths 2,3,1]
+    python -m cafe5_amd.tune [--workload uneven|gamma|errmodel|optimizer] [--paths 2,3,1]
                               [--out cafe5_amd/tune_cache.txt]
 
 A size class is the set of buckets whose vectors have the same number of
 16-row tiles.  For each class and path the candidates are the per-lane
 kernel (where the class can take it) and every instantiated matrix-core
 shape (output tiles per wave x waves >= the class's tiles; partials in LDS
-where they fit, else global scratch; for BATCHED32 classes of up to 16 tiles
+where they fit, else global scratch; for BATCHED32 classes of up to 32 tiles
 also 16- or 32-family tiles) at grid multipliers 4, 8 and 16 (LDS) or 1, 2
 and 4 (global); each is timed as that class's launch in profiled
 executions of the plan (median of 3), with the classes already tuned held at
@@ -36,9 +36,10 @@ MFMA_SHAPES = [(1, 1), (2, 1), (3, 1), (4, 1), (3, 2), (4, 2), (3, 4), (4, 4), (
                (1, 4), (1, 8), (2, 8), (1, 16), (2, 16)]   # the last five: latency shapes
 F16_SHAPES = [s for s in MFMA_SHAPES[:12] if s[1] <= 8]
 # shapes of the FP32 kernel with 32-family tiles (two blocks of 16 per
-# matrix operand), vectors of up to 16 tiles; (1, 16) has global partials only
-FB2_SHAPES = [(2, 4), (1, 8), (2, 8), (1, 16)]
-FB2_MAX_TILES = 16
+# matrix operand), vectors of up to 32 tiles; the 16-wave shapes have global
+# partials only
+FB2_SHAPES = [(2, 4), (1, 8), (2, 8), (1, 16), (2, 16)]
+FB2_MAX_TILES = 32
 LANE_TILES = 3          # the per-lane kernel takes vectors of up to 33 rows (3 tiles)
 
 
@@ -155,6 +156,12 @@ def _workload(name: str):
         w = wl.config_uneven(seed=2, F=20000, T=64)
         lams = np.repeat(w.lambdas[:, None], w.tree.n_nodes, axis=1)
         return w, lams, w.bounds, None, 1
+    if name == "gamma":
+        w = wl.config_gamma(seed=3, F=50000, K=4, alpha=0.5, lam=0.003)
+        from .model import get_gamma
+        _, rate = get_gamma(4, 0.5)
+        lams = np.stack([np.full(w.tree.n_nodes, float(w.lambdas[0]) * r) for r in rate])
+        return w, lams, w.bounds, None, 4
     if name == "errmodel":
         w = wl.config_error_multi(seed=4, F=100000, bound=150)
         return w, w.lambdas[w.node_class][None, :], w.bounds, wl.error_table(0.05, 150), 1
@@ -169,7 +176,7 @@ def _workload(name: str):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--workload", default="uneven", choices=("uneven", "errmodel", "optimizer"))
+    ap.add_argument("--workload", default="uneven", choices=("uneven", "gamma", "errmodel", "optimizer"))
     ap.add_argument("--paths", default="2,3,1")
     ap.add_argument("--out", default=DEFAULT_CACHE)
     ap.add_argument("--append", action="store_true", help="keep the entries already in --out")

@@ -254,7 +254,7 @@ int cafe_gpu_plan_destroy(cafe_gpu_plan *plan);
  * Plans and evaluations pick a launch shape for every size class (the
  * buckets whose vectors have the same number of 16-row tiles): the
  * one-family-per-lane kernel or the matrix-core kernel, and for the latter
- * its families per tile (16, or 32 for the FP32 kernel on vectors of up to 16
+ * its families per tile (16, or 32 for the FP32 kernel on vectors of up to 32
  * tiles: each matrix operand then feeds two 16-family blocks), output tiles
  * per wave, waves per workgroup, partials in LDS or global scratch, and the
  * grid (workgroups per resident slot).  The context holds a

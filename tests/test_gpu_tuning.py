@@ -92,7 +92,8 @@ def test_malformed_cache_is_rejected(setup, tmp_path):
 # 16-family tiles, so the per-family results (after the same certification
 # and FP64 recompute) are bit-identical to the built-in shapes', including
 # the ragged last tile of every (bucket, point).
-@pytest.mark.parametrize("shape", [(2, 4, 1), (2, 4, 0), (1, 8, 1), (1, 8, 0), (2, 8, 1), (2, 8, 0), (1, 16, 0)])
+@pytest.mark.parametrize("shape", [(2, 4, 1), (2, 4, 0), (1, 8, 1), (1, 8, 0), (2, 8, 1), (2, 8, 0), (1, 16, 0),
+                                   (2, 16, 0)])
 def test_32_family_tiles_match_16_family_tiles(setup, shape):
     w, eng, slots, prior, lq, tq = setup
     path = native.PATH_BATCHED32
@@ -118,3 +119,40 @@ def test_32_family_tiles_match_16_family_tiles(setup, shape):
     assert np.array_equal(tuned["lnl"][fin], base["lnl"][fin]), np.abs(tuned["lnl"][fin] - base["lnl"][fin]).max()
     assert np.array_equal(tuned["total"], base["total"])
     assert tuned["n_fallback"] == base["n_fallback"]
+
+
+def test_32_family_tiles_on_501_row_vectors():
+    """The optimizer-scale shape (100 taxa, B = 500: 32-tile vectors, K = 8
+    gamma categories) on 3,000 families: 32-family tiles of sixteen waves
+    (the built-in choice at this size) against 16-family tiles of eight
+    waves of four, bit for bit."""
+    from cafe5_amd.model import ResidentGammaModel
+    w = wl.config_optimizer(seed=5, F=3000, K=8, bound=500, T=200)
+    B, K = 500, 8
+    eng = native.Engine(0)
+    out = []
+    for entry in ((0, 4, 8, 0, 2), None):
+        eng.tuning_clear()
+        if entry:
+            eng.tuning_set(native.PATH_BATCHED32, 32, w.tree.n_leaves, entry[0], *entry[1:3], bool(entry[3]),
+                           entry[4])
+        m = ResidentGammaModel(eng, w.tree, w.counts, B, B, K, priors.uniform(B, B), path=native.PATH_BATCHED32)
+        totals, lnls = [], []
+        for t in (3, 7):
+            totals.append(m.infer_family_likelihoods(float(w.lambdas[t]), max(1.0, float(w.alphas[t]))))
+            lnls.append(m.results["lnl"].copy())
+        m.plan.profile()
+        nts, _, _, launch = m.plan.class_times()
+        m.close()
+        out.append((totals, lnls, {int(n): tuple(int(x) for x in l) for n, l in zip(nts, launch)}))
+    eng.tuning_clear()
+    eng.close()
+    (t16, l16, s16), (t32, l32, s32) = out
+    print(f"16-family {s16}\n32-family {s32}\ntotals {t16} {t32}", flush=True)
+    assert s16[32][:4] == (0, 4, 8, 0) and s16[32][5] == 16
+    assert s32[32][:4] == (0, 2, 16, 0) and s32[32][5] == 32
+    for a, b in zip(l16, l32):
+        assert np.array_equal(np.isfinite(a), np.isfinite(b))
+        fin = np.isfinite(a)
+        assert np.array_equal(a[fin], b[fin])
+    assert t16 == t32

@@ -107,7 +107,8 @@ def test_tuner_candidates_and_cache_format(tmp_path):
     # 16-family tiles: throughput and latency shapes; 32-family tiles
     assert {(ntw, w) for k, ntw, w, _, _ in c13 if k == 0} == {(4, 4), (2, 8), (1, 16)}
     assert {(ntw, w) for k, ntw, w, _, _ in c13 if k == 2} == {(2, 8), (1, 16)}
-    assert not any(k == 2 for k, *_ in tune.candidates(native.PATH_BATCHED32, 17))   # up to 16 tiles
+    assert {(ntw, w) for k, ntw, w, _, _ in tune.candidates(native.PATH_BATCHED32, 32) if k == 2} == {(2, 16)}
+    assert not any(k == 2 for k, *_ in tune.candidates(native.PATH_BATCHED32, 33))   # up to 32 tiles
     assert not any(k == 2 for k, *_ in tune.candidates(native.PATH_BATCHED64, 13))   # FP32 only
     c2 = tune.candidates(native.PATH_BATCHED32, 2)
     assert any(k == 1 for k, *_ in c2) and any(k == 0 for k, *_ in c2)
