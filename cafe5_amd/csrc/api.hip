@@ -19,7 +19,7 @@
 namespace cafe {
 hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const double* d_lg, int n,
                         int first_slot, int N, int Np, long long slot_elems, double* mt64, float* mt32,
-                        float* lf32, hipStream_t stream);
+                        float* lf32, hipStream_t stream, const unsigned char* copies, int top);
 hipError_t launch_fill_zero(void* p, long long bytes, hipStream_t stream);
 hipError_t launch_reconstruct(const ReconNode* nodes, int n_nodes, const int* kids, const int* parent,
                               const double* mt64, int N, int Np, long long slot_elems, const int* counts_T, int F,
@@ -147,7 +147,7 @@ struct cafe_gpu_ctx {
     long long slot_elems = 0;
     DevBuf mt64, mt32, lf32, key_lam, key_t;
     DevBuf mt16h, mt16l;   // split-FP16 halves of mt32 (BATCHED16; made by its evaluations)
-    std::vector<char> built;
+    std::vector<char> built;   // per slot: 0 no, 1 every copy, 2 FP64 and the FP32 copies a plan read
 
     // tree program
     int n_nodes = 0, tree_leaves = 0, n_slots = 0, max_children = 2;
@@ -449,7 +449,7 @@ int cafe_gpu_matrix_build(cafe_gpu_ctx* ctx, int n, const double* lambda_q, cons
     HIP_OR_FAIL(hipEventRecord(ctx->ev[4], ctx->stream));
     HIP_OR_FAIL(launch_build(algo, ctx->key_lam.as<double>(), ctx->key_t.as<double>(), ctx->lg.as<double>(), n,
                              first_slot, ctx->N, ctx->Np, ctx->slot_elems, ctx->mt64.as<double>(),
-                             ctx->mt32.as<float>(), ctx->lf32.as<float>(), ctx->stream));
+                             ctx->mt32.as<float>(), ctx->lf32.as<float>(), ctx->stream, nullptr, 0));
     if (int rc = enqueue_fold(ctx, first_slot, n, ctx->stream)) return rc;
     HIP_OR_FAIL(hipEventRecord(ctx->ev[5], ctx->stream));
     HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
@@ -952,6 +952,10 @@ int upload_eval_inputs(cafe_gpu_ctx* ctx, EvalBufs& b, const cafe_gpu_eval_desc*
         if (slot[i] < 0 || slot[i] >= ctx->capacity || !ctx->built[slot[i]])
             return fail(CAFE_GPU_ENOTFOUND, "node %d of point %lld uses matrix slot %d, which is not built", node,
                         i / ctx->n_nodes, slot[i]);
+        if (ctx->built[slot[i]] == 2 && (d->path == CAFE_GPU_PATH_BATCHED32 || d->path == CAFE_GPU_PATH_BATCHED16))
+            return fail(CAFE_GPU_ENOTFOUND,
+                        "matrix slot %d holds the FP64 matrix and only the FP32 copies a plan read; build it with "
+                        "cafe_gpu_matrix_build for this path", slot[i]);
     }
     const int nb = (int)ctx->buckets.size();
     std::vector<double> pr64((size_t)nb * d->prior_stride, 0.0);
@@ -1589,6 +1593,8 @@ struct cafe_gpu_plan {
     PrunePlan pp;
     int model = 0, path = 0, T = 0, K = 0, n_keys = 0, algo = 0;
     DevBuf d_lam, d_t;
+    DevBuf d_copies;         // per key: the FP32 copies the plan's kernels read (kCopy* bits)
+    int copy_top = 0;        // kCopyColTop: rows and columns of the per-lane classes
     long long pool_gen = 0, input_gen = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1615,13 +1621,16 @@ int plan_enqueue(cafe_gpu_plan* pl, bool timed)
     if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[0], st));
     HIP_OR_FAIL(launch_build(pl->algo, pl->d_lam.as<double>(), pl->d_t.as<double>(), ctx->lg.as<double>(), pl->n_keys,
                              0, ctx->N, ctx->Np, ctx->slot_elems, split ? nullptr : ctx->mt64.as<double>(),
-                             ctx->mt32.as<float>(), ctx->lf32.as<float>(), st));
-    if (int rc = enqueue_fold(ctx, 0, pl->n_keys, st, !split, true)) return rc;
+                             ctx->mt32.as<float>(), ctx->lf32.as<float>(), st, pl->d_copies.as<unsigned char>(),
+                             pl->copy_top));
+    // the folded FP32 leaf pool is read by the reduced paths only
+    const bool reduced = pl->pp.path == CAFE_GPU_PATH_BATCHED32 || pl->pp.path == CAFE_GPU_PATH_BATCHED16;
+    if (int rc = enqueue_fold(ctx, 0, pl->n_keys, st, !split, reduced)) return rc;
     if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[1], st));
     const std::function<int(hipStream_t)> build64 = [&](hipStream_t s) -> int {
         HIP_OR_FAIL(launch_build(pl->algo, pl->d_lam.as<double>(), pl->d_t.as<double>(), ctx->lg.as<double>(),
                                  pl->n_keys, 0, ctx->N, ctx->Np, ctx->slot_elems, ctx->mt64.as<double>(), nullptr,
-                                 nullptr, s));
+                                 nullptr, s, nullptr, 0));
         return enqueue_fold(ctx, 0, pl->n_keys, s, true, false);
     };
     // profile runs issue the size classes one after another (standalone kernel
@@ -1670,10 +1679,37 @@ int plan_current(cafe_gpu_plan* pl)
     return CAFE_GPU_OK;
 }
 
-// Slots 0 .. n_keys-1 hold the plan's matrices once an execution is enqueued.
+// Slots 0 .. n_keys-1 hold the plan's matrices once an execution is enqueued:
+// the FP64 pool and the FP32 copies the plan reads.
 void plan_mark_built(cafe_gpu_plan* pl)
 {
-    for (int i = 0; i < pl->n_keys; ++i) pl->ctx->built[i] = 1;
+    for (int i = 0; i < pl->n_keys; ++i) pl->ctx->built[i] = 2;
+}
+
+// The FP32 copies each key's users read in this plan: a leaf branch's
+// matrix gives leaf columns (c-major); an interior branch's matrix is an
+// MFMA A operand (row-major; also what the split-FP16 halves come from) or,
+// in the per-lane classes, columns of the top-left lane_rows x lane_rows
+// block (c-major).  FP64 paths read no FP32 copy.
+void plan_copies(const cafe_gpu_ctx* ctx, const cafe_gpu_eval_desc* d, const PrunePlan& pp, int P, int n_keys,
+                 std::vector<unsigned char>& copies, int& top)
+{
+    copies.assign(n_keys, 0);
+    top = 0;
+    if (pp.path != CAFE_GPU_PATH_BATCHED32 && pp.path != CAFE_GPU_PATH_BATCHED16) return;
+    bool mfma = false;
+    for (const auto& kc : pp.classes) {
+        if (kc.lane_rows > 0) top = std::max(top, kc.lane_rows);
+        else mfma = true;
+    }
+    const unsigned char interior = (mfma ? kCopyRow : 0) | (top > 0 ? kCopyColTop : 0);
+    const int root = ctx->ops.back().node;
+    for (long long i = 0; i < (long long)P * ctx->n_nodes; ++i) {
+        const int node = (int)(i % ctx->n_nodes);
+        if (node == root) continue;
+        const int s = d->node_slot[i];
+        if (s >= 0 && s < n_keys) copies[s] |= ctx->leaf_col_h[node] >= 0 ? kCopyCol : interior;
+    }
 }
 
 }  // namespace
@@ -1849,6 +1885,15 @@ int cafe_gpu_plan_create(cafe_gpu_ctx* ctx, const cafe_gpu_eval_desc* d, int n_k
     for (auto& e : pl->ev) (void)hipEventCreate(&e);
     pl->cls_ev.assign(pl->pp.classes.size() + 1, nullptr);
     for (auto& e : pl->cls_ev) (void)hipEventCreate(&e);
+    {
+        std::vector<unsigned char> copies;
+        plan_copies(ctx, d, pl->pp, pl->T * pl->K, n_keys, copies, pl->copy_top);
+        if (pl->d_copies.ensure(copies.size()) != hipSuccess ||
+            hipMemcpy(pl->d_copies.p, copies.data(), copies.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            cafe_gpu_plan_destroy(pl);
+            return fail(CAFE_GPU_ENOMEM, "plan copy table");
+        }
+    }
     *out = pl;
     if ((rc = cafe_gpu_plan_set_keys(pl, lambda_q, t_q)) != CAFE_GPU_OK) {
         cafe_gpu_plan_destroy(pl);
@@ -1989,6 +2034,7 @@ int cafe_gpu_plan_destroy(cafe_gpu_plan* pl)
     pl->bufs.release_all();
     pl->d_lam.release();
     pl->d_t.release();
+    pl->d_copies.release();
     delete pl;
     return CAFE_GPU_OK;
 }

@@ -72,7 +72,10 @@ __device__ __forceinline__ double int_pow(double base, int e)
 template <int NB> struct RowGroup { static constexpr int v = NB >= 32 ? 2 : NB >= 16 ? 4 : NB >= 2 ? 64 / NB : 32; };
 
 // Any of the three pools may be null: the FP64 pool (c-major), the FP32 copy
-// (c-major, entries times 2^kMatExp) and its row-major twin.
+// (c-major, entries times 2^kMatExp) and its row-major twin.  copies (may be
+// null: every FP32 copy) says per key which FP32 copies its users read
+// (kCopy* bits); with kCopyColTop alone only the c-major entries of sizes
+// s, c < top are written.
 template <int NB>
 __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restrict__ lam_q,
                                                           const double* __restrict__ t_q,
@@ -80,7 +83,8 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
                                                           long long slot_elems,
                                                           double* __restrict__ mt64,
                                                           float* __restrict__ lf32,
-                                                          float* __restrict__ mt32)
+                                                          float* __restrict__ mt32,
+                                                          const unsigned char* __restrict__ copies, int top)
 {
     constexpr int CW = RowGroup<NB>::v;
     const int key = blockIdx.x;
@@ -89,9 +93,12 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
     const double alpha = lt / (1 + lt);
     const double coeff = 1 - 2 * alpha;
     const long long base = (long long)(first_slot + key) * slot_elems;
+    const int use = copies ? copies[key] : (kCopyRow | kCopyCol);
+    // c-major entries below `col_rows` rows, in columns below `col_cols`
+    const int col_rows = (use & kCopyCol) ? N : (use & kCopyColTop) ? (top < N ? top : N) : 0;
     double* M64 = mt64 ? mt64 + base : nullptr;
-    float* L32 = lf32 ? lf32 + base : nullptr;
-    float* R32 = mt32 ? mt32 + base : nullptr;
+    float* L32 = lf32 && col_rows > 0 ? lf32 + base : nullptr;
+    float* R32 = mt32 && (use & kCopyRow) ? mt32 + base : nullptr;
 
     if (degenerate_key(alpha, coeff)) {
         for (int c = 0; c < N; ++c)
@@ -100,7 +107,7 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
                 if (s < N) {
                     const double v = (s == 0 && c == 0) ? 1.0 : 0.0;
                     if (M64) M64[(long long)c * Np + s] = v;
-                    if (L32) L32[(long long)c * Np + s] = (float)ldexp(v, kMatExp);
+                    if (L32 && s < col_rows && c < col_rows) L32[(long long)c * Np + s] = (float)ldexp(v, kMatExp);
                     if (R32) R32[(long long)s * Np + c] = (float)ldexp(v, kMatExp);
                 }
             }
@@ -165,7 +172,7 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
                 row[b][cc] = s < N ? v32 : 0.f;
                 if (s < N) {
                     if (M64) M64[(long long)c * Np + s] = v;
-                    if (L32) L32[(long long)c * Np + s] = v32;
+                    if (L32 && s < col_rows && c < col_rows) L32[(long long)c * Np + s] = v32;
                 }
             }
         }
@@ -212,7 +219,8 @@ __global__ __launch_bounds__(256) void k_build_closed_form(const double* __restr
                                                            long long slot_elems,
                                                            double* __restrict__ mt64,
                                                            float* __restrict__ mt32,
-                                                           float* __restrict__ lf32)
+                                                           float* __restrict__ lf32,
+                                                           const unsigned char* __restrict__ copies, int top)
 {
     const int key = blockIdx.z;
     const int c = blockIdx.y;
@@ -241,8 +249,11 @@ __global__ __launch_bounds__(256) void k_build_closed_form(const double* __restr
     }
     double* M64 = mt64 + (long long)(first_slot + key) * slot_elems;
     M64[(long long)c * Np + s] = v;
-    if (mt32) mt32[(long long)(first_slot + key) * slot_elems + (long long)s * Np + c] = (float)ldexp(v, kMatExp);
-    if (lf32) lf32[(long long)(first_slot + key) * slot_elems + (long long)c * Np + s] = (float)ldexp(v, kMatExp);
+    const int use = copies ? copies[key] : (kCopyRow | kCopyCol);
+    const bool col = (use & kCopyCol) || ((use & kCopyColTop) && s < top && c < top);
+    if (mt32 && (use & kCopyRow))
+        mt32[(long long)(first_slot + key) * slot_elems + (long long)s * Np + c] = (float)ldexp(v, kMatExp);
+    if (lf32 && col) lf32[(long long)(first_slot + key) * slot_elems + (long long)c * Np + s] = (float)ldexp(v, kMatExp);
 }
 
 // ---------------------------------------------------------------------------
@@ -311,10 +322,11 @@ __global__ void k_fill_zero_u32(uint32_t* __restrict__ p, long long n)
 namespace cafe {
 
 // Matrices of keys 0 .. n-1 into slots first_slot ..: the closed form (algo
-// 0) writes every pool; the recurrence skips the pools passed as null.
+// 0) writes the FP64 pool and the FP32 copies copies[key] names (all of them
+// without copies); the recurrence also skips the pools passed as null.
 hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const double* d_lg,
                         int n, int first_slot, int N, int Np, long long slot_elems, double* mt64,
-                        float* mt32, float* lf32, hipStream_t stream)
+                        float* mt32, float* lf32, hipStream_t stream, const unsigned char* copies, int top)
 {
     if (n <= 0) return hipSuccess;
     if (algo == 0) {
@@ -323,7 +335,8 @@ hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const 
             const int nk = n - k0 < 65535 ? n - k0 : 65535;
             dim3 grid((N + 255) / 256, N, nk);
             hipLaunchKernelGGL(k_build_closed_form, grid, dim3(256), 0, stream, d_lam + k0, d_t + k0,
-                               d_lg, first_slot + k0, N, Np, slot_elems, mt64, mt32, lf32);
+                               d_lg, first_slot + k0, N, Np, slot_elems, mt64, mt32, lf32,
+                               copies ? copies + k0 : nullptr, top);
         }
     } else {
         if (Np % 32 != 0) return hipErrorInvalidValue;   // row groups of up to 32 columns
@@ -331,7 +344,7 @@ hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const 
 #define CAFE_BUILD_NB(NB)                                                                                 \
     if (nb <= NB) {                                                                                       \
         hipLaunchKernelGGL(k_build_recurrence<NB>, dim3(n), dim3(64), 0, stream, d_lam, d_t, first_slot, N, \
-                           Np, slot_elems, mt64, lf32, mt32);                                             \
+                           Np, slot_elems, mt64, lf32, mt32, copies, top);                                \
         return hipGetLastError();                                                                         \
     }
         CAFE_BUILD_NB(1)

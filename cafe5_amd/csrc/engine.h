@@ -24,6 +24,12 @@ constexpr int kLgammaN = 4096;        // log-gamma table (integers < 4096)
 // when k (kMatExp + vec_exp) <= 126 (the host picks vec_exp from the tree's
 // largest child count) while entries far below each vector's max survive.
 constexpr int kMatExp = 27;
+// FP32 copies a key's users read (a plan's build writes only these):
+// kCopyRow the row-major pool (interior branches on the matrix cores and the
+// split-FP16 halves), kCopyCol the c-major pool (leaf columns), kCopyColTop
+// its entries of sizes below the per-lane kernel's rows (interior branches
+// of the per-lane classes).
+enum : unsigned char { kCopyRow = 1, kCopyCol = 2, kCopyColTop = 4 };
 // The split-FP16 pools (hi and lo halves, row-major, see prune_f16.hip) hold
 // every matrix times 2^kMatExp16: entries <= 1 stay below FP16's largest
 // finite value.

@@ -211,7 +211,12 @@ int cafe_gpu_reconstruct_noalloc(cafe_gpu_ctx *ctx, const int *node_slot, const 
  * optimizer scorer (src/optimizer_scorer.cpp:22-36) and lambda-grid scans:
  * only the key values change between executions (plan_set_keys).  The node
  * slots of the descriptor must refer to slots < n_keys.  use_graph = 0 (or a
- * runtime that refuses the capture) runs the same launches eagerly. */
+ * runtime that refuses the capture) runs the same launches eagerly.  A plan's
+ * build writes each slot's FP64 matrix and only the FP32 copies the plan's
+ * own kernels read (none on the FP64 paths); after an execution the FP64
+ * paths, cafe_gpu_matrix_get and the reconstructions may use those slots as
+ * they are, while an eager BATCHED32 or BATCHED16 evaluation of them returns
+ * CAFE_GPU_ENOTFOUND until cafe_gpu_matrix_build rebuilds them. */
 typedef struct cafe_gpu_plan cafe_gpu_plan;
 
 int cafe_gpu_plan_create(cafe_gpu_ctx *ctx, const cafe_gpu_eval_desc *desc, int n_keys,

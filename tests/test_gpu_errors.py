@@ -142,6 +142,29 @@ def test_plan_is_stale_after_inputs_change(eng):
     eng.set_error_model(None)
 
 
+def test_plan_built_slots_serve_the_fp64_paths(eng):
+    """A plan's build writes the FP64 matrices and only the FP32 copies its
+    own kernels read, so an eager FP32 evaluation of its slots asks for a
+    full build first; the FP64 paths use them as they are."""
+    tree, counts, B, R, slots, lq, tq, prior = _small_setup(eng)
+    eng.matrix_reset(B + 1, len(lq))
+    p = eng.plan(slots, prior, lq, tq, path=native.PATH_BATCHED32)
+    p.execute()
+    p.synchronize()
+    planned = p.results()["lnl"][0].copy()
+    p.close()
+    for path in (native.PATH_BATCHED32, native.PATH_BATCHED16):
+        _raises(lambda: eng.evaluate(slots, prior, path=path), "only the FP32 copies a plan read")
+    want = Oracle.evaluate(tree, np.full(tree.n_nodes, 0.02), counts, B, R, prior[0])["lnl"]
+    for path in (native.PATH_PLAIN64, native.PATH_BATCHED64):
+        got = eng.evaluate(slots, prior, path=path)["lnl"][0]
+        assert np.allclose(got, want, rtol=1e-9, atol=0), (path, got, want)
+    assert np.allclose(planned, want, rtol=1e-6, atol=0)
+    eng.matrix_build(lq, tq, 0, native.BUILD_RECURRENCE)
+    got = eng.evaluate(slots, prior, path=native.PATH_BATCHED32)["lnl"][0]
+    assert np.allclose(got, want, rtol=1e-6, atol=0)
+
+
 def test_failed_plan_leaves_slots_unbuilt(eng):
     tree, counts, B, R, slots, lq, tq, prior = _small_setup(eng)
     eng.matrix_reset(B + 1, len(lq))
