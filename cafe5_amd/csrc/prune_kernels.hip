@@ -226,7 +226,8 @@ __global__ void k_finalize(int model, int T, int K, int F, const double* __restr
 }
 
 // total[t] = -sum_f lnl[t][f] with compensated (Neumaier) summation over
-// contiguous chunks and a two-sum tree; +inf when any gamma family failed.
+// each thread's strided share and a two-sum tree; +inf when any gamma family
+// failed.
 __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e)
 {
     s = __dadd_rn(a, b);
@@ -241,13 +242,13 @@ __global__ __launch_bounds__(256) void k_totals(int F, const double* __restrict_
     const double* x = lnl + (long long)t * F;
     const int* fl = fail + (long long)t * F;
     const int tid = threadIdx.x;
-    const int chunk = (F + 255) / 256;
-    const int lo = tid * chunk, hi = lo + chunk < F ? lo + chunk : F;
     // non-finite values are tallied apart (two-sum of an infinity gives NaN):
-    // bit 1 gamma failure, 2 NaN, 4 -inf, 8 +inf
+    // bit 1 gamma failure, 2 NaN, 4 -inf, 8 +inf.  Thread tid takes families
+    // tid, tid + 256, ... (the block's loads coalesce).
     double s = 0.0, c = 0.0;
     int bad = 0;
-    for (int i = lo; i < hi; ++i) {
+#pragma unroll 4
+    for (int i = tid; i < F; i += 256) {
         const double v = x[i];
         if (fl[i]) bad |= 1;
         if (v != v) {
