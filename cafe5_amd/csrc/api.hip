@@ -761,12 +761,17 @@ MfmaShape mfma_shape_fb2(int nt)
 
 // Built-in family-block choice of the FP32 kernel: 32-family tiles for the
 // 17-32-tile vectors (one output tile of sixteen waves per 16 families would
-// otherwise stream each matrix operand for one MFMA: 1.5x slower at B = 500);
-// 16-family tiles below, where the classes run concurrently and the shorter
-// tile list keeps more waves resident.
-int builtin_fb32(int path, int nt)
+// otherwise stream each matrix operand for one MFMA: 1.5x slower at B = 500)
+// and for the only matrix-core class of a plan when it fills the device
+// with at least one 32-family tile per CU (measured 1.1-1.4x faster: B = 100,
+// 150; with 2,000 families, 9% slower); 16-family tiles for smaller vectors
+// when several matrix-core classes run concurrently, where the shorter tile
+// lists leave fewer waves resident beside the other classes and the step
+// gains nothing.
+int builtin_fb32(int path, int nt, bool only_mfma_class, long long fam_points, int n_cu)
 {
-    return path == CAFE_GPU_PATH_BATCHED32 && nt > 16 && nt <= kMaxTilesFb2 ? 2 : 1;
+    if (path != CAFE_GPU_PATH_BATCHED32 || nt > kMaxTilesFb2) return 1;
+    return nt > 16 || (only_mfma_class && fam_points >= 32LL * n_cu) ? 2 : 1;
 }
 
 // FP64 recompute shape: its lists hold a few tiles, so the time is one tile's
@@ -1048,6 +1053,13 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         const int lr = lane_rows_for(key_rows[key]);
         return lr > 0 && ctx->n_slots_lane <= lane_max_slots(lr) ? lr : 0;
     };
+    // size classes the matrix cores take (those the per-lane kernel does not),
+    // and the family-likelihoods of each class
+    int n_mfma_keys = 0;
+    for (const auto& kr : key_rows)
+        if (lane_rows_of(kr.first) == 0) ++n_mfma_keys;
+    std::map<int, long long> key_fams;
+    for (auto& bc : bucket_class) key_fams[bc.first] += (long long)ctx->buckets[bc.second].count * n_points;
     std::vector<TileDesc> tiles;
     std::vector<SegDesc> segs;
     std::vector<int> cls_fams;
@@ -1061,7 +1073,8 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
             // the kernel is chosen here: its tiles carry 64 families (per
             // lane) or 16 (matrix cores)
             kc.lane_rows = lane_rows_of(bc.first);
-            kc.fb32 = kc.lane_rows > 0 ? 1 : builtin_fb32(path, bc.first);
+            kc.fb32 = kc.lane_rows > 0 ? 1
+                                       : builtin_fb32(path, bc.first, n_mfma_keys == 1, key_fams[bc.first], ctx->n_cu);
             if (const cafe_gpu_ctx::TuneEntry* te = find_tuning(ctx, path, bc.first)) {
                 MfmaShape probe;
                 if (tuned_mfma_shape(ctx, path, *te, bc.first, bc.first, probe)) {
