@@ -852,7 +852,16 @@ def main():
         "families_recomputed_plain64_near_subnormal": int(rcplain),
         "batched64_max_rel_dlnL": float(d64.max()) if d64.size else 0.0,
     }
+    # the bars the reduced path is held to (the metric's |dlnL| <= 1e-6 is
+    # the total's, relative; per family the path certifies a looser bar)
+    fam_ok = bool(np.all((d <= 5e-6 + 1e-7 * np.abs(b[fin])) & (d <= 1e-5 * np.abs(b[fin])))) if d.size else True
+    tot_ok = bool(tot_rel.max() <= 1e-6) if tot_rel.size else True
+    accuracy["bars"] = {"total_rel": 1e-6, "per_family": "|dlnL| <= 5e-6 + 1e-7 |lnL| and <= 1e-5 |lnL|",
+                        "total_within": tot_ok, "per_family_within": fam_ok,
+                        "within": bool(tot_ok and fam_ok and same_fin)}
     log(f"accuracy: {accuracy}")
+    if not accuracy["bars"]["within"]:
+        log("WARNING: the reduced path is outside its accuracy bars on this run")
 
     # ---- roofline of the dominant kernel: the FP32 main-pass launch of the
     # size class that takes longest (per-class HIP events, same stream) ----

@@ -235,7 +235,9 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
     e = __dadd_rn(__dsub_rn(a, __dsub_rn(s, bb)), __dsub_rn(b, bb));
 }
 
-__global__ __launch_bounds__(256) void k_totals(int F, const double* __restrict__ lnl,
+constexpr int kTotalsThreads = 1024;
+
+__global__ __launch_bounds__(kTotalsThreads) void k_totals(int F, const double* __restrict__ lnl,
                                                 const int* __restrict__ fail, double* __restrict__ total)
 {
     const int t = blockIdx.x;
@@ -244,11 +246,11 @@ __global__ __launch_bounds__(256) void k_totals(int F, const double* __restrict_
     const int tid = threadIdx.x;
     // non-finite values are tallied apart (two-sum of an infinity gives NaN):
     // bit 1 gamma failure, 2 NaN, 4 -inf, 8 +inf.  Thread tid takes families
-    // tid, tid + 256, ... (the block's loads coalesce).
+    // tid, tid + kTotalsThreads, ... (the block's loads coalesce).
     double s = 0.0, c = 0.0;
     int bad = 0;
 #pragma unroll 4
-    for (int i = tid; i < F; i += 256) {
+    for (int i = tid; i < F; i += kTotalsThreads) {
         const double v = x[i];
         if (fl[i]) bad |= 1;
         if (v != v) {
@@ -264,13 +266,13 @@ __global__ __launch_bounds__(256) void k_totals(int F, const double* __restrict_
             c = __dadd_rn(c, e);
         }
     }
-    __shared__ double ss[256], sc[256];
-    __shared__ int sb[256];
+    __shared__ double ss[kTotalsThreads], sc[kTotalsThreads];
+    __shared__ int sb[kTotalsThreads];
     ss[tid] = s;
     sc[tid] = c;
     sb[tid] = bad;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = kTotalsThreads / 2; w > 0; w >>= 1) {
         if (tid < w) {
             double ns, e;
             two_sum(ss[tid], ss[tid + w], ns, e);
@@ -336,7 +338,7 @@ hipError_t launch_finalize(int model, int T, int K, int F, const double* val, co
                        cat_probs, lnl, cat, post, fail);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_totals, dim3(T), dim3(256), 0, st, F, lnl, fail, total);
+    hipLaunchKernelGGL(k_totals, dim3(T), dim3(kTotalsThreads), 0, st, F, lnl, fail, total);
     return hipGetLastError();
 }
 
