@@ -1388,23 +1388,30 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
     };
 
     // Stage by stage over the classes: every main pass, then every FP64
-    // recompute, then every PLAIN64 recompute.  When concurrent, the classes
-    // of one stage run on the side streams (the main passes on their
-    // longest-first assignment, the recomputes round robin in that order:
-    // measured 1.5% faster per step than placing the recomputes by their
-    // vectors' length) and st waits for
-    // all of them before the next stage: a recompute is a latency-bound chain
-    // on a few tiles, and beside a main pass it would slow that pass down.
+    // recompute followed on the same stream by the class's PLAIN64 recompute.
+    // When concurrent, the classes of one stage run on the side streams (the
+    // main passes on their longest-first assignment, the recomputes round
+    // robin in that order: measured 1.5% faster per step than placing the
+    // recomputes by their vectors' length) and st waits for all of them
+    // before the next stage: a recompute is a latency-bound chain on a few
+    // tiles, and beside a main pass it would slow that pass down.
     const int S = concurrent ? std::min((int)ctx->side.size(), n_cls) : 0;
     if (late_build && S <= 1)
         if (int rc = (*late_build)(st)) return rc;
-    auto stage = [&](int idx, auto&& launch) -> int {
+    // launch2 (may be null) follows every launch of the stage on the same
+    // streams, before the join: the PLAIN64 pass of a class reads only the
+    // flags its own main pass and FP64 pass wrote, on the same stream
+    using Launch = std::function<int(int, hipStream_t)>;
+    auto stage = [&](int idx, const Launch& launch, const Launch* launch2 = nullptr) -> int {
         if (S <= 1) {
             for (int ci = 0; ci < n_cls; ++ci) {
                 if (cls_ev && idx == 0) HIP_OR_FAIL(hipEventRecord((*cls_ev)[ci], st));
                 if (int rc = launch(ci, st)) return rc;
             }
             if (cls_ev && idx == 0) HIP_OR_FAIL(hipEventRecord((*cls_ev)[n_cls], st));
+            if (launch2)
+                for (int ci = 0; ci < n_cls; ++ci)
+                    if (int rc = (*launch2)(ci, st)) return rc;
             return CAFE_GPU_OK;
         }
         hipEvent_t* ev = ctx->join_ev.data() + idx * (kSideStreams + 1);
@@ -1415,6 +1422,9 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
             const int k = idx == 0 ? pp.classes[ci].stream % S : i % S;
             if (int rc = launch(ci, ctx->side[k])) return rc;
         }
+        if (launch2)
+            for (int i = 0; i < n_cls; ++i)
+                if (int rc = (*launch2)(pp.main_order[i], ctx->side[i % S])) return rc;
         if (idx == 0 && late_build)
             if (int rc = (*late_build)(ctx->side[pp.light_stream % S])) return rc;
         for (int k = 0; k < S; ++k) {
@@ -1425,10 +1435,12 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
     };
     if (int rc = stage(0, stage_main)) return rc;
     if (ev_main) HIP_OR_FAIL(hipEventRecord(ev_main, st));
-    if (pp.path == CAFE_GPU_PATH_BATCHED32 || pp.path == CAFE_GPU_PATH_BATCHED16)
-        if (int rc = stage(1, stage_r64)) return rc;
-    if (pp.path != CAFE_GPU_PATH_PLAIN64)
+    const Launch plain = stage_plain;
+    if (pp.path == CAFE_GPU_PATH_BATCHED32 || pp.path == CAFE_GPU_PATH_BATCHED16) {
+        if (int rc = stage(1, stage_r64, &plain)) return rc;
+    } else if (pp.path != CAFE_GPU_PATH_PLAIN64) {
         if (int rc = stage(2, stage_plain)) return rc;
+    }
     if (ev_fb) HIP_OR_FAIL(hipEventRecord(ev_fb, st));
     return CAFE_GPU_OK;
 }
