@@ -81,14 +81,32 @@ __global__ __launch_bounds__(64) void k_prune_plain(PruneParams p)
                             if (s < len) x[q] = Ml[(long long)cnt * Np + s];
                         }
                     } else {
+                        // x[s] += M[s][c] v[c] for c = 0..B in ascending order; the
+                        // loads of U columns are issued before their products, so
+                        // U column reads are in flight instead of one
+                        constexpr int U = 32 / NQ < 8 ? 32 / NQ : 8;
                         const double* V = slots + (long long)ch.in_slot * ldv;
-                        for (int c = 0; c <= B; ++c) {
-                            const double v = V[c];
-                            const double* col = M + (long long)c * Np;
+                        for (int c0 = 0; c0 <= B; c0 += U) {
+                            double vv[U], cv[U][NQ];
 #pragma unroll
-                            for (int q = 0; q < NQ; ++q) {
-                                const int s = lane + 64 * q;
-                                if (s < len) x[q] = __dadd_rn(x[q], __dmul_rn(col[s], v));
+                            for (int u = 0; u < U; ++u) {
+                                const int c = c0 + u <= B ? c0 + u : B;
+                                vv[u] = V[c];
+                                const double* col = M + (long long)c * Np;
+#pragma unroll
+                                for (int q = 0; q < NQ; ++q) {
+                                    const int s = lane + 64 * q;
+                                    cv[u][q] = s < len ? col[s] : 0.0;
+                                }
+                            }
+#pragma unroll
+                            for (int u = 0; u < U; ++u) {
+                                if (c0 + u > B) break;
+#pragma unroll
+                                for (int q = 0; q < NQ; ++q) {
+                                    const int s = lane + 64 * q;
+                                    if (s < len) x[q] = __dadd_rn(x[q], __dmul_rn(cv[u][q], vv[u]));
+                                }
                             }
                         }
                     }
