@@ -19,7 +19,7 @@
 namespace cafe {
 hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const double* d_lg, int n,
                         int first_slot, int N, int Np, long long slot_elems, double* mt64, float* mt32,
-                        float* lf32, hipStream_t stream, const unsigned char* copies, int top);
+                        float* lf32, hipStream_t stream, const unsigned char* copies, int top, int* band);
 hipError_t launch_fill_zero(void* p, long long bytes, hipStream_t stream);
 hipError_t launch_reconstruct(const ReconNode* nodes, int n_nodes, const int* kids, const int* parent,
                               const double* mt64, int N, int Np, long long slot_elems, const int* counts_T, int F,
@@ -146,6 +146,7 @@ struct cafe_gpu_ctx {
     int N = 0, Np = 0, capacity = 0;
     long long slot_elems = 0;
     DevBuf mt64, mt32, lf32, key_lam, key_t;
+    DevBuf band32;         // per slot and 16-row tile of mt32: its nonzero k-tiles (band_entry)
     DevBuf mt16h, mt16l;   // split-FP16 halves of mt32 (BATCHED16; made by its evaluations)
     std::vector<char> built;   // per slot: 0 no, 1 every copy, 2 FP64 and the FP32 copies a plan read
 
@@ -374,7 +375,7 @@ int cafe_gpu_destroy(cafe_gpu_ctx* ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf* bufs[] = {&ctx->lg, &ctx->mt64, &ctx->mt32, &ctx->lf32, &ctx->key_lam, &ctx->key_t, &ctx->d_ops,
                       &ctx->d_kids, &ctx->d_steps, &ctx->d_steps_lane, &ctx->d_counts, &ctx->d_perm, &ctx->d_err64,
-                      &ctx->d_err32, &ctx->mt64e, &ctx->mt32e, &ctx->mt16h, &ctx->mt16l};
+                      &ctx->d_err32, &ctx->mt64e, &ctx->mt32e, &ctx->mt16h, &ctx->mt16l, &ctx->band32};
     for (DevBuf* b : bufs) b->release();
     ctx->eager.release_all();
     for (auto& e : ctx->ev)
@@ -409,6 +410,8 @@ int cafe_gpu_matrix_reset(cafe_gpu_ctx* ctx, int N, int capacity)
     HIP_OR_FAIL(ctx->mt64.ensure(n64));
     HIP_OR_FAIL(ctx->mt32.ensure(n32));
     HIP_OR_FAIL(ctx->lf32.ensure(n32));
+    const size_t nband = sizeof(int) * (size_t)(Np / kTile) * capacity;
+    HIP_OR_FAIL(ctx->band32.ensure(nband));
     if (ctx->n_dev) {
         HIP_OR_FAIL(ctx->mt64e.ensure(n64));
         HIP_OR_FAIL(ctx->mt32e.ensure(n32));
@@ -417,6 +420,7 @@ int cafe_gpu_matrix_reset(cafe_gpu_ctx* ctx, int N, int capacity)
     HIP_OR_FAIL(launch_fill_zero(ctx->mt64.p, (long long)n64, ctx->stream));
     HIP_OR_FAIL(launch_fill_zero(ctx->mt32.p, (long long)n32, ctx->stream));
     HIP_OR_FAIL(launch_fill_zero(ctx->lf32.p, (long long)n32, ctx->stream));
+    HIP_OR_FAIL(launch_fill_zero(ctx->band32.p, (long long)nband, ctx->stream));
     ctx->N = N;
     ctx->Np = Np;
     ctx->slot_elems = slot_elems;
@@ -449,7 +453,8 @@ int cafe_gpu_matrix_build(cafe_gpu_ctx* ctx, int n, const double* lambda_q, cons
     HIP_OR_FAIL(hipEventRecord(ctx->ev[4], ctx->stream));
     HIP_OR_FAIL(launch_build(algo, ctx->key_lam.as<double>(), ctx->key_t.as<double>(), ctx->lg.as<double>(), n,
                              first_slot, ctx->N, ctx->Np, ctx->slot_elems, ctx->mt64.as<double>(),
-                             ctx->mt32.as<float>(), ctx->lf32.as<float>(), ctx->stream, nullptr, 0));
+                             ctx->mt32.as<float>(), ctx->lf32.as<float>(), ctx->stream, nullptr, 0,
+                             ctx->band32.as<int>()));
     if (int rc = enqueue_fold(ctx, first_slot, n, ctx->stream)) return rc;
     HIP_OR_FAIL(hipEventRecord(ctx->ev[5], ctx->stream));
     HIP_OR_FAIL(hipStreamSynchronize(ctx->stream));
@@ -1262,6 +1267,8 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     p.perm = ctx->d_perm.as<int>();
     p.mt64 = ctx->mt64.as<double>();
     p.mt32 = ctx->mt32.as<float>();
+    p.band = ctx->band32.as<int>();
+    p.band_stride = ctx->Np / kTile;
     p.slot_elems = ctx->slot_elems;
     p.Np = ctx->Np;
     p.leaf64 = ctx->n_dev ? ctx->mt64e.as<double>() : ctx->mt64.as<double>();
@@ -1647,7 +1654,7 @@ int plan_enqueue(cafe_gpu_plan* pl, bool timed)
     HIP_OR_FAIL(launch_build(pl->algo, pl->d_lam.as<double>(), pl->d_t.as<double>(), ctx->lg.as<double>(), pl->n_keys,
                              0, ctx->N, ctx->Np, ctx->slot_elems, split ? nullptr : ctx->mt64.as<double>(),
                              ctx->mt32.as<float>(), ctx->lf32.as<float>(), st, pl->d_copies.as<unsigned char>(),
-                             pl->copy_top));
+                             pl->copy_top, ctx->band32.as<int>()));
     // the folded FP32 leaf pool is read by the reduced paths only
     const bool reduced = pl->pp.path == CAFE_GPU_PATH_BATCHED32 || pl->pp.path == CAFE_GPU_PATH_BATCHED16;
     if (int rc = enqueue_fold(ctx, 0, pl->n_keys, st, !split, reduced)) return rc;
@@ -1655,7 +1662,7 @@ int plan_enqueue(cafe_gpu_plan* pl, bool timed)
     const std::function<int(hipStream_t)> build64 = [&](hipStream_t s) -> int {
         HIP_OR_FAIL(launch_build(pl->algo, pl->d_lam.as<double>(), pl->d_t.as<double>(), ctx->lg.as<double>(),
                                  pl->n_keys, 0, ctx->N, ctx->Np, ctx->slot_elems, ctx->mt64.as<double>(), nullptr,
-                                 nullptr, s, nullptr, 0));
+                                 nullptr, s, nullptr, 0, nullptr));
         return enqueue_fold(ctx, 0, pl->n_keys, s, true, false);
     };
     // profile runs issue the size classes one after another (standalone kernel

@@ -75,7 +75,10 @@ template <int NB> struct RowGroup { static constexpr int v = NB >= 32 ? 2 : NB >
 // (c-major, entries times 2^kMatExp) and its row-major twin.  copies (may be
 // null: every FP32 copy) says per key which FP32 copies its users read
 // (kCopy* bits); with kCopyColTop alone only the c-major entries of sizes
-// s, c < top are written.
+// s, c < top are written.  With the row-major copy goes its band (band may be
+// null): per 16-row tile the k-tiles between its first and last nonzero
+// FP32 entry (a lane tracks its rows' first and last nonzero column as it
+// writes them; 16 lanes hold a tile).
 template <int NB>
 __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restrict__ lam_q,
                                                           const double* __restrict__ t_q,
@@ -84,7 +87,8 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
                                                           double* __restrict__ mt64,
                                                           float* __restrict__ lf32,
                                                           float* __restrict__ mt32,
-                                                          const unsigned char* __restrict__ copies, int top)
+                                                          const unsigned char* __restrict__ copies, int top,
+                                                          int* __restrict__ band, int band_stride)
 {
     constexpr int CW = RowGroup<NB>::v;
     const int key = blockIdx.x;
@@ -99,8 +103,12 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
     double* M64 = mt64 ? mt64 + base : nullptr;
     float* L32 = lf32 && col_rows > 0 ? lf32 + base : nullptr;
     float* R32 = mt32 && (use & kCopyRow) ? mt32 + base : nullptr;
+    int* BT = R32 && band ? band + (long long)(first_slot + key) * band_stride : nullptr;
 
     if (degenerate_key(alpha, coeff)) {
+        // only entry (0, 0) is nonzero
+        if (BT)
+            for (int t = lane; t < band_stride; t += 64) BT[t] = t == 0 ? band_entry(0, 1) : band_entry(0, 0);
         for (int c = 0; c < N; ++c)
             for (int b = 0; b < NB; ++b) {
                 const int s = lane + 64 * b;
@@ -125,8 +133,13 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
     const double a_lane = int_pow(alpha, lane + 1);
 
     double prev[NB];   // P_s(c-1) for the lane's rows
+    int first_nz[NB], last_nz[NB];   // the lane's rows' first and last nonzero FP32 column
 #pragma unroll
-    for (int b = 0; b < NB; ++b) prev[b] = 0.0;
+    for (int b = 0; b < NB; ++b) {
+        prev[b] = 0.0;
+        first_nz[b] = Np;
+        last_nz[b] = -1;
+    }
 
     const int n_groups = (N + CW - 1) / CW;
     for (int g = 0; g < n_groups; ++g) {
@@ -170,6 +183,10 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
                 const double v = y < 1.0 ? y : 1.0;   // [0,1] clamp, probability.cpp:147
                 const float v32 = (float)ldexp(v, kMatExp);
                 row[b][cc] = s < N ? v32 : 0.f;
+                if (s < N && v32 != 0.f) {
+                    first_nz[b] = first_nz[b] < c ? first_nz[b] : c;
+                    last_nz[b] = c;
+                }
                 if (s < N) {
                     if (M64) M64[(long long)c * Np + s] = v;
                     if (L32 && s < col_rows && c < col_rows) L32[(long long)c * Np + s] = v32;
@@ -197,6 +214,23 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
             }
         }
     }
+    if (BT) {
+        // tile 4 b + lane / 16 = rows 64 b + 16 (lane / 16) .. + 15
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            int lo = first_nz[b], hi = last_nz[b];
+#pragma unroll
+            for (int d = 1; d < 16; d <<= 1) {
+                const int olo = __shfl_xor(lo, d);
+                const int ohi = __shfl_xor(hi, d);
+                lo = lo < olo ? lo : olo;
+                hi = hi > ohi ? hi : ohi;
+            }
+            const int t = 4 * b + (lane >> 4);
+            if ((lane & 15) == 0 && t < band_stride)
+                BT[t] = hi >= 0 ? band_entry(lo / kTile, hi / kTile + 1) : band_entry(0, 0);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -220,11 +254,17 @@ __global__ __launch_bounds__(256) void k_build_closed_form(const double* __restr
                                                            double* __restrict__ mt64,
                                                            float* __restrict__ mt32,
                                                            float* __restrict__ lf32,
-                                                           const unsigned char* __restrict__ copies, int top)
+                                                           const unsigned char* __restrict__ copies, int top,
+                                                           int* __restrict__ band, int band_stride)
 {
     const int key = blockIdx.z;
     const int c = blockIdx.y;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int use = copies ? copies[key] : (kCopyRow | kCopyCol);
+    // no band scan here: every k-tile of every row tile counts as nonzero
+    if (band && mt32 && (use & kCopyRow) && c == 0 && blockIdx.x == 0)
+        for (int t = threadIdx.x; t < band_stride; t += blockDim.x)
+            band[(long long)(first_slot + key) * band_stride + t] = band_entry(0, band_stride);
     if (s >= N) return;
     const double lt = lam_q[key] * t_q[key];
     const double alpha = lt / (1 + lt);
@@ -249,7 +289,6 @@ __global__ __launch_bounds__(256) void k_build_closed_form(const double* __restr
     }
     double* M64 = mt64 + (long long)(first_slot + key) * slot_elems;
     M64[(long long)c * Np + s] = v;
-    const int use = copies ? copies[key] : (kCopyRow | kCopyCol);
     const bool col = (use & kCopyCol) || ((use & kCopyColTop) && s < top && c < top);
     if (mt32 && (use & kCopyRow))
         mt32[(long long)(first_slot + key) * slot_elems + (long long)s * Np + c] = (float)ldexp(v, kMatExp);
@@ -323,12 +362,16 @@ namespace cafe {
 
 // Matrices of keys 0 .. n-1 into slots first_slot ..: the closed form (algo
 // 0) writes the FP64 pool and the FP32 copies copies[key] names (all of them
-// without copies); the recurrence also skips the pools passed as null.
+// without copies); the recurrence also skips the pools passed as null.  Where
+// the row-major FP32 copy is written, so is its band (Np / 16 entries per
+// slot; band may be null).
 hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const double* d_lg,
                         int n, int first_slot, int N, int Np, long long slot_elems, double* mt64,
-                        float* mt32, float* lf32, hipStream_t stream, const unsigned char* copies, int top)
+                        float* mt32, float* lf32, hipStream_t stream, const unsigned char* copies, int top,
+                        int* band)
 {
     if (n <= 0) return hipSuccess;
+    const int band_stride = Np / kTile;
     if (algo == 0) {
         // grid.z holds the key index: launch at most 65535 keys at a time
         for (int k0 = 0; k0 < n; k0 += 65535) {
@@ -336,7 +379,7 @@ hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const 
             dim3 grid((N + 255) / 256, N, nk);
             hipLaunchKernelGGL(k_build_closed_form, grid, dim3(256), 0, stream, d_lam + k0, d_t + k0,
                                d_lg, first_slot + k0, N, Np, slot_elems, mt64, mt32, lf32,
-                               copies ? copies + k0 : nullptr, top);
+                               copies ? copies + k0 : nullptr, top, band, band_stride);
         }
     } else {
         if (Np % 32 != 0) return hipErrorInvalidValue;   // row groups of up to 32 columns
@@ -344,7 +387,7 @@ hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const 
 #define CAFE_BUILD_NB(NB)                                                                                 \
     if (nb <= NB) {                                                                                       \
         hipLaunchKernelGGL(k_build_recurrence<NB>, dim3(n), dim3(64), 0, stream, d_lam, d_t, first_slot, N, \
-                           Np, slot_elems, mt64, lf32, mt32, copies, top);                                \
+                           Np, slot_elems, mt64, lf32, mt32, copies, top, band, band_stride);             \
         return hipGetLastError();                                                                         \
     }
         CAFE_BUILD_NB(1)

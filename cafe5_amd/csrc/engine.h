@@ -24,6 +24,8 @@ constexpr int kLgammaN = 4096;        // log-gamma table (integers < 4096)
 // when k (kMatExp + vec_exp) <= 126 (the host picks vec_exp from the tree's
 // largest child count) while entries far below each vector's max survive.
 constexpr int kMatExp = 27;
+// Band entry of a row tile: k-tiles [lo, hi) hold its nonzero entries.
+__host__ __device__ inline int band_entry(int lo, int hi) { return lo | hi << 16; }
 // FP32 copies a key's users read (a plan's build writes only these):
 // kCopyRow the row-major pool (interior branches on the matrix cores and the
 // split-FP16 halves), kCopyCol the c-major pool (leaf columns), kCopyColTop
@@ -159,6 +161,11 @@ struct PruneParams {
     int slot_tiles;           // MFMA: 16-row tiles per stored partial vector
     int zero;                 // always 0 (an offset the compiler cannot fold)
     int plain_lds;            // PLAIN64: partial vectors in dynamic LDS
+    // nonzero band of the FP32 row-major pool: per slot and 16-row tile t,
+    // band[slot * band_stride + t] = lo | hi << 16, the k-tiles [lo, hi)
+    // outside of which the tile's entries are exactly 0 (null: no band)
+    const int* band;
+    int band_stride;
 };
 
 // One node of the reconstruction program (children before parents).

@@ -216,9 +216,13 @@ __device__ __forceinline__ void child_gemm(typename Vec4<T>::type (&x)[FB][NTW],
 // s+1 -- its leaf column, or the first PK k-tiles of its A operand -- are in
 // flight in the other half of a ping-pong register buffer (both kinds issue
 // the same loads from per-lane offsets computed once per tile, so the waits
-// count the younger loads exactly).  Wave wv computes output tiles wv + W i,
-// i < NTW (W NTW covers the largest vector of the size class; tiles past a
-// vector's end are computed on a clamped, in-bounds tile and dropped).
+// count the younger loads exactly).  Wave wv computes NTW output tiles (W NTW
+// covers the largest vector of the size class; tiles past a vector's end are
+// computed on a clamped, in-bounds tile and dropped): FP64 tiles wv + W i;
+// FP32 the contiguous run [wv n / W, (wv + 1) n / W) of the n row tiles, so
+// that an interior step multiplies only the k-tiles inside the union of its
+// rows' bands (the FP32 pool's nonzero k-tiles per row tile, recorded by the
+// build): the A blocks it skips are exactly 0, so each sum is unchanged.
 // Partial vectors are kept in fragment layout, in LDS (LDSP) or in the
 // workgroup's region of the global scratch; the leaf counts of the tile's
 // families are staged in LDS.
@@ -334,10 +338,15 @@ void k_prune_mfma(PruneParams p)
         // (separate arrays, chosen with selects: an array indexed by the
         // root flag would be placed in scratch memory)
         int offA0[NTW], offA1[NTW], offL0[NTW], offL1[NTW];
+        // FP32: this wave's run of row tiles, non-root [c0, c1) / root [r0, r1)
+        const int c0 = kF64 ? 0 : wv * nts / W, c1 = kF64 ? 0 : (wv + 1) * nts / W;
+        const int r0 = kF64 ? 0 : wv * rows_root / W, r1 = kF64 ? 0 : (wv + 1) * rows_root / W;
 #pragma unroll
         for (int i = 0; i < NTW; ++i) {
-            const int t0 = wv + W * i < nts ? wv + W * i : nts - 1;
-            const int t1 = wv + W * i < rows_root ? wv + W * i : rows_root - 1;
+            const int u0 = kF64 ? wv + W * i : c0 + i, e0 = kF64 ? nts : c1;
+            const int u1 = kF64 ? wv + W * i : r0 + i, e1 = kF64 ? rows_root : r1;
+            const int t0 = u0 < e0 ? u0 : nts - 1;
+            const int t1 = u1 < e1 ? u1 : rows_root - 1;
             offA0[i] = kF64 ? 4 * g * Np + kTile * t0 + my_arow : (kTile * t0 + my_arow) * Np + 4 * g;
             offA1[i] = kF64 ? 4 * g * Np + kTile * t1 + my_arow : (kTile * t1 + my_arow) * Np + 4 * g;
             offL0[i] = kTile * t0 + 4 * g;
@@ -356,14 +365,45 @@ void k_prune_mfma(PruneParams p)
         // loads of step st (clamped to the last step) into pf; its descriptor
         // into sd.  A leaf step loads its column for family block k into
         // pf[k] (FB = 2), or the same column twice (FB = 1).
-        auto prefetch = [&](V4 (&pf)[PK][NTW], StepDev& sd, int st) {
+        // k-tiles [klo, klo + kn) of an interior step: the union of the bands
+        // of the wave's live row tiles (at least one k-tile: a wave whose
+        // rows are all 0 multiplies one zero block)
+        const __attribute__((address_space(4))) int* band = as_const(p.band);
+        auto kband = [&](int slot, bool root, int& klo, int& kn) {
+            klo = 0;
+            kn = nts;
+            if constexpr (!kF64) {
+                if (!band) return;
+                const int a = root ? r0 : c0, b = root ? r1 : c1;
+                int lo = nts, hi = 0;
+#pragma unroll
+                for (int i = 0; i < NTW; ++i)
+                    if (a + i < b) {
+                        const int e = band[(long long)slot * p.band_stride + a + i];
+                        lo = lo < (e & 0xffff) ? lo : (e & 0xffff);
+                        hi = hi > (e >> 16) ? hi : (e >> 16);
+                    }
+                hi = hi < nts ? hi : nts;
+                if (lo >= hi) {
+                    lo = 0;
+                    hi = 1;
+                }
+                klo = lo;
+                kn = hi - lo;
+            }
+        };
+        auto prefetch = [&](V4 (&pf)[PK][NTW], StepDev& sd, int& klo, int& kn, int st) {
             // wave-uniform index: the descriptor comes through the scalar
             // cache, so waiting for it does not wait for vector loads
             st = __builtin_amdgcn_readfirstlane(st < n_steps ? st : n_steps - 1);
             sd = ld_const(p.steps, st);
             const bool root = (sd.ctl & kStepRoot) != 0;
             const bool leaf = sd.leaf_col >= 0;
-            const T* __restrict__ M = (leaf ? lpool : pool) + (long long)nslot[sd.node] * p.slot_elems;
+            const int slot = nslot[sd.node];
+            const T* __restrict__ M = (leaf ? lpool : pool) + (long long)slot * p.slot_elems;
+            klo = 0;
+            kn = nts;
+            if (!leaf) kband(slot, root, klo, kn);
 #pragma unroll
             for (int k = 0; k < PK; ++k) {
                 const int cn = s_cnt[(leaf ? sd.leaf_col : 0) * FW + (FB > 1 ? 16 * k : 0) + fl] * Np;
@@ -371,7 +411,7 @@ void k_prune_mfma(PruneParams p)
                 for (int i = 0; i < NTW; ++i) {
                     const int oa = root ? offA1[i] : offA0[i];
                     const int ol = cn + (root ? offL1[i] : offL0[i]);
-                    const int kk = k < nts ? k : nts - 1;
+                    const int kk = klo + (k < kn ? k : kn - 1);
                     if constexpr (!kF64) {
                         const int off = leaf ? ol + (FB > 1 ? 0 : 4 * k * zero) : oa + kTile * kk;
                         pf[k][i] = *(const floatx4*)(M + off);
@@ -402,7 +442,7 @@ void k_prune_mfma(PruneParams p)
             cert[f] = 0.0;
         }
 
-        auto consume = [&](const V4 (&pf)[PK][NTW], const StepDev& sd) {
+        auto consume = [&](const V4 (&pf)[PK][NTW], const StepDev& sd, int klo, int kn) {
             const bool root = (sd.ctl & kStepRoot) != 0;
             const int rows = root ? rows_root : nts;
             const int out_slot = sd.ctl & kStepSlotMask;
@@ -410,8 +450,9 @@ void k_prune_mfma(PruneParams p)
             bool live[NTW];
 #pragma unroll
             for (int i = 0; i < NTW; ++i) {
-                live[i] = wv + W * i < rows;
-                tt[i] = live[i] ? wv + W * i : rows - 1;
+                const int u = kF64 ? wv + W * i : (root ? r0 : c0) + i;
+                live[i] = u < (kF64 ? rows : root ? r1 : c1);
+                tt[i] = live[i] ? u : rows - 1;
             }
             if (sd.ctl & kStepFirst) {
 #pragma unroll
@@ -475,7 +516,9 @@ void k_prune_mfma(PruneParams p)
                         }
                     }
                 }
-                const T* __restrict__ M = pool + (long long)nslot[sd.node] * p.slot_elems;
+                // FP32: the step's k-tiles start at klo (column 16 klo of the
+                // row-major rows, fragment klo of the child)
+                const T* __restrict__ M = pool + (long long)nslot[sd.node] * p.slot_elems + (kF64 ? 0 : kTile * klo);
                 int ao[NTW];
 #pragma unroll
                 for (int i = 0; i < NTW; ++i) {
@@ -483,8 +526,8 @@ void k_prune_mfma(PruneParams p)
                     for (int f = 0; f < FB; ++f) x[f][i] = V4{0, 0, 0, 0};
                     ao[i] = root ? offA1[i] : offA0[i];
                 }
-                const V4* bsrc = frag + (long long)sd.in_slot * slot_tiles * FB * 64 + lane;
-                child_gemm<T, NTW, Ring<T, NTW, W, FB>::D, SCALED, FB>(x, M, Np, ao, pf, bsrc, nts, bsh);
+                const V4* bsrc = frag + ((long long)sd.in_slot * slot_tiles + klo) * FB * 64 + lane;
+                child_gemm<T, NTW, Ring<T, NTW, W, FB>::D, SCALED, FB>(x, M, Np, ao, pf, bsrc, kn, bsh);
             }
 #pragma unroll
             for (int f = 0; f < FB; ++f)
@@ -658,13 +701,14 @@ void k_prune_mfma(PruneParams p)
 
         V4 pfa[PK][NTW], pfb[PK][NTW];
         StepDev sda, sdb;
-        prefetch(pfa, sda, 0);
+        int kla, kna, klb, knb;
+        prefetch(pfa, sda, kla, kna, 0);
         for (int st = 0; st < n_steps; st += 2) {
-            prefetch(pfb, sdb, st + 1);
-            consume(pfa, sda);
+            prefetch(pfb, sdb, klb, knb, st + 1);
+            consume(pfa, sda, kla, kna);
             if (st + 1 < n_steps) {
-                prefetch(pfa, sda, st + 2);
-                consume(pfb, sdb);
+                prefetch(pfa, sda, kla, kna, st + 2);
+                consume(pfb, sdb, klb, knb);
             }
         }
         // the last prefetch (past the end) is never consumed: take it here, so

@@ -156,3 +156,32 @@ def test_32_family_tiles_on_501_row_vectors():
         fin = np.isfinite(a)
         assert np.array_equal(a[fin], b[fin])
     assert t16 == t32
+
+
+# Band-limited k-loops: an FP32 interior step multiplies only the k-tiles
+# inside the union of its wave's row tiles' bands (the blocks it skips are
+# exactly 0 in the FP32 pool).  One output tile per wave skips by each tile's
+# own band, the built-in shapes by unions of three or four; results that do
+# not depend on the skipped set are bit-identical.
+@pytest.mark.parametrize("shape", [(1, 16, 0), (1, 4, 1)])
+def test_band_limited_k_loops_are_exact(setup, shape):
+    w, eng, slots, prior, lq, tq = setup
+    path = native.PATH_BATCHED32
+    eng.tuning_clear()
+    base, shapes = _run(eng, slots, prior, lq, tq, path)
+    ntw, wv, lds = shape
+    leaves = w.tree.n_leaves
+    tried = [nt for nt, l in shapes.items() if l[0] == 0 and ntw * wv >= nt]
+    assert tried
+    for nt in tried:
+        eng.tuning_set(path, nt, leaves, 0, ntw, wv, bool(lds), 2)
+    tuned, tshapes = _run(eng, slots, prior, lq, tq, path)
+    eng.tuning_clear()
+    one = [nt for nt in tried if tshapes[nt][:4] == (0, ntw, wv, lds)]
+    print(f"shape {shape}: one tile per wave on classes {one} of {tried}; {tshapes}", flush=True)
+    assert one
+    assert np.array_equal(np.isfinite(tuned["lnl"]), np.isfinite(base["lnl"]))
+    fin = np.isfinite(base["lnl"])
+    assert np.array_equal(tuned["lnl"][fin], base["lnl"][fin]), np.abs(tuned["lnl"][fin] - base["lnl"][fin]).max()
+    assert np.array_equal(tuned["total"], base["total"])
+    assert tuned["n_fallback"] == base["n_fallback"]
