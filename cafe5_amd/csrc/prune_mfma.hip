@@ -367,12 +367,13 @@ void k_prune_mfma(PruneParams p)
         // pf[k] (FB = 2), or the same column twice (FB = 1).
         // k-tiles [klo, klo + kn) of an interior step: the union of the bands
         // of the wave's live row tiles (at least one k-tile: a wave whose
-        // rows are all 0 multiplies one zero block)
+        // rows are all 0 multiplies one zero block).  A one-wave workgroup
+        // holds every row tile, whose union is the whole vector: no lookup.
         const __attribute__((address_space(4))) int* band = as_const(p.band);
         auto kband = [&](int slot, bool root, int& klo, int& kn) {
             klo = 0;
             kn = nts;
-            if constexpr (!kF64) {
+            if constexpr (!kF64 && W > 1) {
                 if (!band) return;
                 const int a = root ? r0 : c0, b = root ? r1 : c1;
                 int lo = nts, hi = 0;
