@@ -2,7 +2,9 @@
 // include/cafe_gpu.h.
 #include <math.h>
 #include <stdarg.h>
+#include <ctype.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -180,6 +182,9 @@ struct cafe_gpu_ctx {
     // measured launch shapes: (path, tiles per vector, leaves) -> shape
     struct TuneEntry {
         int path, nt, leaves, kernel, ntw, w, lds, grid_mult;
+        // the plan's size classes it was measured in (bit nt - 1 per class of
+        // nt tiles); 0: any plan
+        unsigned long long plan_classes;
     };
     std::vector<TuneEntry> tuning;
     // bumped by set_tree, set_families and by set_error_model when it adds or
@@ -850,12 +855,19 @@ size_t mfma_scratch_bytes(const MfmaShape& s, bool fp64, int n_slots, int slot_t
     return (size_t)grid * n_slots * slot_tiles * s.fb * 64 * 4 * (fp64 ? 8 : 4);
 }
 
-// The tuning entry for a class, if the context holds one.
-const cafe_gpu_ctx::TuneEntry* find_tuning(const cafe_gpu_ctx* ctx, int path, int nt)
+// The tuning entry for a class of a plan whose size classes are plan_classes
+// (bit nt - 1 per class of nt tiles), if the context holds one: an entry
+// measured in a plan of the same classes, else one for any plan.
+const cafe_gpu_ctx::TuneEntry* find_tuning(const cafe_gpu_ctx* ctx, int path, int nt,
+                                           unsigned long long plan_classes)
 {
-    for (const auto& e : ctx->tuning)
-        if (e.path == path && e.nt == nt && e.leaves == ctx->n_leaves) return &e;
-    return nullptr;
+    const cafe_gpu_ctx::TuneEntry* any = nullptr;
+    for (const auto& e : ctx->tuning) {
+        if (e.path != path || e.nt != nt || e.leaves != ctx->n_leaves) continue;
+        if (e.plan_classes == plan_classes) return &e;
+        if (e.plan_classes == 0 && !any) any = &e;
+    }
+    return any;
 }
 
 // The matrix-core shape a tuning entry names for a class of nt tiles whose
@@ -897,9 +909,9 @@ bool tuned_mfma_shape(cafe_gpu_ctx* ctx, int path, const cafe_gpu_ctx::TuneEntry
 // Replaces a class's built-in launch shape by its tuning entry where the entry
 // is usable (the kernel itself -- per lane or matrix cores -- was chosen when
 // the class's tiles were made); otherwise the class keeps its shape.
-int apply_tuning(cafe_gpu_ctx* ctx, int path, KernelClass& kc)
+int apply_tuning(cafe_gpu_ctx* ctx, int path, KernelClass& kc, unsigned long long plan_classes)
 {
-    const cafe_gpu_ctx::TuneEntry* e = find_tuning(ctx, path, kc.nt);
+    const cafe_gpu_ctx::TuneEntry* e = find_tuning(ctx, path, kc.nt, plan_classes);
     if (!e || path == CAFE_GPU_PATH_PLAIN64) return CAFE_GPU_OK;
     if (kc.lane_rows > 0) {
         if (e->kernel == 1) {
@@ -1065,6 +1077,10 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         if (lane_rows_of(kr.first) == 0) ++n_mfma_keys;
     std::map<int, long long> key_fams;
     for (auto& bc : bucket_class) key_fams[bc.first] += (long long)ctx->buckets[bc.second].count * n_points;
+    // the plan's size classes, for the tuning entries measured in a plan
+    unsigned long long plan_classes = 0;
+    for (const auto& kr : key_rows)
+        if (kr.first >= 1 && kr.first <= 64) plan_classes |= 1ull << (kr.first - 1);
     std::vector<TileDesc> tiles;
     std::vector<SegDesc> segs;
     std::vector<int> cls_fams;
@@ -1080,7 +1096,7 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
             kc.lane_rows = lane_rows_of(bc.first);
             kc.fb32 = kc.lane_rows > 0 ? 1
                                        : builtin_fb32(path, bc.first, n_mfma_keys == 1, key_fams[bc.first], ctx->n_cu);
-            if (const cafe_gpu_ctx::TuneEntry* te = find_tuning(ctx, path, bc.first)) {
+            if (const cafe_gpu_ctx::TuneEntry* te = find_tuning(ctx, path, bc.first, plan_classes)) {
                 MfmaShape probe;
                 if (tuned_mfma_shape(ctx, path, *te, bc.first, bc.first, probe)) {
                     kc.lane_rows = 0;
@@ -1148,7 +1164,7 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     // Measured shapes from the context's tuning table replace the built-in
     // ones where they exist and fit the class.
     for (auto& kc : pp.classes)
-        if (int rc = apply_tuning(ctx, path, kc)) return rc;
+        if (int rc = apply_tuning(ctx, path, kc, plan_classes)) return rc;
     // Main-pass classes to side streams, longest first onto the least loaded
     // stream.  Estimated cost: contraction work (tiles x nt^2, plus a per-tile
     // overhead) and a floor for the serial chain of one large tile.
@@ -1821,12 +1837,13 @@ int cafe_gpu_tuning_set(cafe_gpu_ctx* ctx, int path, int tiles, int leaves, int 
     if (path < CAFE_GPU_PATH_PLAIN64 || path > CAFE_GPU_PATH_BATCHED16 || tiles < 1 || leaves < 1 || kernel < 0 ||
         kernel > 2 || ntw < 0 || w < 0 || grid_mult < 0)
         return fail(CAFE_GPU_EINVAL, "invalid tuning entry");
-    for (auto& e : ctx->tuning)
-        if (e.path == path && e.nt == tiles && e.leaves == leaves) {
-            e = cafe_gpu_ctx::TuneEntry{path, tiles, leaves, kernel, ntw, w, lds ? 1 : 0, grid_mult};
-            return CAFE_GPU_OK;
-        }
-    ctx->tuning.push_back(cafe_gpu_ctx::TuneEntry{path, tiles, leaves, kernel, ntw, w, lds ? 1 : 0, grid_mult});
+    // the new entry applies to any plan: it replaces every entry of the class
+    ctx->tuning.erase(std::remove_if(ctx->tuning.begin(), ctx->tuning.end(),
+                                     [&](const cafe_gpu_ctx::TuneEntry& e) {
+                                         return e.path == path && e.nt == tiles && e.leaves == leaves;
+                                     }),
+                      ctx->tuning.end());
+    ctx->tuning.push_back(cafe_gpu_ctx::TuneEntry{path, tiles, leaves, kernel, ntw, w, lds ? 1 : 0, grid_mult, 0});
     return CAFE_GPU_OK;
 }
 
@@ -1853,8 +1870,29 @@ int cafe_gpu_tuning_load(cafe_gpu_ctx* ctx, const char* path, int* n_loaded)
         const int n = sscanf(line, "%d %d %d %d %d %d %d %d", &e.path, &e.nt, &e.leaves, &e.kernel, &e.ntw, &e.w,
                              &e.lds, &e.grid_mult);
         if (n <= 0) continue;   // blank or comment
-        if (n != 8 || e.path < CAFE_GPU_PATH_PLAIN64 || e.path > CAFE_GPU_PATH_BATCHED16 || e.nt < 1 ||
-            e.leaves < 1 || e.kernel < 0 || e.kernel > 2 || e.ntw < 0 || e.w < 0 || e.grid_mult < 0) {
+        bool ok = n == 8 && e.path >= CAFE_GPU_PATH_PLAIN64 && e.path <= CAFE_GPU_PATH_BATCHED16 && e.nt >= 1 &&
+                  e.leaves >= 1 && e.kernel >= 0 && e.kernel <= 2 && e.ntw >= 0 && e.w >= 0 && e.grid_mult >= 0;
+        // optional "plan=n1,n2,...": the tiles per vector of every size class
+        // of the plan the entry was measured in (the entry's own among them)
+        if (ok) {
+            if (const char* pc = strstr(line, "plan=")) {
+                pc += 5;
+                for (;;) {
+                    char* end = nullptr;
+                    const long v = strtol(pc, &end, 10);
+                    if (end == pc || v < 1 || v > 64) {
+                        ok = false;
+                        break;
+                    }
+                    e.plan_classes |= 1ull << (v - 1);
+                    pc = end;
+                    if (*pc != ',') break;
+                    ++pc;
+                }
+                ok = ok && (*pc == 0 || isspace((unsigned char)*pc)) && (e.plan_classes >> (e.nt - 1) & 1);
+            }
+        }
+        if (!ok) {
             fclose(fh);
             return fail(CAFE_GPU_EINVAL, "%s:%d: malformed tuning entry", path, lineno);
         }

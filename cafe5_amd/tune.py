@@ -97,7 +97,10 @@ def _step_ms(plan, reps: int = 20, rounds: int = 3) -> float:
 def tune_plan(eng: native.Engine, make_plan, path: int, leaves: int, log=print, objective: str = "step",
               min_gain: float = 0.005) -> List[Tuple]:
     """Measure every class of the plan make_plan(path) builds; returns the
-    chosen entries (path, tiles, leaves, kernel, ntw, w, lds, grid_mult, ms).
+    chosen entries (path, tiles, leaves, kernel, ntw, w, lds, grid_mult, ms,
+    plan): plan is the plan's size classes ("2,3,..." tiles per vector) for
+    the step objective, whose measurements hold only beside those classes,
+    and "" (any plan) for the class objective.
     objective "class": the class's own launch time in profiled executions
     (classes one after another); "step": the whole execution's time with
     the classes running concurrently, as the timed steps run them (a shape
@@ -109,6 +112,7 @@ def tune_plan(eng: native.Engine, make_plan, path: int, leaves: int, log=print, 
     nts, tiles, cms, launch = base.class_times()
     cur_ms = _step_ms(base) if objective == "step" else None
     base.close()
+    plan_key = ",".join(str(int(n)) for n in sorted(nts)) if objective == "step" else ""
     chosen = []
     for nt, default_ms, l in sorted(zip(nts, cms, launch), key=lambda x: -x[1]):
         nt = int(nt)
@@ -133,7 +137,7 @@ def tune_plan(eng: native.Engine, make_plan, path: int, leaves: int, log=print, 
         else:
             best_c = best[1]
         eng.tuning_set(path, nt, leaves, *best_c[:3], bool(best_c[3]), best_c[4])
-        chosen.append((path, nt, leaves, *best_c, best[0]))
+        chosen.append((path, nt, leaves, *best_c, best[0], plan_key))
         what = "class" if objective == "class" else "step"
         log(f"path {native.PATH_NAMES[path]} class {nt} tiles: {what} {float(default_ms if objective == 'class' else cur_ms):.3f} "
             f"ms before, {best[0]:.3f} ms with {best_c} ({tried} candidates)")
@@ -145,10 +149,10 @@ def tune_plan(eng: native.Engine, make_plan, path: int, leaves: int, log=print, 
 def write_cache(path: str, entries, header: str):
     with open(path, "w") as fh:
         fh.write("# cafe_gpu tile-shape cache (cafe5_amd/tune.py): path tiles leaves kernel ntw w lds grid_mult "
-                 "[measured ms]\n")
+                 "[measured ms] [plan=tiles of every size class]\n")
         fh.write(f"# {header}\n")
         for e in entries:
-            fh.write(" ".join(str(int(x)) for x in e[:8]) + f"  {e[8]:.4f}\n")
+            fh.write(" ".join(str(int(x)) for x in e[:8]) + f"  {e[8]:.4f}" + (f"  plan={e[9]}" if e[9] else "") + "\n")
 
 
 def _workload(name: str):
@@ -212,13 +216,15 @@ def main():
             for line in fh:
                 line = line.split("#", 1)[0].split()
                 if len(line) >= 8:
-                    entries.append(tuple(int(x) for x in line[:8]) + (float(line[8]) if len(line) > 8 else 0.0,))
+                    plan = next((t[5:] for t in line[8:] if t.startswith("plan=")), "")
+                    ms = next((float(t) for t in line[8:9] if not t.startswith("plan=")), 0.0)
+                    entries.append(tuple(int(x) for x in line[:8]) + (ms, plan))
     t0 = time.time()
     for path in [int(x) for x in args.paths.split(",") if x]:
         got = tune_plan(eng, make_plan, path, w.tree.n_leaves, log=lambda m: print(m, flush=True),
                         objective=args.objective)
-        keyset = {(e[0], e[1], e[2]) for e in got}
-        entries = [e for e in entries if (e[0], e[1], e[2]) not in keyset] + got
+        keyset = {(e[0], e[1], e[2], e[9]) for e in got}
+        entries = [e for e in entries if (e[0], e[1], e[2], e[9]) not in keyset] + got
     write_cache(args.out, entries, f"measured on {args.workload} ({w.name}), objective {args.objective}, in "
                                    f"{time.time() - t0:.0f} s")
     print(f"wrote {len(entries)} entries to {args.out}", flush=True)

@@ -266,15 +266,20 @@ int cafe_gpu_plan_destroy(cafe_gpu_plan *plan);
  * table of measured choices keyed by (path, tiles per vector, leaves of the
  * tree) that overrides the built-in rules; cafe5_amd/tune.py writes it as a
  * cache file on the GPU, one line per entry:
- *     path tiles leaves kernel ntw w lds grid_mult [measured ms]
+ *     path tiles leaves kernel ntw w lds grid_mult [measured ms] [plan=n1,n2,...]
  * (kernel 0 matrix core, 1 per lane, 2 matrix core with 32-family tiles;
- * '#' starts a comment).  An entry whose
+ * '#' starts a comment).  plan= lists the tiles per vector of every size
+ * class of the plan the entry was measured in (the classes run concurrently,
+ * so a class's best shape depends on the others): such an entry applies only
+ * to plans with exactly those classes, and is preferred there to an entry
+ * without the list, which applies to any plan.  An entry whose
  * shape does not exist or does not fit the class is ignored, and the class
  * keeps its built-in shape. */
 /* Replaces the table with the entries of a cache file; *n_loaded (may be
  * NULL) receives their number. */
 int cafe_gpu_tuning_load(cafe_gpu_ctx *ctx, const char *path, int *n_loaded);
-/* Adds or replaces one entry. */
+/* Adds an entry for any plan, replacing the entries of the same path, tiles
+ * and leaves. */
 int cafe_gpu_tuning_set(cafe_gpu_ctx *ctx, int path, int tiles, int leaves, int kernel, int ntw, int w,
                         int lds, int grid_mult);
 /* Empties the table (built-in shapes only). */

@@ -77,10 +77,12 @@ def test_cached_shapes_are_applied(setup, tmp_path, path):
 def test_malformed_cache_is_rejected(setup, tmp_path):
     _, eng, *_ = setup
     bad = tmp_path / "bad.txt"
-    bad.write_text("2 13 34 0 4\n")
-    with pytest.raises(CafeGpuError) as ei:
-        eng.tuning_load(str(bad))
-    assert "malformed tuning entry" in str(ei.value)
+    for text in ("2 13 34 0 4\n", "2 13 34 0 4 4 1 16  0.8 plan=\n", "2 13 34 0 4 4 1 16 plan=2,x\n",
+                 "2 13 34 0 4 4 1 16 plan=2,7\n", "2 13 34 0 4 4 1 16 plan=13,65\n"):
+        bad.write_text(text)
+        with pytest.raises(CafeGpuError) as ei:
+            eng.tuning_load(str(bad))
+        assert "malformed tuning entry" in str(ei.value), text
     with pytest.raises(CafeGpuError):
         eng.tuning_load(str(tmp_path / "missing.txt"))
     eng.tuning_clear()
@@ -185,3 +187,28 @@ def test_band_limited_k_loops_are_exact(setup, shape):
     assert np.array_equal(tuned["lnl"][fin], base["lnl"][fin]), np.abs(tuned["lnl"][fin] - base["lnl"][fin]).max()
     assert np.array_equal(tuned["total"], base["total"])
     assert tuned["n_fallback"] == base["n_fallback"]
+
+
+# Entries measured in a plan (plan=, the tiles of every size class) apply to
+# plans with exactly those classes and are preferred there to entries for any
+# plan; a plan of other classes ignores them.
+def test_plan_keyed_entries(setup, tmp_path):
+    w, eng, slots, prior, lq, tq = setup
+    path = native.PATH_BATCHED32
+    eng.tuning_clear()
+    base, shapes = _run(eng, slots, prior, lq, tq, path)
+    big = max(shapes)
+    mine = ",".join(str(n) for n in sorted(shapes))
+    leaves = w.tree.n_leaves
+    cache = tmp_path / "plan.txt"
+    cache.write_text(
+        f"{path} {big} {leaves} 0 4 4 1 4\n"                         # any plan: LDS partials
+        f"{path} {big} {leaves} 0 4 4 0 1  0.7  plan={mine}\n"       # this plan's classes: global partials
+        f"{path} 2 {leaves} 0 2 1 1 4  0.5  plan=2,{big}\n")          # other classes: ignored
+    assert eng.tuning_load(str(cache)) == 3
+    tuned, tshapes = _run(eng, slots, prior, lq, tq, path)
+    eng.tuning_clear()
+    print(f"classes {mine}: built-in {shapes}\n   tuned {tshapes}", flush=True)
+    assert tshapes[big][1:4] == (4, 4, 0)       # the plan-keyed entry, not the one for any plan
+    assert tshapes[2] == shapes[2]              # a plan of other classes: built-in
+    assert np.array_equal(tuned["total"], base["total"])   # where partials live changes no arithmetic

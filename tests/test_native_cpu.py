@@ -115,9 +115,10 @@ def test_tuner_candidates_and_cache_format(tmp_path):
     assert not any(k == 1 for k, *_ in tune.candidates(native.PATH_BATCHED64, 2))   # FP64: matrix cores only
     assert all(w <= 8 for _, _, w, _, _ in tune.candidates(native.PATH_BATCHED16, 32))
     out = tmp_path / "cache.txt"
-    tune.write_cache(str(out), [(2, 13, 34, 0, 4, 4, 1, 16, 0.83)], "test")
+    tune.write_cache(str(out), [(2, 13, 34, 0, 4, 4, 1, 16, 0.83, ""), (2, 7, 34, 2, 1, 8, 1, 4, 3.6, "2,7,13")],
+                     "test")
     lines = [l for l in out.read_text().splitlines() if l and not l.startswith("#")]
-    assert lines == ["2 13 34 0 4 4 1 16  0.8300"]
+    assert lines == ["2 13 34 0 4 4 1 16  0.8300", "2 7 34 2 1 8 1 4  3.6000  plan=2,7,13"]
 
 
 def test_committed_tuning_cache_parses():
@@ -127,6 +128,9 @@ def test_committed_tuning_cache_parses():
     for line in open(path):
         f = line.split("#", 1)[0].split()
         if f:
-            assert len(f) in (8, 9), line
+            assert 8 <= len(f) <= 10, line
             path_, tiles, leaves, kernel, ntw, w, lds, gm = (int(x) for x in f[:8])
             assert 0 <= path_ <= 3 and tiles >= 1 and leaves >= 1 and kernel in (0, 1, 2) and lds in (0, 1)
+            plans = [t[5:] for t in f[8:] if t.startswith("plan=")]
+            for pl in plans:   # the plan's classes include the entry's own
+                assert tiles in [int(x) for x in pl.split(",")], line
