@@ -25,11 +25,14 @@ __device__ __forceinline__ bool degenerate_key(double alpha, double coeff)
 // Every term is non-negative, so rounding errors stay relative (~1e-15 per
 // step) and there is no cancellation.
 //
-// One wavefront per matrix.  Lane l owns rows s = l + 64 b.  Columns advance
-// one at a time; along s the recurrence y_s = a y_{s-1} + K_s is a first-order
-// linear scan, done across lanes with a Kogge-Stone affine scan and carried
-// between 64-row blocks.  Stores for a fixed column are contiguous in s; the
-// row-major FP32 copy is written every CW columns from registers.
+// One wavefront per matrix.  Lane l owns the R consecutive rows s = R l ..
+// R l + R - 1 (R = ceil(N / 64)).  Columns advance one at a time; along s the
+// recurrence y_s = a y_{s-1} + K_s is a first-order linear scan: each lane
+// runs it over its own rows, one Kogge-Stone affine scan across the lanes
+// carries the lanes' last values, and a fixup adds the carry to each row.
+// A lane's stores for a fixed column are R contiguous entries (the wave's are
+// contiguous in s); the row-major FP32 copy is written every CW columns from
+// registers.
 // ---------------------------------------------------------------------------
 // Lane moves of a double within the wave (DPP on both 32-bit halves; lanes
 // without a source, or outside the row mask, read 0).
@@ -67,9 +70,9 @@ __device__ __forceinline__ double int_pow(double base, int e)
 }
 
 // Columns per row-major write group: a lane keeps CW consecutive columns of
-// each of its NB rows in registers (64 floats at most) and then writes
-// CW * 4 contiguous bytes per row.
-template <int NB> struct RowGroup { static constexpr int v = NB >= 32 ? 2 : NB >= 16 ? 4 : NB >= 2 ? 64 / NB : 32; };
+// each of its R rows in registers (64 floats at most) and then writes CW * 4
+// contiguous bytes per row.
+template <int R> struct RowGroup { static constexpr int v = R >= 2 ? 64 / R : 32; };
 
 // Any of the three pools may be null: the FP64 pool (c-major), the FP32 copy
 // (c-major, entries times 2^kMatExp) and its row-major twin.  copies (may be
@@ -78,8 +81,8 @@ template <int NB> struct RowGroup { static constexpr int v = NB >= 32 ? 2 : NB >
 // s, c < top are written.  With the row-major copy goes its band (band may be
 // null): per 16-row tile the k-tiles between its first and last nonzero
 // FP32 entry (a lane tracks its rows' first and last nonzero column as it
-// writes them; 16 lanes hold a tile).
-template <int NB>
+// writes them, per tile; the 16 / R lanes of a tile combine theirs).
+template <int R>
 __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restrict__ lam_q,
                                                           const double* __restrict__ t_q,
                                                           int first_slot, int N, int Np,
@@ -90,9 +93,10 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
                                                           const unsigned char* __restrict__ copies, int top,
                                                           int* __restrict__ band, int band_stride)
 {
-    constexpr int CW = RowGroup<NB>::v;
+    constexpr int CW = RowGroup<R>::v;
     const int key = blockIdx.x;
     const int lane = threadIdx.x;
+    const int s0 = R * lane;   // the lane's rows s0 .. s0 + R - 1
     const double lt = lam_q[key] * t_q[key];
     const double alpha = lt / (1 + lt);
     const double coeff = 1 - 2 * alpha;
@@ -110,8 +114,8 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
         if (BT)
             for (int t = lane; t < band_stride; t += 64) BT[t] = t == 0 ? band_entry(0, 1) : band_entry(0, 0);
         for (int c = 0; c < N; ++c)
-            for (int b = 0; b < NB; ++b) {
-                const int s = lane + 64 * b;
+            for (int j = 0; j < R; ++j) {
+                const int s = s0 + j;
                 if (s < N) {
                     const double v = (s == 0 && c == 0) ? 1.0 : 0.0;
                     if (M64) M64[(long long)c * Np + s] = v;
@@ -122,74 +126,119 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
         return;
     }
 
-    // a^d for the scan distances d = 1, 2, 4, 8, and per lane l the factors
-    // a^(l mod 16 + 1), a^(l mod 32 + 1) (row broadcasts) and a^(l + 1)
-    // (carry from the previous 64-row block)
-    double apow[4];
-    apow[0] = alpha;
-    for (int i = 1; i < 4; ++i) apow[i] = apow[i - 1] * apow[i - 1];
-    const double a16 = int_pow(alpha, (lane & 15) + 1);
-    const double a32 = int_pow(alpha, (lane & 31) + 1);
-    const double a_lane = int_pow(alpha, lane + 1);
-
-    double prev[NB];   // P_s(c-1) for the lane's rows
-    int first_nz[NB], last_nz[NB];   // the lane's rows' first and last nonzero FP32 column
+    // Along s, y_s = K_s + a y_{s-1}: a lane runs its R rows in order from 0
+    // (z_j), then the lanes' last values are scanned with factor A = a^R
+    // (Kogge-Stone within rows of 16 lanes, then across rows), and the
+    // carry Y from the lanes before adds a^(j+1) Y to row j.  Factors: A^d
+    // for the shifts d = 1, 2, 4, 8, and per lane A^(l mod 16 + 1),
+    // A^(l mod 32 + 1) (row broadcasts); a^(j+1) for the carry.
+    double apj[R];
+    apj[0] = alpha;
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        prev[b] = 0.0;
-        first_nz[b] = Np;
-        last_nz[b] = -1;
+    for (int j = 1; j < R; ++j) apj[j] = apj[j - 1] * alpha;
+    const double A = apj[R - 1];
+    double Apow[4];
+    Apow[0] = A;
+    for (int i = 1; i < 4; ++i) Apow[i] = Apow[i - 1] * Apow[i - 1];
+    const double A16 = int_pow(A, (lane & 15) + 1);
+    const double A32 = int_pow(A, (lane & 31) + 1);
+
+    // 16-row tiles per lane (R > 16) and lanes per tile (R < 16)
+    constexpr int TPL = R > kTile ? R / kTile : 1;
+    constexpr int LPT = R < kTile ? kTile / R : 1;
+    double prev[R];   // P_s(c-1) for the lane's rows
+    int first_nz[TPL], last_nz[TPL];   // per tile: the lane's rows' first and last nonzero FP32 column
+#pragma unroll
+    for (int j = 0; j < R; ++j) prev[j] = 0.0;
+#pragma unroll
+    for (int q = 0; q < TPL; ++q) {
+        first_nz[q] = Np;
+        last_nz[q] = -1;
     }
+    // whole-lane vector stores of a column: every row of the lane is a
+    // matrix row (the rows are 8 R-byte aligned: Np is a multiple of 32)
+    const bool full = s0 + R <= N;
 
     const int n_groups = (N + CW - 1) / CW;
     for (int g = 0; g < n_groups; ++g) {
-        float row[NB][CW];   // this group's columns of the lane's rows (FP32 copy)
+        float row[R][CW];   // this group's columns of the lane's rows (FP32 copy)
 #pragma unroll
         for (int cc = 0; cc < CW; ++cc) {
             const int c = g * CW + cc;
             if (c >= N) {   // past the last column (wave-uniform): zero padding
 #pragma unroll
-                for (int b = 0; b < NB; ++b) row[b][cc] = 0.f;
+                for (int j = 0; j < R; ++j) row[j][cc] = 0.f;
                 continue;
             }
-            double K[NB];
             // K_s = (1-2a) P_{s-1}(c-1) + a P_s(c-1); row 0 is delta_0 itself
+            const double below0 = dpp_move<kDppWaveShr1, 0xF>(prev[R - 1]);   // lane l-1's last row, 0 at lane 0
+            double z[R];
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                double below = dpp_move<kDppWaveShr1, 0xF>(prev[b]);   // lane l-1, 0 at lane 0
-                const double from_prev_block = b > 0 ? read_lane(prev[b > 0 ? b - 1 : 0], 63) : 0.0;
-                if (lane == 0) below = from_prev_block;
-                K[b] = coeff * below + alpha * prev[b];
+            for (int j = 0; j < R; ++j) {
+                const double below = j == 0 ? below0 : prev[j - 1];
+                double K = coeff * below + alpha * prev[j];
+                if (j == 0 && lane == 0) K = (c == 0) ? 1.0 : 0.0;   // P_0(c) = delta_{c0}
+                z[j] = j == 0 ? K : fma(alpha, z[j - 1], K);
             }
-            if (lane == 0) K[0] = (c == 0) ? 1.0 : 0.0;   // P_0(c) = delta_{c0}
-
-            double carry = 0.0;
+            // inclusive scan of the lanes' last values: Y_l = z_{R-1}(l) + A Y_{l-1}
+            double Y = z[R - 1];
+            Y = fma(Apow[0], dpp_move<kDppRowShr + 1, 0xF>(Y), Y);
+            Y = fma(Apow[1], dpp_move<kDppRowShr + 2, 0xF>(Y), Y);
+            Y = fma(Apow[2], dpp_move<kDppRowShr + 4, 0xF>(Y), Y);
+            Y = fma(Apow[3], dpp_move<kDppRowShr + 8, 0xF>(Y), Y);
+            Y = fma(A16, dpp_move<kDppRowBcast15, 0xA>(Y), Y);
+            Y = fma(A32, dpp_move<kDppRowBcast31, 0xC>(Y), Y);
+            const double carry = dpp_move<kDppWaveShr1, 0xF>(Y);   // Y_{l-1}, 0 at lane 0
+            double v64[R];
+            float v32[R];
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                // y_s = K_s + a y_{s-1}: within rows of 16 lanes (shifts 1..8),
-                // then across rows (broadcast of lane 15 into rows 1 and 3, of
-                // lane 31 into rows 2 and 3)
-                double y = K[b];
-                y = fma(apow[0], dpp_move<kDppRowShr + 1, 0xF>(y), y);
-                y = fma(apow[1], dpp_move<kDppRowShr + 2, 0xF>(y), y);
-                y = fma(apow[2], dpp_move<kDppRowShr + 4, 0xF>(y), y);
-                y = fma(apow[3], dpp_move<kDppRowShr + 8, 0xF>(y), y);
-                y = fma(a16, dpp_move<kDppRowBcast15, 0xA>(y), y);
-                y = fma(a32, dpp_move<kDppRowBcast31, 0xC>(y), y);
-                if (b > 0) y += a_lane * carry;
-                carry = read_lane(y, 63);
-                prev[b] = y;
-                const int s = lane + 64 * b;
+            for (int j = 0; j < R; ++j) {
+                const double y = fma(apj[j], carry, z[j]);
+                prev[j] = y;
                 const double v = y < 1.0 ? y : 1.0;   // [0,1] clamp, probability.cpp:147
-                const float v32 = (float)ldexp(v, kMatExp);
-                row[b][cc] = s < N ? v32 : 0.f;
-                if (s < N && v32 != 0.f) {
-                    first_nz[b] = first_nz[b] < c ? first_nz[b] : c;
-                    last_nz[b] = c;
+                v64[j] = v;
+                v32[j] = (float)ldexp(v, kMatExp);
+                const bool in = s0 + j < N;
+                row[j][cc] = in ? v32[j] : 0.f;
+                if (in && v32[j] != 0.f) {
+                    const int q = j / kTile;
+                    first_nz[q] = first_nz[q] < c ? first_nz[q] : c;
+                    last_nz[q] = c;
                 }
-                if (s < N) {
-                    if (M64) M64[(long long)c * Np + s] = v;
-                    if (L32 && s < col_rows && c < col_rows) L32[(long long)c * Np + s] = v32;
+            }
+            double* m64 = M64 ? M64 + (long long)c * Np + s0 : nullptr;
+            float* l32 = L32 && c < col_rows ? L32 + (long long)c * Np + s0 : nullptr;
+            if (full) {
+                if (m64) {
+                    if constexpr (R % 2 == 0) {
+                        typedef double d2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+                        for (int q = 0; q < R / 2; ++q) *(d2*)(m64 + 2 * q) = d2{v64[2 * q], v64[2 * q + 1]};
+                    } else {
+                        m64[0] = v64[0];
+                    }
+                }
+                if (l32 && s0 + R <= col_rows) {
+                    if constexpr (R % 4 == 0) {
+                        typedef float f4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+                        for (int q = 0; q < R / 4; ++q)
+                            *(f4*)(l32 + 4 * q) = f4{v32[4 * q], v32[4 * q + 1], v32[4 * q + 2], v32[4 * q + 3]};
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < R; ++j) l32[j] = v32[j];
+                    }
+                } else if (l32) {
+#pragma unroll
+                    for (int j = 0; j < R; ++j)
+                        if (s0 + j < col_rows) l32[j] = v32[j];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    if (s0 + j >= N) break;
+                    if (m64) m64[j] = v64[j];
+                    if (l32 && s0 + j < col_rows) l32[j] = v32[j];
                 }
             }
         }
@@ -197,37 +246,37 @@ __global__ __launch_bounds__(64) void k_build_recurrence(const double* __restric
             // row s, columns g CW .. g CW + CW - 1: CW * 4 contiguous bytes
             // (aligned: Np and g CW are multiples of CW)
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const int s = lane + 64 * b;
-                if (s >= N) continue;
+            for (int j = 0; j < R; ++j) {
+                const int s = s0 + j;
+                if (s >= N) break;
                 float* dst = R32 + (long long)s * Np + g * CW;
                 if constexpr (CW % 4 == 0) {
                     typedef float f4 __attribute__((ext_vector_type(4)));
 #pragma unroll
                     for (int q = 0; q < CW / 4; ++q)
-                        *(f4*)(dst + 4 * q) = f4{row[b][4 * q], row[b][4 * q + 1], row[b][4 * q + 2], row[b][4 * q + 3]};
+                        *(f4*)(dst + 4 * q) = f4{row[j][4 * q], row[j][4 * q + 1], row[j][4 * q + 2], row[j][4 * q + 3]};
                 } else {
                     typedef float f2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-                    for (int q = 0; q < CW / 2; ++q) *(f2*)(dst + 2 * q) = f2{row[b][2 * q], row[b][2 * q + 1]};
+                    for (int q = 0; q < CW / 2; ++q) *(f2*)(dst + 2 * q) = f2{row[j][2 * q], row[j][2 * q + 1]};
                 }
             }
         }
     }
     if (BT) {
-        // tile 4 b + lane / 16 = rows 64 b + 16 (lane / 16) .. + 15
+        // tile s0 / 16 + q: the LPT lanes from lane - lane % LPT
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            int lo = first_nz[b], hi = last_nz[b];
+        for (int q = 0; q < TPL; ++q) {
+            int lo = first_nz[q], hi = last_nz[q];
 #pragma unroll
-            for (int d = 1; d < 16; d <<= 1) {
+            for (int d = 1; d < LPT; d <<= 1) {
                 const int olo = __shfl_xor(lo, d);
                 const int ohi = __shfl_xor(hi, d);
                 lo = lo < olo ? lo : olo;
                 hi = hi > ohi ? hi : ohi;
             }
-            const int t = 4 * b + (lane >> 4);
-            if ((lane & 15) == 0 && t < band_stride)
+            const int t = s0 / kTile + q;
+            if (lane % LPT == 0 && t < band_stride)
                 BT[t] = hi >= 0 ? band_entry(lo / kTile, hi / kTile + 1) : band_entry(0, 0);
         }
     }
@@ -383,20 +432,20 @@ hipError_t launch_build(int algo, const double* d_lam, const double* d_t, const 
         }
     } else {
         if (Np % 32 != 0) return hipErrorInvalidValue;   // row groups of up to 32 columns
-        const int nb = (N + 63) / 64;
-#define CAFE_BUILD_NB(NB)                                                                                 \
-    if (nb <= NB) {                                                                                       \
-        hipLaunchKernelGGL(k_build_recurrence<NB>, dim3(n), dim3(64), 0, stream, d_lam, d_t, first_slot, N, \
+        const int rows = (N + 63) / 64;   // rows per lane
+#define CAFE_BUILD_R(R)                                                                                   \
+    if (rows <= R) {                                                                                      \
+        hipLaunchKernelGGL(k_build_recurrence<R>, dim3(n), dim3(64), 0, stream, d_lam, d_t, first_slot, N,  \
                            Np, slot_elems, mt64, lf32, mt32, copies, top, band, band_stride);             \
         return hipGetLastError();                                                                         \
     }
-        CAFE_BUILD_NB(1)
-        CAFE_BUILD_NB(2)
-        CAFE_BUILD_NB(4)
-        CAFE_BUILD_NB(8)
-        CAFE_BUILD_NB(16)
-        CAFE_BUILD_NB(32)
-#undef CAFE_BUILD_NB
+        CAFE_BUILD_R(1)
+        CAFE_BUILD_R(2)
+        CAFE_BUILD_R(4)
+        CAFE_BUILD_R(8)
+        CAFE_BUILD_R(16)
+        CAFE_BUILD_R(32)
+#undef CAFE_BUILD_R
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
