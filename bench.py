@@ -407,7 +407,11 @@ def _class_roofline(run, flops, peak, unit_desc):
             "kernel": f"{KERNEL_KIND[int(dl[0])]}<{int(dl[1])},{int(dl[2])},{'lds' if dl[3] else 'global'}> main-pass "
                       f"launch of the {int(nt_cls[dom])}-tile size class ({int(tiles_cls[dom])} tiles of "
                       f"{int(dl[5])} families, {int(dl[4])} workgroups)",
-            "algorithmic_flops_per_launch": flops, "avg_ms": ms, "flops_per_family_likelihood": unit_desc}
+            "algorithmic_flops_per_launch": flops, "avg_ms": ms, "flops_per_family_likelihood": unit_desc,
+            "executed": "the FP32 matrix-core kernel multiplies only the k-tiles inside each step's nonzero band "
+                        "(matrix blocks that are exactly 0 in FP32 are skipped), so it executes fewer matrix-core "
+                        "flops than the dense algorithmic count above; the counters under profiles/ give the "
+                        "executed rate"}
 
 
 def _ref_cpu_base(w, B, R, err, node_lambda_classes, n_sample, threads, gamma=None):
