@@ -1167,14 +1167,17 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         if (int rc = apply_tuning(ctx, path, kc, plan_classes)) return rc;
     // Main-pass classes to side streams, longest first onto the least loaded
     // stream.  Estimated cost: contraction work (tiles x nt^2, plus a per-tile
-    // overhead) and a floor for the serial chain of one large tile.
+    // overhead) and a floor for the serial chain of one large tile; a
+    // per-lane class counts a quarter (it runs unpadded at the full vector
+    // rate beside the matrix-core classes: measured 3.480 -> 3.459 ms per
+    // step on the uneven-size workload against the full count, 3.52 at half).
     {
         std::vector<double> cost(n_cls);
         std::vector<int> order(n_cls);
         for (int ci = 0; ci < n_cls; ++ci) {
             const KernelClass& kc = pp.classes[ci];
             const double tiles = kc.r16;   // in 16-family tiles
-            cost[ci] = tiles * (kc.nt * kc.nt + 8.0) + 2500.0 * kc.nt * kc.nt;
+            cost[ci] = (tiles * (kc.nt * kc.nt + 8.0) + 2500.0 * kc.nt * kc.nt) * (kc.lane_rows > 0 ? 0.25 : 1.0);
             order[ci] = ci;
         }
         std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return cost[a] > cost[b2]; });
