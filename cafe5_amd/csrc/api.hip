@@ -162,8 +162,10 @@ struct cafe_gpu_ctx {
     std::vector<ChildDev> kids;
     DevBuf d_ops, d_kids, d_steps;
     int n_steps = 0;
-    DevBuf d_steps_lane;     // the same steps with output-reusing slots (lane kernel)
+    DevBuf d_steps_lane;     // the same steps with output-reusing slots (split-FP16 kernel)
     int n_slots_lane = 0;
+    DevBuf d_steps_hot;      // and with the vector the next op reads kept in registers (per-lane kernel)
+    int n_slots_hot = 0;
 
     // families
     int F = 0, n_leaves = 0;
@@ -257,9 +259,14 @@ struct Planner {
     // reuse: a node's output may take one of its children's slots (a kernel
     // whose lanes read every child before writing the output, in order)
     bool reuse = false;
+    // hot (with reuse): the last interior child a node's op reads is the op
+    // just before it, so its vector is read from the kernel's hot registers
+    // (in_slot kHotSlot) and takes no slot; so is every output that the next
+    // op reads (out_slot kHotSlot), and the root's
+    bool hot = false;
 
-    Planner(const std::vector<std::vector<int>>& k, const std::vector<int>& lc, bool r = false)
-        : kids(k), leaf_col(lc), need(k.size(), -1), slot_of(k.size(), -1), reuse(r) {}
+    Planner(const std::vector<std::vector<int>>& k, const std::vector<int>& lc, bool r = false, bool h = false)
+        : kids(k), leaf_col(lc), need(k.size(), -1), slot_of(k.size(), -1), reuse(r || h), hot(h) {}
 
     std::vector<int> interior_order(int u)
     {
@@ -293,13 +300,14 @@ struct Planner {
         return (int)used.size() - 1;
     }
 
-    void emit(int u)
+    void emit(int u, bool store = true)
     {
         auto in = interior_order(u);
-        for (int c : in) emit(c);
+        for (size_t i = 0; i < in.size(); ++i) emit(in[i], !hot || i + 1 < in.size());
         if (reuse)
-            for (int c : in) used[slot_of[c]] = 0;
-        const int out = alloc();
+            for (int c : in)
+                if (slot_of[c] != kHotSlot) used[slot_of[c]] = 0;
+        const int out = hot && !store ? kHotSlot : alloc();
         OpDev op;
         op.node = u;
         op.out_slot = out;
@@ -379,7 +387,7 @@ int cafe_gpu_destroy(cafe_gpu_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf* bufs[] = {&ctx->lg, &ctx->mt64, &ctx->mt32, &ctx->lf32, &ctx->key_lam, &ctx->key_t, &ctx->d_ops,
-                      &ctx->d_kids, &ctx->d_steps, &ctx->d_steps_lane, &ctx->d_counts, &ctx->d_perm, &ctx->d_err64,
+                      &ctx->d_kids, &ctx->d_steps, &ctx->d_steps_lane, &ctx->d_steps_hot, &ctx->d_counts, &ctx->d_perm, &ctx->d_err64,
                       &ctx->d_err32, &ctx->mt64e, &ctx->mt32e, &ctx->mt16h, &ctx->mt16l, &ctx->band32};
     for (DevBuf* b : bufs) b->release();
     ctx->eager.release_all();
@@ -523,6 +531,13 @@ int cafe_gpu_set_tree(cafe_gpu_ctx* ctx, int n_nodes, const int* parent, const i
     pl.compute_need(root);
     pl.used.clear();
     pl.emit(root);
+    // the per-lane kernel's program: the same order, the vector each op reads
+    // from the op just before it in registers, the others in output-reusing
+    // slots
+    Planner ph(kids, lc, true, true);
+    ph.compute_need(root);
+    ph.used.clear();
+    ph.emit(root, false);
     if (pl.max_slot + 1 > kMaxSlots)
         return fail(CAFE_GPU_EINVAL, "tree needs %d live partial vectors (max %d)", pl.max_slot + 1, kMaxSlots);
     ctx->input_gen++;
@@ -569,6 +584,11 @@ int cafe_gpu_set_tree(cafe_gpu_ctx* ctx, int n_nodes, const int* parent, const i
     ctx->n_slots_lane = pr.max_slot + 1;
     HIP_OR_FAIL(ctx->d_steps_lane.ensure(sizeof(StepDev) * lsteps.size()));
     HIP_OR_FAIL(hipMemcpy(ctx->d_steps_lane.p, lsteps.data(), sizeof(StepDev) * lsteps.size(),
+                          hipMemcpyHostToDevice));
+    const std::vector<StepDev> hsteps = flatten(ph.ops, ph.out_kids);
+    ctx->n_slots_hot = std::max(1, ph.max_slot + 1);
+    HIP_OR_FAIL(ctx->d_steps_hot.ensure(sizeof(StepDev) * hsteps.size()));
+    HIP_OR_FAIL(hipMemcpy(ctx->d_steps_hot.p, hsteps.data(), sizeof(StepDev) * hsteps.size(),
                           hipMemcpyHostToDevice));
     return CAFE_GPU_OK;
 }
@@ -1068,7 +1088,7 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     auto lane_rows_of = [&](int key) {
         if (path != CAFE_GPU_PATH_BATCHED32 && path != CAFE_GPU_PATH_BATCHED16) return 0;
         const int lr = lane_rows_for(key_rows[key]);
-        return lr > 0 && ctx->n_slots_lane <= lane_max_slots(lr) ? lr : 0;
+        return lr > 0 && ctx->n_slots_hot + 1 <= lane_max_slots(lr) ? lr : 0;
     };
     // size classes the matrix cores take (those the per-lane kernel does not),
     // and the family-likelihoods of each class
@@ -1104,7 +1124,7 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
                 }
             }
             if (kc.lane_rows > 0) {
-                kc.lane_slots = ctx->n_slots_lane;
+                kc.lane_slots = ctx->n_slots_hot + 1;   // the stored slots and the hot array
                 const PruneParams none{};
                 HIP_OR_FAIL(lane_dispatch(kc.lane_rows, kc.lane_slots, 0, lane_shm(ctx->n_leaves), none, nullptr,
                                           &kc.lane_per_cu));
@@ -1368,8 +1388,8 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
         } else if (pp.path == CAFE_GPU_PATH_BATCHED64) {
             HIP_OR_FAIL(launch_prune_mfma(true, kc.s64, pp.grid(kc.s64, n), q, s));
         } else if (kc.lane_rows > 0) {   // BATCHED32 / BATCHED16: small vectors, one family per lane
-            q.steps = ctx->d_steps_lane.as<StepDev>();
-            q.n_slots = ctx->n_slots_lane;
+            q.steps = ctx->d_steps_hot.as<StepDev>();
+            q.n_slots = ctx->n_slots_hot;
             HIP_OR_FAIL(lane_dispatch(kc.lane_rows, kc.lane_slots, pp.grid_lane(kc.lane_per_cu, n, kc.lane_grid_mult),
                                       lane_shm(ctx->n_leaves), q, s, nullptr));
         } else if (pp.path == CAFE_GPU_PATH_BATCHED16 && kc.nt <= kF16MaxTiles) {

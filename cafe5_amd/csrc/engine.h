@@ -95,7 +95,10 @@ enum : int {
     kStepFirst = 1 << 8,   // first child of its op: the product starts at 1
     kStepLast = 1 << 9,    // last child: the op's vector is complete
     kStepRoot = 1 << 10,   // the op is the root
-    kStepSlotMask = 0xff
+    kStepSlotMask = 0xff,
+    // per-lane kernel: a vector kept in registers (the output the next op
+    // reads, read by that op), as in_slot and out_slot
+    kHotSlot = 0xfe
 };
 
 // Output flags per (point, family).

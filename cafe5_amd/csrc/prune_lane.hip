@@ -16,13 +16,18 @@
 // 16-row tiles of the matrix-core kernel (21 rows: 1.0x the work instead of
 // 2.3x), and a SIMD holds 64 families per wave.
 //
-// Partial vectors are in registers, indexed by the program's slot numbers
-// through selects under wave-uniform masks and weights (per-slot copies of
-// the code under wave-uniform branches run ~6% faster alone but hold ~10%
-// more registers, which costs more beside the other classes).  This
-// kernel's program lets a node's output take the slot of one of its children
-// (the children are all read before the output is written, by the same lane),
-// so a binary tree of 34 leaves needs 3 slots.
+// Partial vectors are in registers, NS arrays of LR floats.  The program
+// (post order) puts each node's op right after the op of its last interior
+// child, so that child's vector is read straight from the last array, the
+// "hot" one, which the op just before wrote (its output is marked kHotSlot,
+// as is the child read): no selects.  Only the other interior children (the
+// first of two interior siblings) are stored, in arrays 0 .. NS-2 indexed by
+// the program's slot numbers through selects under wave-uniform masks and
+// weights (per-slot copies of the code under wave-uniform branches run ~6%
+// faster alone but hold ~10% more registers, which costs more beside the
+// other classes; so does a hot array on top of NS stored ones).  A stored
+// output may take the slot of one of its children (the children are all read
+// before the output is written, by the same lane).
 //
 // Numerics follow the single-wave matrix-core kernel exactly: stored vectors
 // are renormalised to a maximum just below 2^vec_exp, the exponent is carried
@@ -60,31 +65,32 @@ __device__ __forceinline__ float bit_select(unsigned m, float a, float b)
     asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(b));
     return r;
 }
-template <int NS, int LR>
-__device__ __forceinline__ float pick(const float (&sv)[NS][LR], const SlotWeights<NS>& sw, int k)
+// (stored slots are arrays 0 .. NSS-1 of the NS > NSS arrays)
+template <int NSS, int NS, int LR>
+__device__ __forceinline__ float pick(const float (&sv)[NS][LR], const SlotWeights<NSS>& sw, int k)
 {
     float r = sv[0][k];
 #pragma unroll
-    for (int c = 1; c < NS; ++c) r = bit_select(sw.m[c], sv[c][k], r);
+    for (int c = 1; c < NSS; ++c) r = bit_select(sw.m[c], sv[c][k], r);
     return r;
 }
-template <int NS, typename V>
-__device__ __forceinline__ V pick(const V (&a)[NS], const SlotWeights<NS>& sw)
+template <int NSS, int NS, typename V>
+__device__ __forceinline__ V pick(const V (&a)[NS], const SlotWeights<NSS>& sw)
 {
     V r = 0;
 #pragma unroll
-    for (int c = 0; c < NS; ++c) r += sw.w[c] != 0.f ? a[c] : V(0);
+    for (int c = 0; c < NSS; ++c) r += sw.w[c] != 0.f ? a[c] : V(0);
     return r;
 }
 // (the store is a select per element as well: a branch per slot would let
 // the compiler merge the stores into one at a variable index, which moves the
 // arrays to scratch memory)
-template <int NS, int LR>
+template <int NSS, int NS, int LR>
 __device__ __forceinline__ void put(float (&sv)[NS][LR], int (&se)[NS], double (&sc)[NS], int out,
                                     const float (&v)[LR], int e, double c)
 {
 #pragma unroll
-    for (int q = 0; q < NS; ++q) {
+    for (int q = 0; q < NSS; ++q) {
         const bool here = out == q;
 #pragma unroll
         for (int i = 0; i < LR; ++i) sv[q][i] = here ? v[i] : sv[q][i];
@@ -131,7 +137,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LaneWaves<LR
         for (int l = 0; l < p.n_leaves; ++l) s_cnt[l * 64 + lane] = p.counts_T[(long long)l * p.F + fs];
         const __attribute__((address_space(4))) int* nslot = as_const(p.node_slot + (long long)td.point * p.n_nodes);
 
-        float sv[NS][LR];   // stored partial vectors (renormalised)
+        constexpr int H = NS - 1;   // the hot array; stored slots 0 .. NS-2
+        float sv[NS][LR];   // partial vectors (renormalised)
         int se[NS];         // their exponents: true vector = stored 2^se
         double sc[NS];      // their flush bounds, relative to their maximum
         float acc[LR];      // the current node's running product
@@ -195,37 +202,51 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LaneWaves<LR
                         for (int i = 0; i < LR; ++i) acc[i] *= lv[i];
                     }
                 } else {
-                    // interior child in slot s: x = M' v, column k of M' from the
-                    // scalar cache, v[k] the lane's own
-                    const SlotWeights<NS> sw(sd.in_slot);
+                    // interior child, hot or in slot s: x = M' v, column k of M'
+                    // from the scalar cache, v[k] the lane's own
                     const __attribute__((address_space(4))) float* M = as_const(p.cm32 + base);
                     float x[LR];
                     // columns through scalar-register buffers: the loads of the
                     // next D columns are in flight while column k is used
                     // (D = 2 for 21 rows, 1 for larger vectors: SGPR budget)
                     constexpr int D = LR <= 24 ? 2 : 1;
-                    float cbuf[D + 1][LR];
+                    auto contract = [&](auto vread) {
+                        float cbuf[D + 1][LR];
 #pragma unroll
-                    for (int d = 0; d < D; ++d)
+                        for (int d = 0; d < D; ++d)
 #pragma unroll
-                        for (int i = 0; i < LR; ++i) cbuf[d][i] = d < LR ? M[d * Np + i] : 0.f;
+                            for (int i = 0; i < LR; ++i) cbuf[d][i] = d < LR ? M[d * Np + i] : 0.f;
 #pragma unroll
-                    for (int k = 0; k < LR; ++k) {
-                        float (&cur)[LR] = cbuf[k % (D + 1)];
-                        float (&nxt)[LR] = cbuf[(k + D) % (D + 1)];
-                        if (k + D < LR) {
+                        for (int k = 0; k < LR; ++k) {
+                            float (&cur)[LR] = cbuf[k % (D + 1)];
+                            float (&nxt)[LR] = cbuf[(k + D) % (D + 1)];
+                            if (k + D < LR) {
 #pragma unroll
-                            for (int i = 0; i < LR; ++i) nxt[i] = M[(k + D) * Np + i];
+                                for (int i = 0; i < LR; ++i) nxt[i] = M[(k + D) * Np + i];
+                            }
+                            const float vk = vread(k);
+                            // column 0 starts the sums with a product: entries
+                            // are finite and >= 0, so this equals fma(., ., +0)
+#pragma unroll
+                            for (int i = 0; i < LR; ++i)
+                                x[i] = k == 0 ? cur[i] * vk : __builtin_fmaf(cur[i], vk, x[i]);
+                            __builtin_amdgcn_sched_barrier(0);   // no loads further ahead than column k + D
                         }
-                        const float vk = pick(sv, sw, k);
-                        // column 0 starts the sums with a product: entries are
-                        // finite and >= 0, so this equals fma(., ., +0)
-#pragma unroll
-                        for (int i = 0; i < LR; ++i) x[i] = k == 0 ? cur[i] * vk : __builtin_fmaf(cur[i], vk, x[i]);
-                        __builtin_amdgcn_sched_barrier(0);   // no loads further ahead than column k + D
+                    };
+                    int ce;
+                    double cc;
+                    if (sd.in_slot == kHotSlot) {
+                        contract([&](int k) { return sv[H][k]; });
+                        ce = se[H];
+                        cc = sc[H];
+                    } else {
+                        const SlotWeights<NS - 1> sw(sd.in_slot);
+                        contract([&](int k) { return pick<NS - 1>(sv, sw, k); });
+                        ce = pick<NS - 1>(se, sw);
+                        cc = pick<NS - 1>(sc, sw);
                     }
-                    e_sum += pick(se, sw) - kMatExp;
-                    cert += pick(sc, sw) + cint;
+                    e_sum += ce - kMatExp;
+                    cert += cc + cint;
                     log2S += kMatExp + a_exp;
                     if (first) {
 #pragma unroll
@@ -256,10 +277,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LaneWaves<LR
             const int sh = a_exp - e;
             const double cnew =
                 m > 0.f ? (ldexp(cert * 1.01, log2S - e) + ldexp(kTau32, -e)) * rcp_up(mant) : 1e300;
-            float v[LR];
+            if (out == kHotSlot) {   // read by the next op
 #pragma unroll
-            for (int i = 0; i < LR; ++i) v[i] = ldexpf(acc[i], sh);
-            put(sv, se, sc, out, v, e_sum + e - a_exp, cnew);
+                for (int i = 0; i < LR; ++i) sv[H][i] = ldexpf(acc[i], sh);
+                se[H] = e_sum + e - a_exp;
+                sc[H] = cnew;
+            } else {
+                float v[LR];
+#pragma unroll
+                for (int i = 0; i < LR; ++i) v[i] = ldexpf(acc[i], sh);
+                put<NS - 1>(sv, se, sc, out, v, e_sum + e - a_exp, cnew);
+            }
         }
         {
             // root: best x_j * prior_j over root sizes j + 1, j = 0..R-1
@@ -308,7 +336,8 @@ int lane_rows_for(int rows)
     return 0;
 }
 
-// Largest slot count the instances of a row count cover.
+// Largest array count the instances of a row count cover (stored slots + the
+// hot array).
 int lane_max_slots(int lane_rows) { return lane_rows == 21 ? 5 : lane_rows == 33 ? 4 : 0; }
 
 // One instance per (rows, slots); with per_cu set, reports how many
