@@ -186,18 +186,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LaneWaves<LR
                     // multiple of 32) and holds Np >= 4 ceil(LR / 4) entries
                     // (Np >= 32 always; 33-row classes have N >= 22, Np >= 64)
                     typedef float f4 __attribute__((ext_vector_type(4)));
-                    float lv[LR];
+                    auto gather = [&](float (&lv)[LR]) {
 #pragma unroll
-                    for (int q = 0; q < (LR + 3) / 4; ++q) {
-                        const f4 c4 = *(const f4*)(col + 4 * q);
+                        for (int q = 0; q < (LR + 3) / 4; ++q) {
+                            const f4 c4 = *(const f4*)(col + 4 * q);
 #pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            if (4 * q + r < LR) lv[4 * q + r] = c4[r];
-                    }
+                            for (int r = 0; r < 4; ++r)
+                                if (4 * q + r < LR) lv[4 * q + r] = c4[r];
+                        }
+                    };
+                    // a first child's factor is gathered into the product itself
                     if (first) {
-#pragma unroll
-                        for (int i = 0; i < LR; ++i) acc[i] = lv[i];
+                        gather(acc);
                     } else {
+                        float lv[LR];
+                        gather(lv);
 #pragma unroll
                         for (int i = 0; i < LR; ++i) acc[i] *= lv[i];
                     }
@@ -210,7 +213,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LaneWaves<LR
                     // next D columns are in flight while column k is used
                     // (D = 2 for 21 rows, 1 for larger vectors: SGPR budget)
                     constexpr int D = LR <= 24 ? 2 : 1;
-                    auto contract = [&](auto vread) {
+                    auto contract = [&](float (&x)[LR], auto vread) {
                         float cbuf[D + 1][LR];
 #pragma unroll
                         for (int d = 0; d < D; ++d)
@@ -233,15 +236,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LaneWaves<LR
                             __builtin_amdgcn_sched_barrier(0);   // no loads further ahead than column k + D
                         }
                     };
+                    // (summing a first child straight into the product takes four
+                    // copies of the contraction, whose registers spill)
                     int ce;
                     double cc;
                     if (sd.in_slot == kHotSlot) {
-                        contract([&](int k) { return sv[H][k]; });
+                        contract(x, [&](int k) { return sv[H][k]; });
                         ce = se[H];
                         cc = sc[H];
                     } else {
                         const SlotWeights<NS - 1> sw(sd.in_slot);
-                        contract([&](int k) { return pick<NS - 1>(sv, sw, k); });
+                        contract(x, [&](int k) { return pick<NS - 1>(sv, sw, k); });
                         ce = pick<NS - 1>(se, sw);
                         cc = pick<NS - 1>(sc, sw);
                     }
@@ -277,17 +282,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LaneWaves<LR
             const int sh = a_exp - e;
             const double cnew =
                 m > 0.f ? (ldexp(cert * 1.01, log2S - e) + ldexp(kTau32, -e)) * rcp_up(mant) : 1e300;
-            if (out == kHotSlot) {   // read by the next op
+            // the output goes to the hot array (the op it would have fed there
+            // has read it), and from there to its slot when a later op reads it
 #pragma unroll
-                for (int i = 0; i < LR; ++i) sv[H][i] = ldexpf(acc[i], sh);
-                se[H] = e_sum + e - a_exp;
-                sc[H] = cnew;
-            } else {
-                float v[LR];
-#pragma unroll
-                for (int i = 0; i < LR; ++i) v[i] = ldexpf(acc[i], sh);
-                put<NS - 1>(sv, se, sc, out, v, e_sum + e - a_exp, cnew);
-            }
+            for (int i = 0; i < LR; ++i) sv[H][i] = ldexpf(acc[i], sh);
+            se[H] = e_sum + e - a_exp;
+            sc[H] = cnew;
+            if (out != kHotSlot) put<NS - 1>(sv, se, sc, out, sv[H], se[H], sc[H]);
         }
         {
             // root: best x_j * prior_j over root sizes j + 1, j = 0..R-1
