@@ -109,7 +109,12 @@ __device__ __forceinline__ RootBest16 root_combine16(RootBest16 r)
 // ---------------------------------------------------------------------------
 // The kernel: the structure of k_prune_mfma<float, ...> (child steps walked
 // with the next step's operands prefetched into the other half of a
-// ping-pong buffer; wave wv computes output tiles wv + W i, i < NTW).
+// ping-pong buffer; wave wv computes the contiguous run [wv n / W,
+// (wv + 1) n / W) of the n output tiles, and an interior step multiplies the
+// matrix only in the 32-row k-blocks that meet the union of its rows' bands:
+// the FP16 halves of an FP32 zero are zero, so the skipped products add
+// exactly 0.  The all-ones sums over the child's blocks, which the bound
+// needs, still run over every block, in order).
 //
 // Partial vectors are stored ready for use as B operands: per 32-row k-block
 // and lane, three half8 -- hi, lo and the bound ê x 2^30 rounded up -- of the
@@ -192,22 +197,57 @@ __global__ __launch_bounds__(64 * W) void k_prune_f16(PruneParams p)
         const int rows_root = bk.Rp / kTile;
 
         int offA0[NTW], offA1[NTW], offL0[NTW], offL1[NTW];
+        // this wave's run of row tiles, non-root [c0, c1) / root [r0, r1)
+        const int c0 = wv * nts / W, c1 = (wv + 1) * nts / W;
+        const int r0 = wv * rows_root / W, r1 = (wv + 1) * rows_root / W;
 #pragma unroll
         for (int i = 0; i < NTW; ++i) {
-            const int t0 = wv + W * i < nts ? wv + W * i : nts - 1;
-            const int t1 = wv + W * i < rows_root ? wv + W * i : rows_root - 1;
+            const int t0 = c0 + i < c1 ? c0 + i : nts - 1;
+            const int t1 = r0 + i < r1 ? r0 + i : rows_root - 1;
             offA0[i] = (kTile * t0 + fl) * Np + 8 * g;
             offA1[i] = (kTile * t1 + fl) * Np + 8 * g;
             offL0[i] = kTile * t0 + 4 * g;
             offL1[i] = kTile * t1 + 4 * g;
         }
 
-        auto prefetch = [&](floatx4 (&pf)[2][NTW], StepDev& sd, int st) {
+        // k-blocks [kbl, kbh) of an interior step: those that meet the union
+        // of the bands of the wave's live row tiles (at least one block; a
+        // one-wave workgroup holds every row tile: no lookup)
+        const __attribute__((address_space(4))) int* band = as_const(p.band);
+        auto kband = [&](int slot, bool root, int& kbl, int& kbh) {
+            kbl = 0;
+            kbh = nkb;
+            if constexpr (W > 1) {
+                if (!band) return;
+                const int a = root ? r0 : c0, b = root ? r1 : c1;
+                int lo = nts, hi = 0;
+#pragma unroll
+                for (int i = 0; i < NTW; ++i)
+                    if (a + i < b) {
+                        const int e = band[(long long)slot * p.band_stride + a + i];
+                        lo = lo < (e & 0xffff) ? lo : (e & 0xffff);
+                        hi = hi > (e >> 16) ? hi : (e >> 16);
+                    }
+                int bl = lo >> 1, bh = (hi + 1) >> 1;
+                bh = bh < nkb ? bh : nkb;
+                if (bl >= bh) {
+                    bl = 0;
+                    bh = 1;
+                }
+                kbl = bl;
+                kbh = bh;
+            }
+        };
+        auto prefetch = [&](floatx4 (&pf)[2][NTW], StepDev& sd, int& kbl, int& kbh, int st) {
             st = __builtin_amdgcn_readfirstlane(st < n_steps ? st : n_steps - 1);
             sd = ld_const(p.steps, st);
             const bool root = (sd.ctl & kStepRoot) != 0;
             const bool leaf = sd.leaf_col >= 0;
-            const long long mo = (long long)nslot[sd.node] * p.slot_elems;
+            const int slot = nslot[sd.node];
+            const long long mo = (long long)slot * p.slot_elems;
+            kbl = 0;
+            kbh = nkb;
+            if (!leaf) kband(slot, root, kbl, kbh);
             const int cn = s_cnt[(leaf ? sd.leaf_col : 0) * 16 + fl] * Np;
 #pragma unroll
             for (int i = 0; i < NTW; ++i) {
@@ -218,8 +258,8 @@ __global__ __launch_bounds__(64 * W) void k_prune_f16(PruneParams p)
                     pf[0][i] = q[0];
                     pf[1][i] = q[zero];
                 } else {
-                    pf[0][i] = *(const floatx4*)(poolh + mo + oa);
-                    pf[1][i] = *(const floatx4*)(pooll + mo + oa);
+                    pf[0][i] = *(const floatx4*)(poolh + mo + oa + 32 * kbl);
+                    pf[1][i] = *(const floatx4*)(pooll + mo + oa + 32 * kbl);
                 }
             }
         };
@@ -227,7 +267,7 @@ __global__ __launch_bounds__(64 * W) void k_prune_f16(PruneParams p)
         floatx4 acc[NTW], acc_e[NTW];
         int e_sum = 0;
 
-        auto consume = [&](const floatx4 (&pf)[2][NTW], const StepDev& sd) {
+        auto consume = [&](const floatx4 (&pf)[2][NTW], const StepDev& sd, int kbl, int kbh) {
             const bool root = (sd.ctl & kStepRoot) != 0;
             const int rows = root ? rows_root : nts;
             const int out_slot = sd.ctl & kStepSlotMask;
@@ -235,8 +275,9 @@ __global__ __launch_bounds__(64 * W) void k_prune_f16(PruneParams p)
             bool live[NTW];
 #pragma unroll
             for (int i = 0; i < NTW; ++i) {
-                live[i] = wv + W * i < rows;
-                tt[i] = live[i] ? wv + W * i : rows - 1;
+                const int u = (root ? r0 : c0) + i;
+                live[i] = u < (root ? r1 : c1);
+                tt[i] = live[i] ? u : rows - 1;
             }
             const bool first = (sd.ctl & kStepFirst) != 0;
             if (first) e_sum = 0;
@@ -281,16 +322,21 @@ __global__ __launch_bounds__(64 * W) void k_prune_f16(PruneParams p)
                     xs = mfma32(ones, be, xs);   // sums over the block of ê and of vhi
                     xv = mfma32(ones, bh, xv);
                 };
-                block(pf[0], pf[1], fh[b0], flo[b0], fe[b0]);
-                if (nkb > 1) {
-                    // blocks 1.. through a ring of RD A and B operand sets
-                    // (one for eight-wave shapes: registers)
+                // blocks below the band: the sums only
+                for (int kb = 0; kb < kbl; ++kb) {
+                    xs = mfma32(ones, fe[b0 + kb * 64], xs);
+                    xv = mfma32(ones, fh[b0 + kb * 64], xv);
+                }
+                block(pf[0], pf[1], fh[b0 + kbl * 64], flo[b0 + kbl * 64], fe[b0 + kbl * 64]);
+                if (kbh - kbl > 1) {
+                    // blocks kbl + 1 .. kbh - 1 through a ring of RD A and B
+                    // operand sets (one for eight-wave shapes: registers)
                     constexpr int RD = W >= 8 ? 1 : 2;
                     floatx4 rh[RD][NTW], rl[RD][NTW];
                     half8 qh[RD], ql[RD], qe[RD];
 #pragma unroll
                     for (int d = 0; d < RD; ++d) {
-                        const int kb = 1 + d < nkb ? 1 + d : nkb - 1;
+                        const int kb = kbl + 1 + d < kbh ? kbl + 1 + d : kbh - 1;
 #pragma unroll
                         for (int i = 0; i < NTW; ++i) {
                             rh[d][i] = *(const floatx4*)(Mh + ao[i] + 32 * kb);
@@ -300,12 +346,12 @@ __global__ __launch_bounds__(64 * W) void k_prune_f16(PruneParams p)
                         ql[d] = flo[b0 + kb * 64];
                         qe[d] = fe[b0 + kb * 64];
                     }
-                    for (int kb = 1; kb < nkb; kb += RD) {
+                    for (int kb = kbl + 1; kb < kbh; kb += RD) {
 #pragma unroll
                         for (int d = 0; d < RD; ++d) {
                             const int k = kb + d;
-                            if (k < nkb) block(rh[d], rl[d], qh[d], ql[d], qe[d]);
-                            const int kn = k + RD < nkb ? k + RD : nkb - 1;
+                            if (k < kbh) block(rh[d], rl[d], qh[d], ql[d], qe[d]);
+                            const int kn = k + RD < kbh ? k + RD : kbh - 1;
 #pragma unroll
                             for (int i = 0; i < NTW; ++i) {
                                 rh[d][i] = *(const floatx4*)(Mh + ao[i] + 32 * kn);
@@ -328,6 +374,11 @@ __global__ __launch_bounds__(64 * W) void k_prune_f16(PruneParams p)
                         await16(__builtin_bit_cast(floatx4, ql[d]));
                         await16(__builtin_bit_cast(floatx4, qe[d]));
                     }
+                }
+                // blocks above the band: the sums only
+                for (int kb = kbh; kb < nkb; ++kb) {
+                    xs = mfma32(ones, fe[b0 + kb * 64], xs);
+                    xv = mfma32(ones, fh[b0 + kb * 64], xv);
                 }
                 const float s_e = xs[0] * 0x1p-14f;
                 // the matrix's representation floor over a row
@@ -491,13 +542,14 @@ __global__ __launch_bounds__(64 * W) void k_prune_f16(PruneParams p)
 
         floatx4 pfa[2][NTW], pfb[2][NTW];
         StepDev sda, sdb;
-        prefetch(pfa, sda, 0);
+        int kla, kha, klb, khb;
+        prefetch(pfa, sda, kla, kha, 0);
         for (int st = 0; st < n_steps; st += 2) {
-            prefetch(pfb, sdb, st + 1);
-            consume(pfa, sda);
+            prefetch(pfb, sdb, klb, khb, st + 1);
+            consume(pfa, sda, kla, kha);
             if (st + 1 < n_steps) {
-                prefetch(pfa, sda, st + 2);
-                consume(pfb, sdb);
+                prefetch(pfa, sda, kla, kha, st + 2);
+                consume(pfb, sdb, klb, khb);
             }
         }
 #pragma unroll
