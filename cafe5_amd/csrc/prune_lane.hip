@@ -99,8 +99,8 @@ __device__ __forceinline__ void put(float (&sv)[NS][LR], int (&se)[NS], double (
     }
 }
 
-// Three waves per SIMD (168 VGPRs) hold 21-row vectors in up to 3 slots;
-// larger shapes ask for two.
+// Three waves per SIMD (168 VGPRs) hold 21-row vectors in up to 3 arrays (two
+// stored slots and the hot array); larger shapes ask for two.
 template <int LR, int NS> struct LaneWaves { static constexpr int v = LR * (NS + 3) <= 130 ? 3 : 2; };
 
 template <int LR, int NS>
