@@ -131,7 +131,15 @@ clade* parse_newick(const std::string& text, bool parse_to_lambdas)
             cur = c;
             ++i;
         } else if (ch == ',') {
-            if (cur == root.get()) throw std::runtime_error("Newick text has siblings of the root");
+            // text without the outer parentheses (clade.cpp:334-341): a comma
+            // at the top level makes a new root whose first child is the
+            // clade read so far
+            if (cur == root.get()) {
+                clade* old = root.release();
+                root.reset(new clade());
+                root->_is_lambda_clade = parse_to_lambdas;
+                root->add_descendant(old);
+            }
             clade* c = new clade();
             c->_is_lambda_clade = parse_to_lambdas;
             cur->_p_parent->add_descendant(c);

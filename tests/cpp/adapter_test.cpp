@@ -747,6 +747,24 @@ static int run_pvalues(gpu_context& gpu, const std::string& file)
 
 static int run_io()
 {
+    // Newick text without the outer parentheses (clade.cpp:334-341): a
+    // top-level comma makes a new root over the clades read
+    auto kids = [](const clade* c) { return std::vector<const clade*>(c->descendant_begin(), c->descendant_end()); };
+    {
+        std::unique_ptr<clade> t(parse_newick("A:1,B:3;"));
+        const auto d = kids(t.get());
+        report("parse_newick_adds_a_root_over_top_level_siblings",
+               t->is_root() && d.size() == 2 && d[0]->get_taxon_name() == "A" && d[1]->get_taxon_name() == "B" &&
+                   d[0]->get_branch_length() == 1 && d[1]->get_branch_length() == 3 && t->get_taxon_name() == "AB");
+    }
+    {
+        std::unique_ptr<clade> t(parse_newick("(A:1,B:1):2,(C:1,D:1):3;"));
+        const auto d = kids(t.get());
+        report("parse_newick_adds_a_root_over_top_level_clades",
+               d.size() == 2 && kids(d[0]).size() == 2 && kids(d[1]).size() == 2 && d[0]->get_branch_length() == 2 &&
+                   d[1]->get_branch_length() == 3 && t->find_descendant("C") != nullptr &&
+                   t->find_descendant("A")->get_branch_length() == 1 && t->get_taxon_name() == "ABCD");
+    }
     // test.cpp:367-426
     {
         std::istringstream ist("Desc\tFamily ID\tA\tB\tC\tD\n\t (null)1\t5\t10\t2\t6\n\t (null)2\t5\t10\t2\t6\n"

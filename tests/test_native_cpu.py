@@ -91,13 +91,14 @@ def test_cpp_interface_built_and_fails_loudly_without_gpu():
 def test_reference_file_readers_through_cpp_interface():
     """The reference's reader and writer tests (test.cpp:367-426,
     :1552-1608) and its error-model epsilon tests (:1483-1495, :1527-1549)
-    through the C++ interface; no device is used."""
+    through the C++ interface, and Newick text without the outer parentheses
+    (clade.cpp:334-341); no device is used."""
     import subprocess
     driver = os.path.join(ROOT, "tests", "cpp", "adapter_test")
     res = subprocess.run([driver, "io"], capture_output=True, text=True, timeout=60)
     lines = res.stdout.splitlines()
     assert res.returncode == 0 and not [l for l in lines if l.startswith("FAIL")], res.stdout + res.stderr
-    assert sum(l.startswith("PASS") for l in lines) == 9
+    assert sum(l.startswith("PASS") for l in lines) == 11
 
 
 def test_tuner_candidates_and_cache_format(tmp_path):
