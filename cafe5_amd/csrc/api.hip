@@ -696,6 +696,7 @@ struct KernelClass {
     MfmaShape s16;       // launch shape of the split-FP16 kernel (BATCHED16)
     MfmaShape r64;       // FP64 recompute of the FP32 pass (a few tiles: latency)
     int lane_rows = 0;   // > 0: the FP32 main pass runs k_prune_lane with this many rows (64-family tiles)
+    bool plain = false;  // PLAIN64, or vectors over 64 tiles (1,024 rows): the main pass is k_prune_plain's (final FP64)
     int lane_slots = 0;  // its slot instance
     int lane_per_cu = 0; // its workgroups resident per CU
     int lane_grid_mult = 0;   // workgroups per resident slot (0: default)
@@ -720,6 +721,7 @@ struct PrunePlan {
     std::vector<int> plain_off;      // offsets of the one-family PLAIN64 recompute lists
     std::vector<int> main_order;     // main-pass launch order (longest first)
     int light_stream = 0;            // the side stream expected to finish its main-pass work first
+    bool has_plain = false;          // a class runs its main pass on PLAIN64 (reads the FP64 pool)
     int plain_list_size = 0;
     int n_split = 0;                 // BATCHED16: pool slots converted to FP16 halves before pruning
     int n_tiles = 0, n_segs = 0, maxLR = 0;
@@ -846,6 +848,8 @@ void place_partials(MfmaShape& s, bool fp64, int n_slots, int slot_tiles, int n_
 // tiles without spilling registers); BATCHED16 runs larger ones on the FP32
 // matrix-core kernel.
 constexpr int kF16MaxTiles = 32;
+constexpr int kMfmaMaxTiles = 64;     // the matrix-core shapes' largest vectors (1,024 rows)
+constexpr int kPlainMaxRows = 2048;   // k_prune_plain's largest instance: 32 rows per lane
 
 // The split-FP16 kernel's partials: per 32-row block and lane three 16-byte
 // B operands (hi, lo, bound).
@@ -1065,9 +1069,10 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     std::vector<std::pair<int, int>> bucket_class;   // (class key, bucket)
     for (int bi = 0; bi < (int)ctx->buckets.size(); ++bi) {
         const BucketHost& bk = ctx->buckets[bi];
-        // vectors (sizes 0..B) and root rows (sizes 0..R) up to 1024 rows:
-        // PLAIN64 covers 16 x 64, the MFMA shapes 64 tiles of 16
-        if (std::max(bk.B + 1, bk.R + 1) > 1024)
+        // vectors (sizes 0..B) and root rows (sizes 0..R) up to 2,048 rows:
+        // PLAIN64 covers 32 x 64; the MFMA shapes take up to 64 tiles of 16
+        // and larger vectors run on PLAIN64 in every path
+        if (std::max(bk.B + 1, bk.R + 1) > kPlainMaxRows)
             return fail(CAFE_GPU_EINVAL, "bucket B=%d R=%d exceeds the sizes this path supports", bk.B, bk.R);
         bucket_class.push_back({path == CAFE_GPU_PATH_PLAIN64 ? nq_of(bk) : nt_of(bk), bi});
     }
@@ -1113,10 +1118,12 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
             kc.seg_begin = kc.seg_end = (int)segs.size();
             // the kernel is chosen here: its tiles carry 64 families (per
             // lane) or 16 (matrix cores)
-            kc.lane_rows = lane_rows_of(bc.first);
-            kc.fb32 = kc.lane_rows > 0 ? 1
-                                       : builtin_fb32(path, bc.first, n_mfma_keys == 1, key_fams[bc.first], ctx->n_cu);
-            if (const cafe_gpu_ctx::TuneEntry* te = find_tuning(ctx, path, bc.first, plan_classes)) {
+            kc.plain = path == CAFE_GPU_PATH_PLAIN64 || bc.first > kMfmaMaxTiles;
+            kc.lane_rows = kc.plain ? 0 : lane_rows_of(bc.first);
+            kc.fb32 = kc.lane_rows > 0 || kc.plain
+                          ? 1
+                          : builtin_fb32(path, bc.first, n_mfma_keys == 1, key_fams[bc.first], ctx->n_cu);
+            if (const cafe_gpu_ctx::TuneEntry* te = kc.plain ? nullptr : find_tuning(ctx, path, bc.first, plan_classes)) {
                 MfmaShape probe;
                 if (tuned_mfma_shape(ctx, path, *te, bc.first, bc.first, probe)) {
                     kc.lane_rows = 0;
@@ -1153,6 +1160,10 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     }
     const int n_cls = (int)pp.classes.size();
     for (auto& kc : pp.classes) {
+        if (kc.plain) {
+            pp.has_plain = true;
+            continue;
+        }
         kc.s32 = kc.fb32 == 2 ? mfma_shape_fb2(kc.nt) : mfma_shape(kc.nt, false);
         kc.s64 = mfma_shape(kc.nt, true);
         kc.r64 = mfma_shape_chain(kc.nt);
@@ -1184,7 +1195,8 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
     // Measured shapes from the context's tuning table replace the built-in
     // ones where they exist and fit the class.
     for (auto& kc : pp.classes)
-        if (int rc = apply_tuning(ctx, path, kc, plan_classes)) return rc;
+        if (!kc.plain)
+            if (int rc = apply_tuning(ctx, path, kc, plan_classes)) return rc;
     // Main-pass classes to side streams, longest first onto the least loaded
     // stream.  Estimated cost: contraction work (tiles x nt^2, plus a per-tile
     // overhead) and a floor for the serial chain of one large tile; a
@@ -1268,7 +1280,7 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         KernelClass& kc = pp.classes[ci];
         const int n = pp.class_counts[ci];
         size_t mb = 0;
-        if (path == CAFE_GPU_PATH_PLAIN64) mb = plain_bytes(n);
+        if (kc.plain) mb = plain_bytes(n);
         else if (path == CAFE_GPU_PATH_BATCHED64)
             mb = mfma_scratch_bytes(kc.s64, true, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s64, n));
         else if (kc.lane_rows > 0) mb = 0;
@@ -1276,11 +1288,11 @@ int plan_prune(cafe_gpu_ctx* ctx, EvalBufs& b, int path, int model, int n_points
         else mb = mfma_scratch_bytes(kc.s32, false, ctx->n_slots, kc.slot_tiles, pp.grid(kc.s32, n));
         kc.off_main = main_sz;
         main_sz += align256(mb);
-        if (path == CAFE_GPU_PATH_BATCHED32 || path == CAFE_GPU_PATH_BATCHED16) {
+        if ((path == CAFE_GPU_PATH_BATCHED32 || path == CAFE_GPU_PATH_BATCHED16) && !kc.plain) {
             kc.off_r64 = r64_sz;
             r64_sz += align256(mfma_scratch_bytes(kc.r64, true, ctx->n_slots, kc.slot_tiles, pp.grid(kc.r64, kc.r16)));
         }
-        if (path != CAFE_GPU_PATH_PLAIN64) {
+        if (path != CAFE_GPU_PATH_PLAIN64 && !kc.plain) {
             kc.off_rp = rp_sz;
             rp_sz += align256(plain_bytes(pp.plain_off[ci + 1] - pp.plain_off[ci]));
         }
@@ -1382,7 +1394,7 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
         q.tile_counter = d_ctr + ci;
         q.slot_tiles = kc.slot_tiles;
         q.scratch = scratch + kc.off_main;
-        if (pp.path == CAFE_GPU_PATH_PLAIN64) {
+        if (kc.plain) {   // final FP64 values: nothing to recompute
             q.scratch_per_wg = pp.per_wg_plain;
             HIP_OR_FAIL(launch_prune_plain(kc.nq, pp.grid(1, n), q, s));
         } else if (pp.path == CAFE_GPU_PATH_BATCHED64) {
@@ -1404,6 +1416,7 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
     auto stage_r64 = [&](int ci, hipStream_t s) -> int {
         if (pp.path != CAFE_GPU_PATH_BATCHED32 && pp.path != CAFE_GPU_PATH_BATCHED16) return CAFE_GPU_OK;
         const KernelClass& kc = pp.classes[ci];
+        if (kc.plain) return CAFE_GPU_OK;
         HIP_OR_FAIL(launch_pack(segs + kc.seg_begin, kc.seg_end - kc.seg_begin, b.buckets.as<BucketDev>(),
                                 pp.base.perm, b.flag.as<int>(), pp.base.F, kFlagRecompute64, b.fam_list.as<int>(),
                                 fb64, coff, fbc, fbc + fam_off, kTile, s));
@@ -1419,6 +1432,7 @@ int enqueue_prune(cafe_gpu_ctx* ctx, EvalBufs& b, const PrunePlan& pp, hipStream
     auto stage_plain = [&](int ci, hipStream_t s) -> int {
         if (pp.path == CAFE_GPU_PATH_PLAIN64) return CAFE_GPU_OK;
         const KernelClass& kc = pp.classes[ci];
+        if (kc.plain) return CAFE_GPU_OK;
         HIP_OR_FAIL(launch_pack(segs + kc.seg_begin, kc.seg_end - kc.seg_begin, b.buckets.as<BucketDev>(),
                                 pp.base.perm, b.flag.as<int>(), pp.base.F, kFlagRecomputePlain,
                                 b.fam_list.as<int>(), fbplain, coff + n_cls + 1, fbc + n_cls, fbc + fam_off + 1, 1,
@@ -1687,7 +1701,7 @@ int plan_enqueue(cafe_gpu_plan* pl, bool timed)
     // BATCHED32 reads the FP64 pool only in its recompute stages: concurrent
     // runs with the recurrence build make the FP32 pools first and the FP64
     // one beside the main pass
-    const bool split = !timed && pl->algo != 0 &&
+    const bool split = !timed && pl->algo != 0 && !pl->pp.has_plain &&
                        (pl->pp.path == CAFE_GPU_PATH_BATCHED32 || pl->pp.path == CAFE_GPU_PATH_BATCHED16);
     if (timed) HIP_OR_FAIL(hipEventRecord(pl->ev[0], st));
     HIP_OR_FAIL(launch_build(pl->algo, pl->d_lam.as<double>(), pl->d_t.as<double>(), ctx->lg.as<double>(), pl->n_keys,
@@ -2070,7 +2084,7 @@ int cafe_gpu_plan_class_times(cafe_gpu_plan* pl, int cap, int* n_classes, int* c
             const KernelClass& kc = pl->pp.classes[i];
             const int n = pl->pp.class_counts[i];
             int* l = launch + 6 * i;
-            if (pl->path == CAFE_GPU_PATH_PLAIN64) {
+            if (kc.plain) {
                 l[0] = 2;
                 l[1] = kc.nq;
                 l[2] = 1;

@@ -330,6 +330,7 @@ hipError_t launch_prune_plain(int nq, int grid, const PruneParams& p, hipStream_
     else if (nq <= 4) hipLaunchKernelGGL(k_prune_plain<4>, dim3(grid), dim3(64), shm, st, q);
     else if (nq <= 8) hipLaunchKernelGGL(k_prune_plain<8>, dim3(grid), dim3(64), shm, st, q);
     else if (nq <= 16) hipLaunchKernelGGL(k_prune_plain<16>, dim3(grid), dim3(64), shm, st, q);
+    else if (nq <= 32) hipLaunchKernelGGL(k_prune_plain<32>, dim3(grid), dim3(64), shm, st, q);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
