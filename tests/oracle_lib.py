@@ -239,12 +239,14 @@ class Oracle:
 
     @staticmethod
     def reconstruct(tree: Tree, counts_row, lam, B: int, R: int, prior, N: Optional[int] = None,
-                    with_L: bool = False):
+                    with_L: bool = False, matrices=None):
         """Pupko joint reconstruction of one family: states per node (and the
-        nodes' L vectors).  lam: one value or one per node."""
+        nodes' L vectors).  lam: one value or one per node.  matrices: the
+        (pool, node_mat) of node_matrices at these lambdas and N, to share
+        one build between families."""
         N = N or max(B, R) + 1
         node_lambda = np.full(tree.n_nodes, lam) if np.ndim(lam) == 0 else np.asarray(lam)
-        pool, node_mat = Oracle.node_matrices(tree, node_lambda, N)
+        pool, node_mat = matrices if matrices is not None else Oracle.node_matrices(tree, node_lambda, N)
         st, keep = Oracle.tree_struct(tree)
         cnt = np.ascontiguousarray(counts_row, dtype=np.int32)
         pr = np.ascontiguousarray(prior, dtype=np.float64)

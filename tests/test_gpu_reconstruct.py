@@ -75,6 +75,24 @@ def test_large_bound_one_family_per_wave(eng):
         assert np.array_equal(got[f], want), (f, np.flatnonzero(got[f] != want))
 
 
+def test_bound_past_1024_rows(eng):
+    # bounds of 1,100: k_reconstruct on vectors of 1,101 rows, the sizes the
+    # likelihood path runs on its plain kernel; one branch length, so the
+    # oracle builds its one 1,101 x 1,101 matrix once for every family
+    tree = wl.parse_newick("((A:5,B:5):5,(C:5,D:5):5);")
+    B = R = 1100
+    lam = 0.002
+    counts = wl.simulate_families(tree, np.array([1, 350, 700, 900]), np.full(tree.n_nodes, lam), 9, cap=B - 1)
+    assert counts.max() > 850
+    slots = _setup(eng, tree, counts, lam, B, R, native.BUILD_RECURRENCE)
+    prior = priors.uniform(R, R + 1)
+    got = eng.reconstruct(slots, prior, B, R)
+    mats = Oracle.node_matrices(tree, np.full(tree.n_nodes, lam), B + 1)
+    for f in range(counts.shape[0]):
+        want = Oracle.reconstruct(tree, counts[f], lam, B, R, prior, matrices=mats)
+        assert np.array_equal(got[f], want), (f, np.flatnonzero(got[f] != want))
+
+
 def test_many_families_in_chunks(eng):
     # more families than one launch holds (8192): every chunk's rows land in place
     w = wl.config_base(seed=8, F=9000)
