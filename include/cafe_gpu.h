@@ -96,7 +96,11 @@ const char *cafe_gpu_last_error(void);
 
 /* Replaces matrix_cache(int N) (src/matrix_cache.h:91): sets the matrix
  * dimension N = max(R, B) + 1 and the pool capacity in matrices; drops all
- * built matrices. */
+ * built matrices.  2 <= N <= 2047 (the log-gamma table holds 0..4095).  On
+ * every path, size classes whose vectors exceed 1,024 rows (64 tiles of 16)
+ * run their main pass on the PLAIN64 kernel (final FP64 values, the
+ * reference's summation order), the role of the reference's heap-allocated
+ * path for large families (src/probability.cpp:239-310). */
 int cafe_gpu_matrix_reset(cafe_gpu_ctx *ctx, int N, int capacity);
 
 /* Replaces matrix_cache::precalculate_matrices (src/matrix_cache.cpp:113-163):
