@@ -41,6 +41,7 @@ F16_SHAPES = [s for s in MFMA_SHAPES[:12] if s[1] <= 8]
 FB2_SHAPES = [(2, 4), (1, 8), (2, 8), (1, 16), (2, 16)]
 FB2_MAX_TILES = 32
 LANE_TILES = 3          # the per-lane kernel takes vectors of up to 33 rows (3 tiles)
+MAX_SHAPE_TILES = 64    # the matrix-core shapes' largest vectors (1,024 rows)
 
 
 def candidates(path: int, nt: int) -> List[Tuple[int, int, int, int, int]]:
@@ -66,6 +67,13 @@ def candidates(path: int, nt: int) -> List[Tuple[int, int, int, int, int]]:
                 for gm in ((4, 8, 16) if lds else (1, 2, 4)):
                     out.append((2, ntw, w, lds, gm))
     return out
+
+
+def plan_key_of(nts) -> str:
+    """The plan= key of a plan's size classes: their tiles per vector, up to
+    64 (classes of larger vectors run the plain FP64 kernel, which has no
+    launch shape to tune and no place in the engine's key)."""
+    return ",".join(str(int(n)) for n in sorted(nts) if int(n) <= MAX_SHAPE_TILES)
 
 
 def _class_ms(plan, nt: int, reps: int = 3) -> float:
@@ -112,7 +120,7 @@ def tune_plan(eng: native.Engine, make_plan, path: int, leaves: int, log=print, 
     nts, tiles, cms, launch = base.class_times()
     cur_ms = _step_ms(base) if objective == "step" else None
     base.close()
-    plan_key = ",".join(str(int(n)) for n in sorted(nts)) if objective == "step" else ""
+    plan_key = plan_key_of(nts) if objective == "step" else ""
     chosen = []
     for nt, default_ms, l in sorted(zip(nts, cms, launch), key=lambda x: -x[1]):
         nt = int(nt)

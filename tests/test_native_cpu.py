@@ -110,6 +110,9 @@ def test_tuner_candidates_and_cache_format(tmp_path):
     assert {(ntw, w) for k, ntw, w, _, _ in c13 if k == 2} == {(2, 8), (1, 16)}
     assert {(ntw, w) for k, ntw, w, _, _ in tune.candidates(native.PATH_BATCHED32, 32) if k == 2} == {(2, 16)}
     assert not any(k == 2 for k, *_ in tune.candidates(native.PATH_BATCHED32, 33))   # up to 32 tiles
+    for path in (native.PATH_BATCHED64, native.PATH_BATCHED32, native.PATH_BATCHED16):
+        assert tune.candidates(path, 69) == []   # past 64 tiles: the plain kernel, no shape to tune
+    assert tune.plan_key_of([69, 2, 13, 3]) == "2,3,13"   # the engine's plan key holds classes of 1..64 tiles
     assert not any(k == 2 for k, *_ in tune.candidates(native.PATH_BATCHED64, 13))   # FP32 only
     c2 = tune.candidates(native.PATH_BATCHED32, 2)
     assert any(k == 1 for k, *_ in c2) and any(k == 0 for k, *_ in c2)
