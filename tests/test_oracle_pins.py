@@ -385,6 +385,21 @@ def test_ref_prune_bitwise(use_err):
 
 
 @needs_ref
+def test_ref_prune_heap_path_bitwise():
+    # bounds at the reference build's MAX_STACK_FAMILY_SIZE (1,000) switch it to
+    # compute_node_probability_large_families (probability.cpp:239-310, 326-329);
+    # one branch length: one 1,001 x 1,001 matrix.  Counts near the bound, so
+    # the root rows up to size 1,000 carry values
+    tree = wl.parse_newick("((A:5,B:5):5,(C:5,D:5):5);")
+    B = R = 1000
+    counts = np.array([960, 975, 990, 999])
+    ref = Ref.prune(tree, counts, 0.002, 1.0, B, R)
+    got = Oracle.prune(tree, counts, np.full(tree.n_nodes, 0.002), B, R)
+    assert ref[R - 1] > 0 and (ref > 0).sum() > 100
+    assert np.array_equal(ref, got)
+
+
+@needs_ref
 def test_ref_base_model_seeded():
     w = wl.config_base(seed=1, F=150)
     B = R = 40
