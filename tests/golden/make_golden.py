@@ -55,6 +55,13 @@ def main():
                            "root": Ref.prune(tree, counts, 0.004, 1.0, B, R, err if use_err else None).tolist()})
     out["prune"] = {"tree": tree_json(tree), "cases": prunes}
 
+    # the same at bound 1,000, where the reference runs its heap path
+    # (compute_node_probability_large_families, probability.cpp:239-310)
+    tree = wl.parse_newick("((A:5,B:5):5,(C:5,D:5):5);")
+    counts = np.array([960, 975, 990, 999])
+    out["prune_heap"] = {"tree": tree_json(tree), "counts": counts.tolist(), "lambda": 0.002, "B": 1000, "R": 1000,
+                         "root": Ref.prune(tree, counts, 0.002, 1.0, 1000, 1000).tolist()}
+
     # base_model::infer_family_likelihoods (base_model.cpp:53-100): uniform prior
     w = wl.config_base(seed=1, F=60)
     B = R = 40

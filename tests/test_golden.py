@@ -36,6 +36,13 @@ def test_golden_prune():
         assert np.array_equal(got, np.array(c["root"]))
 
 
+def test_golden_prune_heap_path():
+    g = GOLD["prune_heap"]
+    tree = _tree(g["tree"])
+    got = Oracle.prune(tree, np.array(g["counts"]), np.full(tree.n_nodes, g["lambda"]), g["B"], g["R"])
+    assert np.array_equal(got, np.array(g["root"]))
+
+
 def test_golden_base_uniform():
     g = GOLD["base_uniform"]
     tree = _tree(g["tree"])
