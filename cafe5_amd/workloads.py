@@ -182,7 +182,11 @@ def yule_tree(n_taxa: int, seed: int, height: float = 100.0) -> Tree:
 
 
 def parse_newick(text: str) -> Tree:
-    """Minimal Newick reader (names and branch lengths) into a Tree."""
+    """Minimal Newick reader (names and branch lengths) into a Tree.  Text
+    without the outer parentheses is read as the reference reads it
+    (clade.cpp:334-341): a comma at the top level makes a new root whose
+    first child is the clade read so far, and the clades after it are its
+    siblings."""
     text = text.strip().rstrip(";")
     parent: List[int] = []
     length: List[float] = []
@@ -217,7 +221,18 @@ def parse_newick(text: str) -> Tree:
             length[me] = float(text[start:pos])
         return me
 
-    node(-1)
+    root = node(-1)
+    if pos < len(text) and text[pos] == ",":
+        top = len(parent)
+        parent.append(-1)
+        length.append(0.0)
+        raw_names.append("")
+        parent[root] = top
+        while pos < len(text) and text[pos] == ",":
+            pos += 1
+            node(top)
+    if pos != len(text):
+        raise ValueError(f"unexpected {text[pos]!r} at offset {pos} of the Newick text")
     n = len(parent)
     is_leaf = [True] * n
     for p in parent:

@@ -74,6 +74,24 @@ def test_key_table_dedups_quantized_keys():
     assert len(kt.keys) == 3
 
 
+def test_newick_without_outer_parentheses():
+    # clade.cpp:334-341: a top-level comma makes a new root over the clade read
+    # so far and the clades after it
+    from cafe5_amd import workloads as wl
+
+    def same(a, b):
+        return (all(np.array_equal(getattr(a, k), getattr(b, k)) for k in ("parent", "leaf_col", "length"))
+                and list(a.names) == list(b.names))
+
+    assert same(wl.parse_newick("(A:1,B:1):2,C:3;"), wl.parse_newick("((A:1,B:1):2,C:3);"))
+    assert same(wl.parse_newick("(A:1,B:1):2,C:3,D:4;"), wl.parse_newick("((A:1,B:1):2,C:3,D:4);"))
+    assert same(wl.parse_newick("A:1,B:2"), wl.parse_newick("(A:1,B:2);"))
+    t = wl.parse_newick("(A:1,B:1):2,C:3;")
+    assert t.n_nodes == 5 and t.leaf_names() == ["A", "B", "C"] and t.names[t.root] == "ABC"
+    with pytest.raises(ValueError):
+        wl.parse_newick("(A:1,B:1))")   # text past the tree is an error, not dropped
+
+
 def test_cpp_interface_built_and_fails_loudly_without_gpu():
     import subprocess
     import torch
