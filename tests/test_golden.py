@@ -92,3 +92,15 @@ def test_golden_priors():
         assert np.array_equal(Oracle.prior_poisson(c["lambda"], c["max"], c["R"]), np.array(c["p"]))
     m = pr["map"]
     assert np.array_equal(Oracle.prior_map(m["sizes"], m["counts"], m["R"]), np.array(m["p"]))
+
+
+def test_golden_priors_python_package():
+    # the priors the Python package hands the engine (cafe5_amd/priors.py)
+    from cafe5_amd import priors
+    pr = GOLD["priors"]
+    for c in pr["uniform"]:
+        assert np.array_equal(priors.uniform(c["max"], c["R"]), np.array(c["p"]))
+    for c in pr["poisson"]:
+        assert np.array_equal(priors.poisson(c["lambda"], c["max"], c["R"]), np.array(c["p"]))
+    m = pr["map"]
+    assert np.array_equal(priors.from_counts(dict(zip(m["sizes"], m["counts"])), m["R"]), np.array(m["p"]))

@@ -445,6 +445,23 @@ def test_ref_gamma_model_seeded():
 
 
 @needs_ref
+def test_ref_priors_python_package_sweep():
+    # cafe5_amd/priors.py against root_equilibrium_distribution, bitwise
+    from cafe5_amd import priors
+    rng = np.random.default_rng(1)
+    for _ in range(100):
+        lam = float(np.exp(rng.uniform(np.log(0.1), np.log(40))))
+        nv, R = int(rng.integers(5, 300)), int(rng.integers(5, 400))
+        assert np.array_equal(priors.poisson(lam, nv, R), Ref.prior(1, nv, lam, R)), (lam, nv, R)
+        m = int(rng.integers(2, 500))
+        assert np.array_equal(priors.uniform(m, R), Ref.prior(0, m, 0.0, R)), (m, R)
+        sizes = sorted(rng.choice(60, int(rng.integers(1, 8)), replace=False).tolist())
+        counts = rng.integers(1, 50, len(sizes)).tolist()
+        assert np.array_equal(priors.from_counts(dict(zip(sizes, counts)), R),
+                              Ref.prior(2, 0, 0.0, R, sizes, counts)), (sizes, counts, R)
+
+
+@needs_ref
 def test_ref_discrete_gamma_sweep():
     for K in (1, 2, 3, 4, 5, 8, 16):
         for alpha in np.exp(np.linspace(np.log(0.05), np.log(20.0), 23)):
