@@ -84,6 +84,16 @@ def test_golden_discrete_gamma():
         assert np.array_equal(r, np.array(c["multipliers"])), (c["K"], c["alpha"])
 
 
+def test_golden_discrete_gamma_python_package():
+    # the categories the Python package's gamma models use (the C++ mirror's
+    # get_gamma through cafe5_amd.model)
+    from cafe5_amd.model import get_gamma
+    for c in GOLD["discrete_gamma"]:
+        f, r = get_gamma(c["K"], c["alpha"])
+        assert np.array_equal(f, np.array(c["probs"])), c["K"]
+        assert np.array_equal(r, np.array(c["multipliers"])), (c["K"], c["alpha"])
+
+
 def test_golden_priors():
     pr = GOLD["priors"]
     for c in pr["uniform"]:
